@@ -1,0 +1,212 @@
+/*
+ * ipg_oracle.h -- CPU restatement of the IP-Glasma hot path (schenke/ipglasma)
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (ipglasma_amd/, include/ipg.h) never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm in plain C99 (strict IEEE:
+ * built with -O2 -fno-fast-math -ffp-contract=off) and cites the reference
+ * file:line it follows.  Host-visible layouts:
+ *   site index      s = ix*N + iy  (ix <-> x, stride N; iy <-> y, contiguous)
+ *                   (Lattice.cpp:19-28)
+ *   3x3 complex     9 complex = 18 doubles per site, row-major e[3*i+j]
+ *                   (Matrix.h:45), interleaved re,im
+ *   field arrays    [S][18] per field ("site-major")
+ *
+ * Parity pinning (what checks each part, see tests/test_oracle_*.py):
+ *   orc_mt_*         std::mt19937_64 known-answer + the C++ standard's 10000th
+ *                    output KAT; real1/2/3 closed forms
+ *   orc_philox4x32   Random123 published known-answer vectors
+ *   orc_gauss        Marsaglia polar closed form from recorded uniforms
+ *   orc_fft2d        numpy.fft.fft2/ifft2; plane-wave eigenfunction
+ *   orc_expm_coeff   scipy.linalg.expm(i Q^a t^a); unitarity; det=1
+ *   orc_newton_link  residual F_a = 0; U1=1 => U=U2, U2=1 => U=U1; abelian
+ *                    closed form
+ *   orc_evolve_*     vacuum stays vacuum; gauge covariance; Gauss law kept
+ *   orc_tmunu        tracelessness T^tt - T^xx - T^yy - tau^2 T^ee = 0;
+ *                    vacuum T = 0; gauge invariance
+ *   orc_sample_*     Woods-Saxon KS test; d_min; recentring; rotation norm
+ *   orc_tp_sum       single-nucleon Gaussian closed form
+ *   orc_wounding     brute-force pair loop / hard-sphere closed form
+ *   orc_g2mu2        table-node exactness; fixed-point residual
+ * Functions with no check of their own are marked "parity unpinned" below.
+ */
+#ifndef IPG_ORACLE_H
+#define IPG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers (Random.cpp:67-204) ---------------------------------- */
+typedef struct {
+    uint64_t mt[312];
+    int mti;
+    int iset;
+    double gset;
+} orc_mt;
+
+void orc_mt_init(orc_mt *r, uint64_t seed);
+uint64_t orc_mt_u64(orc_mt *r);
+double orc_mt_real1(orc_mt *r);
+double orc_mt_real2(orc_mt *r);
+double orc_mt_real3(orc_mt *r);
+double orc_mt_gauss(orc_mt *r);
+/* fill n draws of kind k (0:u64 as double bits, 1:real1, 2:real2, 3:real3,
+ * 4:gauss) -- convenience for tests */
+void orc_mt_fill(orc_mt *r, int kind, int64_t n, double *out);
+
+/* Philox4x32-10 (Salmon et al., SC'11) */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]);
+/* the 8 colour normals of colour-charge slice (nucleus, k) at site s: the
+ * project's counter map, shared with the device generator */
+void orc_rho_normals_site(uint64_t seed, int nucleus, int k, uint32_t s,
+                          double z[8]);
+void orc_rho_normals_slice(uint64_t seed, int nucleus, int k, int S,
+                           double *z /* [S][8] */);
+/* deterministic restart draws for the tau=0 root finder (per link) */
+void orc_newton_restart_normals(uint64_t seed, uint32_t link, uint32_t restart,
+                                double alpha[8]);
+
+/* ---- SU(3) algebra (Matrix.cpp:310-426, Init.cpp:3483-3499) -------------- */
+void orc_expm_coeff(const double Q[8], double coeff[18]);
+void orc_u_from_exponent(const double Q[8], double U[18]);
+void orc_matmul(const double *A, const double *B, double *C);
+
+/* ---- FFT (FFT.cpp:492-610 semantics, unshifted form) --------------------- */
+/* in-place 2D DFT of complex data[N][N] (interleaved). sign=+1 forward
+ * (exp(-i...)), sign=-1 inverse with 1/N^2 */
+int orc_fft2d(int N, double *data, int sign);
+
+/* ---- nucleons and colour charge (Init.cpp, Glauber.cpp) ----------------- */
+typedef struct {
+    double x, y, z, phi;
+    int collided;
+    int proton;
+} orc_nucleon;
+
+typedef struct {
+    /* projectile (A) and target (B) nucleus data after FindNucleusData2 */
+    int A1, Z1, A2, Z2;
+    double R1, a1, beta2_1, beta3_1, beta4_1, gamma_1, dR1, da1;
+    double R2, a2, beta2_2, beta3_2, beta4_2, gamma_2, dR2, da2;
+    int force_dmin;
+    double d_min;
+    double bmin, bmax;
+    int linear_b;
+    int rotate_rp;
+    double xi; /* proton anisotropy */
+    double sigma_nn_mb;
+    int gaussian_wounding;
+} orc_glauber_cfg;
+
+typedef struct {
+    double b, phiRP;
+    int Ncoll, Npart;
+} orc_event_info;
+
+/* b draw, sampleTA (A then B, rotations), reaction plane shift, wounding.
+ * nucleus arrays must hold A1 / A2 entries.  Returns 0, or <0 on an
+ * unsupported configuration (deuteron, file-based nuclei). */
+int orc_sample_event(orc_mt *r, const orc_glauber_cfg *cfg, orc_nucleon *nucA,
+                     orc_nucleon *nucB, orc_event_info *info);
+
+/* T_p sums (Init.cpp:1009-1115) */
+void orc_tp_sum(int N, double L, double BG, double xi, const orc_nucleon *nuc,
+                int A, double *Tp);
+
+/* Q_s^2 table lookup (Init.cpp:686-749); table Qs2[iT*44+iy], Tlist[200] */
+double orc_nuclear_qs2(const double *Tlist, const double *Qs2, double T,
+                       double y);
+
+/* g2mu2 for both nuclei (Init.cpp:1219-1421) with the serial-chain rapidity
+ * start of a single-threaded run */
+typedef struct {
+    int N;
+    double L, g, BG, rmax, QsmuRatio, QsmuRatioB;
+    double xFactor, roots, rapidity;
+    int useFluctuatingx;
+} orc_qs_cfg;
+void orc_g2mu2(const orc_qs_cfg *c, const double *Tlist, const double *Qs2,
+               const double *TpA, const double *TpB, double *g2mu2A,
+               double *g2mu2B);
+
+/* statistics + success (Init.cpp:1423-1618). out[0..7] = averageQs (max),
+ * averageQsAvg, averageQsmin, Tpp, averageQs2min2*a^2/hbarc^2, count, alphas,
+ * success */
+void orc_stats(const orc_qs_cfg *c, double sigma_nn_mb, int Npart,
+               double minimumQs2ST, const orc_nucleon *nucA, int A1,
+               const orc_nucleon *nucB, int A2, const double *g2mu2A,
+               const double *g2mu2B, const double *TpA, const double *TpB,
+               double out[8]);
+
+/* ---- Wilson lines (Init.cpp:1678-1850) ---------------------------------- */
+/* rho normals: if z != NULL it holds [2][Ny][S][8]; else if mt != NULL the
+ * reference MT Gauss stream is drawn (pos-major, colour-minor, nucleus A
+ * then B); else the project Philox map with `seed` is used.  V[2][S][18]. */
+int orc_setv(int N, int Ny, double g, double m_lat, double uvdamp_lat,
+             const double *g2mu2A, const double *g2mu2B, const double *z,
+             orc_mt *mt, uint64_t seed, double *V);
+
+/* ---- tau=0 fields (Init.cpp:2303-2696, 3344-3481) ----------------------- */
+typedef struct {
+    int iterations; /* total Newton iterations over all restarts */
+    int restarts;
+    int converged;
+    int used_analytic_jacobian;
+} orc_newton_stat;
+
+int orc_newton_link(const double *U1, const double *U2, uint64_t seed,
+                    uint32_t link, double *Usol, orc_newton_stat *st);
+
+/* from V_A, V_B: NaN guard, links, Newton per link, pi.  Output state (all
+ * [S][18]): Ux, Uy, E1=0, E2=0, phi=0, pi.  stats[0]=sum iterations,
+ * stats[1]=max iterations, stats[2]=restarts, stats[3]=unconverged links,
+ * stats[4]=links with analytic Jacobian */
+int orc_init_fields(int N, double g, double *VA, double *VB, uint64_t seed,
+                    double *Ux, double *Uy, double *E1, double *E2,
+                    double *phi, double *pi, double stats[5]);
+
+/* ---- evolution (Evolution.cpp:35-293, 366-431) -------------------------- */
+void orc_evolve_u(int N, double g, double dtau, double tau, const double *E1,
+                  const double *E2, double *Ux, double *Uy);
+void orc_evolve_phi(int N, double dtau, double tau, const double *pi,
+                    double *phi);
+void orc_evolve_pi(int N, double dtau, double tau, const double *Ux,
+                   const double *Uy, const double *phi, double *pi);
+void orc_evolve_e(int N, double g, double dtau, double tau, const double *Ux,
+                  const double *Uy, const double *phi, double *E1,
+                  double *E2);
+
+/* full Evolution::run schedule through itmax; T^{mu nu} (lattice-scaled to
+ * fm units) at it=itmax before the final half step into T[10][S] in the
+ * order tautau, xx, yy, etaeta, taux, tauy, taueta, xy, xeta, yeta.
+ * obs[0] = dE/deta [GeV], obs[1] = Gauss-law max, obs[2] = tau [fm] */
+int orc_run(int N, double g, double L, double dtau, int itmax, double *Ux,
+            double *Uy, double *E1, double *E2, double *phi, double *pi,
+            double *T, double obs[3]);
+
+/* T^{mu nu} at step it (Evolution.cpp:659-1246), physical units */
+void orc_tmunu(int N, double g, double a, double dtau, int it,
+               const double *Ux, const double *Uy, const double *E1,
+               const double *E2, const double *phi, const double *pi,
+               double *T);
+double orc_gauss_law(int N, const double *Ux, const double *Uy,
+                     const double *E1, const double *E2, const double *phi,
+                     const double *pi);
+
+/* dE/deta = tau * a^2 * hbarc * sum T^tautau  (GeV) */
+double orc_dedeta(int N, double a, double tau_fm, const double *Ttautau);
+
+/* lattice-unit derived step count and dtau (main.cpp:531-534,
+ * Evolution.cpp:396) */
+void orc_time_steps(double L, int N, double maxtime, double *dtau, int *itmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,202 @@
+"""Benchmark: IP-Glasma events/s on one MI355X, batched vs one launch
+sequence per event, Pb+Pb at N=512, FP64.
+
+A "step" is one pass of the hot path over one batch of synthetic events:
+nucleons -> colour charge -> Wilson lines (Ny=50 FFT Poisson solves) ->
+tau=0 root finder -> 69 leapfrog steps -> T^{mu nu} -> dE/deta, for every
+event of the batch, timed from the host.  Inputs are generated on the device
+(counter-based colour-charge normals); the Q_s table is resident.
+
+Printed JSON fields beyond the driver contract:
+  plain_events_per_s  one launch sequence per event (same events, same code)
+  batched_over_plain  the ratio
+  roofline            the leapfrog step kernel: 1728 algorithmic bytes per
+                      site-step (read+write Ux, Uy, E1, E2, phi, pi once),
+                      HIP-event timed on the library's stream
+  cpu_baseline        the C oracle (OpenMP) on a bounded sample of the same
+                      event, extrapolated to one event
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SITE_STEP = 1728
+
+
+def pbpb_params(I, N=512, L=30.0, Ny=50, seed=1):
+    p = I.default_params()
+    p.set(size=N, Ny=Ny, L=L, g=2.0, m=0.2, BG=4.0, UVdamp=0.0, QsmuRatio=0.8, rmax=1e7, roots=5020.0,
+          Rapidity=0.0, useFluctuatingx=1, xFromThisFactorTimesQs=1.0, SigmaNN=70.0, maxtime=0.4,
+          Projectile="Pb", Target="Pb", setWSDeformParams=1, R_WS=6.62, a_WS=0.546, beta2=0.0,
+          beta3=0.0, beta4=0.0, gamma=0.0, force_dmin_flag=0, d_min=0.9, dR_np=0.0, da_np=0.0,
+          bmin=0.0, bmax=0.0, samplebFromLinearDistribution=1, rotateReactionPlane=1,
+          gaussianWounding=1, computeGluonMultiplicity=0, writeOutputs=0, useTimeForSeed=0,
+          seed=seed, minimumQs2ST=0)
+    return p
+
+
+def run_plain(I, p, seeds):
+    t0 = time.perf_counter()
+    out = []
+    for s in seeds:
+        ev = I.Event(p, seed=s)
+        tries = 0
+        while not ev.init():
+            tries += 1
+            if tries > 50:
+                raise RuntimeError("event never succeeded")
+        ev.run()
+        out.append(ev.observables())
+        ev.close()
+    return time.perf_counter() - t0, out
+
+
+def run_batched(I, p, seeds, use_graph=True):
+    t0 = time.perf_counter()
+    b = I.Batch([p] * len(seeds), seeds)
+    b.run(use_graph=use_graph)
+    out = [b.observables(e) for e in range(len(seeds))]
+    b.close()
+    return time.perf_counter() - t0, out
+
+
+def roofline(I, p):
+    ev = I.Event(p, seed=12345)
+    while not ev.init():
+        pass
+    ev.enable_timing(True)
+    ev.run()
+    t = ev.timing()
+    ev.close()
+    S = p.size * p.size
+    avg_ms = t["step_ms_total"] / max(1, t["nsteps"])
+    achieved = BYTES_PER_SITE_STEP * S / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5), "launches": t["nsteps"],
+            "bytes_per_launch": BYTES_PER_SITE_STEP * S}
+
+
+def cpu_baseline(p):
+    """Oracle timing on a bounded sample of the event, extrapolated."""
+    import ctypes as C
+    import numpy as np
+    import _cases as K
+    import _oracle as O
+    import ipglasma_amd as I
+    N, S = p.size, p.size ** 2
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tab = K.load_table(I.QS_TABLE)
+    t0 = time.perf_counter()
+    st, nA, nB, info = K.oracle_event(p, 3)
+    TpA, TpB, gA, gB = K.oracle_color_charge(p, nA, nB, tab)
+    t_cc = time.perf_counter() - t0
+    ny_sample = 1
+    V = np.zeros((2, S, 18))
+    m_lat = p.m * (p.L / N) / 0.1973269718
+    t0 = time.perf_counter()
+    O.lib().orc_setv(N, ny_sample, p.g, m_lat, 0.0, gA * (ny_sample / p.Ny), gB * (ny_sample / p.Ny),
+                     None, None, 99, V.reshape(-1))
+    t_setv = (time.perf_counter() - t0) * p.Ny / ny_sample
+    # root finder on a sample of links (single thread), scaled to 2S links
+    # over the OpenMP thread count the full oracle uses
+    VA, VB = O.from_c(V[0]), O.from_c(V[1])
+    sample = list(range(N * (N // 2), N * (N // 2) + 4 * N, 3))
+    out18 = np.zeros(18)
+    t0 = time.perf_counter()
+    for s in sample:
+        i, j = divmod(s, N)
+        px = min(i + 1, N - 1) * N + j
+        U1 = O.to_c(VA[s] @ VA[px].conj().T)
+        U2 = O.to_c(VB[s] @ VB[px].conj().T)
+        O.lib().orc_newton_link(U1, U2, 7, 2 * s, out18, None)
+    t_newton = (time.perf_counter() - t0) * (2 * S) / len(sample) / max(1, threads)
+    # leapfrog: 2 full steps on random-free fields
+    st2 = {k: np.zeros((S, 18)) for k in O.FIELDS}
+    one = np.zeros(18)
+    one[0] = one[8] = one[16] = 1.0
+    st2["Ux"][:] = one
+    st2["Uy"][:] = one
+    t0 = time.perf_counter()
+    nst = 2
+    for it in range(1, nst + 1):
+        O.lib().orc_evolve_pi(N, p.dtau, it * p.dtau, st2["Ux"], st2["Uy"], st2["phi"], st2["pi"])
+        O.lib().orc_evolve_e(N, p.g, p.dtau, it * p.dtau, st2["Ux"], st2["Uy"], st2["phi"], st2["E1"], st2["E2"])
+        O.lib().orc_evolve_phi(N, p.dtau, it * p.dtau, st2["pi"], st2["phi"])
+        O.lib().orc_evolve_u(N, p.g, p.dtau, it * p.dtau, st2["E1"], st2["E2"], st2["Ux"], st2["Uy"])
+    t_step = (time.perf_counter() - t0) / nst
+    T = np.zeros((10, S))
+    t0 = time.perf_counter()
+    O.lib().orc_tmunu(N, p.g, p.L / N, p.dtau, p.itmax, *[st2[k] for k in O.FIELDS], T.reshape(-1))
+    t_tmunu = time.perf_counter() - t0
+    total = t_cc + t_setv + t_newton + p.itmax * t_step + t_tmunu
+    return {"value": round(1.0 / total, 6), "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": (f"C oracle, Pb+Pb N={N}: colour charge once, setV 1 of {p.Ny} slices x{p.Ny}, "
+                       f"root finder on {len(sample)} central links scaled to {2 * S} over {threads} "
+                       f"threads, 2 leapfrog steps x{p.itmax}, T^munu once; "
+                       f"seconds per event {total:.2f} (cc {t_cc:.2f}, setV {t_setv:.2f}, "
+                       f"newton {t_newton:.2f}, evolve {p.itmax * t_step:.2f}, tmunu {t_tmunu:.2f})")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: events are independent and this build has no multi-GPU "
+              "path (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import ipglasma_amd as I
+    p = pbpb_params(I, N=a.size)
+    seeds = list(range(1000, 1000 + a.batch))
+    for w in range(a.warmup):
+        run_batched(I, p, seeds)
+        run_plain(I, p, seeds[:2])
+    tb = 0.0
+    tp = 0.0
+    obs_b = obs_p = None
+    for k in range(a.steps):
+        dt, obs_b = run_batched(I, p, seeds)
+        tb += dt
+        print(f"[bench] batched step {k}: {dt:.3f} s for {a.batch} events", file=sys.stderr, flush=True)
+    for k in range(a.steps):
+        dt, obs_p = run_plain(I, p, seeds)
+        tp += dt
+        print(f"[bench] plain step {k}: {dt:.3f} s for {a.batch} events", file=sys.stderr, flush=True)
+    ev_b = a.batch * a.steps / tb
+    ev_p = a.batch * a.steps / tp
+    # batched and plain must agree event by event
+    maxrel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(obs_b, obs_p))
+    rf = roofline(I, p)
+    line = {
+        "metric": "events/s: batched vs per-event launch, N=512 FP64, dE/deta rel err<=1e-9",
+        "value": round(ev_b, 4), "unit": "events/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1e3 * tb / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded Glauber nucleons, device normals)",
+        "config": {"workload": f"Pb+Pb b=0, N={a.size}, L=30 fm, Ny=50, tau_max=0.4 fm/c "
+                               f"(itmax={p.itmax}), batch of {a.batch} events",
+                   "events_per_step": a.batch},
+        "plain_events_per_s": round(ev_p, 4), "batched_over_plain": round(ev_b / ev_p, 3),
+        "batched_vs_plain_max_rel_dEdeta": maxrel,
+        "dEdeta_GeV": obs_b[0]["dEdeta"],
+        "roofline": rf,
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(p)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

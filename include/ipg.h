@@ -1,0 +1,181 @@
+/*
+ * ipg.h -- C ABI of the MI355X-native IP-Glasma solver core.
+ *
+ * The reference (schenke/ipglasma) has no FFI: its drop-in seam is the C++
+ * call surface driven by main.cpp:129-360 plus the `key value ... EndOfFile`
+ * input file.  Each entry point below names the reference interface it
+ * replaces.  All functions are extern "C", take plain pointers and sizes,
+ * never exit(), and return IPG_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is ipg_last_error().
+ *
+ * Host-visible layouts (caller-owned buffers):
+ *   site index   s = ix*N + iy            (Lattice.cpp:19-28: pos = i*N + j)
+ *   3x3 complex  18 doubles per site, row-major, interleaved re,im
+ *                (Matrix.h:45 e[j+3i]), i.e. field arrays are [S][18]
+ *   T^{mu nu}    [10][S] doubles, order tautau, xx, yy, etaeta, taux, tauy,
+ *                taueta, xy, xeta, yeta (Cell.h:42-51), physical units fm^-4
+ *                (taueta, xeta, yeta: fm^-5; etaeta: fm^-6)
+ * Device memory is structure-of-arrays FP64 and never exposed.
+ */
+#ifndef IPG_H
+#define IPG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPG_OK 0
+#define IPG_ERR_ARG -1       /* bad argument / unsupported option          */
+#define IPG_ERR_IO -2        /* input or table file missing or malformed   */
+#define IPG_ERR_DEVICE -3    /* HIP runtime error or no device             */
+#define IPG_ERR_STATE -4     /* call out of order                          */
+#define IPG_ERR_NOMEM -5
+
+/* Parsed input file: the keys of readInput (main.cpp:448-578) that the hot
+ * path reads, with the same names.  dtau and itmax are derived exactly as
+ * main.cpp:531-534 and Evolution.cpp:396 derive them. */
+typedef struct ipg_params {
+    int size, Nc, Ny;
+    double L, g, m, UVdamp, BG, QsmuRatio, rmax, roots, Rapidity;
+    double xFromThisFactorTimesQs, SigmaNN, maxtime;
+    int inverseQsForMaxTime, useFluctuatingx, usePseudoRapidity, useNucleus;
+    int useSmoothNucleus, useConstituentQuarkProton, smearQs, runningCoupling;
+    int gaussianWounding, nucleonPositionsFromFile, rotateReactionPlane;
+    int samplebFromLinearDistribution, setWSDeformParams, force_dmin_flag;
+    int useFixedNpart, averageOverThisManyNuclei, useSeedList, useTimeForSeed;
+    int computeGluonMultiplicity, writeOutputs, readInitialWilsonLines;
+    int writeInitialWilsonLines, SubNucleonParamType;
+    double R_WS, a_WS, beta2, beta3, beta4, gamma, d_min, dR_np, da_np;
+    double bmin, bmax, protonAnisotropy, minimumQs2ST, smearingWidth;
+    double Jacobianm, g2mu;
+    unsigned long long seed;
+    char Projectile[16], Target[16];
+    char NucleusQsTableFileName[512];
+    /* derived */
+    double dtau; /* lattice units */
+    int itmax;
+    /* project options (not in the reference input; default 0) */
+    int rho_rng; /* 0: device Philox counter map, 1: the reference's
+                    MT19937-64 Gauss stream drawn on the host in its order */
+} ipg_params;
+
+typedef struct ipg_observables {
+    int success;       /* Init::init success flag (Init.cpp:1614-1618)      */
+    int Npart, Ncoll;
+    double b, phiRP;   /* impact parameter [fm], reaction plane angle       */
+    double averageQs, averageQsAvg, averageQsmin, Tpp, Qs2minST, alphas;
+    double dEdeta;     /* tau a^2 hbarc sum_x T^tautau(x)  [GeV]             */
+    double gauss_max;  /* checkGaussLaw maximum (Evolution.cpp:295-364)     */
+    double tau_fm;     /* time of the T^{mu nu} evaluation [fm]             */
+    long long newton_iterations; /* summed over links                      */
+    int newton_max_iterations, newton_restarts, newton_unconverged;
+    int newton_analytic_jacobian;
+    int qs_serial_fallback;
+    uint64_t rho_seed;    /* key of the device colour-charge normals        */
+    uint64_t newton_seed; /* key of the root-finder restart draws           */
+} ipg_observables;
+
+typedef struct ipg_ctx ipg_ctx;
+typedef struct ipg_batch ipg_batch;
+
+/* ---- configuration (Setup.cpp:24-104, main.cpp:448-578) ----------------- */
+/* Fill defaults (the values of the reference's `input` file). */
+void ipg_default_params(ipg_params *p);
+/* Parse a reference-format input file: whitespace separated key/value pairs
+ * up to EndOfFile; integer keys truncate (Setup::IFind), the seed rounds
+ * (Setup::ULLIFind).  Keys absent from the file keep their current value in
+ * *p, unlike the reference, which exits on a missing key. */
+int ipg_read_input(const char *path, ipg_params *p);
+/* Recompute dtau and itmax from L, size, maxtime (main.cpp:531-534). */
+void ipg_derive(ipg_params *p);
+const char *ipg_last_error(void);
+int ipg_version(void);
+/* sizeof(ipg_params) (which=0) or sizeof(ipg_observables) (which=1), so that
+ * bindings can check their struct layout */
+int ipg_struct_size(int which);
+
+/* ---- one event: Init(nn) + Evolution(nn) + Lattice (main.cpp:167-187) --- */
+ipg_ctx *ipg_create(const ipg_params *p);
+void ipg_destroy(ipg_ctx *c);
+/* Q_s^2(T_p, y) table (Init::readNuclearQs, Init.cpp:441-480): 200x44 rows
+ * of "y T Qs2" */
+int ipg_load_qs_table(ipg_ctx *c, const char *path);
+/* Random::init_genrand64(seed) of the event's host generator (main.cpp:98);
+ * also keys the device counter-based streams */
+int ipg_seed(ipg_ctx *c, uint64_t seed);
+
+/* Init::init (Init.cpp:2303-2696): b, nucleons, colour charge, success
+ * test, Wilson lines, links, tau=0 root finder, pi.  *success receives the
+ * reference's success flag; on 0 the caller retries (main.cpp:339-360). */
+int ipg_init(ipg_ctx *c, int *success);
+/* Init::init split into its stages (same order, for drop-in callers and
+ * parity checks) */
+int ipg_init_nucleons(ipg_ctx *c);      /* b, sampleTA, reaction plane, wounding */
+int ipg_init_color_charge(ipg_ctx *c, int *success); /* T_p, g2mu2, stats */
+int ipg_init_wilson_lines(ipg_ctx *c);  /* setV */
+int ipg_init_fields(ipg_ctx *c);        /* links, root finder, pi */
+
+/* Evolution::run (Evolution.cpp:366-657): leapfrog to itmax, T^{mu nu} at
+ * itmax, final half step, Gauss law, dE/deta */
+int ipg_evolve(ipg_ctx *c);
+/* single schedule pieces: step 0 is the tau=0 phi/U kick, steps 1..itmax-1
+ * are full leapfrog steps, step itmax evaluates T^{mu nu} and then takes the
+ * half step and the Gauss-law check */
+int ipg_evolve_step(ipg_ctx *c, int it);
+
+int ipg_get_observables(ipg_ctx *c, ipg_observables *o);
+
+/* field ids for ipg_get_field / ipg_set_field */
+#define IPG_F_UX 0
+#define IPG_F_UY 1
+#define IPG_F_E1 2
+#define IPG_F_E2 3
+#define IPG_F_PHI 4
+#define IPG_F_PI 5
+#define IPG_F_VA 6      /* Wilson lines after setV (until fields are built) */
+#define IPG_F_VB 7
+#define IPG_F_G2MU2A 8  /* scalar fields: [S] doubles */
+#define IPG_F_G2MU2B 9
+#define IPG_F_TPA 10
+#define IPG_F_TPB 11
+/* current state (after the last completed step) as [S][18] or [S] */
+int ipg_get_field(ipg_ctx *c, int which, double *out);
+int ipg_set_field(ipg_ctx *c, int which, const double *in);
+int ipg_get_tmunu(ipg_ctx *c, double *out);
+/* nucleons after the +-b/2 shift: per nucleon x, y, z, phi, collided,
+ * proton; A1 rows of nucleus A then A2 rows of nucleus B */
+int ipg_get_nucleons(ipg_ctx *c, double *out, int *A1, int *A2);
+int ipg_set_nucleons(ipg_ctx *c, const double *in, int A1, int A2);
+/* per-link root-finder iteration counts [2S] (x links then y links) */
+int ipg_get_newton_iterations(ipg_ctx *c, int *out);
+
+/* colour-charge normals: inject z[2][Ny][S][8] (host copy; NULL clears), or
+ * read back the device counter-map normals of slice (nucleus, k) as [S][8] */
+int ipg_set_rho_normals(ipg_ctx *c, const double *z);
+int ipg_get_rho_normals(ipg_ctx *c, int nucleus, int k, double *out);
+
+/* GPU time of the last ipg_evolve in ms: out[0] total of the leapfrog
+ * launches, out[1] the T^{mu nu} launch, out[2] number of timed step
+ * launches.  Timing is recorded only after ipg_enable_timing(c, 1). */
+int ipg_enable_timing(ipg_ctx *c, int on);
+int ipg_get_timing(ipg_ctx *c, double *out);
+
+/* ---- many events in one launch sequence (north_star batched form) ------ */
+/* nev events, each with its own parameters and seed; lattice sizes and
+ * collision systems may differ.  Every kernel launch covers all events. */
+ipg_batch *ipg_batch_create(const ipg_params *p, int nev, const uint64_t *seeds);
+void ipg_batch_destroy(ipg_batch *b);
+int ipg_batch_load_qs_table(ipg_batch *b, const char *path);
+/* full Init::init + Evolution::run for all events; events that fail the
+ * success test are re-drawn from their generator until they succeed or
+ * max_retries is reached.  use_graph: capture the evolution launches in a
+ * HIP graph. */
+int ipg_batch_run(ipg_batch *b, int use_graph, int max_retries);
+int ipg_batch_get_observables(ipg_batch *b, int ev, ipg_observables *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

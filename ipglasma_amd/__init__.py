@@ -1,0 +1,309 @@
+"""ipglasma_amd -- MI355X-native IP-Glasma solver core (Python binding).
+
+The product is the C-ABI shared library ``libipglasma_amd.so`` (HIP kernels
+for gfx950 plus host C++), declared in ``include/ipg.h``.  This module binds
+it with ctypes for tests, the benchmark and scripting.  There is no CPU
+fallback: every call that needs the device fails loudly when the library is
+missing or no HIP device is present.
+
+The classes mirror the reference's call surface (schenke/ipglasma):
+``Setup`` (Setup.cpp:24-104) parses the input file into ``Parameters``;
+``Event`` owns one Init/Evolution/Lattice triple (main.cpp:167-187) with
+``init()`` (Init::init) and ``run()`` (Evolution::run); ``Batch`` runs many
+events through one launch sequence.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libipglasma_amd.so")
+QS_TABLE = os.path.join(os.path.dirname(_HERE), "tests", "golden", "qs2Adj_vs_Tp_vs_Y_200.in")
+
+FIELDS = {"Ux": 0, "Uy": 1, "E1": 2, "E2": 3, "phi": 4, "pi": 5, "VA": 6, "VB": 7,
+          "g2mu2A": 8, "g2mu2B": 9, "TpA": 10, "TpB": 11}
+TMUNU = ("tautau", "xx", "yy", "etaeta", "taux", "tauy", "taueta", "xy", "xeta", "yeta")
+
+
+class IPGError(RuntimeError):
+    pass
+
+
+_INT_FIELDS_1 = ("size", "Nc", "Ny")
+_DBL_FIELDS_1 = ("L", "g", "m", "UVdamp", "BG", "QsmuRatio", "rmax", "roots", "Rapidity",
+                 "xFromThisFactorTimesQs", "SigmaNN", "maxtime")
+_INT_FIELDS_2 = ("inverseQsForMaxTime", "useFluctuatingx", "usePseudoRapidity", "useNucleus",
+                 "useSmoothNucleus", "useConstituentQuarkProton", "smearQs", "runningCoupling",
+                 "gaussianWounding", "nucleonPositionsFromFile", "rotateReactionPlane",
+                 "samplebFromLinearDistribution", "setWSDeformParams", "force_dmin_flag",
+                 "useFixedNpart", "averageOverThisManyNuclei", "useSeedList", "useTimeForSeed",
+                 "computeGluonMultiplicity", "writeOutputs", "readInitialWilsonLines",
+                 "writeInitialWilsonLines", "SubNucleonParamType")
+_DBL_FIELDS_2 = ("R_WS", "a_WS", "beta2", "beta3", "beta4", "gamma", "d_min", "dR_np", "da_np",
+                 "bmin", "bmax", "protonAnisotropy", "minimumQs2ST", "smearingWidth",
+                 "Jacobianm", "g2mu")
+
+
+class Parameters(C.Structure):
+    """ipg_params (include/ipg.h); the keys of readInput (main.cpp:448-578)."""
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS_1] + [(n, C.c_double) for n in _DBL_FIELDS_1] +
+                [(n, C.c_int) for n in _INT_FIELDS_2] + [(n, C.c_double) for n in _DBL_FIELDS_2] +
+                [("seed", C.c_ulonglong), ("Projectile", C.c_char * 16), ("Target", C.c_char * 16),
+                 ("NucleusQsTableFileName", C.c_char * 512), ("dtau", C.c_double), ("itmax", C.c_int),
+                 ("rho_rng", C.c_int)])
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if k in ("Projectile", "Target", "NucleusQsTableFileName"):
+                v = v.encode() if isinstance(v, str) else v
+            setattr(self, k, v)
+        lib().ipg_derive(C.byref(self))
+        return self
+
+    def copy(self):
+        c = Parameters()
+        C.pointer(c)[0] = self
+        return c
+
+
+class Observables(C.Structure):
+    _fields_ = ([("success", C.c_int), ("Npart", C.c_int), ("Ncoll", C.c_int)] +
+                [(n, C.c_double) for n in ("b", "phiRP", "averageQs", "averageQsAvg", "averageQsmin",
+                                           "Tpp", "Qs2minST", "alphas", "dEdeta", "gauss_max",
+                                           "tau_fm")] +
+                [("newton_iterations", C.c_longlong)] +
+                [(n, C.c_int) for n in ("newton_max_iterations", "newton_restarts",
+                                        "newton_unconverged", "newton_analytic_jacobian",
+                                        "qs_serial_fallback")] +
+                [("rho_seed", C.c_uint64), ("newton_seed", C.c_uint64)])
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IPGError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        sig = {
+            "ipg_default_params": (None, [C.POINTER(Parameters)]),
+            "ipg_read_input": (C.c_int, [C.c_char_p, C.POINTER(Parameters)]),
+            "ipg_derive": (None, [C.POINTER(Parameters)]),
+            "ipg_last_error": (C.c_char_p, []),
+            "ipg_version": (C.c_int, []),
+            "ipg_struct_size": (C.c_int, [C.c_int]),
+            "ipg_create": (vp, [C.POINTER(Parameters)]),
+            "ipg_destroy": (None, [vp]),
+            "ipg_load_qs_table": (C.c_int, [vp, C.c_char_p]),
+            "ipg_seed": (C.c_int, [vp, C.c_uint64]),
+            "ipg_init": (C.c_int, [vp, ip]),
+            "ipg_init_nucleons": (C.c_int, [vp]),
+            "ipg_init_color_charge": (C.c_int, [vp, ip]),
+            "ipg_init_wilson_lines": (C.c_int, [vp]),
+            "ipg_init_fields": (C.c_int, [vp]),
+            "ipg_evolve": (C.c_int, [vp]),
+            "ipg_evolve_step": (C.c_int, [vp, C.c_int]),
+            "ipg_get_observables": (C.c_int, [vp, C.POINTER(Observables)]),
+            "ipg_get_field": (C.c_int, [vp, C.c_int, dp]),
+            "ipg_set_field": (C.c_int, [vp, C.c_int, dp]),
+            "ipg_get_tmunu": (C.c_int, [vp, dp]),
+            "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
+            "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
+            "ipg_get_newton_iterations": (C.c_int, [vp, ip]),
+            "ipg_set_rho_normals": (C.c_int, [vp, dp]),
+            "ipg_get_rho_normals": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+            "ipg_enable_timing": (C.c_int, [vp, C.c_int]),
+            "ipg_get_timing": (C.c_int, [vp, dp]),
+            "ipg_batch_create": (vp, [C.POINTER(Parameters), C.c_int, C.POINTER(C.c_uint64)]),
+            "ipg_batch_destroy": (None, [vp]),
+            "ipg_batch_load_qs_table": (C.c_int, [vp, C.c_char_p]),
+            "ipg_batch_run": (C.c_int, [vp, C.c_int, C.c_int]),
+            "ipg_batch_get_observables": (C.c_int, [vp, C.c_int, C.POINTER(Observables)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        if L.ipg_struct_size(0) != C.sizeof(Parameters) or L.ipg_struct_size(1) != C.sizeof(Observables):
+            raise IPGError("ipg_params / ipg_observables layout mismatch between ipg.h and the binding")
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise IPGError(f"status {rc}: {lib().ipg_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**kw):
+    p = Parameters()
+    lib().ipg_default_params(C.byref(p))
+    return p.set(**kw)
+
+
+class Setup:
+    """Input-file reader with the reference's key/value semantics
+    (Setup::StringFind/DFind/IFind/ULLIFind, Setup.cpp:24-104)."""
+
+    @staticmethod
+    def read(path, base=None):
+        p = base.copy() if base is not None else default_params()
+        _check(lib().ipg_read_input(path.encode(), C.byref(p)))
+        return p
+
+
+class Event:
+    """One event: the reference's Init(nn), Evolution(nn) and Lattice for one
+    parameter set (main.cpp:167-187) on the GPU."""
+
+    def __init__(self, params, qs_table=None, seed=None):
+        self.p = params
+        self.N = params.size
+        self.S = self.N * self.N
+        self._c = lib().ipg_create(C.byref(params))
+        if not self._c:
+            raise IPGError(lib().ipg_last_error().decode())
+        table = qs_table or QS_TABLE
+        _check(lib().ipg_load_qs_table(self._c, table.encode()))
+        _check(lib().ipg_seed(self._c, params.seed if seed is None else seed))
+
+    def close(self):
+        if self._c:
+            lib().ipg_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # Init::init and its stages
+    def init(self):
+        ok = C.c_int(0)
+        _check(lib().ipg_init(self._c, C.byref(ok)))
+        return bool(ok.value)
+
+    def init_nucleons(self):
+        _check(lib().ipg_init_nucleons(self._c))
+
+    def init_color_charge(self):
+        ok = C.c_int(0)
+        _check(lib().ipg_init_color_charge(self._c, C.byref(ok)))
+        return bool(ok.value)
+
+    def init_wilson_lines(self):
+        _check(lib().ipg_init_wilson_lines(self._c))
+
+    def init_fields(self):
+        _check(lib().ipg_init_fields(self._c))
+
+    # Evolution::run
+    def run(self):
+        _check(lib().ipg_evolve(self._c))
+
+    def step(self, it):
+        _check(lib().ipg_evolve_step(self._c, it))
+
+    def observables(self):
+        o = Observables()
+        _check(lib().ipg_get_observables(self._c, C.byref(o)))
+        return o.as_dict()
+
+    def field(self, name):
+        which = FIELDS[name]
+        out = np.zeros((self.S, 18) if which < 8 else self.S)
+        _check(lib().ipg_get_field(self._c, which, _dptr(out)))
+        return out
+
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(lib().ipg_set_field(self._c, FIELDS[name], _dptr(a)))
+
+    def tmunu(self):
+        out = np.zeros((10, self.S))
+        _check(lib().ipg_get_tmunu(self._c, _dptr(out)))
+        return out
+
+    def nucleons(self):
+        a1, a2 = C.c_int(), C.c_int()
+        _check(lib().ipg_get_nucleons(self._c, None, C.byref(a1), C.byref(a2)))
+        out = np.zeros((a1.value + a2.value, 6))
+        _check(lib().ipg_get_nucleons(self._c, _dptr(out), C.byref(a1), C.byref(a2)))
+        return out[:a1.value], out[a1.value:]
+
+    def set_nucleons(self, A, B):
+        arr = np.ascontiguousarray(np.vstack([A, B]), dtype=np.float64)
+        _check(lib().ipg_set_nucleons(self._c, _dptr(arr), len(A), len(B)))
+
+    def newton_iterations(self):
+        out = np.zeros(2 * self.S, dtype=np.int32)
+        _check(lib().ipg_get_newton_iterations(self._c, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def set_rho_normals(self, z):
+        if z is None:
+            _check(lib().ipg_set_rho_normals(self._c, None))
+            return
+        a = np.ascontiguousarray(z, dtype=np.float64)
+        assert a.size == 2 * self.p.Ny * self.S * 8
+        self._z = a
+        _check(lib().ipg_set_rho_normals(self._c, _dptr(a)))
+
+    def rho_normals(self, nucleus, k):
+        out = np.zeros((self.S, 8))
+        _check(lib().ipg_get_rho_normals(self._c, nucleus, k, _dptr(out)))
+        return out
+
+    def enable_timing(self, on=True):
+        _check(lib().ipg_enable_timing(self._c, int(on)))
+
+    def timing(self):
+        out = np.zeros(3)
+        _check(lib().ipg_get_timing(self._c, _dptr(out)))
+        return {"step_ms_total": out[0], "tmunu_ms": out[1], "nsteps": int(out[2])}
+
+
+class Batch:
+    """Many events in one launch sequence (heterogeneous sizes allowed)."""
+
+    def __init__(self, params_list, seeds, qs_table=None):
+        n = len(params_list)
+        arr = (Parameters * n)(*params_list)
+        sd = (C.c_uint64 * n)(*seeds)
+        self.n = n
+        self._b = lib().ipg_batch_create(arr, n, sd)
+        if not self._b:
+            raise IPGError(lib().ipg_last_error().decode())
+        _check(lib().ipg_batch_load_qs_table(self._b, (qs_table or QS_TABLE).encode()))
+
+    def run(self, use_graph=True, max_retries=20):
+        _check(lib().ipg_batch_run(self._b, int(use_graph), max_retries))
+
+    def observables(self, e):
+        o = Observables()
+        _check(lib().ipg_batch_get_observables(self._b, e, C.byref(o)))
+        return o.as_dict()
+
+    def close(self):
+        if self._b:
+            lib().ipg_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,898 @@
+// api.cpp -- the C ABI (include/ipg.h) and the event engine behind it.
+//
+// One Engine holds nev events.  ipg_ctx is an Engine with one event: its
+// launch sequence is the plain "one event per launch sequence" path.
+// ipg_batch is an Engine with many events: every kernel launch covers all
+// of them (heterogeneous lattice sizes share each site-parallel launch; the
+// FFT launches are grouped by lattice size), and the leapfrog launches can be
+// replayed from a HIP graph.
+//
+// Schedule (main.cpp:129-360 with Init::init, Init.cpp:2303-2696, and
+// Evolution::run, Evolution.cpp:366-657):
+//   host:   b, nucleons, reaction plane, wounding     (MT19937-64)
+//   device: T_p -> Q_s^2 fixed point -> g2mu2 -> statistics -> success
+//   device: setV (Ny slices: rho, FFT, Poisson kernel, inverse FFT, exp)
+//   device: links, root finder, pi, E = phi = 0
+//   device: step 0, steps 1..itmax-1, T^{mu nu} and the half step at itmax,
+//           Gauss law, dE/deta
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "ipg_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(IPG_ERR_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+int pad8(long long n) { return (int)(((n + 7) / 8) * 8); }
+int log2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
+struct Plan {
+    int N = 0, logN = 0, C = 0;
+    double2* tw = nullptr;
+    double* s2 = nullptr;
+};
+
+struct Event {
+    ipg_params p;
+    ipg::MT64 rng;
+    uint64_t seed = 0;
+    uint64_t attempt = 0;
+    ipg::NucleusData proj, targ;
+    std::vector<ipg::Nucleon> A, B;
+    ipg::EventInfo info;
+    ipg_observables obs;
+    double* mem = nullptr;   // one device allocation per event
+    double* d_nuc = nullptr;
+    double* d_z = nullptr;   // injected or MT normals
+    std::vector<double> hz;  // host copy of injected normals
+    int success = 0;
+    int parity = 0;          // buffer holding the current U, phi
+    int last_it = -1;
+    double stats[8] = {0};
+};
+
+}  // namespace
+
+struct Engine {
+    std::vector<Event> ev;
+    std::vector<EvDesc> desc;
+    EvDesc* d_desc = nullptr;
+    hipStream_t st = nullptr;
+    std::map<int, Plan> plans;
+    double* d_Tl = nullptr;
+    double* d_Q = nullptr;
+    bool have_table = false;
+    int* d_flag = nullptr;
+    double* d_out = nullptr;  // 8 doubles per event
+    int nblk = 0, nlinkblk = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    double t_steps_ms = 0., t_tmunu_ms = 0.;
+    int t_nsteps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_itm = -1;
+
+    ~Engine() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        for (Event& e : ev) {
+            if (e.mem) hipFree(e.mem);
+            if (e.d_nuc) hipFree(e.d_nuc);
+            if (e.d_z) hipFree(e.d_z);
+        }
+        for (auto& kv : plans) {
+            hipFree(kv.second.tw);
+            hipFree(kv.second.s2);
+        }
+        if (d_desc) hipFree(d_desc);
+        if (d_Tl) hipFree(d_Tl);
+        if (d_Q) hipFree(d_Q);
+        if (d_flag) hipFree(d_flag);
+        if (d_out) hipFree(d_out);
+        for (hipEvent_t t : tev) hipEventDestroy(t);
+        if (st) hipStreamDestroy(st);
+    }
+
+    int setup(const ipg_params* ps, int nev, const uint64_t* seeds) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(IPG_ERR_DEVICE, "no HIP device");
+        HIPCHK(hipStreamCreate(&st));
+        ev.resize(nev);
+        desc.resize(nev);
+        int b0 = 0, lb0 = 0;
+        for (int e = 0; e < nev; e++) {
+            Event& E = ev[e];
+            E.p = ps[e];
+            ipg::derive(&E.p);
+            const ipg_params& p = E.p;
+            if (p.Nc != 3) return fail(IPG_ERR_ARG, "only Nc = 3 is supported");
+            const int N = p.size;
+            if (N < 4 || (N & (N - 1)) != 0 || N > 1024)
+                return fail(IPG_ERR_ARG, "lattice size must be a power of two in [4, 1024]");
+            if (p.useNucleus != 1 || p.useSmoothNucleus != 0 || p.usePseudoRapidity != 0 || p.runningCoupling != 0 ||
+                p.readInitialWilsonLines != 0 || p.useFixedNpart != 0)
+                return fail(IPG_ERR_ARG, "option outside the supported hot path (useNucleus=1, useSmoothNucleus=0, "
+                                         "usePseudoRapidity=0, runningCoupling=0, readInitialWilsonLines=0, "
+                                         "useFixedNpart=0)");
+            if (!ipg::find_nucleus(p.Projectile, p, &E.proj)) return fail(IPG_ERR_ARG, std::string("unknown nucleus ") + p.Projectile);
+            if (!ipg::find_nucleus(p.Target, p, &E.targ)) return fail(IPG_ERR_ARG, std::string("unknown nucleus ") + p.Target);
+            E.seed = seeds ? seeds[e] : p.seed;
+            E.rng.seed(E.seed);
+            const long long S = (long long)N * N;
+            EvDesc& d = desc[e];
+            std::memset(&d, 0, sizeof(d));
+            d.N = N;
+            d.Ny = p.Ny;
+            d.S = S;
+            d.itmax = p.itmax;
+            d.nblocks = pad8((S + IPG_BLOCK - 1) / IPG_BLOCK);
+            d.block0 = b0;
+            b0 += d.nblocks;
+            d.lnblocks = (int)((2 * S + 63) / 64);
+            d.lblock0 = lb0;
+            lb0 += d.lnblocks;
+            d.active = 1;
+            d.g = p.g;
+            d.L = p.L;
+            d.a = p.L / N;
+            d.dtau = p.dtau;
+            d.m_lat = p.m * d.a / ipg::HBARC;
+            d.uvdamp_lat = p.UVdamp / d.a * ipg::HBARC;
+            d.BG = p.BG;
+            d.rmax = p.rmax;
+            d.QsmuRatio = p.QsmuRatio;
+            d.QsmuRatioB = p.QsmuRatio;
+            d.xFactor = p.xFromThisFactorTimesQs;
+            d.roots = p.roots;
+            d.rapidity = p.Rapidity;
+            d.xi = p.protonAnisotropy;
+            d.rcut = std::sqrt(0.1 * p.SigmaNN / M_PI);
+            d.useFluctuatingx = p.useFluctuatingx;
+            // device memory: 9 matrix fields (162 planes), 5 scalars, 2 int
+            // planes, FFT buffer (18), T (10), partials, root-finder counters
+            const size_t nd = (size_t)S * (162 + 5 + 18 + 10) + (size_t)8 * d.nblocks + (size_t)S * 2;
+            HIPCHK(hipMalloc(&E.mem, nd * sizeof(double)));
+            double* q = E.mem;
+            auto take = [&](size_t n) {
+                double* r = q;
+                q += n;
+                return r;
+            };
+            const size_t F = 18 * (size_t)S;
+            d.U[0][0] = take(F);
+            d.U[0][1] = take(F);
+            d.U[1][0] = take(F);
+            d.U[1][1] = take(F);
+            d.phi[0] = take(F);
+            d.phi[1] = take(F);
+            d.E1 = take(F);
+            d.E2 = take(F);
+            d.pi = take(F);
+            d.VA = d.U[1][0];
+            d.VB = d.U[1][1];
+            d.L1x = d.E1;
+            d.L1y = d.E2;
+            d.L2x = d.phi[0];
+            d.L2y = d.phi[1];
+            d.TpA = take(S);
+            d.TpB = take(S);
+            d.g2A = take(S);
+            d.g2B = take(S);
+            d.yB = take(S);
+            // FFT buffer [2][4][S] complex; also the 18 S-double scratch of the
+            // field layout transposes
+            d.fftbuf = (double2*)take(18 * (size_t)S);
+            d.T = take(10 * (size_t)S);
+            d.partial = take(8 * (size_t)d.nblocks);
+            int* ip = (int*)take(2 * (size_t)S);
+            d.check2 = ip;
+            d.prev2 = ip + S;
+            d.newton_stat = (int*)d.check2;  // reused after the colour charge is done
+            if (!plans.count(N)) {
+                Plan pl;
+                pl.N = N;
+                pl.logN = log2i(N);
+                pl.C = std::max(1, std::min(N, 4096 / N));
+                std::vector<double2> tw(N / 2);
+                for (int m = 0; m < N / 2; m++) {
+                    double ang = -2.0 * M_PI * (double)m / (double)N;
+                    tw[m] = make_double2(std::cos(ang), std::sin(ang));
+                }
+                std::vector<double> s2(N);
+                for (int qq = 0; qq < N; qq++) {
+                    double k = 2. * M_PI * (double)qq / (double)N;
+                    s2[qq] = 4. * (std::sin(k / 2.) * std::sin(k / 2.));
+                }
+                HIPCHK(hipMalloc(&pl.tw, sizeof(double2) * tw.size()));
+                HIPCHK(hipMalloc(&pl.s2, sizeof(double) * N));
+                HIPCHK(hipMemcpy(pl.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(pl.s2, s2.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+                plans[N] = pl;
+            }
+            std::memset(&E.obs, 0, sizeof(E.obs));
+        }
+        nblk = b0;
+        nlinkblk = lb0;
+        HIPCHK(hipMalloc(&d_desc, sizeof(EvDesc) * nev));
+        HIPCHK(hipMalloc(&d_flag, sizeof(int) * nev));
+        HIPCHK(hipMalloc(&d_out, sizeof(double) * 8 * nev));
+        return upload_desc();
+    }
+
+    int upload_desc() {
+        HIPCHK(hipMemcpyAsync(d_desc, desc.data(), sizeof(EvDesc) * desc.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return IPG_OK;
+    }
+
+    int load_table(const char* path) {
+        std::ifstream in(path);
+        if (!in) return fail(IPG_ERR_IO, std::string("Q_s table ") + path + " does not exist");
+        std::vector<double> Tl(200), Q(200 * 44);
+        for (int iT = 0; iT < 200; iT++)
+            for (int iy = 0; iy < 44; iy++) {
+                std::string y, T, q;
+                if (!(in >> y >> T >> q)) return fail(IPG_ERR_IO, "Q_s table ended prematurely");
+                Tl[iT] = std::atof(T.c_str());
+                Q[iT * 44 + iy] = std::atof(q.c_str());
+            }
+        if (!d_Tl) HIPCHK(hipMalloc(&d_Tl, sizeof(double) * 200));
+        if (!d_Q) HIPCHK(hipMalloc(&d_Q, sizeof(double) * 200 * 44));
+        HIPCHK(hipMemcpy(d_Tl, Tl.data(), sizeof(double) * 200, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_Q, Q.data(), sizeof(double) * 200 * 44, hipMemcpyHostToDevice));
+        have_table = true;
+        return IPG_OK;
+    }
+
+    // host nucleon sampling for event e, upload of the nucleon list
+    int nucleons(int e) {
+        Event& E = ev[e];
+        std::string err;
+        int rc = ipg::sample_event(E.rng, E.p, E.proj, E.targ, E.A, E.B, E.info, err);
+        if (rc) return fail(rc, err);
+        E.attempt++;
+        return upload_nucleons(e);
+    }
+
+    int upload_nucleons(int e) {
+        Event& E = ev[e];
+        const int n = (int)(E.A.size() + E.B.size());
+        std::vector<double> h(4 * (size_t)n);
+        int k = 0;
+        for (auto* V : {&E.A, &E.B})
+            for (const ipg::Nucleon& q : *V) {
+                h[4 * k] = q.x;
+                h[4 * k + 1] = q.y;
+                h[4 * k + 2] = q.phi;
+                h[4 * k + 3] = q.collided;
+                k++;
+            }
+        if (E.d_nuc) hipFree(E.d_nuc);
+        E.d_nuc = nullptr;
+        HIPCHK(hipMalloc(&E.d_nuc, sizeof(double) * std::max(4, 4 * n)));
+        HIPCHK(hipMemcpy(E.d_nuc, h.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+        desc[e].nuc = E.d_nuc;
+        desc[e].A1 = (int)E.A.size();
+        desc[e].A2 = (int)E.B.size();
+        // counter-based device streams of this attempt
+        desc[e].rho_seed = splitmix64(E.seed ^ (E.attempt * 0xD1B54A32D192ED03ULL));
+        desc[e].newton_seed = splitmix64(desc[e].rho_seed);
+        E.obs.rho_seed = desc[e].rho_seed;
+        E.obs.newton_seed = desc[e].newton_seed;
+        E.obs.b = E.info.b;
+        E.obs.phiRP = E.info.phiRP;
+        E.obs.Npart = E.info.Npart;
+        E.obs.Ncoll = E.info.Ncoll;
+        return IPG_OK;
+    }
+
+    // T_p, Q_s fixed point, statistics for all active events
+    int color_charge() {
+        if (!have_table) return fail(IPG_ERR_STATE, "Q_s table not loaded (ipg_load_qs_table)");
+        int rc = upload_desc();
+        if (rc) return rc;
+        ipg_launch_tp(d_desc, (int)ev.size(), nblk, st);
+        ipg_launch_qs(d_desc, (int)ev.size(), nblk, d_Tl, d_Q, d_flag, st);
+        ipg_launch_stats(d_desc, (int)ev.size(), nblk, d_out, st);
+        std::vector<double> hout(8 * ev.size());
+        std::vector<int> hflag(ev.size());
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(hout.data(), d_out, sizeof(double) * hout.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hflag.data(), d_flag, sizeof(int) * hflag.size(), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < ev.size(); e++) {
+            if (!desc[e].active) continue;
+            std::memcpy(ev[e].stats, &hout[8 * e], sizeof(double) * 7);
+            finish_stats((int)e, hflag[e]);
+        }
+        return IPG_OK;
+    }
+
+    // averages and the success test (Init.cpp:1520-1618)
+    void finish_stats(int e, int flag) {
+        Event& E = ev[e];
+        const ipg_params& p = E.p;
+        const double a = p.L / p.size;
+        const double count = E.stats[5];
+        double aQs2 = E.stats[1] / count, aQs2Avg = E.stats[2] / count, aQs2min = E.stats[3] / count;
+        E.obs.averageQs = std::sqrt(aQs2);
+        E.obs.averageQsAvg = std::sqrt(aQs2Avg);
+        E.obs.averageQsmin = std::sqrt(aQs2min);
+        E.obs.Tpp = E.stats[6];
+        E.obs.Qs2minST = E.stats[4] * a * a / ipg::HBARC / ipg::HBARC;
+        E.obs.alphas = p.g * p.g / 4. / M_PI;
+        E.obs.qs_serial_fallback = flag & 1;
+        E.success = E.obs.averageQs > 0 && E.obs.averageQsAvg > 0 && aQs2 > 0 && E.obs.averageQsmin > 0 &&
+                    aQs2Avg > 0 && E.obs.alphas > 0 && E.info.Npart >= 2 && E.obs.Qs2minST > p.minimumQs2ST;
+        E.obs.success = E.success;
+        // maximal evolution time 1/<Q_s> (Evolution.cpp:385-389)
+        if (p.inverseQsForMaxTime == 1 && E.obs.averageQs > 0) {
+            double maxtime = 1. / E.obs.averageQs * ipg::HBARC;
+            desc[e].itmax = static_cast<int>(maxtime / (a * p.dtau) + 0.1);
+        }
+    }
+
+    // rho normals for events that need host-side normals (injected or MT)
+    int prepare_normals() {
+        for (size_t e = 0; e < ev.size(); e++) {
+            Event& E = ev[e];
+            desc[e].znorm = nullptr;
+            if (!desc[e].active) continue;
+            const size_t S = desc[e].S, n = 2 * (size_t)E.p.Ny * S * 8;
+            const double* src = nullptr;
+            std::vector<double> mt;
+            if (!E.hz.empty()) {
+                if (E.hz.size() != n) return fail(IPG_ERR_ARG, "injected normals have the wrong size");
+                src = E.hz.data();
+            } else if (E.p.rho_rng == 1) {
+                // the reference's draw order: nucleus A then B, slice, site, colour
+                mt.resize(n);
+                for (size_t t = 0; t < n; t++) mt[t] = E.rng.gauss();
+                src = mt.data();
+            }
+            if (!src) continue;
+            if (!E.d_z) HIPCHK(hipMalloc(&E.d_z, n * sizeof(double)));
+            HIPCHK(hipMemcpy(E.d_z, src, n * sizeof(double), hipMemcpyHostToDevice));
+            desc[e].znorm = E.d_z;
+        }
+        return upload_desc();
+    }
+
+    int wilson_lines() {
+        int rc = prepare_normals();
+        if (rc) return rc;
+        const int nev = (int)ev.size();
+        ipg_launch_unit_v(d_desc, nev, nblk, st);
+        // group events by lattice size; each group is contiguous in a copy
+        std::map<int, std::vector<int>> groups;
+        for (int e = 0; e < nev; e++)
+            if (desc[e].active) groups[desc[e].N].push_back(e);
+        for (auto& kv : groups) {
+            const Plan& pl = plans[kv.first];
+            FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
+            // contiguous descriptor run for this size: if the events of the
+            // group are not contiguous, launch per event
+            std::vector<int>& es = kv.second;
+            bool contiguous = es.back() - es.front() + 1 == (int)es.size();
+            int maxNy = 0;
+            for (int e : es) maxNy = std::max(maxNy, desc[e].Ny);
+            if (contiguous) {
+                const int ng = (int)es.size();
+                for (int k = 0; k < maxNy; k++)
+                    ipg_launch_setv_slice(d_desc + es.front(), ng, dp, k, ng * 2 * pl.N, ng * 8 * (pl.N / pl.C), st);
+            } else {
+                for (int e : es)
+                    for (int k = 0; k < desc[e].Ny; k++)
+                        ipg_launch_setv_slice(d_desc + e, 1, dp, k, 2 * pl.N, 8 * (pl.N / pl.C), st);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        return IPG_OK;
+    }
+
+    int fields() {
+        const int nev = (int)ev.size();
+        ipg_launch_links(d_desc, nev, nblk, st);
+        ipg_launch_newton(d_desc, nev, nlinkblk, st);
+        ipg_launch_pi_init(d_desc, nev, nblk, st);
+        ipg_launch_zero_fields(d_desc, nev, nblk, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        // root-finder statistics
+        for (int e = 0; e < nev; e++) {
+            if (!desc[e].active) continue;
+            std::vector<int> it(2 * (size_t)desc[e].S);
+            HIPCHK(hipMemcpy(it.data(), desc[e].newton_stat, sizeof(int) * it.size(), hipMemcpyDeviceToHost));
+            long long sum = 0;
+            int mx = 0, rs = 0, un = 0, an = 0;
+            for (int v : it) {
+                int n = v & 0xFFFFF;
+                sum += n;
+                mx = std::max(mx, n);
+                rs += (v >> 20) & 0xFF;
+                un += (v >> 28) & 1;
+                an += (v >> 29) & 1;
+            }
+            ipg_observables& o = ev[e].obs;
+            o.newton_iterations = sum;
+            o.newton_max_iterations = mx;
+            o.newton_restarts = rs;
+            o.newton_unconverged = un;
+            o.newton_analytic_jacobian = an;
+            ev[e].parity = 0;
+            ev[e].last_it = -1;
+        }
+        return IPG_OK;
+    }
+
+    int max_itmax() const {
+        int m = 0;
+        for (const EvDesc& d : desc)
+            if (d.active) m = std::max(m, d.itmax);
+        return m;
+    }
+
+    // launches of leapfrog iteration it for all events (it = 0: tau=0 kick)
+    void launch_iteration(int it) {
+        const int nev = (int)ev.size();
+        if (it == 0) {
+            ipg_launch_step0(d_desc, nev, nblk, st);
+            return;
+        }
+        ipg_launch_tmunu(d_desc, nev, nblk, it, st);
+        ipg_launch_step(d_desc, nev, nblk, it, st);
+        ipg_launch_gauss(d_desc, nev, nblk, it, st);
+        ipg_launch_finish(d_desc, nev, it, d_out, st);
+    }
+
+    int evolve(bool use_graph) {
+        int rc = upload_desc();
+        if (rc) return rc;
+        const int itm = max_itmax();
+        const int nev = (int)ev.size();
+        if (use_graph) {
+            if (gexec && graph_itm != itm) {
+                hipGraphExecDestroy(gexec);
+                hipGraphDestroy(graph);
+                gexec = nullptr;
+                graph = nullptr;
+            }
+            if (!gexec) {
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (int it = 0; it <= itm; it++) launch_iteration(it);
+                HIPCHK(hipStreamEndCapture(st, &graph));
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                graph_itm = itm;
+            }
+            HIPCHK(hipGraphLaunch(gexec, st));
+        } else {
+            if (timing) {
+                for (hipEvent_t t : tev) hipEventDestroy(t);
+                tev.assign(3 * (size_t)(itm + 1), nullptr);
+                for (hipEvent_t& t : tev) HIPCHK(hipEventCreate(&t));
+            }
+            ipg_launch_step0(d_desc, nev, nblk, st);
+            for (int it = 1; it <= itm; it++) {
+                if (timing) HIPCHK(hipEventRecord(tev[3 * it], st));
+                ipg_launch_tmunu(d_desc, nev, nblk, it, st);
+                if (timing) HIPCHK(hipEventRecord(tev[3 * it + 1], st));
+                ipg_launch_step(d_desc, nev, nblk, it, st);
+                if (timing) HIPCHK(hipEventRecord(tev[3 * it + 2], st));
+                ipg_launch_gauss(d_desc, nev, nblk, it, st);
+                ipg_launch_finish(d_desc, nev, it, d_out, st);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (timing && !use_graph) {
+            t_steps_ms = 0.;
+            t_tmunu_ms = 0.;
+            t_nsteps = 0;
+            for (int it = 1; it <= itm; it++) {
+                // full steps only (the half step at itmax moves fewer bytes)
+                bool full = false, tm = false;
+                for (const EvDesc& d : desc) {
+                    if (d.active && it < d.itmax) full = true;
+                    if (d.active && it == d.itmax) tm = true;
+                }
+                float ms = 0.f;
+                if (full && !tm) {
+                    HIPCHK(hipEventElapsedTime(&ms, tev[3 * it + 1], tev[3 * it + 2]));
+                    t_steps_ms += ms;
+                    t_nsteps++;
+                }
+                if (tm) {
+                    HIPCHK(hipEventElapsedTime(&ms, tev[3 * it], tev[3 * it + 1]));
+                    t_tmunu_ms += ms;
+                }
+            }
+        }
+        std::vector<double> hout(8 * ev.size());
+        HIPCHK(hipMemcpy(hout.data(), d_out, sizeof(double) * hout.size(), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < ev.size(); e++) {
+            if (!desc[e].active) continue;
+            const EvDesc& d = desc[e];
+            const double tau_fm = d.itmax * d.dtau * d.a;
+            ev[e].obs.tau_fm = tau_fm;
+            ev[e].obs.dEdeta = tau_fm * d.a * d.a * ipg::HBARC * hout[4 * e];
+            ev[e].obs.gauss_max = hout[4 * e + 1];
+            ev[e].parity = (d.itmax - 1) & 1;
+            ev[e].last_it = d.itmax;
+        }
+        return IPG_OK;
+    }
+
+    int single_step(int e, int it) {
+        // one schedule piece for a single-event engine
+        int rc = upload_desc();
+        if (rc) return rc;
+        launch_iteration(it);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        const EvDesc& d = desc[e];
+        if (it == 0)
+            ev[e].parity = 0;
+        else
+            ev[e].parity = (it < d.itmax) ? (it & 1) : ((it - 1) & 1);
+        ev[e].last_it = it;
+        if (it == d.itmax) {
+            std::vector<double> hout(8 * ev.size());
+            HIPCHK(hipMemcpy(hout.data(), d_out, sizeof(double) * hout.size(), hipMemcpyDeviceToHost));
+            const double tau_fm = d.itmax * d.dtau * d.a;
+            ev[e].obs.tau_fm = tau_fm;
+            ev[e].obs.dEdeta = tau_fm * d.a * d.a * ipg::HBARC * hout[4 * e];
+            ev[e].obs.gauss_max = hout[4 * e + 1];
+        }
+        return IPG_OK;
+    }
+};
+
+struct ipg_ctx {
+    Engine eng;
+};
+struct ipg_batch {
+    Engine eng;
+};
+
+// ---- field access ------------------------------------------------------------------
+namespace {
+
+__global__ void k_soa_to_aos(const double* __restrict__ f, long long S, double* __restrict__ out) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 18 * S) return;
+    long long s = t / 18, c = t - s * 18;
+    out[t] = f[c * S + s];
+}
+__global__ void k_aos_to_soa(const double* __restrict__ in, long long S, double* __restrict__ f) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 18 * S) return;
+    long long s = t / 18, c = t - s * 18;
+    f[c * S + s] = in[t];
+}
+
+double* field_ptr(Engine& g, int which, bool* matrix) {
+    EvDesc& d = g.desc[0];
+    const int p = g.ev[0].parity;
+    *matrix = true;
+    switch (which) {
+        case IPG_F_UX: return d.U[p][0];
+        case IPG_F_UY: return d.U[p][1];
+        case IPG_F_E1: return d.E1;
+        case IPG_F_E2: return d.E2;
+        case IPG_F_PHI: return d.phi[p];
+        case IPG_F_PI: return d.pi;
+        case IPG_F_VA: return d.VA;
+        case IPG_F_VB: return d.VB;
+    }
+    *matrix = false;
+    switch (which) {
+        case IPG_F_G2MU2A: return d.g2A;
+        case IPG_F_G2MU2B: return d.g2B;
+        case IPG_F_TPA: return d.TpA;
+        case IPG_F_TPB: return d.TpB;
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ipg_last_error(void) { return g_err.c_str(); }
+int ipg_version(void) { return 100; }
+int ipg_struct_size(int which) { return which == 0 ? (int)sizeof(ipg_params) : (int)sizeof(ipg_observables); }
+void ipg_default_params(ipg_params* p) { ipg::default_params(p); }
+void ipg_derive(ipg_params* p) { ipg::derive(p); }
+int ipg_read_input(const char* path, ipg_params* p) {
+    std::string err;
+    int rc = ipg::read_input(path, p, err);
+    if (rc) fail(rc, err);
+    return rc;
+}
+
+ipg_ctx* ipg_create(const ipg_params* p) {
+    if (!p) {
+        fail(IPG_ERR_ARG, "null parameters");
+        return nullptr;
+    }
+    ipg_ctx* c = new ipg_ctx;
+    if (c->eng.setup(p, 1, nullptr) != IPG_OK) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+void ipg_destroy(ipg_ctx* c) { delete c; }
+
+int ipg_load_qs_table(ipg_ctx* c, const char* path) {
+    if (!c || !path) return fail(IPG_ERR_ARG, "null argument");
+    return c->eng.load_table(path);
+}
+int ipg_seed(ipg_ctx* c, uint64_t seed) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    c->eng.ev[0].seed = seed;
+    c->eng.ev[0].attempt = 0;
+    c->eng.ev[0].rng.seed(seed);
+    return IPG_OK;
+}
+
+int ipg_init_nucleons(ipg_ctx* c) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    c->eng.desc[0].active = 1;
+    return c->eng.nucleons(0);
+}
+int ipg_init_color_charge(ipg_ctx* c, int* success) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    int rc = c->eng.color_charge();
+    if (success) *success = c->eng.ev[0].success;
+    return rc;
+}
+int ipg_init_wilson_lines(ipg_ctx* c) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    int rc = c->eng.wilson_lines();
+    if (rc) return rc;
+    if (hipStreamSynchronize(c->eng.st) != hipSuccess) return fail(IPG_ERR_DEVICE, "setV failed");
+    return IPG_OK;
+}
+int ipg_init_fields(ipg_ctx* c) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    return c->eng.fields();
+}
+int ipg_init(ipg_ctx* c, int* success) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    int rc = ipg_init_nucleons(c);
+    if (rc) return rc;
+    int ok = 0;
+    rc = ipg_init_color_charge(c, &ok);
+    if (rc) return rc;
+    if (success) *success = ok;
+    if (!ok) return IPG_OK;  // the caller re-draws (main.cpp:339-360)
+    rc = ipg_init_wilson_lines(c);
+    if (rc) return rc;
+    return ipg_init_fields(c);
+}
+int ipg_evolve(ipg_ctx* c) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    return c->eng.evolve(false);
+}
+int ipg_evolve_step(ipg_ctx* c, int it) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    if (it < 0 || it > c->eng.desc[0].itmax) return fail(IPG_ERR_ARG, "step outside 0..itmax");
+    return c->eng.single_step(0, it);
+}
+int ipg_get_observables(ipg_ctx* c, ipg_observables* o) {
+    if (!c || !o) return fail(IPG_ERR_ARG, "null argument");
+    *o = c->eng.ev[0].obs;
+    return IPG_OK;
+}
+
+int ipg_get_field(ipg_ctx* c, int which, double* out) {
+    if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
+    Engine& g = c->eng;
+    bool mat;
+    double* f = field_ptr(g, which, &mat);
+    if (!f) return fail(IPG_ERR_ARG, "unknown field id");
+    const long long S = g.desc[0].S;
+    if (!mat) {
+        HIPCHK(hipMemcpy(out, f, sizeof(double) * S, hipMemcpyDeviceToHost));
+        return IPG_OK;
+    }
+    double* tmp = (double*)g.desc[0].fftbuf;  // 18 S doubles of scratch
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, f, S, tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.st));
+    HIPCHK(hipMemcpy(out, tmp, sizeof(double) * 18 * S, hipMemcpyDeviceToHost));
+    return IPG_OK;
+}
+int ipg_set_field(ipg_ctx* c, int which, const double* in) {
+    if (!c || !in) return fail(IPG_ERR_ARG, "null argument");
+    Engine& g = c->eng;
+    bool mat;
+    double* f = field_ptr(g, which, &mat);
+    if (!f) return fail(IPG_ERR_ARG, "unknown field id");
+    const long long S = g.desc[0].S;
+    if (!mat) {
+        HIPCHK(hipMemcpy(f, in, sizeof(double) * S, hipMemcpyHostToDevice));
+        return IPG_OK;
+    }
+    double* tmp = (double*)g.desc[0].fftbuf;
+    HIPCHK(hipMemcpy(tmp, in, sizeof(double) * 18 * S, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, tmp, S, f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.st));
+    g.desc[0].active = 1;
+    return IPG_OK;
+}
+int ipg_get_tmunu(ipg_ctx* c, double* out) {
+    if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpy(out, c->eng.desc[0].T, sizeof(double) * 10 * c->eng.desc[0].S, hipMemcpyDeviceToHost));
+    return IPG_OK;
+}
+int ipg_get_nucleons(ipg_ctx* c, double* out, int* A1, int* A2) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    Event& E = c->eng.ev[0];
+    if (A1) *A1 = (int)E.A.size();
+    if (A2) *A2 = (int)E.B.size();
+    if (out) {
+        int k = 0;
+        for (auto* V : {&E.A, &E.B})
+            for (const ipg::Nucleon& q : *V) {
+                double* r = out + 6 * k++;
+                r[0] = q.x;
+                r[1] = q.y;
+                r[2] = q.z;
+                r[3] = q.phi;
+                r[4] = q.collided;
+                r[5] = q.proton;
+            }
+    }
+    return IPG_OK;
+}
+int ipg_set_nucleons(ipg_ctx* c, const double* in, int A1, int A2) {
+    if (!c || !in || A1 < 1 || A2 < 1) return fail(IPG_ERR_ARG, "bad nucleon list");
+    Event& E = c->eng.ev[0];
+    E.A.resize(A1);
+    E.B.resize(A2);
+    int Np = 0;
+    for (int k = 0; k < A1 + A2; k++) {
+        const double* r = in + 6 * k;
+        ipg::Nucleon q{r[0], r[1], r[2], r[3], (int)r[4], (int)r[5]};
+        (k < A1 ? E.A[k] : E.B[k - A1]) = q;
+        Np += q.collided == 1;
+    }
+    E.info.Npart = Np;
+    c->eng.desc[0].active = 1;
+    return c->eng.upload_nucleons(0);
+}
+int ipg_get_newton_iterations(ipg_ctx* c, int* out) {
+    if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
+    const EvDesc& d = c->eng.desc[0];
+    std::vector<int> v(2 * (size_t)d.S);
+    HIPCHK(hipMemcpy(v.data(), d.newton_stat, sizeof(int) * v.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < v.size(); i++) out[i] = v[i] & 0xFFFFF;
+    return IPG_OK;
+}
+int ipg_set_rho_normals(ipg_ctx* c, const double* z) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    Event& E = c->eng.ev[0];
+    if (!z) {
+        E.hz.clear();
+        return IPG_OK;
+    }
+    const size_t n = 2 * (size_t)E.p.Ny * c->eng.desc[0].S * 8;
+    E.hz.assign(z, z + n);
+    return IPG_OK;
+}
+int ipg_get_rho_normals(ipg_ctx* c, int nucleus, int k, double* out) {
+    if (!c || !out || nucleus < 0 || nucleus > 1 || k < 0 || k >= c->eng.desc[0].Ny)
+        return fail(IPG_ERR_ARG, "bad argument");
+    Engine& g = c->eng;
+    int rc = g.upload_desc();
+    if (rc) return rc;
+    double* tmp = (double*)g.desc[0].fftbuf;
+    ipg_launch_rho_normals(g.d_desc, nucleus, k, tmp, g.st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.st));
+    HIPCHK(hipMemcpy(out, tmp, sizeof(double) * 8 * g.desc[0].S, hipMemcpyDeviceToHost));
+    return IPG_OK;
+}
+int ipg_enable_timing(ipg_ctx* c, int on) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    c->eng.timing = on != 0;
+    return IPG_OK;
+}
+int ipg_get_timing(ipg_ctx* c, double* out) {
+    if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
+    out[0] = c->eng.t_steps_ms;
+    out[1] = c->eng.t_tmunu_ms;
+    out[2] = c->eng.t_nsteps;
+    return IPG_OK;
+}
+
+// ---- batched form ------------------------------------------------------------------
+ipg_batch* ipg_batch_create(const ipg_params* p, int nev, const uint64_t* seeds) {
+    if (!p || nev < 1) {
+        fail(IPG_ERR_ARG, "bad batch arguments");
+        return nullptr;
+    }
+    ipg_batch* b = new ipg_batch;
+    if (b->eng.setup(p, nev, seeds) != IPG_OK) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+void ipg_batch_destroy(ipg_batch* b) { delete b; }
+int ipg_batch_load_qs_table(ipg_batch* b, const char* path) {
+    if (!b || !path) return fail(IPG_ERR_ARG, "null argument");
+    return b->eng.load_table(path);
+}
+int ipg_batch_run(ipg_batch* b, int use_graph, int max_retries) {
+    if (!b) return fail(IPG_ERR_ARG, "null batch");
+    Engine& g = b->eng;
+    const int nev = (int)g.ev.size();
+    std::vector<int> pending(nev, 1);
+    for (int attempt = 0; attempt <= max_retries; attempt++) {
+        bool any = false;
+        for (int e = 0; e < nev; e++) {
+            g.desc[e].active = pending[e];
+            if (!pending[e]) continue;
+            any = true;
+            int rc = g.nucleons(e);
+            if (rc) return rc;
+        }
+        if (!any) break;
+        int rc = g.color_charge();
+        if (rc) return rc;
+        for (int e = 0; e < nev; e++)
+            if (pending[e] && g.ev[e].success) pending[e] = 0;
+    }
+    for (int e = 0; e < nev; e++) g.desc[e].active = g.ev[e].success;
+    int rc = g.upload_desc();
+    if (rc) return rc;
+    rc = g.wilson_lines();
+    if (rc) return rc;
+    rc = g.fields();
+    if (rc) return rc;
+    return g.evolve(use_graph != 0);
+}
+int ipg_batch_get_observables(ipg_batch* b, int e, ipg_observables* o) {
+    if (!b || !o || e < 0 || e >= (int)b->eng.ev.size()) return fail(IPG_ERR_ARG, "bad argument");
+    *o = b->eng.ev[e].obs;
+    return IPG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,524 @@
+// host.cpp -- input parsing, MT19937-64, nucleus table, nucleon sampling.
+//
+// Follows (draw order is part of the contract: nucleon positions and the
+// binary-collision list are bit-exact targets):
+//   Setup.cpp:24-104, main.cpp:448-578   input keys, truncation/rounding rules
+//   main.cpp:531-534, Evolution.cpp:396  derived dtau and step count
+//   Random.cpp:67-204                    MT19937-64 and its transforms
+//   Glauber.cpp:19-287                   nucleus constants
+//   Init.cpp:2320-2345                   impact parameter
+//   Init.cpp:65-439, 2721-3178           sampleTA and its samplers
+//   Init.cpp:868-901, 1117-1216          reaction plane, shift, wounding
+#include "host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+
+namespace ipg {
+
+// ---- MT19937-64 ---------------------------------------------------------------
+void MT64::seed(uint64_t s) {
+    mt_[0] = s;
+    for (int i = 1; i < 312; i++) mt_[i] = 6364136223846793005ULL * (mt_[i - 1] ^ (mt_[i - 1] >> 62)) + (uint64_t)i;
+    mti_ = 312;
+    iset_ = 0;
+    gset_ = 0.;
+}
+
+uint64_t MT64::next() {
+    constexpr uint64_t kA = 0xB5026F5AA96619E9ULL, kUp = 0xFFFFFFFF80000000ULL, kLo = 0x7FFFFFFFULL;
+    if (mti_ >= 312) {
+        if (mti_ == 313) seed(5489ULL);
+        for (int i = 0; i < 312; i++) {
+            uint64_t y = (mt_[i] & kUp) | (mt_[(i + 1) % 312] & kLo);
+            mt_[i] = mt_[(i + 156) % 312] ^ (y >> 1) ^ ((y & 1ULL) ? kA : 0ULL);
+        }
+        mti_ = 0;
+    }
+    uint64_t x = mt_[mti_++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= (x >> 43);
+    return x;
+}
+double MT64::real1() { return (double)(next() >> 11) * (1.0 / 9007199254740991.0); }
+double MT64::real2() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+double MT64::real3() { return ((double)(next() >> 12) + 0.5) * (1.0 / 4503599627370496.0); }
+double MT64::gauss() {
+    if (iset_) {
+        iset_ = 0;
+        return gset_;
+    }
+    double v1, v2, r2;
+    do {
+        v1 = 2.0 * real3() - 1.0;
+        v2 = 2.0 * real3() - 1.0;
+        r2 = v1 * v1 + v2 * v2;
+    } while (r2 > 1. || r2 == 0.);
+    const double f = std::sqrt(-2.0 * std::log(r2) / r2);
+    gset_ = v1 * f;
+    iset_ = 1;
+    return v2 * f;
+}
+
+// ---- nucleus table ---------------------------------------------------------------
+namespace {
+struct Row {
+    const char* name;
+    int A, Z;
+    double R, a, b2, b3, b4;
+    int user_b2;  // beta2 taken from the input (U, Xe)
+};
+// Glauber.cpp:19-245 (Woods-Saxon nuclei; deuteron and He3 need samplers
+// that are not on this path)
+const Row kNuclei[] = {
+    {"Au", 197, 79, 6.37, 0.535, -0.13, 0., -0.03, 0}, {"Pb", 208, 82, 6.62, 0.546, 0., 0., 0., 0},
+    {"p", 1, 1, 1., 1., 0., 0., 0., 0},                {"d", 2, 1, 1.0, 0.228, 0., 0., 0., 0},
+    {"C", 12, 6, 2.44, 1.635, 0., 0., 0., 0},          {"O", 16, 8, 2.608, 0.513, -0.01, 0., -0.122, 0},
+    {"Ne", 20, 10, 2.8, 0.57, 0., 0., 0., 0},           {"S", 32, 16, 2.54, 2.191, 0., 0., 0., 0},
+    {"Ar", 40, 18, 3.61, 0.516, 0.1668, 0., 0.00695, 0}, {"W", 184, 74, 6.51, 0.535, 0., 0., 0., 0},
+    {"Al", 27, 13, 3.07, 0.519, 0., 0., 0., 0},         {"Ca", 40, 20, 3.766, 0.586, 0., 0., 0., 0},
+    {"Cu", 63, 29, 4.163, 0.606, 0.162, 0., 0.006, 0},  {"Fe", 56, 26, 4.106, 0.519, 0., 0., 0., 0},
+    {"Pt", 195, 78, 6.78, 0.54, 0., 0., 0., 0},         {"U", 238, 92, 6.81, 0.55, 0., 0., 0.093, 1},
+    {"Ru", 96, 44, 5.085, 0.46, 0.158, 0., 0., 0},      {"Zr", 96, 40, 5.02, 0.46, 0., 0., 0., 0},
+    {"Xe", 129, 54, 5.42, 0.57, 0., 0., -0.003, 1},
+};
+}  // namespace
+
+bool find_nucleus(const std::string& name, const ipg_params& p, NucleusData* out) {
+    for (const Row& r : kNuclei) {
+        if (name != r.name) continue;
+        NucleusData d;
+        d.A = r.A;
+        d.Z = r.Z;
+        d.R = r.R;
+        d.a = r.a;
+        d.beta2 = r.user_b2 ? p.beta2 : r.b2;
+        d.beta3 = r.b3;
+        d.beta4 = r.b4;
+        d.gamma = 0.;
+        d.dR = 0.;
+        d.da = 0.;
+        if (p.setWSDeformParams) {
+            d.R = p.R_WS;
+            d.a = p.a_WS;
+            d.beta2 = p.beta2;
+            d.beta3 = p.beta3;
+            d.beta4 = p.beta4;
+            d.gamma = p.gamma;
+            d.dR = p.dR_np;
+            d.da = p.da_np;
+        }
+        d.force_dmin = p.force_dmin_flag;
+        d.d_min = p.d_min;
+        *out = d;
+        return true;
+    }
+    return false;
+}
+
+// ---- nucleon sampling ---------------------------------------------------------------
+namespace {
+
+double fermi(double r, double R, double a) { return 1. / (1. + std::exp((r - R) / a)); }
+
+double ws_radius(MT64& rng, double a, double R) {
+    const double rmax = R + 10. * a;
+    double r;
+    do {
+        r = rmax * std::pow(rng.real3(), 1.0 / 3.0);
+    } while (rng.real3() > fermi(r, R, a));
+    return r;
+}
+
+double ylm(int l, double ct) {
+    if (l == 2) return (3.0 * ct * ct - 1.0) * 0.31539156525252005;
+    if (l == 3) return (5.0 * ct * ct * ct - 3.0 * ct) * 0.3731763325901154;
+    return (35.0 * ct * ct * ct * ct - 30.0 * ct * ct + 3.0) * 0.10578554691520431;
+}
+
+void recentre(std::vector<double>& x, std::vector<double>& y, std::vector<double>& z) {
+    double mx = 0., my = 0., mz = 0.;
+    for (size_t i = 0; i < x.size(); i++) {
+        mx += x[i];
+        my += y[i];
+        mz += z[i];
+    }
+    mx /= (double)x.size();
+    my /= (double)y.size();
+    mz /= (double)z.size();
+    for (size_t i = 0; i < x.size(); i++) {
+        x[i] -= mx;
+        y[i] -= my;
+        z[i] -= mz;
+    }
+}
+
+void generate_nucleus(MT64& rng, const NucleusData& d, std::vector<Nucleon>& out) {
+    const int A = d.A, Z = d.Z;
+    std::vector<double> x(A, 0.), y(A, 0.), z(A, 0.);
+    std::vector<int> proton(A, 1);
+    const bool spherical = std::abs(d.beta2) < 1e-15 && std::abs(d.beta4) < 1e-15 && std::abs(d.beta3) < 1e-15 &&
+                           std::abs(d.gamma) < 1e-15;
+    if (spherical || (!d.force_dmin && std::abs(d.gamma) <= 1e-15)) {
+        // radii (and polar angles) first, sorted by radius, then placed
+        // with the d_min rejection (Init.cpp:2749-2816, 2833-2904)
+        std::vector<double> r(A, 0.), ct(A, 0.);
+        std::vector<int> idx(A);
+        for (int i = 0; i < A; i++) {
+            const double ai = i < Z ? d.a : d.a + d.da, Ri = i < Z ? d.R : d.R + d.dR;
+            if (spherical) {
+                r[i] = ws_radius(rng, ai, Ri);
+            } else {
+                const double rmax = Ri + 10. * ai;
+                double Rth;
+                do {
+                    r[i] = rmax * std::pow(rng.real3(), 1.0 / 3.0);
+                    ct[i] = 1.0 - 2.0 * rng.real3();
+                    Rth = Ri * (1.0 + d.beta2 * ylm(2, ct[i]) + d.beta3 * ylm(3, ct[i]) + d.beta4 * ylm(4, ct[i]));
+                } while (rng.real3() > fermi(r[i], Rth, ai));
+            }
+            idx[i] = i;
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&r](int i1, int i2) { return r[i1] < r[i2]; });
+        std::vector<double> rs(A);
+        for (int i = 0; i < A; i++) rs[i] = r[idx[i]];
+        const double dmin2 = d.d_min * d.d_min;
+        for (int i = 0; i < A; i++) {
+            const double ri = rs[i];
+            const double th_fixed = spherical ? 0. : std::acos(ct[idx[i]]);
+            double xi = 0, yi = 0, zi = 0;
+            int iter = 0, reject;
+            do {
+                iter++;
+                reject = 0;
+                const double phi = 2. * M_PI * rng.real3();
+                const double th = spherical ? std::acos(1. - 2. * rng.real3()) : th_fixed;
+                xi = ri * std::sin(th) * std::cos(phi);
+                yi = ri * std::sin(th) * std::sin(phi);
+                zi = ri * std::cos(th);
+                for (int j = i - 1; j >= 0; j--) {
+                    if ((ri - rs[j]) * (ri - rs[j]) > dmin2) break;
+                    const double dsq = (xi - x[j]) * (xi - x[j]) + (yi - y[j]) * (yi - y[j]) + (zi - z[j]) * (zi - z[j]);
+                    if (dsq < dmin2) {
+                        reject = 1;
+                        break;
+                    }
+                }
+            } while (reject == 1 && iter < 100);
+            x[i] = xi;
+            y[i] = yi;
+            z[i] = zi;
+            // the spherical sampler marks every nucleon a proton (Init.cpp:2809-2813)
+            proton[i] = spherical ? 1 : (idx[i] < Z);
+        }
+    } else {
+        // Init.cpp:2906-2982 (forced d_min) and 2984-3037 (triaxial)
+        const double rmax = d.R + d.dR + 10. * (d.a + d.da);
+        const double dmin2 = d.d_min * d.d_min;
+        for (int i = 0; i < A; i++) {
+            const double Ri = i >= Z ? d.R + d.dR : d.R, ai = i >= Z ? d.a + d.da : d.a;
+            bool again;
+            double xi, yi, zi;
+            do {
+                double r, ct, phi, Rth;
+                do {
+                    r = rmax * std::pow(rng.real3(), 1.0 / 3.0);
+                    ct = 1.0 - 2.0 * rng.real3();
+                    phi = 2. * M_PI * rng.real3();
+                    const double y22 = (1.0 - ct * ct) * std::cos(2. * phi) * 0.5462742152960397;
+                    Rth = Ri * (1.0 + d.beta2 * (std::cos(d.gamma) * ylm(2, ct) + std::sin(d.gamma) * y22) +
+                                d.beta3 * ylm(3, ct) + d.beta4 * ylm(4, ct));
+                } while (rng.real3() > fermi(r, Rth, ai));
+                const double st = std::sqrt(1. - ct * ct);
+                xi = r * st * std::cos(phi);
+                yi = r * st * std::sin(phi);
+                zi = r * ct;
+                again = false;
+                if (d.force_dmin) {
+                    for (int j = i - 1; j >= 0; j--) {
+                        const double dsq = (xi - x[j]) * (xi - x[j]) + (yi - y[j]) * (yi - y[j]) + (zi - z[j]) * (zi - z[j]);
+                        if (dsq < dmin2) {
+                            again = true;
+                            break;
+                        }
+                    }
+                }
+            } while (again);
+            x[i] = xi;
+            y[i] = yi;
+            z[i] = zi;
+            proton[i] = i < Z;
+        }
+    }
+    recentre(x, y, z);
+    out.resize(A);
+    for (int i = 0; i < A; i++) out[i] = Nucleon{x[i], y[i], z[i], std::atan2(y[i], x[i]), 0, proton[i]};
+}
+
+void rotate_3d(MT64& rng, std::vector<Nucleon>& n) {
+    const double al = 2 * M_PI * rng.real3();
+    const double be = std::acos(1. - 2. * rng.real3());
+    const double ga = 2 * M_PI * rng.real3();
+    const double c1 = std::cos(al), s1 = std::sin(al), c2 = std::cos(be), s2 = std::sin(be);
+    const double c3 = std::cos(ga), s3 = std::sin(ga);
+    for (Nucleon& q : n) {
+        const double x = q.x, y = q.y, z = q.z;
+        q.x = c2 * x - c3 * s2 * y + s2 * s3 * z;
+        q.y = c1 * s2 * x + (c1 * c2 * c3 - s1 * s3) * y + (-c3 * s1 - c1 * c2 * s3) * z;
+        q.z = s1 * s2 * x + (c1 * s3 + c2 * c3 * s1) * y + (c1 * c3 - c2 * s1 * s3) * z;
+    }
+}
+
+}  // namespace
+
+int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
+                 std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err) {
+    if (p.nucleonPositionsFromFile != 0) {
+        err = "nucleonPositionsFromFile != 0 is not supported";
+        return IPG_ERR_ARG;
+    }
+    if (p.useConstituentQuarkProton > 0 || p.smearQs == 1 || p.averageOverThisManyNuclei != 1) {
+        err = "constituent quarks, smearQs and nucleus averaging are not supported";
+        return IPG_ERR_ARG;
+    }
+    if (proj.A == 2 || targ.A == 2 || proj.A == 3 || targ.A == 3) {
+        err = "deuteron and He3 samplers are not supported";
+        return IPG_ERR_ARG;
+    }
+    const double xb = rng.real1();
+    const double b = p.samplebFromLinearDistribution == 1
+                         ? std::sqrt((p.bmax * p.bmax - p.bmin * p.bmin) * xb + p.bmin * p.bmin)
+                         : (p.bmax - p.bmin) * xb + p.bmin;
+    A.clear();
+    B.clear();
+    if (proj.A == 1)
+        A.push_back(Nucleon{0., 0., 0., 0., 0, 1});
+    else
+        generate_nucleus(rng, proj, A);
+    if (targ.A == 1)
+        B.push_back(Nucleon{0., 0., 0., 0., 0, 1});
+    else
+        generate_nucleus(rng, targ, B);
+    rotate_3d(rng, A);
+    rotate_3d(rng, B);
+
+    double phiRP = 0.;
+    if (p.rotateReactionPlane) phiRP = 2 * M_PI * rng.real2();
+    for (Nucleon& q : A) {
+        q.x -= b / 2. * std::cos(phiRP);
+        q.y -= b / 2. * std::sin(phiRP);
+    }
+    for (Nucleon& q : B) {
+        q.x += b / 2. * std::cos(phiRP);
+        q.y += b / 2. * std::sin(phiRP);
+    }
+    const double xi = p.protonAnisotropy;
+    if (xi != 0.) {
+        for (Nucleon& q : A) q.phi = 2 * M_PI * rng.real2();
+        for (Nucleon& q : B) q.phi = 2 * M_PI * rng.real2();
+    } else {
+        for (Nucleon& q : A) q.phi = 0.;
+        for (Nucleon& q : B) q.phi = 0.;
+    }
+    // binary collisions (Init.cpp:1117-1216)
+    const double d2 = p.SigmaNN / (M_PI * 10.);
+    int Ncoll = 0;
+    for (size_t i = 0; i < A.size(); i++) {
+        for (size_t j = 0; j < B.size(); j++) {
+            const double dx = B[j].x - A[i].x, dy = B[j].y - A[i].y;
+            const double dij = dx * dx + dy * dy;
+            bool hit;
+            if (p.gaussianWounding == 0) {
+                hit = dij < d2;
+            } else {
+                const double G = 0.92;
+                const double pr = G * std::exp(-G * dij / d2);
+                hit = rng.real1() < pr;
+            }
+            if (hit) {
+                Ncoll++;
+                B[j].collided = 1;
+                A[i].collided = 1;
+            }
+        }
+    }
+    if (A.size() == 1 && B.size() == 1) {
+        A[0].collided = 1;
+        B[0].collided = 1;
+    }
+    int Npart = 0;
+    for (const Nucleon& q : A) Npart += q.collided == 1;
+    for (const Nucleon& q : B) Npart += q.collided == 1;
+    info.b = b;
+    info.phiRP = phiRP;
+    info.Ncoll = Ncoll;
+    info.Npart = Npart;
+    return IPG_OK;
+}
+
+// ---- input file -----------------------------------------------------------------
+void default_params(ipg_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    // the reference's shipped `input` file
+    p->size = 64;
+    p->Nc = 3;
+    p->Ny = 50;
+    p->L = 8;
+    p->g = 1.;
+    p->m = 0.4;
+    p->UVdamp = 0.;
+    p->BG = 4.;
+    p->QsmuRatio = 0.8;
+    p->rmax = 1e7;
+    p->roots = 7000;
+    p->Rapidity = 0.;
+    p->xFromThisFactorTimesQs = 1;
+    p->SigmaNN = 73.;
+    p->maxtime = 0.1;
+    p->useFluctuatingx = 1;
+    p->useNucleus = 1;
+    p->gaussianWounding = 1;
+    p->rotateReactionPlane = 1;
+    p->samplebFromLinearDistribution = 1;
+    p->averageOverThisManyNuclei = 1;
+    p->useTimeForSeed = 1;
+    p->computeGluonMultiplicity = 1;
+    p->R_WS = 6.6;
+    p->a_WS = 0.52;
+    p->beta2 = 0.28;
+    p->beta4 = 0.08;
+    p->d_min = 0.9;
+    p->bmin = 0;
+    p->bmax = 5;
+    p->smearingWidth = 0.6;
+    p->Jacobianm = 0.35;
+    p->g2mu = 0.1;
+    p->seed = 13;
+    std::strcpy(p->Projectile, "p");
+    std::strcpy(p->Target, "p");
+    std::strcpy(p->NucleusQsTableFileName, "qs2Adj_vs_Tp_vs_Y_200.in");
+    derive(p);
+}
+
+void derive(ipg_params* p) {
+    const double a = p->L / static_cast<double>(p->size);
+    const int steps = static_cast<int>(10 * p->maxtime / a) + 1;
+    p->dtau = p->maxtime / (steps * a);
+    p->itmax = static_cast<int>(p->maxtime / (a * p->dtau) + 0.1);
+}
+
+int read_input(const char* path, ipg_params* p, std::string& err) {
+    std::ifstream in(path);
+    if (!in) {
+        err = std::string("input file ") + path + " is absent";
+        return IPG_ERR_IO;
+    }
+    // pairwise scan up to EndOfFile; the first occurrence of a key wins
+    std::map<std::string, std::string> kv;
+    std::string k, v;
+    while (in >> k) {
+        if (k == "EndOfFile") break;
+        if (!(in >> v)) break;
+        if (!kv.count(k)) kv[k] = v;
+    }
+    auto has = [&](const char* key) { return kv.count(key) > 0; };
+    auto D = [&](const char* key, double& dst) {
+        if (has(key)) dst = std::atof(kv[key].c_str());
+    };
+    auto I = [&](const char* key, int& dst) {
+        if (has(key)) dst = static_cast<int>(std::atof(kv[key].c_str()));
+    };
+    auto Sx = [&](const char* key, char* dst, size_t n) {
+        if (has(key)) {
+            std::strncpy(dst, kv[key].c_str(), n - 1);
+            dst[n - 1] = 0;
+        }
+    };
+    Sx("NucleusQsTableFileName", p->NucleusQsTableFileName, sizeof(p->NucleusQsTableFileName));
+    I("nucleonPositionsFromFile", p->nucleonPositionsFromFile);
+    Sx("Target", p->Target, sizeof(p->Target));
+    Sx("Projectile", p->Projectile, sizeof(p->Projectile));
+    I("runningCoupling", p->runningCoupling);
+    D("L", p->L);
+    D("BG", p->BG);
+    I("size", p->size);
+    I("useFluctuatingx", p->useFluctuatingx);
+    I("Nc", p->Nc);
+    I("inverseQsForMaxTime", p->inverseQsForMaxTime);
+    if (has("seed")) p->seed = (unsigned long long)(std::atof(kv["seed"].c_str()) + 0.5);
+    I("useSeedList", p->useSeedList);
+    I("Ny", p->Ny);
+    D("roots", p->roots);
+    D("g", p->g);
+    D("m", p->m);
+    D("Jacobianm", p->Jacobianm);
+    D("SigmaNN", p->SigmaNN);
+    D("rmax", p->rmax);
+    D("UVdamp", p->UVdamp);
+    I("setWSDeformParams", p->setWSDeformParams);
+    if (p->setWSDeformParams) {
+        D("R_WS", p->R_WS);
+        D("a_WS", p->a_WS);
+        D("beta2", p->beta2);
+        D("beta3", p->beta3);
+        D("beta4", p->beta4);
+        D("gamma", p->gamma);
+        double f = p->force_dmin_flag;
+        D("force_dmin_flag", f);
+        p->force_dmin_flag = f != 0.;
+        D("d_min", p->d_min);
+        D("dR_np", p->dR_np);
+        D("da_np", p->da_np);
+    }
+    D("bmin", p->bmin);
+    D("bmax", p->bmax);
+    I("rotateReactionPlane", p->rotateReactionPlane);
+    I("computeGluonMultiplicity", p->computeGluonMultiplicity);
+    D("QsmuRatio", p->QsmuRatio);
+    {
+        double f = p->usePseudoRapidity;
+        D("usePseudoRapidity", f);
+        p->usePseudoRapidity = (int)f;
+    }
+    D("Rapidity", p->Rapidity);
+    I("useNucleus", p->useNucleus);
+    D("g2mu", p->g2mu);
+    D("maxtime", p->maxtime);
+    D("xFromThisFactorTimesQs", p->xFromThisFactorTimesQs);
+    I("samplebFromLinearDistribution", p->samplebFromLinearDistribution);
+    I("writeOutputs", p->writeOutputs);
+    I("writeInitialWilsonLines", p->writeInitialWilsonLines);
+    I("readInitialWilsonLines", p->readInitialWilsonLines);
+    I("averageOverThisManyNuclei", p->averageOverThisManyNuclei);
+    I("useTimeForSeed", p->useTimeForSeed);
+    I("useFixedNpart", p->useFixedNpart);
+    I("smearQs", p->smearQs);
+    D("smearingWidth", p->smearingWidth);
+    I("gaussianWounding", p->gaussianWounding);
+    D("protonAnisotropy", p->protonAnisotropy);
+    {
+        double f = p->useConstituentQuarkProton;
+        D("useConstituentQuarkProton", f);
+        p->useConstituentQuarkProton = (int)f;
+    }
+    I("useSmoothNucleus", p->useSmoothNucleus);
+    {
+        // Setup::IFind truncates, and the reference stores it as an int
+        int q = (int)p->minimumQs2ST;
+        I("minimumQs2ST", q);
+        p->minimumQs2ST = q;
+    }
+    I("SubNucleonParamType", p->SubNucleonParamType);
+    I("rho_rng", p->rho_rng);
+    derive(p);
+    return IPG_OK;
+}
+
+}  // namespace ipg

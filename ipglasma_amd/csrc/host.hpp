@@ -1,0 +1,66 @@
+// host.hpp -- host-side C++ of the solver core: input parsing, MT19937-64,
+// nucleus data, nucleon sampling and wounding.  These are the parts of the
+// hot path that are sequential by definition (one random stream per event)
+// and bit-exact targets (nucleon positions, N_coll, N_part).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ipg.h"
+
+namespace ipg {
+
+// MT19937-64 with the reference's transforms (Random.cpp:67-204)
+class MT64 {
+   public:
+    void seed(uint64_t s);
+    uint64_t next();
+    double real1();  // [0,1]
+    double real2();  // [0,1)
+    double real3();  // (0,1)
+    double gauss();  // Marsaglia polar, second deviate cached
+
+   private:
+    uint64_t mt_[312];
+    int mti_ = 313;
+    int iset_ = 0;
+    double gset_ = 0.;
+};
+
+struct Nucleon {
+    double x, y, z, phi;
+    int collided;
+    int proton;
+};
+
+struct NucleusData {
+    int A = 0, Z = 0;
+    double R = 0, a = 0, beta2 = 0, beta3 = 0, beta4 = 0, gamma = 0, dR = 0, da = 0;
+    int force_dmin = 0;
+    double d_min = 0;
+};
+
+// Glauber::FindNucleusData2 (Glauber.cpp:19-287) with the setWSDeformParams
+// override; returns false for an unknown name
+bool find_nucleus(const std::string& name, const ipg_params& p, NucleusData* out);
+
+struct EventInfo {
+    double b = 0, phiRP = 0;
+    int Ncoll = 0, Npart = 0;
+};
+
+// Init::init's draws up to the colour-charge density: b (Init.cpp:2320-2345),
+// sampleTA (65-439), reaction plane and shift (868-901), wounding
+// (1117-1216).  Returns IPG_OK or IPG_ERR_ARG for unsupported options.
+int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
+                 std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err);
+
+int read_input(const char* path, ipg_params* p, std::string& err);
+void default_params(ipg_params* p);
+void derive(ipg_params* p);
+
+constexpr double HBARC = 0.1973269718;  // Phys_consts.h:6
+
+}  // namespace ipg

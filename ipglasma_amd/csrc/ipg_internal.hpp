@@ -1,0 +1,37 @@
+// ipg_internal.hpp -- host-side launchers shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ipg_device.hpp"
+
+// site-parallel helpers: nblk = total blocks over all events of the launch
+void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
+void ipg_launch_tmunu(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
+void ipg_launch_gauss(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
+void ipg_launch_finish(const EvDesc* d_evs, int nev, int it, double* d_out, hipStream_t st);
+
+// colour charge
+void ipg_launch_tp(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+void ipg_launch_qs(const EvDesc* d_evs, int nev, int nblk, const double* d_Tlist, const double* d_Qs2,
+                   int* d_flag_host_visible, hipStream_t st);
+void ipg_launch_stats(const EvDesc* d_evs, int nev, int nblk, double* d_out, hipStream_t st);
+
+// Wilson lines: one colour-charge slice k for every event of the launch
+struct FftPlanDev {
+    int N;
+    int logN;
+    int C;                 // columns per block in the column pass
+    const double2* tw;     // exp(-2 pi i m / N), m < N/2
+    const double* s2;      // 4 sin^2(pi q / N), q < N
+};
+void ipg_launch_setv_slice(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k, int nrowblk,
+                           int ncolblk, hipStream_t st);
+void ipg_launch_unit_v(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+
+// tau = 0 fields
+void ipg_launch_links(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+void ipg_launch_newton(const EvDesc* d_evs, int nev, int nlinkblk, hipStream_t st);
+void ipg_launch_pi_init(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+void ipg_launch_zero_fields(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
+void ipg_launch_rho_normals(const EvDesc* d_evs, int nucleus, int k, double* d_out, hipStream_t st);

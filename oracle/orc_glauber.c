@@ -383,7 +383,7 @@ void orc_g2mu2(const orc_qs_cfg *c, const double *Tl, const double *Q, const dou
             if (check != 2) continue;
             if (c->useFluctuatingx == 1) {
                 double ydev = 10000;
-                while (fabs(ydev) > 0.001) {
+                for (int guard = 0; fabs(ydev) > 0.001 && guard < 100000; guard++) {
                     if (ly >= 0) {
                         QsA = sqrt(orc_nuclear_qs2(Tl, Q, TpA[s], fabs(ly)));
                     } else {
@@ -407,7 +407,7 @@ void orc_g2mu2(const orc_qs_cfg *c, const double *Tl, const double *Q, const dou
                 if (gA[s] != gA[s]) gA[s] = 0.;
                 ly = c->rapidity;
                 ydev = 10000;
-                while (fabs(ydev) > 0.001) {
+                for (int guard = 0; fabs(ydev) > 0.001 && guard < 100000; guard++) {
                     if (ly >= 0) {
                         QsB = sqrt(orc_nuclear_qs2(Tl, Q, TpB[s], fabs(ly)));
                     } else {

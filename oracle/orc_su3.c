@@ -50,7 +50,11 @@ void orc_expm_coeff(const double Q[8], double coeff[18]) {
     for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
     c1 *= 0.25;
 
-    double c0max = fmax(1e-15, 2.0 * pow(c1 / 3.0, 1.5));
+    /* pow(c1/3, 1.5) written as x*sqrt(x): two correctly rounded operations,
+     * so the degenerate-eigenvalue case |c0/c0max| ~ 1 (where acos turns NaN)
+     * is decided identically on every platform */
+    const double x3 = c1 / 3.0;
+    double c0max = fmax(1e-15, 2.0 * (x3 * sqrt(x3)));
     double th3 = acos(c0 / c0max) / 3.0;
     double u = sqrt(c1 / 3.0) * cos(th3);
     double w = sqrt(c1) * sin(th3);

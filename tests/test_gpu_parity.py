@@ -1,0 +1,259 @@
+"""GPU parity: the HIP path through the C ABI against the C oracle on the same
+seeded inputs, stage by stage and end to end.
+
+Tolerances (north star): dE/deta 1e-9 relative, pointwise T^{mu nu} 1e-6
+relative (of the largest |T| of each component), bit-exact nucleon positions
+and collision counts.  Intermediate fields are held to FP64 round-off scaled
+by the depth of the computation.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TABLE = None
+
+
+def table():
+    global TABLE
+    if TABLE is None:
+        import ipglasma_amd as I
+        TABLE = K.load_table(I.QS_TABLE)
+    return TABLE
+
+
+def event(p, seed):
+    import ipglasma_amd as I
+    return I.Event(p, seed=seed)
+
+
+@pytest.mark.parametrize("system,seed", [("AuAu", 5), ("PbPb", 7), ("OO", 3), ("pPb", 2)])
+def test_nucleons_bit_exact(system, seed):
+    p = K.params(system, N=32)
+    ev = event(p, seed)
+    ev.init_nucleons()
+    A, B = ev.nucleons()
+    o = ev.observables()
+    st, nA, nB, info = K.oracle_event(p, seed)
+    np.testing.assert_array_equal(A, K.nucleon_array(nA))
+    np.testing.assert_array_equal(B, K.nucleon_array(nB))
+    assert o["Ncoll"] == info.Ncoll and o["Npart"] == info.Npart
+    assert o["b"] == info.b and o["phiRP"] == info.phiRP
+
+
+@pytest.mark.parametrize("system,seed", [("AuAu", 5), ("PbPb", 7), ("pPb", 2)])
+def test_color_charge(system, seed):
+    p = K.params(system, N=48)
+    ev = event(p, seed)
+    ev.init_nucleons()
+    ok = ev.init_color_charge()
+    st, nA, nB, info = K.oracle_event(p, seed)
+    TpA, TpB, gA, gB = K.oracle_color_charge(p, nA, nB, table())
+    np.testing.assert_allclose(ev.field("TpA"), TpA, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(ev.field("TpB"), TpB, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(ev.field("g2mu2A"), gA, rtol=1e-9, atol=0)
+    np.testing.assert_allclose(ev.field("g2mu2B"), gB, rtol=1e-9, atol=0)
+    out = np.zeros(8)
+    qc = K.qs_cfg(p)
+    O.lib().orc_stats(C.byref(qc), p.SigmaNN, info.Npart, p.minimumQs2ST, nA, len(nA), nB, len(nB),
+                      gA, gB, TpA, TpB, out)
+    o = ev.observables()
+    assert ok == bool(out[7])
+    np.testing.assert_allclose([o["averageQs"], o["averageQsAvg"], o["averageQsmin"], o["Tpp"],
+                                o["Qs2minST"]], out[:5], rtol=1e-9)
+
+
+def test_rho_normals_match_counter_map():
+    p = K.params("AuAu", N=32)
+    ev = event(p, 5)
+    ev.init_nucleons()
+    seed = ev.observables()["rho_seed"]
+    for nuc, k in ((0, 0), (1, 3)):
+        z = ev.rho_normals(nuc, k)
+        np.testing.assert_allclose(z, O.rho_normals(seed, nuc, k, p.size ** 2), rtol=0, atol=1e-13)
+
+
+def _init_to_fields(p, seed):
+    ev = event(p, seed)
+    ev.init_nucleons()
+    assert ev.init_color_charge()
+    ev.init_wilson_lines()
+    return ev
+
+
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2)])
+def test_wilson_lines(system, N, seed):
+    p = K.params(system, N=N, Ny=3)
+    ev = _init_to_fields(p, seed)
+    VA, VB = ev.field("VA"), ev.field("VB")
+    gA, gB = ev.field("g2mu2A"), ev.field("g2mu2B")
+    rs = ev.observables()["rho_seed"]
+    V = np.zeros((2, N * N, 18))
+    m_lat = p.m * (p.L / N) / 0.1973269718
+    O.lib().orc_setv(N, p.Ny, p.g, m_lat, 0.0, gA, gB, None, None, rs, V.reshape(-1))
+    np.testing.assert_allclose(VA, V[0], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(VB, V[1], rtol=0, atol=1e-11)
+
+
+def test_wilson_lines_injected_normals_mt_order():
+    """Reference draw order from the MT stream, injected on the host."""
+    p = K.params("AuAu", N=16, Ny=2, L=20.0)
+    ev = event(p, 9)
+    ev.init_nucleons()
+    assert ev.init_color_charge()
+    S = 16 * 16
+    z = O.mt_stream(4242, 4, 2 * p.Ny * S * 8)
+    ev.set_rho_normals(z)
+    ev.init_wilson_lines()
+    gA, gB = ev.field("g2mu2A"), ev.field("g2mu2B")
+    V = np.zeros((2, S, 18))
+    m_lat = p.m * (p.L / 16) / 0.1973269718
+    O.lib().orc_setv(16, p.Ny, p.g, m_lat, 0.0, gA, gB, z.ctypes.data, None, 0, V.reshape(-1))
+    np.testing.assert_allclose(ev.field("VA"), V[0], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(ev.field("VB"), V[1], rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("PbPb", 32, 7)])
+def test_tau0_fields(system, N, seed):
+    p = K.params(system, N=N, Ny=3)
+    ev = _init_to_fields(p, seed)
+    VA, VB = ev.field("VA").copy(), ev.field("VB").copy()
+    ev.init_fields()
+    o = ev.observables()
+    S = N * N
+    st = {k: np.zeros((S, 18)) for k in O.FIELDS}
+    stats = np.zeros(5)
+    O.lib().orc_init_fields(N, p.g, VA, VB, o["newton_seed"], *[st[k] for k in O.FIELDS], stats)
+    assert o["newton_unconverged"] == 0 and stats[3] == 0
+    for k in ("Ux", "Uy", "pi"):
+        np.testing.assert_allclose(ev.field(k), st[k], rtol=0, atol=1e-9, err_msg=k)
+    for k in ("E1", "E2", "phi"):
+        assert np.all(ev.field(k) == 0.0)
+    it_gpu = ev.newton_iterations()
+    assert abs(int(it_gpu.sum()) - int(stats[0])) <= 0.01 * stats[0]
+
+
+def _oracle_state_from(ev):
+    return {k: ev.field(k).copy() for k in O.FIELDS}
+
+
+def _tmunu_check(Tg, To):
+    for c in range(10):
+        scale = np.abs(To[c]).max()
+        if scale == 0:
+            assert np.abs(Tg[c]).max() == 0
+            continue
+        err = np.abs(Tg[c] - To[c]).max() / scale
+        assert err < 1e-6, (c, err)
+
+
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2)])
+def test_evolution_tmunu_dedeta(system, N, seed):
+    p = K.params(system, N=N, Ny=3)
+    ev = _init_to_fields(p, seed)
+    ev.init_fields()
+    st = _oracle_state_from(ev)
+    ev.run()
+    o = ev.observables()
+    T = np.zeros((10, N * N))
+    obs = np.zeros(3)
+    O.lib().orc_run(N, p.g, p.L, p.dtau, p.itmax, *[st[k] for k in O.FIELDS], T.reshape(-1), obs)
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-9 * abs(obs[0]), (o["dEdeta"], obs[0])
+    _tmunu_check(ev.tmunu(), T)
+    for k in O.FIELDS:
+        np.testing.assert_allclose(ev.field(k), st[k], rtol=0, atol=1e-10, err_msg=k)
+    assert o["gauss_max"] == pytest.approx(obs[1], rel=1e-6, abs=1e-14)
+
+
+def test_random_state_evolution_and_tmunu():
+    """Non-trivial random SU(3) links and Hermitian E, phi, pi (with trace
+    parts) stress every term of the step and of all ten T^{mu nu}."""
+    import ipglasma_amd as I
+    import scipy.linalg as sl
+    N = 16
+    p = K.params("AuAu", N=N, maxtime=0.6, L=8.0)
+    ev = I.Event(p, seed=1)
+    rng = np.random.default_rng(3)
+    S = N * N
+
+    def herm(scale):
+        a = rng.normal(size=(S, 3, 3)) + 1j * rng.normal(size=(S, 3, 3))
+        return scale * (a + a.conj().transpose(0, 2, 1)) / 2
+
+    def su3():
+        H = herm(0.7)
+        return np.array([sl.expm(1j * h) for h in H])
+
+    st = {"Ux": O.to_c(su3()), "Uy": O.to_c(su3()), "E1": O.to_c(herm(0.3)), "E2": O.to_c(herm(0.3)),
+          "phi": O.to_c(herm(0.2)), "pi": O.to_c(herm(0.2))}
+    for k, v in st.items():
+        ev.set_field(k, v)
+    ev.run()
+    T = np.zeros((10, S))
+    obs = np.zeros(3)
+    ref = {k: v.copy() for k, v in st.items()}
+    O.lib().orc_run(N, p.g, p.L, p.dtau, p.itmax, *[ref[k] for k in O.FIELDS], T.reshape(-1), obs)
+    _tmunu_check(ev.tmunu(), T)
+    o = ev.observables()
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-9 * abs(obs[0])
+    for k in O.FIELDS:
+        np.testing.assert_allclose(ev.field(k), ref[k], rtol=0, atol=1e-10, err_msg=k)
+
+
+def test_end_to_end_event():
+    """Full Init::init + Evolution::run on the GPU against the oracle chain
+    fed the device's nucleons, normals and restart keys."""
+    p = K.params("AuAu", N=32, Ny=4)
+    import ipglasma_amd as I
+    ev = I.Event(p, seed=21)
+    ok = ev.init()
+    assert ok
+    ev.run()
+    o = ev.observables()
+    st, nA, nB, info = K.oracle_event(p, 21)
+    TpA, TpB, gA, gB = K.oracle_color_charge(p, nA, nB, table())
+    N = p.size
+    S = N * N
+    V = np.zeros((2, S, 18))
+    m_lat = p.m * (p.L / N) / 0.1973269718
+    O.lib().orc_setv(N, p.Ny, p.g, m_lat, 0.0, gA, gB, None, None, o["rho_seed"], V.reshape(-1))
+    ref = {k: np.zeros((S, 18)) for k in O.FIELDS}
+    stats = np.zeros(5)
+    O.lib().orc_init_fields(N, p.g, V[0].copy(), V[1].copy(), o["newton_seed"], *[ref[k] for k in O.FIELDS],
+                            stats)
+    T = np.zeros((10, S))
+    obs = np.zeros(3)
+    O.lib().orc_run(N, p.g, p.L, p.dtau, p.itmax, *[ref[k] for k in O.FIELDS], T.reshape(-1), obs)
+    assert o["Npart"] == info.Npart and o["Ncoll"] == info.Ncoll
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-9 * abs(obs[0]), (o["dEdeta"], obs[0])
+    _tmunu_check(ev.tmunu(), T)
+
+
+def test_batch_matches_plain():
+    import ipglasma_amd as I
+    ps = [K.params("AuAu", N=32, Ny=3), K.params("pPb", N=16, Ny=3), K.params("PbPb", N=32, Ny=3)]
+    seeds = [5, 2, 7]
+    plain = []
+    for p, s in zip(ps, seeds):
+        ev = I.Event(p, seed=s)
+        ok = ev.init()
+        tries = 0
+        while not ok and tries < 20:
+            ok = ev.init()
+            tries += 1
+        ev.run()
+        plain.append(ev.observables())
+        ev.close()
+    for graph in (False, True):
+        b = I.Batch(ps, seeds)
+        b.run(use_graph=graph)
+        for e in range(len(ps)):
+            o = b.observables(e)
+            assert o["Ncoll"] == plain[e]["Ncoll"]
+            assert o["dEdeta"] == pytest.approx(plain[e]["dEdeta"], rel=1e-12)
+        b.close()

@@ -1,0 +1,178 @@
+// ipglasma.hpp -- C++ call surface of the reference (schenke/ipglasma) over
+// the C ABI of include/ipg.h.  Header only.
+//
+// The reference drives one event through (main.cpp:167-360):
+//     Setup setup; readInput(&setup, param, ...);          Setup.cpp:24-104
+//     Random random; random.init_genrand64(seed);          Random.cpp:67-73
+//     Init init(nn); Group group(Nc); Glauber glauber;     Init.h:51
+//     glauber.initGlauber(...);                            Glauber.cpp:1135
+//     Evolution evolution(nn); Lattice lat(param, Nc, N);  Evolution.h:23
+//     while (!param->getSuccess()) {
+//         init.init(&lat, &group, param, &random, &glauber, 0);  Init.cpp:2303
+//         if (param->getSuccess()) evolution.run(&lat, &group, param);
+//     }
+// The classes below keep those names, argument order and meaning.  The
+// differences are the error convention (an ipg::Error exception instead of
+// exit(1)) and that the event state lives on the GPU inside Lattice.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ipg.h"
+
+namespace ipglasma {
+
+struct Error : std::runtime_error {
+    explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+inline void check(int rc) {
+    if (rc != IPG_OK) throw Error(ipg_last_error());
+}
+
+// Parameters: the parsed input file (Parameters.h getters for the hot path)
+class Parameters {
+   public:
+    Parameters() { ipg_default_params(&p_); }
+    ipg_params& raw() { return p_; }
+    const ipg_params& raw() const { return p_; }
+    int getSize() const { return p_.size; }
+    int getNc() const { return p_.Nc; }
+    int getNy() const { return p_.Ny; }
+    double getL() const { return p_.L; }
+    double getg() const { return p_.g; }
+    double getdtau() const { return p_.dtau; }
+    double getMaxtime() const { return p_.maxtime; }
+    unsigned long long getSeed() const { return p_.seed; }
+    int getUseTimeForSeed() const { return p_.useTimeForSeed; }
+    std::string getNucleusQsTableFileName() const { return p_.NucleusQsTableFileName; }
+    // filled by Init::init (Init.cpp:1614-1644)
+    int getSuccess() const { return success_; }
+    void setSuccess(int s) { success_ = s; }
+    double getb() const { return obs_.b; }
+    int getNpart() const { return obs_.Npart; }
+    double getAverageQs() const { return obs_.averageQs; }
+    double getAverageQsAvg() const { return obs_.averageQsAvg; }
+    double getAverageQsmin() const { return obs_.averageQsmin; }
+    double getTpp() const { return obs_.Tpp; }
+    double getalphas() const { return obs_.alphas; }
+    const ipg_observables& observables() const { return obs_; }
+    void setObservables(const ipg_observables& o) { obs_ = o; }
+
+   private:
+    ipg_params p_;
+    int success_ = 0;
+    ipg_observables obs_{};
+};
+
+// Setup: the reference's key/value input reader (Setup.cpp:24-104)
+class Setup {
+   public:
+    void read(const std::string& file_name, Parameters* param) {
+        check(ipg_read_input(file_name.c_str(), &param->raw()));
+    }
+};
+
+// Random: the event's MT19937-64 stream (Random.cpp:67-73); the generator
+// itself runs inside the library so that the draw order of Init::init is
+// kept exactly
+class Random {
+   public:
+    void init_genrand64(unsigned long long seed) {
+        seed_ = seed;
+        seeded_ = true;
+    }
+    unsigned long long seed() const { return seed_; }
+    bool seeded() const { return seeded_; }
+
+   private:
+    unsigned long long seed_ = 5489ULL;
+    bool seeded_ = false;
+};
+
+// Group and Glauber keep their places in the call sequence; SU(3) and the
+// nucleus table are part of the library
+class Group {
+   public:
+    explicit Group(int Nc) {
+        if (Nc != 3) throw Error("only Nc = 3 is supported");
+    }
+};
+class Glauber {
+   public:
+    void initGlauber(double, std::string, std::string, double, bool, double, double, double, double, double,
+                     double, bool, double, double, double, int) {}
+};
+
+// Lattice: owns the device state of one event (Lattice.cpp:4-30)
+class Lattice {
+   public:
+    Lattice(Parameters* param, int Nc, int N) {
+        (void)Nc;
+        n_ = N;
+        param->raw().size = N;
+        ctx_ = ipg_create(&param->raw());
+        if (!ctx_) throw Error(ipg_last_error());
+        check(ipg_load_qs_table(ctx_, param->raw().NucleusQsTableFileName));
+    }
+    ~Lattice() { ipg_destroy(ctx_); }
+    Lattice(const Lattice&) = delete;
+    Lattice& operator=(const Lattice&) = delete;
+    ipg_ctx* ctx() { return ctx_; }
+    int N() const { return n_; }
+    // cell fields as [S][18] (3x3 complex row-major) or [S]
+    std::vector<double> field(int which, size_t S) {
+        std::vector<double> v(which < IPG_F_G2MU2A ? 18 * S : S);
+        check(ipg_get_field(ctx_, which, v.data()));
+        return v;
+    }
+    std::vector<double> tmunu(size_t S) {
+        std::vector<double> v(10 * S);
+        check(ipg_get_tmunu(ctx_, v.data()));
+        return v;
+    }
+    bool seeded = false;
+
+   private:
+    ipg_ctx* ctx_ = nullptr;
+    int n_ = 0;
+};
+
+// Init: Init::init (Init.cpp:2303-2696)
+class Init {
+   public:
+    explicit Init(const int nn[]) { (void)nn; }
+    void init(Lattice* lat, Group* group, Parameters* param, Random* random, Glauber* glauber,
+              int READFROMFILE) {
+        (void)group;
+        (void)glauber;
+        if (READFROMFILE != 0) throw Error("reading Wilson lines from file is not supported");
+        if (!lat->seeded) {
+            check(ipg_seed(lat->ctx(), random->seed()));
+            lat->seeded = true;
+        }
+        int ok = 0;
+        check(ipg_init(lat->ctx(), &ok));
+        ipg_observables o;
+        check(ipg_get_observables(lat->ctx(), &o));
+        param->setObservables(o);
+        param->setSuccess(ok);
+    }
+};
+
+// Evolution: Evolution::run (Evolution.cpp:366-657)
+class Evolution {
+   public:
+    explicit Evolution(const int nn[]) { (void)nn; }
+    void run(Lattice* lat, Group* group, Parameters* param) {
+        (void)group;
+        check(ipg_evolve(lat->ctx()));
+        ipg_observables o;
+        check(ipg_get_observables(lat->ctx(), &o));
+        param->setObservables(o);
+    }
+};
+
+}  // namespace ipglasma

@@ -53,6 +53,10 @@ uint64_t splitmix64(uint64_t x) {
 }
 
 int pad8(long long n) { return (int)(((n + 7) / 8) * 8); }
+
+// colour-charge slices per setV chunk: the Wilson lines are read and written
+// once per chunk; the FFT buffer holds 256 bytes per site and slice
+constexpr int kSetvChunk = 8;
 int log2i(int n) {
     int l = 0;
     while ((1 << l) < n) l++;
@@ -184,7 +188,14 @@ struct Engine {
             d.useFluctuatingx = p.useFluctuatingx;
             // device memory: 9 matrix fields (162 planes), 5 scalars, 2 int
             // planes, FFT buffer (18), T (10), partials, root-finder counters
-            const size_t nd = (size_t)S * (162 + 5 + 18 + 10) + (size_t)8 * d.nblocks + (size_t)S * 2;
+            // setV chunk: kc slices of 2 nuclei x 4 packed complex planes; the
+            // same region holds the four tau=0 link fields (72 planes) after
+            // the Wilson lines are done
+            d.kc = std::max(1, std::min(p.Ny, kSetvChunk));
+            const size_t nfft = std::max<size_t>(72, 16 * (size_t)d.kc);
+            // U (4 x 18 planes), phi pair, E1, E2, pi (5 x 9 Hermitian planes),
+            // 5 scalar planes, FFT/link region, T (10), partials, int planes
+            const size_t nd = (size_t)S * (72 + 45 + 5 + nfft + 10) + (size_t)8 * d.nblocks + (size_t)S * 2;
             HIPCHK(hipMalloc(&E.mem, nd * sizeof(double)));
             double* q = E.mem;
             auto take = [&](size_t n) {
@@ -192,30 +203,28 @@ struct Engine {
                 q += n;
                 return r;
             };
-            const size_t F = 18 * (size_t)S;
+            const size_t F = 18 * (size_t)S, H = 9 * (size_t)S;
             d.U[0][0] = take(F);
             d.U[0][1] = take(F);
             d.U[1][0] = take(F);
             d.U[1][1] = take(F);
-            d.phi[0] = take(F);
-            d.phi[1] = take(F);
-            d.E1 = take(F);
-            d.E2 = take(F);
-            d.pi = take(F);
+            d.phi[0] = take(H);
+            d.phi[1] = take(H);
+            d.E1 = take(H);
+            d.E2 = take(H);
+            d.pi = take(H);
             d.VA = d.U[1][0];
             d.VB = d.U[1][1];
-            d.L1x = d.E1;
-            d.L1y = d.E2;
-            d.L2x = d.phi[0];
-            d.L2y = d.phi[1];
             d.TpA = take(S);
             d.TpB = take(S);
             d.g2A = take(S);
             d.g2B = take(S);
             d.yB = take(S);
-            // FFT buffer [2][4][S] complex; also the 18 S-double scratch of the
-            // field layout transposes
-            d.fftbuf = (double2*)take(18 * (size_t)S);
+            d.fftbuf = (double2*)take(nfft * (size_t)S);
+            d.L1x = (double*)d.fftbuf;
+            d.L1y = d.L1x + F;
+            d.L2x = d.L1y + F;
+            d.L2y = d.L2x + F;
             d.T = take(10 * (size_t)S);
             d.partial = take(8 * (size_t)d.nblocks);
             int* ip = (int*)take(2 * (size_t)S);
@@ -403,20 +412,21 @@ struct Engine {
         for (auto& kv : groups) {
             const Plan& pl = plans[kv.first];
             FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
-            // contiguous descriptor run for this size: if the events of the
-            // group are not contiguous, launch per event
+            // contiguous descriptor run for this size, else one event at a time
             std::vector<int>& es = kv.second;
-            bool contiguous = es.back() - es.front() + 1 == (int)es.size();
-            int maxNy = 0;
-            for (int e : es) maxNy = std::max(maxNy, desc[e].Ny);
+            const bool contiguous = es.back() - es.front() + 1 == (int)es.size();
+            int maxNy = 0, kc = 1 << 30;
+            for (int e : es) {
+                maxNy = std::max(maxNy, desc[e].Ny);
+                kc = std::min(kc, desc[e].kc);
+            }
             if (contiguous) {
-                const int ng = (int)es.size();
-                for (int k = 0; k < maxNy; k++)
-                    ipg_launch_setv_slice(d_desc + es.front(), ng, dp, k, ng * 2 * pl.N, ng * 8 * (pl.N / pl.C), st);
+                for (int k0 = 0; k0 < maxNy; k0 += kc)
+                    ipg_launch_setv_chunk(d_desc + es.front(), (int)es.size(), dp, k0, kc, st);
             } else {
                 for (int e : es)
-                    for (int k = 0; k < desc[e].Ny; k++)
-                        ipg_launch_setv_slice(d_desc + e, 1, dp, k, 2 * pl.N, 8 * (pl.N / pl.C), st);
+                    for (int k0 = 0; k0 < desc[e].Ny; k0 += desc[e].kc)
+                        ipg_launch_setv_chunk(d_desc + e, 1, dp, k0, desc[e].kc, st);
             }
         }
         HIPCHK(hipGetLastError());
@@ -465,17 +475,29 @@ struct Engine {
         return m;
     }
 
-    // launches of leapfrog iteration it for all events (it = 0: tau=0 kick)
+    // does any active event evaluate T^{mu nu} (its last step) at iteration it
+    bool any_final(int it) const {
+        for (const EvDesc& d : desc)
+            if (d.active && d.itmax == it) return true;
+        return false;
+    }
+
+    // launches of leapfrog iteration it for all events (it = 0: tau=0 kick);
+    // the T^{mu nu}, Gauss-law and reduction kernels are launched only at
+    // iterations where some event takes its final half step
     void launch_iteration(int it) {
         const int nev = (int)ev.size();
         if (it == 0) {
             ipg_launch_step0(d_desc, nev, nblk, st);
             return;
         }
-        ipg_launch_tmunu(d_desc, nev, nblk, it, st);
+        const bool fin = any_final(it);
+        if (fin) ipg_launch_tmunu(d_desc, nev, nblk, it, st);
         ipg_launch_step(d_desc, nev, nblk, it, st);
-        ipg_launch_gauss(d_desc, nev, nblk, it, st);
-        ipg_launch_finish(d_desc, nev, it, d_out, st);
+        if (fin) {
+            ipg_launch_gauss(d_desc, nev, nblk, it, st);
+            ipg_launch_finish(d_desc, nev, it, d_out, st);
+        }
     }
 
     int evolve(bool use_graph) {
@@ -506,13 +528,16 @@ struct Engine {
             }
             ipg_launch_step0(d_desc, nev, nblk, st);
             for (int it = 1; it <= itm; it++) {
+                const bool fin = any_final(it);
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it], st));
-                ipg_launch_tmunu(d_desc, nev, nblk, it, st);
+                if (fin) ipg_launch_tmunu(d_desc, nev, nblk, it, st);
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it + 1], st));
                 ipg_launch_step(d_desc, nev, nblk, it, st);
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it + 2], st));
-                ipg_launch_gauss(d_desc, nev, nblk, it, st);
-                ipg_launch_finish(d_desc, nev, it, d_out, st);
+                if (fin) {
+                    ipg_launch_gauss(d_desc, nev, nblk, it, st);
+                    ipg_launch_finish(d_desc, nev, it, d_out, st);
+                }
             }
         }
         HIPCHK(hipGetLastError());
@@ -602,22 +627,43 @@ __global__ void k_aos_to_soa(const double* __restrict__ in, long long S, double*
     long long s = t / 18, c = t - s * 18;
     f[c * S + s] = in[t];
 }
+// Hermitian 9-plane storage <-> [S][18]
+__global__ void k_herm_to_aos(const double* __restrict__ f, long long S, double* __restrict__ out) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    M3 m = ldh(f, (size_t)S, (size_t)s);
+    for (int k = 0; k < 9; k++) {
+        out[18 * s + 2 * k] = m.e[k].r;
+        out[18 * s + 2 * k + 1] = m.e[k].i;
+    }
+}
+__global__ void k_aos_to_herm(const double* __restrict__ in, long long S, double* __restrict__ f) {
+    long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    M3 m;
+    for (int k = 0; k < 9; k++) m.e[k] = cd{in[18 * s + 2 * k], in[18 * s + 2 * k + 1]};
+    sth(f, (size_t)S, (size_t)s, m);
+}
 
-double* field_ptr(Engine& g, int which, bool* matrix) {
+// kind: 0 full 3x3 complex (18 planes), 1 Hermitian (9 planes), 2 scalar
+double* field_ptr(Engine& g, int which, int* kind) {
     EvDesc& d = g.desc[0];
     const int p = g.ev[0].parity;
-    *matrix = true;
+    *kind = 0;
     switch (which) {
         case IPG_F_UX: return d.U[p][0];
         case IPG_F_UY: return d.U[p][1];
+        case IPG_F_VA: return d.VA;
+        case IPG_F_VB: return d.VB;
+    }
+    *kind = 1;
+    switch (which) {
         case IPG_F_E1: return d.E1;
         case IPG_F_E2: return d.E2;
         case IPG_F_PHI: return d.phi[p];
         case IPG_F_PI: return d.pi;
-        case IPG_F_VA: return d.VA;
-        case IPG_F_VB: return d.VB;
     }
-    *matrix = false;
+    *kind = 2;
     switch (which) {
         case IPG_F_G2MU2A: return d.g2A;
         case IPG_F_G2MU2B: return d.g2B;
@@ -722,16 +768,19 @@ int ipg_get_observables(ipg_ctx* c, ipg_observables* o) {
 int ipg_get_field(ipg_ctx* c, int which, double* out) {
     if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
     Engine& g = c->eng;
-    bool mat;
-    double* f = field_ptr(g, which, &mat);
+    int kind;
+    double* f = field_ptr(g, which, &kind);
     if (!f) return fail(IPG_ERR_ARG, "unknown field id");
     const long long S = g.desc[0].S;
-    if (!mat) {
+    if (kind == 2) {
         HIPCHK(hipMemcpy(out, f, sizeof(double) * S, hipMemcpyDeviceToHost));
         return IPG_OK;
     }
-    double* tmp = (double*)g.desc[0].fftbuf;  // 18 S doubles of scratch
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, f, S, tmp);
+    double* tmp = (double*)g.desc[0].fftbuf;  // >= 72 S doubles of scratch
+    if (kind == 0)
+        hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, f, S, tmp);
+    else
+        hipLaunchKernelGGL(k_herm_to_aos, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, g.st, f, S, tmp);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(g.st));
     HIPCHK(hipMemcpy(out, tmp, sizeof(double) * 18 * S, hipMemcpyDeviceToHost));
@@ -740,17 +789,20 @@ int ipg_get_field(ipg_ctx* c, int which, double* out) {
 int ipg_set_field(ipg_ctx* c, int which, const double* in) {
     if (!c || !in) return fail(IPG_ERR_ARG, "null argument");
     Engine& g = c->eng;
-    bool mat;
-    double* f = field_ptr(g, which, &mat);
+    int kind;
+    double* f = field_ptr(g, which, &kind);
     if (!f) return fail(IPG_ERR_ARG, "unknown field id");
     const long long S = g.desc[0].S;
-    if (!mat) {
+    if (kind == 2) {
         HIPCHK(hipMemcpy(f, in, sizeof(double) * S, hipMemcpyHostToDevice));
         return IPG_OK;
     }
     double* tmp = (double*)g.desc[0].fftbuf;
     HIPCHK(hipMemcpy(tmp, in, sizeof(double) * 18 * S, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, tmp, S, f);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, g.st, tmp, S, f);
+    else
+        hipLaunchKernelGGL(k_aos_to_herm, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, g.st, tmp, S, f);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(g.st));
     g.desc[0].active = 1;

@@ -57,7 +57,8 @@ struct EvDesc {
     int useFluctuatingx;
     int qs_flags;  // bit0: serial fallback needed
     // setV
-    double2* fftbuf;     // [2 nuclei][4 packed][S] complex
+    double2* fftbuf;     // [2 nuclei][kc slices][4 packed][S] complex
+    int kc;              // slices per setV chunk
     const double* znorm;  // optional host-provided normals [2][Ny][S][8]
     uint64_t rho_seed;
     uint64_t newton_seed;

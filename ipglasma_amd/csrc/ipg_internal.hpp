@@ -17,7 +17,7 @@ void ipg_launch_qs(const EvDesc* d_evs, int nev, int nblk, const double* d_Tlist
                    int* d_flag_host_visible, hipStream_t st);
 void ipg_launch_stats(const EvDesc* d_evs, int nev, int nblk, double* d_out, hipStream_t st);
 
-// Wilson lines: one colour-charge slice k for every event of the launch
+// Wilson lines
 struct FftPlanDev {
     int N;
     int logN;
@@ -25,8 +25,8 @@ struct FftPlanDev {
     const double2* tw;     // exp(-2 pi i m / N), m < N/2
     const double* s2;      // 4 sin^2(pi q / N), q < N
 };
-void ipg_launch_setv_slice(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k, int nrowblk,
-                           int ncolblk, hipStream_t st);
+// slices [k0, k0 + kc) of every event of the launch (one lattice size)
+void ipg_launch_setv_chunk(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k0, int kc, hipStream_t st);
 void ipg_launch_unit_v(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 
 // tau = 0 fields

@@ -11,7 +11,10 @@
 // pi and E depend on the old U and phi only, so the kernel updates pi and E in
 // place, then phi and U from the new pi and E, writing phi and U to the other
 // half of a ping-pong pair (neighbours still read the old values).  Each site
-// reads and writes the six fields once: 1728 algorithmic bytes per site-step.
+// reads and writes the six fields once: 1728 algorithmic bytes per site-step
+// in the reference's representation (six full 3x3 complex matrices); E1, E2,
+// phi and pi are Hermitian and stored as 9 reals, so the kernel moves
+// 2 x (2 x 144 + 4 x 72) = 1152 bytes per site-step.
 // Neighbours are clamped (Lattice.cpp:19-28).
 #include "ipg_device.hpp"
 #include "ipg_internal.hpp"
@@ -35,46 +38,49 @@ __device__ __forceinline__ Nb neighbours(int s, int N) {
     return n;
 }
 
-// pi and E update at one site with step dtau at time tau (lattice units);
-// returns the new pi, E1, E2 in registers
-__device__ __forceinline__ void kick(const double* __restrict__ Ux, const double* __restrict__ Uy,
-                                     const double* __restrict__ phi, const double* __restrict__ E1,
-                                     const double* __restrict__ E2, const double* __restrict__ pi,
-                                     size_t S, const Nb& n, double g, double dtau, double tau, M3& pin,
-                                     M3& e1n, M3& e2n) {
-    const M3 ph = ldm(phi, S, n.s);
-    const M3 ux = ldm(Ux, S, n.s);
-    const M3 uy = ldm(Uy, S, n.s);
-    // parallel transported neighbours of phi
-    M3 phX = ux * mul_bd(ldm(phi, S, n.pX), ux);
-    M3 phY = uy * mul_bd(ldm(phi, S, n.pY), uy);
-    M3 br = phX;
-    {
-        M3 u = ldm(Ux, S, n.mX);
-        br = br + mul_ad(u, ldm(phi, S, n.mX)) * u;
+// Anti-Hermitian 3x3 in compact form: imaginary parts of the diagonal and the
+// upper off-diagonal entries (the lower ones are -conj of them).
+struct AH {
+    double d[3];
+    cd o[3];  // (0,1), (0,2), (1,2)
+};
+// X - X^+
+__device__ __forceinline__ AH ah_of(const M3& X) {
+    AH a;
+    a.d[0] = 2. * X.e[0].i;
+    a.d[1] = 2. * X.e[4].i;
+    a.d[2] = 2. * X.e[8].i;
+    a.o[0] = X.e[1] - cconj(X.e[3]);
+    a.o[1] = X.e[2] - cconj(X.e[6]);
+    a.o[2] = X.e[5] - cconj(X.e[7]);
+    return a;
+}
+// traceless part of an anti-Hermitian matrix
+__device__ __forceinline__ AH ah_tl(AH a) {
+    const double t = (a.d[0] + a.d[1] + a.d[2]) / 3.0;
+    a.d[0] -= t;
+    a.d[1] -= t;
+    a.d[2] -= t;
+    return a;
+}
+// traceless Hermitian part of  E + i c1 A + i c2 B  (E Hermitian, A, B anti-Hermitian)
+__device__ __forceinline__ M3 e_update(const M3& E, double c1, const AH& A, double c2, const AH& B) {
+    M3 h;
+    double dd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dd[k] = E.e[4 * k].r - c1 * A.d[k] - c2 * B.d[k];
+    const double t = (dd[0] + dd[1] + dd[2]) / 3.0;
+    h.e[0] = cd{dd[0] - t, 0.};
+    h.e[4] = cd{dd[1] - t, 0.};
+    h.e[8] = cd{dd[2] - t, 0.};
+    const int up[3] = {1, 2, 5}, lo[3] = {3, 6, 7};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        cd v = E.e[up[q]] + cd{-c1 * A.o[q].i - c2 * B.o[q].i, c1 * A.o[q].r + c2 * B.o[q].r};
+        h.e[up[q]] = v;
+        h.e[lo[q]] = cconj(v);
     }
-    br = br + phY;
-    {
-        M3 u = ldm(Uy, S, n.mY);
-        br = br + mul_ad(u, ldm(phi, S, n.mY)) * u;
-    }
-    br = br - 4. * ph;
-    pin = ldm(pi, S, n.s) + (dtau / tau) * br;
-
-    // commutators [phi~, phi]
-    M3 c1 = phX * ph - ph * phX;
-    M3 c2 = phY * ph - ph * phY;
-    // plaquettes
-    M3 U12 = (ux * ldm(Uy, S, n.pX)) * mul_bd(dag(ldm(Ux, S, n.pY)), uy);
-    M3 U12d = dag(U12);
-    M3 P = mul_bd(ux, ldm(Uy, S, n.pXmY)) * mul_ad(ldm(Ux, S, n.mY), ldm(Uy, S, n.mY));
-    M3 t1 = traceless(((U12 + P) - U12d) - dag(P));
-    const double a1 = tau * dtau / (2. * g * g);
-    const double a2 = dtau / tau;
-    e1n = traceless(ldm(E1, S, n.s) + (mul_i(a1, t1) + mul_i(a2, c1)));
-    P = mul_bd(uy, ldm(Ux, S, n.mXpY)) * mul_ad(ldm(Uy, S, n.mX), ldm(Ux, S, n.mX));
-    t1 = traceless(((U12d + P) - U12) - dag(P));
-    e2n = traceless(ldm(E2, S, n.s) + (mul_i(a1, t1) + mul_i(a2, c2)));
+    return h;
 }
 
 // (1 + X + X^2/2) with X = i c E  (the n=2 loop of Evolution.cpp:54-68)
@@ -83,6 +89,8 @@ __device__ __forceinline__ M3 taylor2(double c, const M3& E) {
     M3 t = m_one() + 0.5 * X;
     return m_one() + X * t;
 }
+
+#define IPG_PHASE() __builtin_amdgcn_sched_barrier(0)
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_step(const EvDesc* __restrict__ evs, int nev, int it) {
     const int e = find_event(evs, nev, blockIdx.x);
@@ -97,19 +105,58 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_step(const EvDesc* __restrict__ e
     const bool half = it == ev.itmax;
     const double dt = half ? ev.dtau / 2. : ev.dtau;
     const Nb n = neighbours((int)s, ev.N);
-    M3 pin, e1n, e2n;
-    kick(ev.U[p][0], ev.U[p][1], ev.phi[p], ev.E1, ev.E2, ev.pi, S, n, ev.g, dt, tau, pin, e1n, e2n);
-    stm(ev.pi, S, s, pin);
-    stm(ev.E1, S, s, e1n);
-    stm(ev.E2, S, s, e2n);
-    if (half) return;
-    // drift: phi and U from the new pi and E
-    const double cphi = (tau + dt / 2.) * dt;
-    M3 ph = ldm(ev.phi[p], S, s) + cphi * pin;
-    stm(ev.phi[p ^ 1], S, s, ph);
+    const double* __restrict__ Ux = ev.U[p][0];
+    const double* __restrict__ Uy = ev.U[p][1];
+    const double* __restrict__ phi = ev.phi[p];
+
+    // phase 1: pi (evolvePi) and the commutators [phi~_i, phi] of evolveE
+    M3 ux = ldm(Ux, S, s), uy;
+    AH c1, c2;
+    M3 pin;
+    {
+        const M3 ph = ldh(phi, S, s);
+        M3 t = ux * mul_bd(ldh(phi, S, n.pX), ux);  // phi~_x
+        c1 = ah_of(t * ph);
+        M3 br = t;
+        {
+            M3 u = ldm(Ux, S, n.mX);
+            br = br + mul_ad(u, ldh(phi, S, n.mX)) * u;
+        }
+        uy = ldm(Uy, S, s);
+        t = uy * mul_bd(ldh(phi, S, n.pY), uy);  // phi~_y
+        c2 = ah_of(t * ph);
+        br = br + t;
+        {
+            M3 u = ldm(Uy, S, n.mY);
+            br = br + mul_ad(u, ldh(phi, S, n.mY)) * u;
+        }
+        br = br - 4. * ph;
+        pin = herm_part(ldh(ev.pi, S, s) + (dt / tau) * br);
+        sth(ev.pi, S, s, pin);
+        if (!half) sth(ev.phi[p ^ 1], S, s, ph + ((tau + dt / 2.) * dt) * pin);  // evolvePhi
+    }
+    IPG_PHASE();
+    const double a1 = tau * dt / (2. * ev.g * ev.g);
+    const double a2 = dt / tau;
     const double cu = ev.g * ev.g * dt / (tau + dt / 2.);
-    stm(ev.U[p ^ 1][0], S, s, taylor2(cu, e1n) * ldm(ev.U[p][0], S, s));
-    stm(ev.U[p ^ 1][1], S, s, taylor2(cu, e2n) * ldm(ev.U[p][1], S, s));
+    // phase 2: E1 from the plaquettes U_12 and U_{1,-2} (evolveE), then U_x
+    M3 U12 = (ux * ldm(Uy, S, n.pX)) * mul_bd(dag(ldm(Ux, S, n.pY)), uy);
+    {
+        M3 P = mul_bd(ux, ldm(Uy, S, n.pXmY)) * mul_ad(ldm(Ux, S, n.mY), ldm(Uy, S, n.mY));
+        const AH t1 = ah_tl(ah_of(U12 + P));
+        const M3 e1n = e_update(ldh(ev.E1, S, s), a1, t1, a2, c1);
+        sth(ev.E1, S, s, e1n);
+        if (!half) stm(ev.U[p ^ 1][0], S, s, taylor2(cu, e1n) * ux);  // evolveU
+    }
+    IPG_PHASE();
+    // phase 3: E2 from U_21 = U_12^+ and U_{2,-1}, then U_y
+    {
+        M3 P = mul_bd(uy, ldm(Ux, S, n.mXpY)) * mul_ad(ldm(Uy, S, n.mX), ldm(Ux, S, n.mX));
+        const AH t1 = ah_tl(ah_of(dag(U12) + P));
+        const M3 e2n = e_update(ldh(ev.E2, S, s), a1, t1, a2, c2);
+        sth(ev.E2, S, s, e2n);
+        if (!half) stm(ev.U[p ^ 1][1], S, s, taylor2(cu, e2n) * uy);
+    }
 }
 
 // evolvePhi(dtau, 0) and evolveU(dtau, 0) before the loop (Evolution.cpp:393-394)
@@ -122,11 +169,11 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_step0(const EvDesc* __restrict__ 
     if (s >= ev.S) return;
     const size_t S = ev.S;
     const double dt = ev.dtau, tau = 0.;
-    M3 ph = ldm(ev.phi[0], S, s) + ((tau + dt / 2.) * dt) * ldm(ev.pi, S, s);
-    stm(ev.phi[0], S, s, ph);
+    M3 ph = ldh(ev.phi[0], S, s) + ((tau + dt / 2.) * dt) * ldh(ev.pi, S, s);
+    sth(ev.phi[0], S, s, ph);
     const double cu = ev.g * ev.g * dt / (tau + dt / 2.);
-    stm(ev.U[0][0], S, s, taylor2(cu, ldm(ev.E1, S, s)) * ldm(ev.U[0][0], S, s));
-    stm(ev.U[0][1], S, s, taylor2(cu, ldm(ev.E2, S, s)) * ldm(ev.U[0][1], S, s));
+    stm(ev.U[0][0], S, s, taylor2(cu, ldh(ev.E1, S, s)) * ldm(ev.U[0][0], S, s));
+    stm(ev.U[0][1], S, s, taylor2(cu, ldh(ev.E2, S, s)) * ldm(ev.U[0][1], S, s));
 }
 
 // ---- reductions ---------------------------------------------------------------
@@ -194,8 +241,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
         const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
         double* T = ev.T;
         // electric and longitudinal
-        const M3 e1 = ldm(ev.E1, S, s), e2 = ldm(ev.E2, S, s), e1p = ldm(ev.E1, S, pY), e2p = ldm(ev.E2, S, pX);
-        const M3 p0 = ldm(ev.pi, S, s), pXm = ldm(ev.pi, S, pX), pYm = ldm(ev.pi, S, pY), pXYm = ldm(ev.pi, S, pXY);
+        const M3 e1 = ldh(ev.E1, S, s), e2 = ldh(ev.E2, S, s), e1p = ldh(ev.E1, S, pY), e2p = ldh(ev.E2, S, pX);
+        const M3 p0 = ldh(ev.pi, S, s), pXm = ldh(ev.pi, S, pX), pYm = ldh(ev.pi, S, pY), pXYm = ldh(ev.pi, S, pXY);
         const double t11 = retrprod(e1, e1), t1p = retrprod(e1p, e1p), t22 = retrprod(e2, e2), t2p = retrprod(e2p, e2p);
         const double lng = (retrprod(p0, p0) + retrprod(pXm, pXm) + retrprod(pYm, pYm) + retrprod(pXYm, pXYm)) / 4.;
         const double gt = g * g / tau / tau;
@@ -208,7 +255,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
         const M3 UxpY = ldm(Ux, S, pY), UypX = ldm(Uy, S, pX);
         double rplaq = 3.0;
         if (i != N - 1) rplaq = retrprod(ux, UypX * (mul_bd(dag(UxpY), uy)));
-        const M3 ph = ldm(phi, S, s), phX = ldm(phi, S, pX), phY = ldm(phi, S, pY), phXY = ldm(phi, S, pXY);
+        const M3 ph = ldh(phi, S, s), phX = ldh(phi, S, pX), phY = ldh(phi, S, pY), phXY = ldh(phi, S, pXY);
         const M3 Udx = dag(ux), Udy = dag(uy), UDxpY = dag(UxpY), UDypX = dag(UypX);
         const M3 gX = m3(ux, phX, Udx) - ph;   // Ux phiX Ux^+ - phi
         const M3 gY = m3(uy, phY, Udy) - ph;
@@ -260,8 +307,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
 
         // T^{tau x}
         {
-            const M3 phmX = ldm(phi, S, mX), phmXpY = ldm(phi, S, mXpY), ph2X = ldm(phi, S, p2X);
-            const M3 ph2XY = ldm(phi, S, p2XY);
+            const M3 phmX = ldh(phi, S, mX), phmXpY = ldh(phi, S, mXpY), ph2X = ldh(phi, S, p2X);
+            const M3 ph2XY = ldh(phi, S, p2XY);
             double im = imtrprod(e2, tlquad(Pa, Pb, Pc, Pd)) + imtrprod(e2p, tlquad(Pe, Pf, Pg, Ph));
             double re = retrprod(p0, m3(ux, phX, Udx) - m3(UDxmX, phmX, UxmX)) +
                         retrprod(pYm, m3(UxpY, phXY, UDxpY) - m3(UDxmXpY, phmXpY, UxmXpY)) +
@@ -271,8 +318,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
         }
         // T^{tau y}
         {
-            const M3 phmY = ldm(phi, S, mY), phpXmY = ldm(phi, S, pXmY), ph2Y = ldm(phi, S, p2Y);
-            const M3 phX2Y = ldm(phi, S, pX2Y);
+            const M3 phmY = ldh(phi, S, mY), phpXmY = ldh(phi, S, pXmY), ph2Y = ldh(phi, S, p2Y);
+            const M3 phX2Y = ldh(phi, S, pX2Y);
             double im = imtrprod(e1, tlquad(Pb, Pa, Qc, Qd)) + imtrprod(e1p, tlquad(Qe, Qf, Qg, Qh));
             double re = retrprod(p0, m3(uy, phY, Udy) - m3(UDymY, phmY, UymY)) +
                         retrprod(pXm, m3(UypX, phXY, UDypX) - m3(UDypXmY, phpXmY, UypXmY)) +
@@ -323,9 +370,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gauss(const EvDesc* __restrict__ 
         const int p = (it - 1) & 1;
         const Nb n = neighbours((int)s, ev.N);
         const M3 uxm = ldm(ev.U[p][0], S, n.mX), uym = ldm(ev.U[p][1], S, n.mY);
-        const M3 ph = ldm(ev.phi[p], S, s), pp = ldm(ev.pi, S, s);
-        M3 G = mul_ad(uxm, ldm(ev.E1, S, n.mX)) * uxm - ldm(ev.E1, S, s);
-        G = G + mul_ad(uym, ldm(ev.E2, S, n.mY)) * uym - ldm(ev.E2, S, s);
+        const M3 ph = ldh(ev.phi[p], S, s), pp = ldh(ev.pi, S, s);
+        M3 G = mul_ad(uxm, ldh(ev.E1, S, n.mX)) * uxm - ldh(ev.E1, S, s);
+        G = G + mul_ad(uym, ldh(ev.E2, S, n.mY)) * uym - ldh(ev.E2, S, s);
         G = G - mul_i(1.0, ph * pp - pp * ph);
         v = square(G);
     }

@@ -399,14 +399,23 @@ __device__ void lds_ifft_dif(double2* x, int nrow, int N, int logN, const double
     }
 }
 
-// rows forward: generate rho for row i of nucleus n, 4 packed complex rows
+// A launch covers a chunk of kc slices [k0, k0+kc) of every event of one
+// lattice size.  fftbuf holds [nucleus][slice in chunk][4 packed pairs][S]
+// complex values.
+
+// rows forward: generate rho for row i of (nucleus n, slice k), 4 packed
+// complex rows, forward FFT along y
 __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
-                                                            int k) {
+                                                            int k0, int kc) {
     extern __shared__ double2 lds[];
     const int N = pl.N;
-    const int e = blockIdx.x / (2 * N);
-    const int n = (blockIdx.x / N) & 1;
-    const int i = blockIdx.x % N;
+    const int per_ev = 2 * kc * N;
+    const int e = blockIdx.x / per_ev;
+    const int rem = blockIdx.x - e * per_ev;
+    const int n = rem / (kc * N);
+    const int kk = (rem / N) % kc;
+    const int i = rem % N;
+    const int k = k0 + kk;
     const EvDesc& ev = evs[e];
     if (!ev.active || k >= ev.Ny) return;
     const size_t S = ev.S;
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
     }
     __syncthreads();
     lds_fft_dit(lds, 4, N, pl.logN, pl.tw);
-    double2* out = ev.fftbuf + (size_t)n * 4 * S;
+    double2* out = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
         out[(size_t)p * S + (size_t)i * N + j] = lds[q];
@@ -436,19 +445,22 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
 }
 
 // columns: forward FFT, multiply by 1/(k^2+m^2) e^{-|k| UVdamp} / N^2, inverse
-__global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k) {
+__global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+                                                        int kc) {
     extern __shared__ double2 lds[];
     const int N = pl.N, C = pl.C;
     const int per_pack = N / C;
-    const int e = blockIdx.x / (8 * per_pack);
-    const int rem = blockIdx.x - e * 8 * per_pack;
-    const int n = rem / (4 * per_pack);
+    const int per_ev = 2 * kc * 4 * per_pack;
+    const int e = blockIdx.x / per_ev;
+    const int rem = blockIdx.x - e * per_ev;
+    const int nk = rem / (4 * per_pack);  // n * kc + kk
     const int p = (rem / per_pack) & 3;
     const int j0 = (rem % per_pack) * C;
+    const int k = k0 + nk % kc;
     const EvDesc& ev = evs[e];
     if (!ev.active || k >= ev.Ny) return;
     const size_t S = ev.S;
-    double2* buf = ev.fftbuf + ((size_t)n * 4 + p) * S;
+    double2* buf = ev.fftbuf + ((size_t)nk * 4 + p) * S;
     // load C columns: lds[c * N + bitrev(r)]
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
@@ -477,37 +489,56 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
     }
 }
 
-// rows inverse, then V <- exp(-i A^a t^a) V at every site of the row
-__global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl,
-                                                            int k) {
+// rows inverse for every slice of the chunk in order; each thread keeps the
+// Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V
+template <int SPT>
+__global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+                                                            int kc) {
     extern __shared__ double2 lds[];
     const int N = pl.N;
     const int e = blockIdx.x / (2 * N);
     const int n = (blockIdx.x / N) & 1;
     const int i = blockIdx.x % N;
     const EvDesc& ev = evs[e];
-    if (!ev.active || k >= ev.Ny) return;
+    if (!ev.active || k0 >= ev.Ny) return;
     const size_t S = ev.S;
-    const double2* in = ev.fftbuf + (size_t)n * 4 * S;
-    for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
-        const int p = q / N, j = q - p * N;
-        lds[q] = in[(size_t)p * S + (size_t)i * N + j];
-    }
-    __syncthreads();
-    lds_ifft_dif(lds, 4, N, pl.logN, pl.tw);
-    double* V = n ? ev.VB : ev.VA;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const size_t s = (size_t)i * N + j;
-        const unsigned jr = bitrev(j, pl.logN);
-        double Q[8];
+    double* Vg = n ? ev.VB : ev.VA;
+    M3 V[SPT];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            double2 v = lds[p * N + jr];
-            Q[2 * p] = -v.x;
-            Q[2 * p + 1] = -v.y;
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * IPG_BLOCK;
+        V[t] = j < N ? ldm(Vg, S, (size_t)i * N + j) : m_one();
+    }
+    const int kend = min(kc, ev.Ny - k0);
+    for (int kk = 0; kk < kend; kk++) {
+        const double2* in = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
+        for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
+            const int p = q / N, j = q - p * N;
+            lds[q] = in[(size_t)p * S + (size_t)i * N + j];
         }
-        M3 u = exp_iQt(Q);
-        stm(V, S, s, u * ldm(V, S, s));
+        __syncthreads();
+        lds_ifft_dif(lds, 4, N, pl.logN, pl.tw);
+#pragma unroll
+        for (int t = 0; t < SPT; t++) {
+            const int j = threadIdx.x + t * IPG_BLOCK;
+            if (j < N) {
+                const unsigned jr = bitrev(j, pl.logN);
+                double Q[8];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    double2 v = lds[p * N + jr];
+                    Q[2 * p] = -v.x;
+                    Q[2 * p + 1] = -v.y;
+                }
+                V[t] = exp_iQt(Q) * V[t];
+            }
+        }
+        __syncthreads();  // the next slice overwrites the LDS rows
+    }
+#pragma unroll
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * IPG_BLOCK;
+        if (j < N) stm(Vg, S, (size_t)i * N + j, V[t]);
     }
 }
 
@@ -747,7 +778,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_pi_init(const EvDesc* __restrict_
         acc = (((acc - mul_ad(U, D)) + D) + Dd * U) - Dd;
     }
     // pi = (-2i/g) (1/8) acc
-    stm(ev.pi, S, s, mul_i(-2. / ev.g, (1. / 8.) * acc));
+    sth(ev.pi, S, s, mul_i(-2. / ev.g, (1. / 8.) * acc));
 }
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_zero_fields(const EvDesc* __restrict__ evs, int nev) {
@@ -757,9 +788,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_zero_fields(const EvDesc* __restr
     const long long s = (long long)xcd_remap(blockIdx.x - ev.block0, ev.nblocks) * IPG_BLOCK + threadIdx.x;
     if (s >= ev.S) return;
     const M3 z = m_zero();
-    stm(ev.E1, ev.S, s, z);
-    stm(ev.E2, ev.S, s, z);
-    stm(ev.phi[0], ev.S, s, z);
+    sth(ev.E1, ev.S, s, z);
+    sth(ev.E2, ev.S, s, z);
+    sth(ev.phi[0], ev.S, s, z);
 }
 
 }  // namespace
@@ -778,14 +809,21 @@ void ipg_launch_stats(const EvDesc* d, int nev, int nblk, double* out, hipStream
     hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
     hipLaunchKernelGGL(k_stats_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, out);
 }
-void ipg_launch_setv_slice(const EvDesc* d, int nev, const FftPlanDev& pl, int k, int nrowblk, int ncolblk,
-                           hipStream_t st) {
+void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, hipStream_t st) {
     const size_t lrow = sizeof(double2) * 4 * pl.N;
     const size_t lcol = sizeof(double2) * pl.C * pl.N;
-    hipLaunchKernelGGL(k_setv_rows_fwd, dim3(nrowblk), dim3(IPG_BLOCK), lrow, st, d, pl, k);
-    hipLaunchKernelGGL(k_setv_cols, dim3(ncolblk), dim3(IPG_BLOCK), lcol, st, d, pl, k);
-    hipLaunchKernelGGL(k_setv_rows_inv, dim3(nrowblk), dim3(IPG_BLOCK), lrow, st, d, pl, k);
-    (void)nev;
+    const int nrow_fwd = nev * 2 * kc * pl.N;
+    const int ncol = nev * 2 * kc * 4 * (pl.N / pl.C);
+    const int nrow_inv = nev * 2 * pl.N;
+    hipLaunchKernelGGL(k_setv_rows_fwd, dim3(nrow_fwd), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_setv_cols, dim3(ncol), dim3(IPG_BLOCK), lcol, st, d, pl, k0, kc);
+    const int spt = (pl.N + IPG_BLOCK - 1) / IPG_BLOCK;
+    if (spt <= 1)
+        hipLaunchKernelGGL(k_setv_rows_inv<1>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
+    else if (spt == 2)
+        hipLaunchKernelGGL(k_setv_rows_inv<2>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL(k_setv_rows_inv<4>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
 }
 void ipg_launch_unit_v(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_unit_v, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);

@@ -158,22 +158,88 @@ IPG_HD double square(const M3& a) {
 }
 
 // ---- SoA field access ------------------------------------------------------
-template <typename P>
-__device__ __forceinline__ M3 ldm(P f, size_t S, size_t s) {
+// Field pointers are cast to the global address space so that each plane is
+// addressed as (uniform plane base in SGPRs) + (32-bit lane offset): one
+// offset register per site instead of a 64-bit address per plane.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ const gdouble* gp(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* gp(double* p) { return (gdouble*)p; }
+// plane k at byte offset ob (< 4 GiB, one plane of at most 2^29 sites)
+__device__ __forceinline__ double gld(const double* f, size_t plane, unsigned ob) {
+    return *(const gdouble*)((const gchar*)gp(f + plane) + ob);
+}
+__device__ __forceinline__ void gst(double* f, size_t plane, unsigned ob, double v) {
+    *(gdouble*)((gchar*)gp(f + plane) + ob) = v;
+}
+
+__device__ __forceinline__ M3 ldm(const double* f, size_t S, size_t s) {
+    const unsigned ob = (unsigned)s * 8u;
     M3 m;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        m.e[k].r = f[(2 * k) * S + s];
-        m.e[k].i = f[(2 * k + 1) * S + s];
+        m.e[k].r = gld(f, (2 * k) * S, ob);
+        m.e[k].i = gld(f, (2 * k + 1) * S, ob);
     }
     return m;
 }
-__device__ __forceinline__ void stm(double* __restrict__ f, size_t S, size_t s, const M3& m) {
+__device__ __forceinline__ void stm(double* f, size_t S, size_t s, const M3& m) {
+    const unsigned ob = (unsigned)s * 8u;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        f[(2 * k) * S + s] = m.e[k].r;
-        f[(2 * k + 1) * S + s] = m.e[k].i;
+        gst(f, (2 * k) * S, ob, m.e[k].r);
+        gst(f, (2 * k + 1) * S, ob, m.e[k].i);
     }
+}
+
+// Hermitian fields (E^i, phi, pi) are stored as 9 planes: the real diagonal
+// h00 h11 h22, then Re/Im of h01, h02, h12.  They are Hermitian by
+// construction (Evolution.cpp:245-290 builds E from i(X - X^+) and
+// commutators of Hermitian matrices; pi = -(2i/g) A^eta with A^eta
+// anti-Hermitian, Init.cpp:2669-2673); storing the upper triangle drops only
+// the round-off-level anti-Hermitian part.  pi and phi keep their trace.
+__device__ __forceinline__ M3 ldh(const double* f, size_t S, size_t s0) {
+    const unsigned s = (unsigned)s0 * 8u;
+    M3 m;
+    const double r01 = gld(f, 3 * S, s), i01 = gld(f, 4 * S, s);
+    const double r02 = gld(f, 5 * S, s), i02 = gld(f, 6 * S, s);
+    const double r12 = gld(f, 7 * S, s), i12 = gld(f, 8 * S, s);
+    m.e[0] = cd{gld(f, 0, s), 0.};
+    m.e[4] = cd{gld(f, S, s), 0.};
+    m.e[8] = cd{gld(f, 2 * S, s), 0.};
+    m.e[1] = cd{r01, i01};
+    m.e[3] = cd{r01, -i01};
+    m.e[2] = cd{r02, i02};
+    m.e[6] = cd{r02, -i02};
+    m.e[5] = cd{r12, i12};
+    m.e[7] = cd{r12, -i12};
+    return m;
+}
+__device__ __forceinline__ void sth(double* f, size_t S, size_t s0, const M3& m) {
+    const unsigned s = (unsigned)s0 * 8u;
+    gst(f, 0, s, m.e[0].r);
+    gst(f, S, s, m.e[4].r);
+    gst(f, 2 * S, s, m.e[8].r);
+    gst(f, 3 * S, s, m.e[1].r);
+    gst(f, 4 * S, s, m.e[1].i);
+    gst(f, 5 * S, s, m.e[2].r);
+    gst(f, 6 * S, s, m.e[2].i);
+    gst(f, 7 * S, s, m.e[5].r);
+    gst(f, 8 * S, s, m.e[5].i);
+}
+// the stored (upper-triangle) form of a matrix, expanded again
+IPG_HD M3 herm_part(const M3& m) {
+    M3 h;
+    h.e[0] = cd{m.e[0].r, 0.};
+    h.e[4] = cd{m.e[4].r, 0.};
+    h.e[8] = cd{m.e[8].r, 0.};
+    h.e[1] = m.e[1];
+    h.e[2] = m.e[2];
+    h.e[5] = m.e[5];
+    h.e[3] = cconj(m.e[1]);
+    h.e[6] = cconj(m.e[2]);
+    h.e[7] = cconj(m.e[5]);
+    return h;
 }
 
 // ---- SU(3) exponential -------------------------------------------------------

@@ -55,3 +55,32 @@ def test_read_input_missing_file_is_an_error():
 def test_version():
     import ipglasma_amd as I
     assert I.lib().ipg_version() >= 100
+
+
+def test_cxx_call_surface_compiles_and_links(tmp_path):
+    """A driver written against the reference's class names (main.cpp:167-360)
+    compiles against include/ipglasma.hpp and links to the library."""
+    src = tmp_path / "drv.cpp"
+    src.write_text('''
+#include "ipglasma.hpp"
+using namespace ipglasma;
+int main(int argc, char** argv) {
+    Parameters param; Setup setup;
+    if (argc > 1) setup.read(argv[1], &param);
+    Random random; random.init_genrand64(param.getSeed());
+    int nn[2] = {param.getSize(), param.getSize()};
+    Init init(nn); Group group(param.getNc()); Glauber glauber;
+    Evolution evolution(nn);
+    Lattice lat(&param, param.getNc(), param.getSize());
+    while (param.getSuccess() == 0) {
+        init.init(&lat, &group, &param, &random, &glauber, 0);
+        if (param.getSuccess() == 0) continue;
+        evolution.run(&lat, &group, &param);
+    }
+    return 0;
+}
+''')
+    import ipglasma_amd as I
+    libdir = os.path.dirname(I.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "drv"),
+                    "-L", libdir, "-lipglasma_amd", "-Wl,-rpath," + libdir], check=True)

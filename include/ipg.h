@@ -75,6 +75,23 @@ typedef struct ipg_observables {
     int qs_serial_fallback;
     uint64_t rho_seed;    /* key of the device colour-charge normals        */
     uint64_t newton_seed; /* key of the root-finder restart draws           */
+    /* Evolution::eccentricity(lat, param, itmax, 0.0, 0) on the Landau-frame
+     * eps and u^tau (Evolution.cpp:1280-1810, gfactor = 1): eps_1..eps_6 and
+     * Psi_1..Psi_6 as written to eccentricities<id>.dat; param->setArea and
+     * the Rbar, <eps>, <Q_sA^2 Q_sB^2>, sqrt<r^2> columns of that file */
+    double eccentricity[6];
+    double Psi[6];
+    double area;          /* fm^2 of the sites with eps >= 0               */
+    double Rbar;
+    double avg_eps;       /* GeV/fm^3                                      */
+    double avg_qs2A_qs2B; /* GeV^4 * fm (the reference's units)            */
+    double rms_r;         /* fm                                            */
+    double centroid[2];   /* eps u^tau weighted <x>, <y> [fm]              */
+    /* MyEigen::flowVelocity4D printout: sqrt(sum u_x^2 eps / sum eps),
+     * sqrt(sum u_y^2 eps / sum eps), sqrt(sum (tau u^eta)^2 eps / sum eps) */
+    double u_rms[3];
+    int landau_failures;    /* sites whose 4x4 QR iteration did not converge */
+    int landau_no_timelike; /* sites without a timelike eigenvector          */
 } ipg_observables;
 
 typedef struct ipg_ctx ipg_ctx;
@@ -126,6 +143,18 @@ int ipg_evolve(ipg_ctx *c);
 int ipg_evolve_step(ipg_ctx *c, int it);
 
 int ipg_get_observables(ipg_ctx *c, ipg_observables *o);
+
+/* Landau frame after ipg_evolve (MyEigen::flowVelocity4D, MyEigen.cpp:25-321,
+ * called from Evolution.cpp:409 at itmax): out[15][S] = eps [fm^-4], u^tau,
+ * u^x, u^y, u^eta [fm^-1], pi^{mu nu} in T's order (tautau, xx, yy, etaeta,
+ * taux, tauy, taueta, xy, xeta, yeta) -- the cell fields epsilon, utau, ux,
+ * uy, ueta, pi* of Cell.h */
+int ipg_get_landau(ipg_ctx *c, double *out);
+/* Recompute the Landau frame and the eccentricities at step it >= 1
+ * (tau = it dtau a) from T[10][S] (or from the event's current T^{mu nu} when
+ * T is NULL); updates the observables.  For callers that hold T^{mu nu}
+ * themselves (Evolution::u + Evolution::eccentricity on given cells). */
+int ipg_landau(ipg_ctx *c, const double *T, int it);
 
 /* field ids for ipg_get_field / ipg_set_field */
 #define IPG_F_UX 0

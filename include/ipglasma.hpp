@@ -133,6 +133,13 @@ class Lattice {
         check(ipg_get_tmunu(ctx_, v.data()));
         return v;
     }
+    // Cell epsilon, utau, ux, uy, ueta, pi^{mu nu} after Evolution::run
+    // (MyEigen.cpp:25-321), [15][S]
+    std::vector<double> landau(size_t S) {
+        std::vector<double> v(15 * S);
+        check(ipg_get_landau(ctx_, v.data()));
+        return v;
+    }
     bool seeded = false;
 
    private:

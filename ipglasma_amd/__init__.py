@@ -76,10 +76,18 @@ class Observables(C.Structure):
                 [(n, C.c_int) for n in ("newton_max_iterations", "newton_restarts",
                                         "newton_unconverged", "newton_analytic_jacobian",
                                         "qs_serial_fallback")] +
-                [("rho_seed", C.c_uint64), ("newton_seed", C.c_uint64)])
+                [("rho_seed", C.c_uint64), ("newton_seed", C.c_uint64)] +
+                [("eccentricity", C.c_double * 6), ("Psi", C.c_double * 6)] +
+                [(n, C.c_double) for n in ("area", "Rbar", "avg_eps", "avg_qs2A_qs2B", "rms_r")] +
+                [("centroid", C.c_double * 2), ("u_rms", C.c_double * 3)] +
+                [("landau_failures", C.c_int), ("landau_no_timelike", C.c_int)])
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if isinstance(v, C.Array) else v
+        return d
 
 
 _lib = None
@@ -115,6 +123,8 @@ def lib():
             "ipg_get_field": (C.c_int, [vp, C.c_int, dp]),
             "ipg_set_field": (C.c_int, [vp, C.c_int, dp]),
             "ipg_get_tmunu": (C.c_int, [vp, dp]),
+            "ipg_get_landau": (C.c_int, [vp, dp]),
+            "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
             "ipg_get_newton_iterations": (C.c_int, [vp, ip]),
@@ -236,6 +246,17 @@ class Event:
         out = np.zeros((10, self.S))
         _check(lib().ipg_get_tmunu(self._c, _dptr(out)))
         return out
+
+    def landau(self):
+        """[15, S]: eps, u^tau, u^x, u^y, u^eta, pi^{mu nu} (T's order)"""
+        out = np.zeros((15, self.S))
+        _check(lib().ipg_get_landau(self._c, _dptr(out)))
+        return out
+
+    def landau_from(self, T, it):
+        """Landau frame and eccentricities of a given T[10, S] at step it"""
+        a = None if T is None else np.ascontiguousarray(T, dtype=np.float64)
+        _check(lib().ipg_landau(self._c, None if a is None else _dptr(a), it))
 
     def nucleons(self):
         a1, a2 = C.c_int(), C.c_int()

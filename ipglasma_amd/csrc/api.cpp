@@ -57,6 +57,8 @@ int pad8(long long n) { return (int)(((n + 7) / 8) * 8); }
 // colour-charge slices per setV chunk: the Wilson lines are read and written
 // once per chunk; the FFT buffer holds 256 bytes per site and slice
 constexpr int kSetvChunk = 8;
+// doubles of end-of-evolution observables per event (kernels_landau.hip)
+constexpr int kObs = 32;
 int log2i(int n) {
     int l = 0;
     while ((1 << l) < n) l++;
@@ -101,6 +103,7 @@ struct Engine {
     bool have_table = false;
     int* d_flag = nullptr;
     double* d_out = nullptr;  // 8 doubles per event
+    double* d_obs = nullptr;  // kObs doubles per event (Landau, eccentricities)
     int nblk = 0, nlinkblk = 0;
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -127,6 +130,7 @@ struct Engine {
         if (d_Q) hipFree(d_Q);
         if (d_flag) hipFree(d_flag);
         if (d_out) hipFree(d_out);
+        if (d_obs) hipFree(d_obs);
         for (hipEvent_t t : tev) hipEventDestroy(t);
         if (st) hipStreamDestroy(st);
     }
@@ -194,8 +198,10 @@ struct Engine {
             d.kc = std::max(1, std::min(p.Ny, kSetvChunk));
             const size_t nfft = std::max<size_t>(72, 16 * (size_t)d.kc);
             // U (4 x 18 planes), phi pair, E1, E2, pi (5 x 9 Hermitian planes),
-            // 5 scalar planes, FFT/link region, T (10), partials, int planes
-            const size_t nd = (size_t)S * (72 + 45 + 5 + nfft + 10) + (size_t)8 * d.nblocks + (size_t)S * 2;
+            // 5 scalar planes, FFT/link region, T (10), Landau frame (15 + 1),
+            // partials, int planes (2 + 1)
+            const size_t nd = (size_t)S * (72 + 45 + 5 + nfft + 10 + 16) + (size_t)32 * d.nblocks +
+                              (size_t)S * 2 + ((size_t)S + 1) / 2;
             HIPCHK(hipMalloc(&E.mem, nd * sizeof(double)));
             double* q = E.mem;
             auto take = [&](size_t n) {
@@ -226,10 +232,13 @@ struct Engine {
             d.L2x = d.L1y + F;
             d.L2y = d.L2x + F;
             d.T = take(10 * (size_t)S);
-            d.partial = take(8 * (size_t)d.nblocks);
+            d.land = take(15 * (size_t)S);
+            d.land_epsv = take(S);
+            d.partial = take(32 * (size_t)d.nblocks);
             int* ip = (int*)take(2 * (size_t)S);
             d.check2 = ip;
             d.prev2 = ip + S;
+            d.land_flag = (int*)take(((size_t)S + 1) / 2);
             d.newton_stat = (int*)d.check2;  // reused after the colour charge is done
             if (!plans.count(N)) {
                 Plan pl;
@@ -259,6 +268,9 @@ struct Engine {
         HIPCHK(hipMalloc(&d_desc, sizeof(EvDesc) * nev));
         HIPCHK(hipMalloc(&d_flag, sizeof(int) * nev));
         HIPCHK(hipMalloc(&d_out, sizeof(double) * 8 * nev));
+        HIPCHK(hipMalloc(&d_obs, sizeof(double) * kObs * nev));
+        HIPCHK(hipMemset(d_obs, 0, sizeof(double) * kObs * nev));
+        for (int e = 0; e < nev; e++) desc[e].obs = d_obs + (size_t)kObs * e;
         return upload_desc();
     }
 
@@ -497,6 +509,7 @@ struct Engine {
         if (fin) {
             ipg_launch_gauss(d_desc, nev, nblk, it, st);
             ipg_launch_finish(d_desc, nev, it, d_out, st);
+            ipg_launch_landau(d_desc, nev, nblk, it, 0, st);
         }
     }
 
@@ -537,6 +550,7 @@ struct Engine {
                 if (fin) {
                     ipg_launch_gauss(d_desc, nev, nblk, it, st);
                     ipg_launch_finish(d_desc, nev, it, d_out, st);
+                    ipg_launch_landau(d_desc, nev, nblk, it, 0, st);
                 }
             }
         }
@@ -577,7 +591,43 @@ struct Engine {
             ev[e].parity = (d.itmax - 1) & 1;
             ev[e].last_it = d.itmax;
         }
+        return read_obs();
+    }
+
+    // end-of-evolution observables of the active events (k_ecc_finish layout)
+    int read_obs() {
+        std::vector<double> h((size_t)kObs * ev.size());
+        HIPCHK(hipMemcpy(h.data(), d_obs, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < ev.size(); e++) {
+            if (!desc[e].active) continue;
+            const double* o = &h[(size_t)kObs * e];
+            ipg_observables& q = ev[e].obs;
+            for (int n = 0; n < 6; n++) {
+                q.eccentricity[n] = o[n];
+                q.Psi[n] = o[6 + n];
+            }
+            q.area = o[12];
+            q.Rbar = o[13];
+            q.avg_eps = o[14];
+            q.avg_qs2A_qs2B = o[15];
+            q.rms_r = o[16];
+            q.centroid[0] = o[17];
+            q.centroid[1] = o[18];
+            for (int k = 0; k < 3; k++) q.u_rms[k] = std::sqrt(o[19 + k] / o[22]);
+            q.landau_failures = (int)o[23];
+            q.landau_no_timelike = (int)o[24];
+        }
         return IPG_OK;
+    }
+
+    // Landau frame + eccentricities of event 0 from its current T at step it
+    int landau(int it) {
+        int rc = upload_desc();
+        if (rc) return rc;
+        ipg_launch_landau(d_desc, (int)ev.size(), nblk, it, 1, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        return read_obs();
     }
 
     int single_step(int e, int it) {
@@ -600,6 +650,7 @@ struct Engine {
             ev[e].obs.tau_fm = tau_fm;
             ev[e].obs.dEdeta = tau_fm * d.a * d.a * ipg::HBARC * hout[4 * e];
             ev[e].obs.gauss_max = hout[4 * e + 1];
+            return read_obs();
         }
         return IPG_OK;
     }
@@ -812,6 +863,19 @@ int ipg_get_tmunu(ipg_ctx* c, double* out) {
     if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
     HIPCHK(hipMemcpy(out, c->eng.desc[0].T, sizeof(double) * 10 * c->eng.desc[0].S, hipMemcpyDeviceToHost));
     return IPG_OK;
+}
+int ipg_get_landau(ipg_ctx* c, double* out) {
+    if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpy(out, c->eng.desc[0].land, sizeof(double) * 15 * c->eng.desc[0].S, hipMemcpyDeviceToHost));
+    return IPG_OK;
+}
+int ipg_landau(ipg_ctx* c, const double* T, int it) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    if (it < 1) return fail(IPG_ERR_ARG, "the Landau frame needs tau > 0 (it >= 1)");
+    Engine& g = c->eng;
+    if (T) HIPCHK(hipMemcpy(g.desc[0].T, T, sizeof(double) * 10 * g.desc[0].S, hipMemcpyHostToDevice));
+    g.desc[0].active = 1;
+    return g.landau(it);
 }
 int ipg_get_nucleons(ipg_ctx* c, double* out, int* A1, int* A2) {
     if (!c) return fail(IPG_ERR_ARG, "null context");

@@ -64,8 +64,13 @@ struct EvDesc {
     uint64_t newton_seed;
     // outputs
     double* T;        // 10 planes of S doubles
-    double* partial;  // per-block reduction partials
+    double* partial;  // per-block reduction partials (32 slots of nblocks)
     int* newton_stat;  // per-link iteration counts (2S) and flags
+    // Landau frame: eps, u^tau, u^x, u^y, u^eta, pi^{mu nu} (T's order)
+    double* land;       // 15 planes of S doubles
+    double* land_epsv;  // running eps of the reference after each site
+    int* land_flag;     // bit0 eps set, bit1 u^mu set, bit2 QR failed
+    double* obs;        // 32 doubles of end-of-evolution observables
 };
 
 // block -> (event, local block) for site-parallel launches

@@ -10,6 +10,9 @@ void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t
 void ipg_launch_tmunu(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
 void ipg_launch_gauss(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
 void ipg_launch_finish(const EvDesc* d_evs, int nev, int it, double* d_out, hipStream_t st);
+// Landau frame + eccentricities of the events that end at it (all events if
+// all != 0)
+void ipg_launch_landau(const EvDesc* d_evs, int nev, int nblk, int it, int all, hipStream_t st);
 
 // colour charge
 void ipg_launch_tp(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);

@@ -219,6 +219,10 @@ __device__ __forceinline__ M3 tlquad(const M3& P1, const M3& P2, const M3& P3, c
     return traceless(P1 - P2) + traceless(P3 - P4);
 }
 
+// Each block below reloads its inputs (L1/L2 hits) after a compiler memory
+// barrier, so that only the matrices of one term are live at a time.
+#define IPG_FENCE() asm volatile("" ::: "memory")
+
 __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ evs, int nev, int it) {
     const int e = find_event(evs, nev, blockIdx.x);
     const EvDesc& ev = evs[e];
@@ -233,6 +237,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
         const double* Ux = ev.U[p][0];
         const double* Uy = ev.U[p][1];
         const double* phi = ev.phi[p];
+        const double* E1 = ev.E1;
+        const double* E2 = ev.E2;
+        const double* PI = ev.pi;
         const double tau = it * ev.dtau, g = ev.g, a = ev.a;
         const int i = (int)(s / N), j = (int)(s - (s / N) * N);
         auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
@@ -240,117 +247,204 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
         const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
         const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
         double* T = ev.T;
-        // electric and longitudinal
-        const M3 e1 = ldh(ev.E1, S, s), e2 = ldh(ev.E2, S, s), e1p = ldh(ev.E1, S, pY), e2p = ldh(ev.E2, S, pX);
-        const M3 p0 = ldh(ev.pi, S, s), pXm = ldh(ev.pi, S, pX), pYm = ldh(ev.pi, S, pY), pXYm = ldh(ev.pi, S, pXY);
-        const double t11 = retrprod(e1, e1), t1p = retrprod(e1p, e1p), t22 = retrprod(e2, e2), t2p = retrprod(e2p, e2p);
-        const double lng = (retrprod(p0, p0) + retrprod(pXm, pXm) + retrprod(pYm, pYm) + retrprod(pXYm, pXYm)) / 4.;
-        const double gt = g * g / tau / tau;
-        double tt = gt * (t11 + t1p + t22 + t2p) / 2. + lng;
-        double xx = gt * (-t11 - t1p + t22 + t2p) / 2. + lng;
-        double yy = gt * (t11 + t1p - t22 - t2p) / 2. + lng;
-        double ee = 1. / tau / tau * (gt * (t11 + t1p + t22 + t2p) / 2. - lng);
-        // magnetic and transverse phi gradients
-        const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
-        const M3 UxpY = ldm(Ux, S, pY), UypX = ldm(Uy, S, pX);
-        double rplaq = 3.0;
-        if (i != N - 1) rplaq = retrprod(ux, UypX * (mul_bd(dag(UxpY), uy)));
-        const M3 ph = ldh(phi, S, s), phX = ldh(phi, S, pX), phY = ldh(phi, S, pY), phXY = ldh(phi, S, pXY);
-        const M3 Udx = dag(ux), Udy = dag(uy), UDxpY = dag(UxpY), UDypX = dag(UypX);
-        const M3 gX = m3(ux, phX, Udx) - ph;   // Ux phiX Ux^+ - phi
-        const M3 gY = m3(uy, phY, Udy) - ph;
-        const M3 gXY1 = m3(UxpY, phXY, UDxpY) - phY;
-        const M3 gXY2 = m3(UypX, phXY, UDypX) - phX;
-        const double A = retrprod(gX, gX), B = retrprod(gXY1, gXY1), C = retrprod(gY, gY), D = retrprod(gXY2, gXY2);
-        const double mag = 2. / (g * g) * (3.0 - rplaq);
-        const double h = 0.5 / tau / tau;
-        tt = (tt + mag) + h * (A + B + C + D);
-        xx = (xx + mag) + h * (A + B - C - D);
-        yy = (yy + mag) + h * (-A - B + C + D);
-        ee = ee + 1. / tau / tau * (-mag + h * (A + B + C + D));
         const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+        // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
+        // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
+        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
+        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
+        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
+        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
+
+        // electric and longitudinal (Evolution.cpp:754-814)
+        double tt, xx, yy, ee;
+        {
+            const M3 e1 = ldh(E1, S, s), e1p = ldh(E1, S, pY), e2 = ldh(E2, S, s), e2p = ldh(E2, S, pX);
+            const double t11 = retrprod(e1, e1), t1p = retrprod(e1p, e1p), t22 = retrprod(e2, e2),
+                         t2p = retrprod(e2p, e2p);
+            double lng = 0.;
+            {
+                M3 q = ldh(PI, S, s);
+                lng += retrprod(q, q);
+                q = ldh(PI, S, pX);
+                lng += retrprod(q, q);
+                q = ldh(PI, S, pY);
+                lng += retrprod(q, q);
+                q = ldh(PI, S, pXY);
+                lng += retrprod(q, q);
+                lng /= 4.;
+            }
+            const double gt = g * g / tau / tau;
+            tt = gt * (t11 + t1p + t22 + t2p) / 2. + lng;
+            xx = gt * (-t11 - t1p + t22 + t2p) / 2. + lng;
+            yy = gt * (t11 + t1p - t22 - t2p) / 2. + lng;
+            ee = 1. / tau / tau * (gt * (t11 + t1p + t22 + t2p) / 2. - lng);
+        }
+        IPG_FENCE();
+        // magnetic and transverse phi gradients (Evolution.cpp:816-905)
+        {
+            double rplaq = 3.0;
+            if (i != N - 1) {
+                const M3 ux = ldm(Ux, S, s);
+                rplaq = retrprod(ux, ldm(Uy, S, pX) * mul_bd(dag(ldm(Ux, S, pY)), ldm(Uy, S, s)));
+            }
+            double A, B, C, D;
+            {
+                M3 q = gX();
+                A = retrprod(q, q);
+            }
+            {
+                M3 q = gXY1();
+                B = retrprod(q, q);
+            }
+            {
+                M3 q = gY();
+                C = retrprod(q, q);
+            }
+            {
+                M3 q = gXY2();
+                D = retrprod(q, q);
+            }
+            const double mag = 2. / (g * g) * (3.0 - rplaq);
+            const double h = 0.5 / tau / tau;
+            tt = (tt + mag) + h * (A + B + C + D);
+            xx = (xx + mag) + h * (A + B - C - D);
+            yy = (yy + mag) + h * (-A - B + C + D);
+            ee = ee + 1. / tau / tau * (-mag + h * (A + B + C + D));
+        }
         T[0 * S + s] = tt / a4;
         T[1 * S + s] = xx / a4;
         T[2 * S + s] = yy / a4;
         T[3 * S + s] = ee / a6;
         ttsum = tt / a4;
-
-        // flux components
-        const M3 UxmX = ldm(Ux, S, mX), UDxmX = dag(UxmX);
-        const M3 UxmXpY = ldm(Ux, S, mXpY), UDxmXpY = dag(UxmXpY);
-        const M3 UxpX = ldm(Ux, S, pX), UDxpX = dag(UxpX);
-        const M3 UxpXpY = ldm(Ux, S, pXY), UDxpXpY = dag(UxpXpY);
-        const M3 UymY = ldm(Uy, S, mY), UDymY = dag(UymY);
-        const M3 UypXmY = ldm(Uy, S, pXmY), UDypXmY = dag(UypXmY);
-        const M3 UypY = ldm(Uy, S, pY), UDypY = dag(UypY);
-        const M3 Uyp2X = ldm(Uy, S, p2X), UDyp2X = dag(Uyp2X);
-        const M3 Uxp2Y = ldm(Ux, S, p2Y), UDxp2Y = dag(Uxp2Y);
-        const M3 UypXpY = ldm(Uy, S, pXY), UDypXpY = dag(UypXpY);
-        const M3 UymX = ldm(Uy, S, mX), UDymX = dag(UymX);
-        const M3 UxmY = ldm(Ux, S, mY), UDxmY = dag(UxmY);
-
-        // plaquette-type clovers shared by several components
-        const M3 Pa = m4(uy, UxpY, UDypX, Udx);       // U_y U_x(+y) U_y(+x)^+ U_x^+
-        const M3 Pb = m4(ux, UypX, UDxpY, Udy);       // U_x U_y(+x) U_x(+y)^+ U_y^+
-        const M3 Pc = m4(UDxmX, UymX, UxmXpY, Udy);
-        const M3 Pd = m4(uy, UDxmXpY, UDymX, UxmX);
-        const M3 Pe = m4(UypX, UxpXpY, UDyp2X, UDxpX);
-        const M3 Pf = m4(UxpX, Uyp2X, UDxpXpY, UDypX);
-        const M3 Pg = m4(Udx, uy, UxpY, UDypX);
-        const M3 Ph = m4(UypX, UDxpY, Udy, ux);
-        const M3 Qc = m4(UDymY, UxmY, UypXmY, Udx);
-        const M3 Qd = m4(ux, UDypXmY, UDxmY, UymY);
-        const M3 Qe = m4(UxpY, UypXpY, UDxp2Y, UDypY);
-        const M3 Qf = m4(UypY, Uxp2Y, UDypXpY, UDxpY);
-        const M3 Qg = m4(Udy, ux, UypX, UDxpY);
-        const M3 Qh = m4(UxpY, UDypX, Udx, uy);
-
-        // T^{tau x}
-        {
-            const M3 phmX = ldh(phi, S, mX), phmXpY = ldh(phi, S, mXpY), ph2X = ldh(phi, S, p2X);
-            const M3 ph2XY = ldh(phi, S, p2XY);
-            double im = imtrprod(e2, tlquad(Pa, Pb, Pc, Pd)) + imtrprod(e2p, tlquad(Pe, Pf, Pg, Ph));
-            double re = retrprod(p0, m3(ux, phX, Udx) - m3(UDxmX, phmX, UxmX)) +
-                        retrprod(pYm, m3(UxpY, phXY, UDxpY) - m3(UDxmXpY, phmXpY, UxmXpY)) +
-                        retrprod(pXm, m3(UxpX, ph2X, UDxpX) - m3(Udx, ph, ux)) +
-                        retrprod(pXYm, m3(UxpXpY, ph2XY, UDxpXpY) - m3(UDxpY, phY, UxpY));
-            T[4 * S + s] = (2. / tau / 8. * im - 2. / 8. / tau * re) / a4;
-        }
-        // T^{tau y}
-        {
-            const M3 phmY = ldh(phi, S, mY), phpXmY = ldh(phi, S, pXmY), ph2Y = ldh(phi, S, p2Y);
-            const M3 phX2Y = ldh(phi, S, pX2Y);
-            double im = imtrprod(e1, tlquad(Pb, Pa, Qc, Qd)) + imtrprod(e1p, tlquad(Qe, Qf, Qg, Qh));
-            double re = retrprod(p0, m3(uy, phY, Udy) - m3(UDymY, phmY, UymY)) +
-                        retrprod(pXm, m3(UypX, phXY, UDypX) - m3(UDypXmY, phpXmY, UypXmY)) +
-                        retrprod(pYm, m3(UypY, ph2Y, UDypY) - m3(Udy, ph, uy)) +
-                        retrprod(pXYm, m3(UypXpY, phX2Y, UDypXpY) - m3(UDypX, phX, UypX));
-            T[5 * S + s] = (2. / tau / 8. * im - 2. / 8. / tau * re) / a4;
-        }
+        IPG_FENCE();
         // T^{tau eta}
         {
-            double re = retrprod(e1, gX) + retrprod(e1p, gXY1) + retrprod(e2, gY) + retrprod(e2p, gXY2);
+            double re = retrprod(ldh(E1, S, s), gX());
+            re += retrprod(ldh(E1, S, pY), gXY1());
+            re += retrprod(ldh(E2, S, s), gY());
+            re += retrprod(ldh(E2, S, pX), gXY2());
             T[6 * S + s] = (g / tau / tau / tau * re) / a5;
         }
+        IPG_FENCE();
         // T^{xy}
         {
-            M3 el = (-1. / 4. * g * g * (e1 + m3(uy, e1p, Udy))) * (e2 + m3(ux, e2p, Udx));
-            M3 yl = (uy * gXY1) * Udy;        // U_y gXY1 U_y^+
-            M3 xl = (ux * gXY2) * Udx;        // U_x gXY2 U_x^+
-            M3 b = gX * gY + yl * gY + gX * xl + yl * xl;
-            T[7 * S + s] = (2. / tau / tau * (retrprod(el, m_one()) + 0.25 * retrprod(b, m_one()))) / a4;
+            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
+            double b;
+            {
+                const M3 el = (-1. / 4. * g * g * (ldh(E1, S, s) + m3(uy, ldh(E1, S, pY), dag(uy)))) *
+                              (ldh(E2, S, s) + m3(ux, ldh(E2, S, pX), dag(ux)));
+                b = 4. * tr(el).r;
+            }
+            const M3 yl = m3(uy, gXY1(), dag(uy));  // U_y gXY1 U_y^+
+            const M3 xl = m3(ux, gXY2(), dag(ux));  // U_x gXY2 U_x^+
+            const M3 gx = gX(), gy = gY();
+            b += retrprod(gx, gy) + retrprod(yl, gy) + retrprod(gx, xl) + retrprod(yl, xl);
+            T[7 * S + s] = (2. / tau / tau * (0.25 * b)) / a4;
         }
-        // T^{x eta}
+        IPG_FENCE();
+        // T^{tau x}, T^{x eta}, T^{tau y}, T^{y eta} (Evolution.cpp:1047-1234).
+        // The clover combinations enter with opposite signs, e.g. the T^{x eta}
+        // clover (Pb - Pa)_TL + (Pd - Pc)_TL is -X1.
+        double ttx_im, ttx_re, tty_im, tty_re, xe_im, ye_im;
         {
-            double r1 = 1. / 4. * g * (retrprod(e1, p0 + m3(ux, pXm, Udx)) + retrprod(e1p, pYm + m3(UxpY, pXYm, UDxpY)));
-            double r2 = 1. / 8. / g * (imtrprod(tlquad(Pb, Pa, Pd, Pc), gY) + imtrprod(tlquad(Ph, Pg, Pf, Pe), gXY2));
-            T[8 * S + s] = (-2. / tau / tau * (r1 + r2)) / a5;
+            // X1 = (Pa - Pb)_TL + (Pc - Pd)_TL,
+            // Pa = Uy Ux(+y) Uy(+x)^+ Ux^+, Pb = Ux Uy(+x) Ux(+y)^+ Uy^+
+            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
+            M3 X = traceless(m4(uy, ldm(Ux, S, pY), dag(ldm(Uy, S, pX)), dag(ux)) -
+                             m4(ux, ldm(Uy, S, pX), dag(ldm(Ux, S, pY)), dag(uy)));
+            {
+                const M3 UxmX = ldm(Ux, S, mX), UxmXpY = ldm(Ux, S, mXpY), UymX = ldm(Uy, S, mX);
+                X = X + traceless(m4(dag(UxmX), UymX, UxmXpY, dag(uy)) - m4(uy, dag(UxmXpY), dag(UymX), UxmX));
+            }
+            ttx_im = imtrprod(ldh(E2, S, s), X);
+            xe_im = -imtrprod(X, gY());
         }
-        // T^{y eta}
+        IPG_FENCE();
         {
-            double r1 = 1. / 4. * g * (retrprod(e2, p0 + m3(uy, pYm, Udy)) + retrprod(e2p, pXm + m3(UypX, pXYm, UDypX)));
-            double r2 = 1. / 8. / g * (imtrprod(tlquad(Pa, Pb, Qd, Qc), gX) + imtrprod(tlquad(Qh, Qg, Qf, Qe), gXY1));
-            T[9 * S + s] = (-2. / tau / tau * (r1 + r2)) / a5;
+            // Y1 = (Pb - Pa)_TL + (Qc - Qd)_TL
+            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
+            M3 Y = traceless(m4(ux, ldm(Uy, S, pX), dag(ldm(Ux, S, pY)), dag(uy)) -
+                             m4(uy, ldm(Ux, S, pY), dag(ldm(Uy, S, pX)), dag(ux)));
+            {
+                const M3 UymY = ldm(Uy, S, mY), UxmY = ldm(Ux, S, mY), UypXmY = ldm(Uy, S, pXmY);
+                Y = Y + traceless(m4(dag(UymY), UxmY, UypXmY, dag(ux)) - m4(ux, dag(UypXmY), dag(UxmY), UymY));
+            }
+            tty_im = imtrprod(ldh(E1, S, s), Y);
+            ye_im = -imtrprod(Y, gX());
+        }
+        IPG_FENCE();
+        {
+            // X2 = (Pe - Pf)_TL + (Pg - Ph)_TL
+            const M3 UxpX = ldm(Ux, S, pX), UxpXpY = ldm(Ux, S, pXY), Uyp2X = ldm(Uy, S, p2X);
+            const M3 UypX = ldm(Uy, S, pX);
+            M3 X = traceless(m4(UypX, UxpXpY, dag(Uyp2X), dag(UxpX)) - m4(UxpX, Uyp2X, dag(UxpXpY), dag(UypX)));
+            {
+                const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s), UxpY = ldm(Ux, S, pY);
+                X = X + traceless(m4(dag(ux), uy, UxpY, dag(UypX)) - m4(UypX, dag(UxpY), dag(uy), ux));
+            }
+            ttx_im += imtrprod(ldh(E2, S, pX), X);
+            xe_im -= imtrprod(X, gXY2());
+        }
+        IPG_FENCE();
+        {
+            // Y2 = (Qe - Qf)_TL + (Qg - Qh)_TL
+            const M3 UxpY = ldm(Ux, S, pY), UypXpY = ldm(Uy, S, pXY), Uxp2Y = ldm(Ux, S, p2Y);
+            const M3 UypY = ldm(Uy, S, pY);
+            M3 Y = traceless(m4(UxpY, UypXpY, dag(Uxp2Y), dag(UypY)) - m4(UypY, Uxp2Y, dag(UypXpY), dag(UxpY)));
+            {
+                const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s), UypX = ldm(Uy, S, pX);
+                Y = Y + traceless(m4(dag(uy), ux, UypX, dag(UxpY)) - m4(UxpY, dag(UypX), dag(ux), uy));
+            }
+            tty_im += imtrprod(ldh(E1, S, pY), Y);
+            ye_im -= imtrprod(Y, gXY1());
+        }
+        IPG_FENCE();
+        // pi phi-gradient parts of T^{tau x} and T^{tau y}
+        {
+            M3 u = ldm(Ux, S, s), v = ldm(Ux, S, mX);
+            ttx_re = retrprod(ldh(PI, S, s), m3(u, ldh(phi, S, pX), dag(u)) - m3(dag(v), ldh(phi, S, mX), v));
+            u = ldm(Ux, S, pY);
+            v = ldm(Ux, S, mXpY);
+            ttx_re += retrprod(ldh(PI, S, pY), m3(u, ldh(phi, S, pXY), dag(u)) - m3(dag(v), ldh(phi, S, mXpY), v));
+            u = ldm(Ux, S, pX);
+            v = ldm(Ux, S, s);
+            ttx_re += retrprod(ldh(PI, S, pX), m3(u, ldh(phi, S, p2X), dag(u)) - m3(dag(v), ldh(phi, S, s), v));
+            u = ldm(Ux, S, pXY);
+            v = ldm(Ux, S, pY);
+            ttx_re += retrprod(ldh(PI, S, pXY), m3(u, ldh(phi, S, p2XY), dag(u)) - m3(dag(v), ldh(phi, S, pY), v));
+        }
+        IPG_FENCE();
+        {
+            M3 u = ldm(Uy, S, s), v = ldm(Uy, S, mY);
+            tty_re = retrprod(ldh(PI, S, s), m3(u, ldh(phi, S, pY), dag(u)) - m3(dag(v), ldh(phi, S, mY), v));
+            u = ldm(Uy, S, pX);
+            v = ldm(Uy, S, pXmY);
+            tty_re += retrprod(ldh(PI, S, pX), m3(u, ldh(phi, S, pXY), dag(u)) - m3(dag(v), ldh(phi, S, pXmY), v));
+            u = ldm(Uy, S, pY);
+            v = ldm(Uy, S, s);
+            tty_re += retrprod(ldh(PI, S, pY), m3(u, ldh(phi, S, p2Y), dag(u)) - m3(dag(v), ldh(phi, S, s), v));
+            u = ldm(Uy, S, pXY);
+            v = ldm(Uy, S, pX);
+            tty_re += retrprod(ldh(PI, S, pXY), m3(u, ldh(phi, S, pX2Y), dag(u)) - m3(dag(v), ldh(phi, S, pX), v));
+        }
+        T[4 * S + s] = (2. / tau / 8. * ttx_im - 2. / 8. / tau * ttx_re) / a4;
+        T[5 * S + s] = (2. / tau / 8. * tty_im - 2. / 8. / tau * tty_re) / a4;
+        IPG_FENCE();
+        // E pi parts of T^{x eta}, T^{y eta}
+        {
+            M3 u = ldm(Ux, S, s);
+            double r1 = retrprod(ldh(E1, S, s), ldh(PI, S, s) + m3(u, ldh(PI, S, pX), dag(u)));
+            u = ldm(Ux, S, pY);
+            r1 += retrprod(ldh(E1, S, pY), ldh(PI, S, pY) + m3(u, ldh(PI, S, pXY), dag(u)));
+            T[8 * S + s] = (-2. / tau / tau * (1. / 4. * g * r1 + 1. / 8. / g * xe_im)) / a5;
+        }
+        IPG_FENCE();
+        {
+            M3 u = ldm(Uy, S, s);
+            double r1 = retrprod(ldh(E2, S, s), ldh(PI, S, s) + m3(u, ldh(PI, S, pY), dag(u)));
+            u = ldm(Uy, S, pX);
+            r1 += retrprod(ldh(E2, S, pX), ldh(PI, S, pX) + m3(u, ldh(PI, S, pXY), dag(u)));
+            T[9 * S + s] = (-2. / tau / tau * (1. / 4. * g * r1 + 1. / 8. / g * ye_im)) / a5;
         }
     }
     double bs = block_sum(ttsum);

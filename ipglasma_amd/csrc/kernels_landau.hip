@@ -1,0 +1,267 @@
+// kernels_landau.hip -- Landau frame and spatial eccentricities at the end of
+// the evolution.
+//
+//   k_landau        MyEigen::flowVelocity4D (MyEigen.cpp:25-321): per site the
+//                   4x4 eigenproblem of T^mu_nu, eps, u^mu and pi^{mu nu}; block
+//                   partials of the u^mu averages (MyEigen.cpp:199-206) and of
+//                   the first eccentricity pass (Evolution.cpp:1346-1460)
+//   k_landau_carry  pi^{mu nu} at sites whose eps or u^mu the reference takes
+//                   from an earlier site (its running variables), by a scan
+//                   over the site order
+//   k_ecc_mid       centroid and the per-event sums of the first pass
+//   k_ecc           moments about the centroid (Evolution.cpp:1478-1660) and
+//                   the second centroid pass (1670-1790)
+//   k_ecc_finish    eps_n, Psi_n, Rbar, sqrt<r^2>
+//
+// Built without floating-point contraction (see Makefile): the per-site Landau
+// results are the bit pattern of the same code on the host.
+#include "ipg_device.hpp"
+#include "ipg_internal.hpp"
+#include "landau.hpp"
+
+namespace {
+
+constexpr double kHbarc = 0.1973269718;
+// Evolution.cpp:646 calls eccentricity(lat, param, it, 0.0, 0)
+constexpr double kCutoff = 0.;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+// deterministic block sum (fixed tree), valid in thread 0
+template <int NT>
+__device__ double bsum(double v) {
+    __shared__ double w[NT / 64];
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NT / 64; k++) r += w[k];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void load_T(const EvDesc& ev, long long s, double (&T)[10]) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) T[k] = ev.T[k * ev.S + s];
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_landau(const EvDesc* __restrict__ evs, int nev, int it, int all) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active || (!all && it != ev.itmax)) return;
+    const int lb = xcd_remap(blockIdx.x - ev.block0, ev.nblocks);
+    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
+    const long long S = ev.S;
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = 0.;
+    if (s < S) {
+        const int N = ev.N;
+        const int si = (int)(s / N), sj = (int)(s - (long long)si * N);
+        double T[10];
+        load_T(ev, s, T);
+        ipgl::LandauSite o;
+        ipgl::landau_site(T, N, ev.a, ev.dtau, it, si, sj, o);
+        ev.land[0 * S + s] = o.eps;
+#pragma unroll
+        for (int k = 0; k < 4; k++) ev.land[(1 + k) * S + s] = o.u[k];
+        ev.land_epsv[s] = o.eps_var;
+        ev.land_flag[s] = o.flags;
+        if ((o.flags & 3) == 3) {
+            double pi[10];
+            ipgl::landau_pi(T, o.eps_var, o.u[0], o.u[1], o.u[2], o.u[3], it, ev.dtau, ev.a, pi);
+#pragma unroll
+            for (int k = 0; k < 10; k++) ev.land[(5 + k) * S + s] = pi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] = o.acc[k];
+        acc[4] = (o.flags & 4) ? 1. : 0.;
+        acc[5] = (o.flags & 2) ? 0. : 1.;
+        // first eccentricity pass: weight eps u^tau unless eps < cutoff
+        const double L = ev.L, a = ev.a;
+        const double x = -L / 2. + a * si, y = -L / 2. + a * sj;
+        double w = 0.;
+        if (!(o.eps < kCutoff)) {
+            w = o.eps * o.u[0];
+            acc[8] = 1.;
+            acc[9] = o.eps * kHbarc;
+            acc[10] = ev.g2A[s] * ev.QsmuRatio * ev.QsmuRatio * ev.g2B[s] * ev.QsmuRatioB * ev.QsmuRatioB / a / a /
+                      a / a;
+        }
+        acc[6] = x * w;
+        acc[7] = y * w;
+        acc[11] = w;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const double r = bsum<IPG_BLOCK>(acc[k]);
+        if (threadIdx.x == 0) ev.partial[(size_t)k * ev.nblocks + lb] = r;
+    }
+}
+
+// pi^{mu nu} at sites without their own (eps, u^mu): the reference forms it
+// from its running variables, last set at an earlier site in row-major order
+__global__ __launch_bounds__(1024) void k_landau_carry(const EvDesc* __restrict__ evs, int nev, int it, int all) {
+    const EvDesc& ev = evs[blockIdx.x];
+    if (!ev.active || (!all && it != ev.itmax)) return;
+    __shared__ int ce[1024], cu[1024];
+    const long long S = ev.S;
+    const long long chunk = (S + 1023) / 1024;
+    const long long b = threadIdx.x * chunk, en = min(S, b + chunk);
+    int le = -1, lu = -1;
+    for (long long s = b; s < en; s++) {
+        const int f = ev.land_flag[s];
+        if (f & 1) le = (int)s;
+        if (f & 2) lu = (int)s;
+    }
+    ce[threadIdx.x] = le;
+    cu[threadIdx.x] = lu;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int ve = threadIdx.x >= o ? ce[threadIdx.x - o] : -1;
+        const int vu = threadIdx.x >= o ? cu[threadIdx.x - o] : -1;
+        __syncthreads();
+        ce[threadIdx.x] = max(ce[threadIdx.x], ve);
+        cu[threadIdx.x] = max(cu[threadIdx.x], vu);
+        __syncthreads();
+    }
+    int re = threadIdx.x > 0 ? ce[threadIdx.x - 1] : -1;
+    int ru = threadIdx.x > 0 ? cu[threadIdx.x - 1] : -1;
+    for (long long s = b; s < en; s++) {
+        const int f = ev.land_flag[s];
+        if (f & 1) re = (int)s;
+        if (f & 2) ru = (int)s;
+        if ((f & 3) == 3) continue;
+        const double eps = re >= 0 ? ev.land_epsv[re] : 0.;
+        double u[4] = {0., 0., 0., 0.};
+        if (ru >= 0)
+            for (int k = 0; k < 4; k++) u[k] = ev.land[(1 + k) * S + ru];
+        double T[10], pi[10];
+        load_T(ev, s, T);
+        ipgl::landau_pi(T, eps, u[0], u[1], u[2], u[3], it, ev.dtau, ev.a, pi);
+        for (int k = 0; k < 10; k++) ev.land[(5 + k) * S + s] = pi[k];
+    }
+}
+
+// per-event sum of partial slot k over the event's (unpadded) blocks
+__device__ double slot_sum(const EvDesc& ev, int k) {
+    const int nb = (int)((ev.S + IPG_BLOCK - 1) / IPG_BLOCK);
+    double v = 0.;
+    for (int l = threadIdx.x; l < nb; l += IPG_BLOCK) v += ev.partial[(size_t)k * ev.nblocks + l];
+    return bsum<IPG_BLOCK>(v);
+}
+
+// obs layout: 0-5 eps_1..6, 6-11 Psi_1..6, 12 area, 13 Rbar, 14 <eps> hbarc,
+// 15 <Q_sA^2 Q_sB^2> hbarc, 16 sqrt<r^2>, 17-18 centroid, 19-22 u^mu average
+// sums, 23 QR failures, 24 sites without a timelike eigenvector, 25 sum w
+__global__ __launch_bounds__(IPG_BLOCK) void k_ecc_mid(const EvDesc* __restrict__ evs, int nev, int it, int all) {
+    const EvDesc& ev = evs[blockIdx.x];
+    if (!ev.active || (!all && it != ev.itmax)) return;
+    double r[12];
+    for (int k = 0; k < 12; k++) r[k] = slot_sum(ev, k);
+    if (threadIdx.x == 0) {
+        double* o = ev.obs;
+        for (int k = 0; k < 4; k++) o[19 + k] = r[k];
+        o[23] = r[4];
+        o[24] = r[5];
+        const double tot = r[11];
+        o[25] = tot;
+        o[17] = r[6] / tot;
+        o[18] = r[7] / tot;
+        o[12] = r[8] * ev.a * ev.a;
+        o[14] = r[9] / r[8];
+        o[15] = r[10] / r[8] * kHbarc;
+    }
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_ecc(const EvDesc* __restrict__ evs, int nev, int it, int all) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active || (!all && it != ev.itmax)) return;
+    const int lb = xcd_remap(blockIdx.x - ev.block0, ev.nblocks);
+    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
+    const long long S = ev.S;
+    double acc[19];
+#pragma unroll
+    for (int k = 0; k < 19; k++) acc[k] = 0.;
+    if (s < S) {
+        const int N = ev.N;
+        const int si = (int)(s / N), sj = (int)(s - (long long)si * N);
+        const double L = ev.L, a = ev.a;
+        const double avx = ev.obs[17], avy = ev.obs[18];
+        const double x = -L / 2. + a * si - avx, y = -L / 2. + a * sj - avy;
+        const double eps = ev.land[s], ut = ev.land[S + s];
+        const double w = eps < kCutoff ? 0. : eps * ut;
+        acc[17] = x * x * w;
+        acc[18] = y * y * w;
+        if (si >= 2 && si < N - 2 && sj >= 2 && sj < N - 2) {
+            // r^k e^{i n phi} = r^(k-n) z^n, z = x + i y
+            const double r2 = x * x + y * y, r = sqrt(r2);
+            double zr = x, zi = y;  // z^n
+            double p[7][2];
+            p[1][0] = x;
+            p[1][1] = y;
+#pragma unroll
+            for (int n = 2; n <= 6; n++) {
+                const double nr = zr * x - zi * y, ni = zr * y + zi * x;
+                zr = nr;
+                zi = ni;
+                p[n][0] = zr;
+                p[n][1] = zi;
+            }
+            acc[0] = r2 * r * w;        // r^3 (n = 1 and n = 3)
+            acc[1] = r2 * w;            // r^2
+            acc[2] = r2 * r2 * w;       // r^4
+            acc[3] = r2 * r2 * r * w;   // r^5
+            acc[4] = r2 * r2 * r2 * w;  // r^6
+            acc[5] = r2 * p[1][0] * w;  // r^3 cos(phi)
+            acc[6] = r2 * p[1][1] * w;
+#pragma unroll
+            for (int n = 2; n <= 6; n++) {
+                acc[7 + 2 * (n - 2)] = p[n][0] * w;
+                acc[8 + 2 * (n - 2)] = p[n][1] * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 19; k++) {
+        const double r = bsum<IPG_BLOCK>(acc[k]);
+        if (threadIdx.x == 0) ev.partial[(size_t)(12 + k) * ev.nblocks + lb] = r;
+    }
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_ecc_finish(const EvDesc* __restrict__ evs, int nev, int it,
+                                                         int all) {
+    const EvDesc& ev = evs[blockIdx.x];
+    if (!ev.active || (!all && it != ev.itmax)) return;
+    double r[19];
+    for (int k = 0; k < 19; k++) r[k] = slot_sum(ev, 12 + k);
+    if (threadIdx.x == 0) {
+        double* o = ev.obs;
+        const double rn[7] = {0., r[0], r[1], r[0], r[2], r[3], r[4]};
+        const double c[7] = {0., r[5], r[7], r[9], r[11], r[13], r[15]};
+        const double sn[7] = {0., r[6], r[8], r[10], r[12], r[14], r[16]};
+        for (int n = 1; n <= 6; n++) {
+            o[n - 1] = sqrt(c[n] * c[n] + sn[n] * sn[n]) / rn[n];
+            o[5 + n] = (atan(sn[n] / c[n]) + M_PI) / n;
+        }
+        const double tot = o[25];
+        const double axx = r[17] / tot, ayy = r[18] / tot;
+        o[13] = 1. / sqrt(1. / axx + 1. / ayy);
+        o[16] = sqrt(r[1] / tot);
+    }
+}
+
+}  // namespace
+
+void ipg_launch_landau(const EvDesc* d, int nev, int nblk, int it, int all, hipStream_t st) {
+    hipLaunchKernelGGL(k_landau, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
+    hipLaunchKernelGGL(k_landau_carry, dim3(nev), dim3(1024), 0, st, d, nev, it, all);
+    hipLaunchKernelGGL(k_ecc_mid, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
+    hipLaunchKernelGGL(k_ecc, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
+    hipLaunchKernelGGL(k_ecc_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
+}

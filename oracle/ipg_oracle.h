@@ -30,6 +30,12 @@
  *   orc_tp_sum       single-nucleon Gaussian closed form
  *   orc_wounding     brute-force pair loop / hard-sphere closed form
  *   orc_g2mu2        table-node exactness; fixed-point residual
+ *   orc_eig4         numpy.linalg.eig (LAPACK dgeev) eigenvalues and
+ *                    eigenvector directions; A v = lambda v residual
+ *   orc_landau       boosted perfect fluid and anisotropic fluid closed forms
+ *                    (eps, u^mu, pi^{mu nu}); u.u = 1; sign u^tau > 0
+ *   orc_eccentricity elliptic and rotated Gaussian closed forms; moments of
+ *                    point sources; translation invariance
  * Functions with no check of their own are marked "parity unpinned" below.
  */
 #ifndef IPG_ORACLE_H
@@ -201,6 +207,27 @@ double orc_gauss_law(int N, const double *Ux, const double *Uy,
 
 /* dE/deta = tau * a^2 * hbarc * sum T^tautau  (GeV) */
 double orc_dedeta(int N, double a, double tau_fm, const double *Ttautau);
+
+/* real nonsymmetric 4x4 eigenproblem (row-major A): eigenvalues wr + i wi,
+ * eigenvector k in column k of vr + i vi (row-major), unit Euclidean norm;
+ * a complex pair occupies adjacent columns, positive imaginary part first.
+ * Returns nonzero when the QR iteration does not converge. */
+int orc_eig4(const double *A, double *wr, double *wi, double *vr, double *vi);
+
+/* Landau frame at step it (MyEigen::flowVelocity4D): T[10][S] as above ->
+ * out[15][S] = eps, u^tau, u^x, u^y, u^eta, pi^{mu nu} (T's order);
+ * avg = sum u_x^2 eps, sum u_y^2 eps, sum (tau u^eta)^2 eps, sum eps over
+ * the timelike eigenvectors found */
+void orc_landau(int N, double L, double dtau, int it, const double *T,
+                double *out, double avg[4], int *nfail);
+
+/* Evolution::eccentricity with gfactor = 1: res[0..5] = eps_1..eps_6,
+ * res[6..11] = Psi_1..Psi_6, res[12] = area, res[13] = Rbar,
+ * res[14] = <eps> hbarc, res[15] = <Q_sA^2 Q_sB^2> hbarc, res[16] = sqrt<r^2>,
+ * res[17], res[18] = centroid */
+void orc_eccentricity(int N, double L, double QsmuRatio, double QsmuRatioB,
+                      double cutoff, const double *eps, const double *utau,
+                      const double *g2A, const double *g2B, double *res);
 
 /* lattice-unit derived step count and dtau (main.cpp:531-534,
  * Evolution.cpp:396) */

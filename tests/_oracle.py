@@ -106,6 +106,11 @@ def lib():
         L.orc_dedeta.restype = C.c_double
         L.orc_time_steps.argtypes = [C.c_double, C.c_int, C.c_double,
                                      C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.orc_eig4.argtypes = [_dp] * 5
+        L.orc_eig4.restype = C.c_int
+        L.orc_landau.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
+                                 C.POINTER(C.c_int)]
+        L.orc_eccentricity.argtypes = [C.c_int] + [C.c_double] * 4 + [_dp] * 5
         _lib = L
     return _lib
 
@@ -177,6 +182,35 @@ def time_steps(L, N, maxtime):
     it = C.c_int()
     lib().orc_time_steps(L, N, maxtime, C.byref(d), C.byref(it))
     return d.value, it.value
+
+
+def eig4(A):
+    """eigenvalues (complex [4]) and eigenvectors (complex columns [4, 4])"""
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(16)
+    wr, wi, vr, vi = (np.zeros(4), np.zeros(4), np.zeros(16), np.zeros(16))
+    rc = lib().orc_eig4(A, wr, wi, vr, vi)
+    return rc, wr + 1j * wi, (vr + 1j * vi).reshape(4, 4)
+
+
+def landau(N, L, dtau, it, T):
+    """T [10, S] -> (out [15, S], avg [4], failures)"""
+    T = np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+    out = np.zeros(15 * N * N)
+    avg = np.zeros(4)
+    nf = C.c_int()
+    lib().orc_landau(N, L, dtau, it, T, out, avg, C.byref(nf))
+    return out.reshape(15, N * N), avg, nf.value
+
+
+def eccentricity(N, L, eps, utau, g2A=None, g2B=None, QsmuRatio=0.8, QsmuRatioB=0.8, cutoff=0.):
+    S = N * N
+    g2A = np.zeros(S) if g2A is None else np.ascontiguousarray(g2A, dtype=np.float64)
+    g2B = np.zeros(S) if g2B is None else np.ascontiguousarray(g2B, dtype=np.float64)
+    res = np.zeros(19)
+    lib().orc_eccentricity(N, L, QsmuRatio, QsmuRatioB, cutoff,
+                           np.ascontiguousarray(eps, dtype=np.float64),
+                           np.ascontiguousarray(utau, dtype=np.float64), g2A, g2B, res)
+    return res
 
 
 def new_state(N):

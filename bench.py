@@ -84,6 +84,19 @@ def roofline(I, p):
             "bytes_per_launch": BYTES_PER_SITE_STEP * S}
 
 
+def copy_probe(mib):
+    """One plain device-to-device copy kernel of known size: what FETCH_SIZE
+    and WRITE_SIZE read for it scales the counters of the other kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(p):
     """Oracle timing on a bounded sample of the event, extrapolated."""
     import ctypes as C
@@ -153,6 +166,10 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the timed single event of the roofline figure (for rocprofv3)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (the HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: events are independent and this build has no multi-GPU "
@@ -160,6 +177,11 @@ def main():
         sys.exit(2)
     import ipglasma_amd as I
     p = pbpb_params(I, N=a.size)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+    if a.roofline_only:
+        print(json.dumps(roofline(I, p)))
+        return
     seeds = list(range(1000, 1000 + a.batch))
     for w in range(a.warmup):
         run_batched(I, p, seeds)

@@ -71,9 +71,11 @@ def roofline(I, p):
     ev = I.Event(p, seed=12345)
     while not ev.init():
         pass
+    print("[bench] roofline event: initial conditions done", file=sys.stderr, flush=True)
     ev.enable_timing(True)
     ev.run()
     t = ev.timing()
+    print("[bench] roofline event: evolution done", file=sys.stderr, flush=True)
     ev.close()
     S = p.size * p.size
     avg_ms = t["step_ms_total"] / max(1, t["nsteps"])

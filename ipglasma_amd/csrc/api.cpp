@@ -102,6 +102,7 @@ struct Engine {
     double* d_Q = nullptr;
     bool have_table = false;
     int* d_flag = nullptr;
+    int* d_work = nullptr;    // root-finder work counter
     double* d_out = nullptr;  // 8 doubles per event
     double* d_obs = nullptr;  // kObs doubles per event (Landau, eccentricities)
     int nblk = 0, nlinkblk = 0;
@@ -129,6 +130,7 @@ struct Engine {
         if (d_Tl) hipFree(d_Tl);
         if (d_Q) hipFree(d_Q);
         if (d_flag) hipFree(d_flag);
+        if (d_work) hipFree(d_work);
         if (d_out) hipFree(d_out);
         if (d_obs) hipFree(d_obs);
         for (hipEvent_t t : tev) hipEventDestroy(t);
@@ -237,9 +239,8 @@ struct Engine {
             d.partial = take(32 * (size_t)d.nblocks);
             int* ip = (int*)take(2 * (size_t)S);
             d.check2 = ip;
-            d.prev2 = ip + S;
+            d.newton_stat = ip;  // 2S link counters, reused after the colour charge
             d.land_flag = (int*)take(((size_t)S + 1) / 2);
-            d.newton_stat = (int*)d.check2;  // reused after the colour charge is done
             if (!plans.count(N)) {
                 Plan pl;
                 pl.N = N;
@@ -267,6 +268,7 @@ struct Engine {
         nlinkblk = lb0;
         HIPCHK(hipMalloc(&d_desc, sizeof(EvDesc) * nev));
         HIPCHK(hipMalloc(&d_flag, sizeof(int) * nev));
+        HIPCHK(hipMalloc(&d_work, sizeof(int)));
         HIPCHK(hipMalloc(&d_out, sizeof(double) * 8 * nev));
         HIPCHK(hipMalloc(&d_obs, sizeof(double) * kObs * nev));
         HIPCHK(hipMemset(d_obs, 0, sizeof(double) * kObs * nev));
@@ -448,7 +450,7 @@ struct Engine {
     int fields() {
         const int nev = (int)ev.size();
         ipg_launch_links(d_desc, nev, nblk, st);
-        ipg_launch_newton(d_desc, nev, nlinkblk, st);
+        ipg_launch_newton(d_desc, nev, nlinkblk, d_work, st);
         ipg_launch_pi_init(d_desc, nev, nblk, st);
         ipg_launch_zero_fields(d_desc, nev, nblk, st);
         HIPCHK(hipGetLastError());

@@ -47,7 +47,6 @@ struct EvDesc {
     double* g2B;
     double* yB;      // final rapidity of the B fixed point per site
     int* check2;     // site inside both nuclei (Init.cpp:1255-1301)
-    int* prev2;      // last check2 site before this one (-1: none)
     // Glauber nucleons (x, y, phi, collided) for A then B
     double* nuc;
     int A1, A2;

@@ -34,7 +34,7 @@ void ipg_launch_unit_v(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 
 // tau = 0 fields
 void ipg_launch_links(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
-void ipg_launch_newton(const EvDesc* d_evs, int nev, int nlinkblk, hipStream_t st);
+void ipg_launch_newton(const EvDesc* d_evs, int nev, int nlinkblk, int* d_counter, hipStream_t st);
 void ipg_launch_pi_init(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_zero_fields(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_rho_normals(const EvDesc* d_evs, int nucleus, int k, double* d_out, hipStream_t st);

@@ -170,54 +170,61 @@ __device__ __forceinline__ int site_check2(const EvDesc& ev, long long s) {
     return check == 2;
 }
 
-// pass 1: nucleus B from the input rapidity at every site
-__global__ __launch_bounds__(IPG_BLOCK) void k_qs_b(const EvDesc* __restrict__ evs, int nev, const double* Tl,
-                                                   const double* Q) {
-    const int e = find_event(evs, nev, blockIdx.x);
-    const EvDesc& ev = evs[e];
-    if (!ev.active) return;
-    const long long s = (long long)xcd_remap(blockIdx.x - ev.block0, ev.nblocks) * IPG_BLOCK + threadIdx.x;
-    if (s >= ev.S) return;
-    int c2 = site_check2(ev, s);
-    ev.check2[s] = c2;
-    ev.g2A[s] = 0.;
-    ev.g2B[s] = 0.;
-    ev.yB[s] = ev.rapidity;
-    if (!c2) return;
-    if (ev.useFluctuatingx == 1) {
-        double ly = ev.rapidity, Qs = 1.;
-        ev.g2B[s] = qs_fixed_point(Tl, Q, ev.TpB[s], ly, Qs, ev.rapidity, -1., ev.QsmuRatioB, ev);
-        ev.yB[s] = ly;
-    }
+// block maximum of an int (all threads of the block call it), valid in thread 0
+__device__ int block_max_int(int v) {
+    __shared__ int w[IPG_BLOCK / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_down(v, o, 64));
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = -1;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < IPG_BLOCK / 64; k++) r = max(r, w[k]);
+    __syncthreads();
+    return r;
 }
 
-// last site before s (single-thread order) whose check2 flag is set; one
-// block per event, chunked prefix-max scan
-__global__ __launch_bounds__(1024) void k_prev2(const EvDesc* __restrict__ evs, int nev, int* flag) {
-    const EvDesc& ev = evs[blockIdx.x];
-    if (threadIdx.x == 0) flag[blockIdx.x] = 0;
+// partial slot holding, per block, the last site whose check2 flag is set
+constexpr int kSlotCheck2 = 8;
+
+// pass 1: nucleus B from the input rapidity at every site; records per block
+// the last site inside both nuclei (for the site order of pass 2)
+__global__ __launch_bounds__(IPG_BLOCK) void k_qs_b(const EvDesc* __restrict__ evs, int nev, const double* Tl,
+                                                   const double* Q, int* flag) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (blockIdx.x == (unsigned)ev.block0 && threadIdx.x == 0) flag[e] = 0;
     if (!ev.active) return;
-    __shared__ int carry[1024];
-    const long long S = ev.S;
-    const long long chunk = (S + 1023) / 1024;
-    const long long b = threadIdx.x * chunk, en = min(S, b + chunk);
-    int last = -1;
-    for (long long s = b; s < en; s++)
-        if (ev.check2[s]) last = (int)s;
-    carry[threadIdx.x] = last;
-    __syncthreads();
-    // inclusive max-scan over thread carries (Hillis-Steele)
-    for (int o = 1; o < 1024; o <<= 1) {
-        int v = threadIdx.x >= o ? carry[threadIdx.x - o] : -1;
-        __syncthreads();
-        carry[threadIdx.x] = max(carry[threadIdx.x], v);
-        __syncthreads();
+    const int lb = xcd_remap(blockIdx.x - ev.block0, ev.nblocks);
+    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
+    int c2 = 0;
+    if (s < ev.S) {
+        c2 = site_check2(ev, s);
+        ev.check2[s] = c2;
+        ev.g2A[s] = 0.;
+        ev.g2B[s] = 0.;
+        ev.yB[s] = ev.rapidity;
+        if (c2 && ev.useFluctuatingx == 1) {
+            double ly = ev.rapidity, Qs = 1.;
+            ev.g2B[s] = qs_fixed_point(Tl, Q, ev.TpB[s], ly, Qs, ev.rapidity, -1., ev.QsmuRatioB, ev);
+            ev.yB[s] = ly;
+        }
     }
-    int run = threadIdx.x > 0 ? carry[threadIdx.x - 1] : -1;
-    for (long long s = b; s < en; s++) {
-        ev.prev2[s] = run;
-        if (ev.check2[s]) run = (int)s;
+    const int last = block_max_int(c2 ? (int)s : -1);
+    if (threadIdx.x == 0) ev.partial[(size_t)kSlotCheck2 * ev.nblocks + lb] = (double)last;
+}
+
+// the last site before s inside both nuclei (-1: none): a walk back through
+// s's own block, then through the per-block maxima of earlier blocks
+__device__ long long prev_check2(const EvDesc& ev, long long s) {
+    const long long b0 = (s / IPG_BLOCK) * IPG_BLOCK;
+    for (long long q = s - 1; q >= b0; q--)
+        if (ev.check2[q]) return q;
+    for (long long b = s / IPG_BLOCK - 1; b >= 0; b--) {
+        const double m = ev.partial[(size_t)kSlotCheck2 * ev.nblocks + b];
+        if (m >= 0.) return (long long)m;
     }
+    return -1;
 }
 
 // pass 2: nucleus A starting from the previous check2 site's final B rapidity
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_qs_a(const EvDesc* __restrict__ e
     if (s >= ev.S) return;
     if (!ev.check2[s]) return;
     if (ev.useFluctuatingx == 1) {
-        int p = ev.prev2[s];
+        const long long p = prev_check2(ev, s);
         double ly = p >= 0 ? ev.yB[p] : ev.rapidity;
         if (ly < 0 || ev.rapidity < 0) {
             // the first iteration would read Q_s carried from the previous site
@@ -607,142 +614,194 @@ __device__ __forceinline__ double residual(const M3& W, const M3& Us, const cd t
     return Fz;
 }
 
-__global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, int nev) {
-    // augmented system [J | F] of every lane, column-interleaved across lanes
-    __shared__ double Jl[72 * NWB];
-    const int e = find_event_links(evs, nev, blockIdx.x);
-    const EvDesc& ev = evs[e];
-    if (!ev.active) return;
-    const long long l = (long long)(blockIdx.x - ev.lblock0) * NWB + threadIdx.x;
-    const long long S = ev.S;
-    if (l >= 2 * S) return;
-    const int dir = l >= S;
-    const long long s = dir ? l - S : l;
-    const uint32_t link = (uint32_t)(2 * s + dir);
-    double* J = Jl + threadIdx.x;
+// One Newton iteration of one link, Init.cpp:3386-3476 (the body of the
+// reference's while loop); returns the loop condition for the next round.
+struct NewtonLink {
+    M3 U0, W, Us;
+    double alpha[8], best_alpha[8];
+    double Fzero, Fmin;
+    int iter, nRestart, total, analytic, best_one;
+    uint32_t ndraw;
+};
+
+__device__ __forceinline__ bool newton_iteration(NewtonLink& k, double* J, uint64_t seed, uint32_t link) {
 #define JA(a, b) J[((a) * 9 + (b)) * NWB]
-    const M3 U1 = ldm(dir ? ev.L1y : ev.L1x, S, s);
-    const M3 U2 = ldm(dir ? ev.L2y : ev.L2x, S, s);
-    const M3 U0 = U1 * U2;
-    const M3 W = U1 + U2;
-    cd trc[8];
-    tr_gen(W - dag(W), trc);
-    double alpha[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    M3 Us = exp_iQt(alpha) * U0;
-    M3 best = m_one();
-    double Fzero = 10., Fmin = 1e6;
-    int iter = 0, nRestart = 0, total = 0, analytic = 0;
-    uint32_t ndraw = 0;
     const int maxIt = 2000, maxRe = 200;
-    while (Fzero > 1e-6 && iter < maxIt && nRestart < maxRe) {
-        iter++;
-        total++;
-        double F[8];
-        Fzero = residual(W, Us, trc, F);
+    k.iter++;
+    k.total++;
+    cd trc[8];
+    tr_gen(k.W - dag(k.W), trc);
+    double F[8];
+    k.Fzero = residual(k.W, k.Us, trc, F);
 #pragma unroll
-        for (int a = 0; a < 8; a++) JA(a, 8) = F[a];
-        const M3 Ud = dag(Us);
-        bool good = true;
-        // forward-difference Jacobian, column by column
+    for (int a = 0; a < 8; a++) JA(a, 8) = F[a];
+    const M3 Ud = dag(k.Us);
+    bool good = true;
+    // forward-difference Jacobian, column by column
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const double d = fmax(0.001, fmin(10., 0.01 * fabs(alpha[b])));
-            const double ab = alpha[b];
-            alpha[b] = ab + d;
-            M3 Mp = W * (dag(exp_iQt(alpha) * U0) - Ud);
-            cd t[8];
-            tr_gen(Mp, t);
-            double chk = 0.;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                double v = 2. * t[a].i / d;
-                JA(a, b) = v;
-                chk += fabs(v);
-                if (chk < 1e-15) good = false;
-            }
-            // the reference adds and then subtracts the step in place
-            alpha[b] = alpha[b] - d;
-        }
-        if (!good) {
-            analytic = 1;
-            for (int b = 0; b < 8; b++) {
-                M3 Mb = gen_mul(b, Ud);
-                for (int a = 0; a < 8; a++) JA(a, b) = -2. * retrprod(gen_mul(a, W), Mb);
-            }
-        }
-        // Gaussian elimination with partial pivoting (first maximal pivot);
-        // row swaps carry the right-hand side along
-        for (int j = 0; j < 7; j++) {
-            double mx = fabs(JA(j, j));
-            int ip = j;
-            for (int r = j + 1; r < 8; r++) {
-                double v = fabs(JA(r, j));
-                if (v > mx) {
-                    mx = v;
-                    ip = r;
-                }
-            }
-            if (ip != j) {
-                for (int c = 0; c < 9; c++) {
-                    double t = JA(j, c);
-                    JA(j, c) = JA(ip, c);
-                    JA(ip, c) = t;
-                }
-            }
-            double ajj = JA(j, j);
-            if (ajj != 0.0) {
-                for (int r = j + 1; r < 8; r++) {
-                    double arj = JA(r, j) / ajj;
-                    JA(r, j) = arj;
-                    for (int c = j + 1; c < 8; c++) JA(r, c) = JA(r, c) - arj * JA(j, c);
-                }
-            }
-        }
-        // unit-lower forward, upper back substitution on column 8
-        for (int r = 1; r < 8; r++) {
-            double t = JA(r, 8);
-            for (int c = 0; c < r; c++) t -= JA(r, c) * JA(c, 8);
-            JA(r, 8) = t;
-        }
-        JA(7, 8) = JA(7, 8) / JA(7, 7);
-        for (int r = 6; r >= 0; r--) {
-            double t = JA(r, 8);
-            for (int c = r + 1; c < 8; c++) t -= JA(r, c) * JA(c, 8);
-            JA(r, 8) = t / JA(r, r);
-        }
-        double y[8];
-        bool ok = true;
+    for (int b = 0; b < 8; b++) {
+        const double d = fmax(0.001, fmin(10., 0.01 * fabs(k.alpha[b])));
+        const double ab = k.alpha[b];
+        k.alpha[b] = ab + d;
+        M3 Mp = k.W * (dag(exp_iQt(k.alpha) * k.U0) - Ud);
+        cd t[8];
+        tr_gen(Mp, t);
+        double chk = 0.;
 #pragma unroll
         for (int a = 0; a < 8; a++) {
-            y[a] = JA(a, 8);
-            if (fabs(y[a]) > 1e5) ok = false;
+            double v = 2. * t[a].i / d;
+            JA(a, b) = v;
+            chk += fabs(v);
+            if (chk < 1e-15) good = false;
         }
-        if (ok) {
+        // the reference adds and then subtracts the step in place
+        k.alpha[b] = k.alpha[b] - d;
+    }
+    if (!good) {
+        k.analytic = 1;
+        for (int b = 0; b < 8; b++) {
+            M3 Mb = gen_mul(b, Ud);
+            for (int a = 0; a < 8; a++) JA(a, b) = -2. * retrprod(gen_mul(a, k.W), Mb);
+        }
+    }
+    // Gaussian elimination with partial pivoting (first maximal pivot); row
+    // swaps carry the right-hand side along
+    for (int j = 0; j < 7; j++) {
+        double mx = fabs(JA(j, j));
+        int ip = j;
+        for (int r = j + 1; r < 8; r++) {
+            double v = fabs(JA(r, j));
+            if (v > mx) {
+                mx = v;
+                ip = r;
+            }
+        }
+        if (ip != j) {
+            for (int c = 0; c < 9; c++) {
+                double t = JA(j, c);
+                JA(j, c) = JA(ip, c);
+                JA(ip, c) = t;
+            }
+        }
+        double ajj = JA(j, j);
+        if (ajj != 0.0) {
+            for (int r = j + 1; r < 8; r++) {
+                double arj = JA(r, j) / ajj;
+                JA(r, j) = arj;
+                for (int c = j + 1; c < 8; c++) JA(r, c) = JA(r, c) - arj * JA(j, c);
+            }
+        }
+    }
+    // unit-lower forward, upper back substitution on column 8
+    for (int r = 1; r < 8; r++) {
+        double t = JA(r, 8);
+        for (int c = 0; c < r; c++) t -= JA(r, c) * JA(c, 8);
+        JA(r, 8) = t;
+    }
+    JA(7, 8) = JA(7, 8) / JA(7, 7);
+    for (int r = 6; r >= 0; r--) {
+        double t = JA(r, 8);
+        for (int c = r + 1; c < 8; c++) t -= JA(r, c) * JA(c, 8);
+        JA(r, 8) = t / JA(r, r);
+    }
+    double y[8];
+    bool ok = true;
 #pragma unroll
-            for (int a = 0; a < 8; a++) alpha[a] = alpha[a] + y[a];
-        } else {
-            restart_alpha(ev.newton_seed, link, ndraw++, alpha);
-        }
-        Us = exp_iQt(alpha) * U0;
-        if (Fzero < Fmin) {
-            Fmin = Fzero;
-            best = Us;
-        }
-        if (iter == maxIt) {
-            restart_alpha(ev.newton_seed, link, ndraw++, alpha);
-            Us = exp_iQt(alpha) * U0;
-            nRestart++;
-            iter = 0;
-        }
+    for (int a = 0; a < 8; a++) {
+        y[a] = JA(a, 8);
+        if (fabs(y[a]) > 1e5) ok = false;
     }
 #undef JA
-    int conv = 1;
-    if (nRestart == maxRe) {
-        Us = best;
-        conv = 0;
+    if (ok) {
+#pragma unroll
+        for (int a = 0; a < 8; a++) k.alpha[a] = k.alpha[a] + y[a];
+    } else {
+        restart_alpha(seed, link, k.ndraw++, k.alpha);
     }
-    stm(ev.U[0][dir], S, s, Us);
-    ev.newton_stat[l] = total | (nRestart << 20) | ((!conv) << 28) | (analytic << 29);
+    k.Us = exp_iQt(k.alpha) * k.U0;
+    if (k.Fzero < k.Fmin) {
+        // best = Us, kept as its exponent (Us is exp(i alpha t) U0)
+        k.Fmin = k.Fzero;
+#pragma unroll
+        for (int a = 0; a < 8; a++) k.best_alpha[a] = k.alpha[a];
+        k.best_one = 0;
+    }
+    if (k.iter == maxIt) {
+        restart_alpha(seed, link, k.ndraw++, k.alpha);
+        k.Us = exp_iQt(k.alpha) * k.U0;
+        k.nRestart++;
+        k.iter = 0;
+    }
+    return k.Fzero > 1e-6 && k.iter < maxIt && k.nRestart < maxRe;
+}
+
+// Persistent root finder: the iteration count per link is heavy-tailed
+// (median 2, p99 ~400, restarts after 2000), so each lane takes a new link
+// from a wave-aggregated work counter as soon as its link is done instead of
+// waiting for the slowest lane of a fixed wavefront of links.  Work items are
+// the link slots of all events of the launch (lblock0 * NWB + l).
+__global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, int nev, int nitems,
+                                                int* __restrict__ counter) {
+    // augmented system [J | F] of every lane, column-interleaved across lanes
+    __shared__ double Jl[72 * NWB];
+    double* J = Jl + threadIdx.x;
+    const int lane = threadIdx.x;
+    int state = 0;  // 0: needs a link, 1: iterating, 2: no work left
+    int e = 0, dir = 0;
+    long long l = 0, s = 0;
+    uint32_t link = 0;
+    NewtonLink k;
+    for (;;) {
+        const unsigned long long need = __ballot(state == 0);
+        if (need) {
+            const int first = __ffsll((long long)need) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(counter, __popcll(need));
+            base = __shfl(base, first, NWB);
+            if (state == 0) {
+                const int g = base + __popcll(need & ((1ull << lane) - 1ull));
+                if (g >= nitems) {
+                    state = 2;
+                } else {
+                    e = find_event_links(evs, nev, g / NWB);
+                    const EvDesc& ev = evs[e];
+                    l = (long long)g - (long long)ev.lblock0 * NWB;
+                    const long long S = ev.S;
+                    if (ev.active && l < 2 * S) {
+                        dir = l >= S;
+                        s = dir ? l - S : l;
+                        link = (uint32_t)(2 * s + dir);
+                        const M3 U1 = ldm(dir ? ev.L1y : ev.L1x, S, s);
+                        const M3 U2 = ldm(dir ? ev.L2y : ev.L2x, S, s);
+                        k.U0 = U1 * U2;
+                        k.W = U1 + U2;
+#pragma unroll
+                        for (int a = 0; a < 8; a++) k.alpha[a] = 0.;
+                        k.Us = exp_iQt(k.alpha) * k.U0;
+                        k.best_one = 1;
+                        k.Fzero = 10.;
+                        k.Fmin = 1e6;
+                        k.iter = k.nRestart = k.total = k.analytic = 0;
+                        k.ndraw = 0;
+                        state = 1;
+                    }
+                    // padding slot or inactive event: fetch again
+                }
+            }
+        }
+        if (__ballot(state != 2) == 0) break;
+        if (state == 1) {
+            const EvDesc& ev = evs[e];
+            if (!newton_iteration(k, J, ev.newton_seed, link)) {
+                const int conv = k.nRestart != 200;
+                M3 out = k.Us;
+                if (!conv) out = k.best_one ? m_one() : exp_iQt(k.best_alpha) * k.U0;
+                stm(ev.U[0][dir], ev.S, s, out);
+                ev.newton_stat[l] = k.total | (k.nRestart << 20) | ((!conv) << 28) | (k.analytic << 29);
+                state = 0;
+            }
+        }
+    }
 }
 
 // A^eta from the four links touching the site, pi = -(2i/g) A^eta
@@ -800,8 +859,7 @@ void ipg_launch_tp(const EvDesc* d, int nev, int nblk, hipStream_t st) {
 }
 void ipg_launch_qs(const EvDesc* d, int nev, int nblk, const double* Tl, const double* Q, int* flag,
                    hipStream_t st) {
-    hipLaunchKernelGGL(k_qs_b, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, Tl, Q);
-    hipLaunchKernelGGL(k_prev2, dim3(nev), dim3(1024), 0, st, d, nev, flag);
+    hipLaunchKernelGGL(k_qs_b, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, Tl, Q, flag);
     hipLaunchKernelGGL(k_qs_a, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, Tl, Q, flag);
     hipLaunchKernelGGL(k_qs_serial, dim3(nev), dim3(64), 0, st, d, nev, Tl, Q, (const int*)flag);
 }
@@ -831,8 +889,14 @@ void ipg_launch_unit_v(const EvDesc* d, int nev, int nblk, hipStream_t st) {
 void ipg_launch_links(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_links, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
 }
-void ipg_launch_newton(const EvDesc* d, int nev, int nlinkblk, hipStream_t st) {
-    hipLaunchKernelGGL(k_newton, dim3(nlinkblk), dim3(NWB), 0, st, d, nev);
+__global__ void k_zero_counter(int* c) { *c = 0; }
+
+void ipg_launch_newton(const EvDesc* d, int nev, int nlinkblk, int* d_counter, hipStream_t st) {
+    // enough single-wave blocks to fill the machine (the kernel runs at one
+    // wave per SIMD: 256 CUs x 4); each loops until the work counter runs out
+    const int grid = nlinkblk < 2048 ? nlinkblk : 2048;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, d_counter);
+    hipLaunchKernelGGL(k_newton, dim3(grid), dim3(NWB), 0, st, d, nev, nlinkblk * NWB, d_counter);
 }
 void ipg_launch_pi_init(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_pi_init, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);

@@ -6,8 +6,8 @@
 //                   partials of the u^mu averages (MyEigen.cpp:199-206) and of
 //                   the first eccentricity pass (Evolution.cpp:1346-1460)
 //   k_landau_carry  pi^{mu nu} at sites whose eps or u^mu the reference takes
-//                   from an earlier site (its running variables), by a scan
-//                   over the site order
+//                   from an earlier site (its running variables): a walk back
+//                   through the site's block, then through per-block maxima
 //   k_ecc_mid       centroid and the per-event sums of the first pass
 //   k_ecc           moments about the centroid (Evolution.cpp:1478-1660) and
 //                   the second centroid pass (1670-1790)
@@ -44,6 +44,24 @@ __device__ double bsum(double v) {
     return r;
 }
 
+// block maximum of an int, valid in thread 0
+__device__ int bmax_int(int v) {
+    __shared__ int w[IPG_BLOCK / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_down(v, o, 64));
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = -1;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < IPG_BLOCK / 64; k++) r = max(r, w[k]);
+    __syncthreads();
+    return r;
+}
+
+// partial slots of k_landau: 0-11 sums, 12/13 the last site of the block that
+// set eps / u^mu (read by k_landau_carry before k_ecc reuses 12-30)
+constexpr int kSlotEps = 12, kSlotU = 13;
+
 __device__ __forceinline__ void load_T(const EvDesc& ev, long long s, double (&T)[10]) {
 #pragma unroll
     for (int k = 0; k < 10; k++) T[k] = ev.T[k * ev.S + s];
@@ -59,6 +77,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_landau(const EvDesc* __restrict__
     double acc[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = 0.;
+    int flags = 0;
     if (s < S) {
         const int N = ev.N;
         const int si = (int)(s / N), sj = (int)(s - (long long)si * N);
@@ -71,6 +90,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_landau(const EvDesc* __restrict__
         for (int k = 0; k < 4; k++) ev.land[(1 + k) * S + s] = o.u[k];
         ev.land_epsv[s] = o.eps_var;
         ev.land_flag[s] = o.flags;
+        flags = o.flags;
         if ((o.flags & 3) == 3) {
             double pi[10];
             ipgl::landau_pi(T, o.eps_var, o.u[0], o.u[1], o.u[2], o.u[3], it, ev.dtau, ev.a, pi);
@@ -101,50 +121,46 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_landau(const EvDesc* __restrict__
         const double r = bsum<IPG_BLOCK>(acc[k]);
         if (threadIdx.x == 0) ev.partial[(size_t)k * ev.nblocks + lb] = r;
     }
+    const int le = bmax_int((flags & 1) ? (int)s : -1);
+    const int lu = bmax_int((flags & 2) ? (int)s : -1);
+    if (threadIdx.x == 0) {
+        ev.partial[(size_t)kSlotEps * ev.nblocks + lb] = (double)le;
+        ev.partial[(size_t)kSlotU * ev.nblocks + lb] = (double)lu;
+    }
+}
+
+// last site q <= s whose flag has `bit` (-1: none)
+__device__ long long last_set(const EvDesc& ev, long long s, int bit, int slot) {
+    const long long b0 = (s / IPG_BLOCK) * IPG_BLOCK;
+    for (long long q = s; q >= b0; q--)
+        if (ev.land_flag[q] & bit) return q;
+    for (long long b = s / IPG_BLOCK - 1; b >= 0; b--) {
+        const double m = ev.partial[(size_t)slot * ev.nblocks + b];
+        if (m >= 0.) return (long long)m;
+    }
+    return -1;
 }
 
 // pi^{mu nu} at sites without their own (eps, u^mu): the reference forms it
 // from its running variables, last set at an earlier site in row-major order
-__global__ __launch_bounds__(1024) void k_landau_carry(const EvDesc* __restrict__ evs, int nev, int it, int all) {
-    const EvDesc& ev = evs[blockIdx.x];
+__global__ __launch_bounds__(IPG_BLOCK) void k_landau_carry(const EvDesc* __restrict__ evs, int nev, int it,
+                                                           int all) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
     if (!ev.active || (!all && it != ev.itmax)) return;
-    __shared__ int ce[1024], cu[1024];
+    const long long s = (long long)xcd_remap(blockIdx.x - ev.block0, ev.nblocks) * IPG_BLOCK + threadIdx.x;
     const long long S = ev.S;
-    const long long chunk = (S + 1023) / 1024;
-    const long long b = threadIdx.x * chunk, en = min(S, b + chunk);
-    int le = -1, lu = -1;
-    for (long long s = b; s < en; s++) {
-        const int f = ev.land_flag[s];
-        if (f & 1) le = (int)s;
-        if (f & 2) lu = (int)s;
-    }
-    ce[threadIdx.x] = le;
-    cu[threadIdx.x] = lu;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int ve = threadIdx.x >= o ? ce[threadIdx.x - o] : -1;
-        const int vu = threadIdx.x >= o ? cu[threadIdx.x - o] : -1;
-        __syncthreads();
-        ce[threadIdx.x] = max(ce[threadIdx.x], ve);
-        cu[threadIdx.x] = max(cu[threadIdx.x], vu);
-        __syncthreads();
-    }
-    int re = threadIdx.x > 0 ? ce[threadIdx.x - 1] : -1;
-    int ru = threadIdx.x > 0 ? cu[threadIdx.x - 1] : -1;
-    for (long long s = b; s < en; s++) {
-        const int f = ev.land_flag[s];
-        if (f & 1) re = (int)s;
-        if (f & 2) ru = (int)s;
-        if ((f & 3) == 3) continue;
-        const double eps = re >= 0 ? ev.land_epsv[re] : 0.;
-        double u[4] = {0., 0., 0., 0.};
-        if (ru >= 0)
-            for (int k = 0; k < 4; k++) u[k] = ev.land[(1 + k) * S + ru];
-        double T[10], pi[10];
-        load_T(ev, s, T);
-        ipgl::landau_pi(T, eps, u[0], u[1], u[2], u[3], it, ev.dtau, ev.a, pi);
-        for (int k = 0; k < 10; k++) ev.land[(5 + k) * S + s] = pi[k];
-    }
+    if (s >= S) return;
+    if ((ev.land_flag[s] & 3) == 3) return;
+    const long long re = last_set(ev, s, 1, kSlotEps), ru = last_set(ev, s, 2, kSlotU);
+    const double eps = re >= 0 ? ev.land_epsv[re] : 0.;
+    double u[4] = {0., 0., 0., 0.};
+    if (ru >= 0)
+        for (int k = 0; k < 4; k++) u[k] = ev.land[(1 + k) * S + ru];
+    double T[10], pi[10];
+    load_T(ev, s, T);
+    ipgl::landau_pi(T, eps, u[0], u[1], u[2], u[3], it, ev.dtau, ev.a, pi);
+    for (int k = 0; k < 10; k++) ev.land[(5 + k) * S + s] = pi[k];
 }
 
 // per-event sum of partial slot k over the event's (unpadded) blocks
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_ecc_finish(const EvDesc* __restri
 
 void ipg_launch_landau(const EvDesc* d, int nev, int nblk, int it, int all, hipStream_t st) {
     hipLaunchKernelGGL(k_landau, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
-    hipLaunchKernelGGL(k_landau_carry, dim3(nev), dim3(1024), 0, st, d, nev, it, all);
+    hipLaunchKernelGGL(k_landau_carry, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
     hipLaunchKernelGGL(k_ecc_mid, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
     hipLaunchKernelGGL(k_ecc, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
     hipLaunchKernelGGL(k_ecc_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);

@@ -87,15 +87,21 @@ def roofline(I, p):
 
 
 def copy_probe(mib):
-    """One plain device-to-device copy kernel of known size: what FETCH_SIZE
-    and WRITE_SIZE read for it scales the counters of the other kernels."""
-    import torch
-    n = mib * (1 << 20) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
+    """One plain device-to-device copy of known size (the runtime's copy
+    kernel): what FETCH_SIZE and WRITE_SIZE read for it scales the counters
+    of the other kernels."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, n)
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
     print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 

@@ -29,6 +29,7 @@
 
 #include "host.hpp"
 #include "ipg_internal.hpp"
+#include "fft_lds.hpp"
 
 namespace {
 
@@ -151,8 +152,8 @@ struct Engine {
             const ipg_params& p = E.p;
             if (p.Nc != 3) return fail(IPG_ERR_ARG, "only Nc = 3 is supported");
             const int N = p.size;
-            if (N < 4 || (N & (N - 1)) != 0 || N > 1024)
-                return fail(IPG_ERR_ARG, "lattice size must be a power of two in [4, 1024]");
+            if (!ipgfft::supported(N))
+                return fail(IPG_ERR_ARG, "lattice size must be 2^a or 3 * 2^a in [16, 1024]");
             if (p.useNucleus != 1 || p.useSmoothNucleus != 0 || p.usePseudoRapidity != 0 || p.runningCoupling != 0 ||
                 p.readInitialWilsonLines != 0 || p.useFixedNpart != 0)
                 return fail(IPG_ERR_ARG, "option outside the supported hot path (useNucleus=1, useSmoothNucleus=0, "
@@ -245,9 +246,9 @@ struct Engine {
                 Plan pl;
                 pl.N = N;
                 pl.logN = log2i(N);
-                pl.C = std::max(1, std::min(N, 4096 / N));
-                std::vector<double2> tw(N / 2);
-                for (int m = 0; m < N / 2; m++) {
+                pl.C = ipgfft::cols_per_block(N);
+                std::vector<double2> tw(N);
+                for (int m = 0; m < N; m++) {
                     double ang = -2.0 * M_PI * (double)m / (double)N;
                     tw[m] = make_double2(std::cos(ang), std::sin(ang));
                 }

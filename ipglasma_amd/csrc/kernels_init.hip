@@ -13,6 +13,7 @@
 //   k_pi_init     pi = -(2i/g) A^eta (Init.cpp:2521-2584, 2664-2680)
 #include "ipg_device.hpp"
 #include "ipg_internal.hpp"
+#include "fft_lds.hpp"
 
 namespace {
 
@@ -64,14 +65,20 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tp(const EvDesc* __restrict__ evs
     const int ix = (int)(s / N), iy = (int)(s - (long long)ix * N);
     const double x = -ev.L / 2. + ev.a * ix, y = -ev.L / 2. + ev.a * iy;
     const double xi = ev.xi, BG = ev.BG;
+    const double pre = sqrt(1 + xi), nrm = 2. * M_PI * BG;
     double tA = 0., tB = 0.;
     for (int n = 0; n < ev.A1 + ev.A2; n++) {
-        const double xm = ev.nuc[4 * n], ym = ev.nuc[4 * n + 1], ph = ev.nuc[4 * n + 2];
+        const double xm = ev.nuc[4 * n], ym = ev.nuc[4 * n + 1];
         double dx = xm - x, dy = ym - y;
-        double pr = dx * cos(ph) + dy * sin(ph);
-        double bp2 = dx * dx + dy * dy + xi * (pr * pr);
+        double bp2 = dx * dx + dy * dy;
+        // the anisotropy term xi (b.n)^2 is exactly +0 for xi = 0
+        if (xi != 0.) {
+            const double ph = ev.nuc[4 * n + 2];
+            const double pr = dx * cos(ph) + dy * sin(ph);
+            bp2 = bp2 + xi * (pr * pr);
+        }
         bp2 /= HBARC * HBARC;
-        double T = sqrt(1 + xi) * exp(-bp2 / (2. * BG)) / (2. * M_PI * BG);
+        double T = pre * exp(-bp2 / (2. * BG)) / nrm;
         if (n < ev.A1)
             tA = tA + T;
         else
@@ -360,62 +367,17 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_stats_finish(const EvDesc* __rest
 }
 
 // ---- setV: LDS-staged 2D FFT ------------------------------------------------------
-__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cmulc2(double2 a, double2 b) {  // a * conj(b)
-    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
-__device__ __forceinline__ unsigned bitrev(unsigned v, int logN) { return __brev(v) >> (32 - logN); }
-
-// decimation in time on `nrow` rows of length N held in LDS (bit-reversed in,
-// natural out), forward sign exp(-i...)
-__device__ void lds_fft_dit(double2* x, int nrow, int N, int logN, const double2* __restrict__ tw) {
-    const int nb = nrow * (N >> 1);
-    for (int lg = 1; lg <= logN; lg++) {
-        const int half = 1 << (lg - 1);
-        const int str = N >> lg;  // twiddle stride
-        for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-            const int r = q / (N >> 1), bi = q - r * (N >> 1);
-            const int grp = bi >> (lg - 1), k2 = bi & (half - 1);
-            const int i0 = r * N + (grp << lg) + k2, i1 = i0 + half;
-            double2 w = tw[k2 * str];
-            double2 t = cmul2(x[i1], w);
-            double2 a = x[i0];
-            x[i0] = make_double2(a.x + t.x, a.y + t.y);
-            x[i1] = make_double2(a.x - t.x, a.y - t.y);
-        }
-        __syncthreads();
-    }
-}
-// decimation in frequency, inverse sign exp(+i...) (natural in, bit-reversed out)
-__device__ void lds_ifft_dif(double2* x, int nrow, int N, int logN, const double2* __restrict__ tw) {
-    const int nb = nrow * (N >> 1);
-    for (int lg = logN; lg >= 1; lg--) {
-        const int half = 1 << (lg - 1);
-        const int str = N >> lg;
-        for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-            const int r = q / (N >> 1), bi = q - r * (N >> 1);
-            const int grp = bi >> (lg - 1), k2 = bi & (half - 1);
-            const int i0 = r * N + (grp << lg) + k2, i1 = i0 + half;
-            double2 a = x[i0], b = x[i1];
-            x[i0] = make_double2(a.x + b.x, a.y + b.y);
-            x[i1] = cmulc2(make_double2(a.x - b.x, a.y - b.y), tw[k2 * str]);
-        }
-        __syncthreads();
-    }
-}
-
 // A launch covers a chunk of kc slices [k0, k0+kc) of every event of one
-// lattice size.  fftbuf holds [nucleus][slice in chunk][4 packed pairs][S]
-// complex values.
+// lattice size N (a template argument: the transform plan is compile-time).
+// fftbuf holds [nucleus][slice in chunk][4 packed pairs][S] complex values,
+// rows in natural order.
 
 // rows forward: generate rho for row i of (nucleus n, slice k), 4 packed
-// complex rows, forward FFT along y
+// complex rows, forward transform along y
+template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
                                                             int k0, int kc) {
-    extern __shared__ double2 lds[];
-    const int N = pl.N;
+    __shared__ double2 lds[4 * N];
     const int per_ev = 2 * kc * N;
     const int e = blockIdx.x / per_ev;
     const int rem = blockIdx.x - e * per_ev;
@@ -438,12 +400,11 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
             normals8(ev.rho_seed, (uint32_t)k, (uint32_t)n, (uint32_t)s, z);
         }
         const double amp = ev.g * sqrt(g2[s] / (double)ev.Ny);
-        const unsigned jr = bitrev(j, pl.logN);
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + jr] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[p * N + j] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
     }
     __syncthreads();
-    lds_fft_dit(lds, 4, N, pl.logN, pl.tw);
+    ipgfft::lds_fft<N, 4, IPG_BLOCK, false>(lds, pl.tw);
     double2* out = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
@@ -451,12 +412,14 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
     }
 }
 
-// columns: forward FFT, multiply by 1/(k^2+m^2) e^{-|k| UVdamp} / N^2, inverse
+// columns: forward transform, multiply by 1/(k^2+m^2) e^{-|k| UVdamp} / N^2,
+// inverse transform; C columns per block
+template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                         int kc) {
-    extern __shared__ double2 lds[];
-    const int N = pl.N, C = pl.C;
-    const int per_pack = N / C;
+    constexpr int C = ipgfft::cols_per_block(N);
+    __shared__ double2 lds[C * N];
+    constexpr int per_pack = N / C;
     const int per_ev = 2 * kc * 4 * per_pack;
     const int e = blockIdx.x / per_ev;
     const int rem = blockIdx.x - e * per_ev;
@@ -468,13 +431,13 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
     if (!ev.active || k >= ev.Ny) return;
     const size_t S = ev.S;
     double2* buf = ev.fftbuf + ((size_t)nk * 4 + p) * S;
-    // load C columns: lds[c * N + bitrev(r)]
+    // load C columns: lds[c * N + r]
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
-        lds[c * N + bitrev(r, pl.logN)] = buf[(size_t)r * N + j0 + c];
+        lds[c * N + r] = buf[(size_t)r * N + j0 + c];
     }
     __syncthreads();
-    lds_fft_dit(lds, C, N, pl.logN, pl.tw);
+    ipgfft::lds_fft<N, C, IPG_BLOCK, false>(lds, pl.tw);
     const double m2 = ev.m_lat * ev.m_lat, uv = ev.uvdamp_lat;
     const double invN2 = 1.0 / ((double)N * (double)N);
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
@@ -489,20 +452,21 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
         lds[q] = make_double2(lds[q].x * f, lds[q].y * f);
     }
     __syncthreads();
-    lds_ifft_dif(lds, C, N, pl.logN, pl.tw);
+    ipgfft::lds_fft<N, C, IPG_BLOCK, true>(lds, pl.tw);
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
-        buf[(size_t)r * N + j0 + c] = lds[c * N + bitrev(r, pl.logN)];
+        buf[(size_t)r * N + j0 + c] = lds[c * N + r];
     }
 }
 
 // rows inverse for every slice of the chunk in order; each thread keeps the
 // Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V
-template <int SPT>
-__global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+constexpr int kInvBlock = 512;
+template <int N>
+__global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                             int kc) {
-    extern __shared__ double2 lds[];
-    const int N = pl.N;
+    constexpr int SPT = (N + kInvBlock - 1) / kInvBlock;
+    __shared__ double2 lds[4 * N];
     const int e = blockIdx.x / (2 * N);
     const int n = (blockIdx.x / N) & 1;
     const int i = blockIdx.x % N;
@@ -513,7 +477,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __res
     M3 V[SPT];
 #pragma unroll
     for (int t = 0; t < SPT; t++) {
-        const int j = threadIdx.x + t * IPG_BLOCK;
+        const int j = threadIdx.x + t * kInvBlock;
         V[t] = j < N ? ldm(Vg, S, (size_t)i * N + j) : m_one();
     }
     const int kend = min(kc, ev.Ny - k0);
@@ -524,16 +488,15 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __res
             lds[q] = in[(size_t)p * S + (size_t)i * N + j];
         }
         __syncthreads();
-        lds_ifft_dif(lds, 4, N, pl.logN, pl.tw);
+        ipgfft::lds_fft<N, 4, kInvBlock, true>(lds, pl.tw);
 #pragma unroll
         for (int t = 0; t < SPT; t++) {
-            const int j = threadIdx.x + t * IPG_BLOCK;
+            const int j = threadIdx.x + t * kInvBlock;
             if (j < N) {
-                const unsigned jr = bitrev(j, pl.logN);
                 double Q[8];
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
-                    double2 v = lds[p * N + jr];
+                    double2 v = lds[p * N + j];
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
@@ -544,7 +507,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_inv(const EvDesc* __res
     }
 #pragma unroll
     for (int t = 0; t < SPT; t++) {
-        const int j = threadIdx.x + t * IPG_BLOCK;
+        const int j = threadIdx.x + t * kInvBlock;
         if (j < N) stm(Vg, S, (size_t)i * N + j, V[t]);
     }
 }
@@ -867,21 +830,32 @@ void ipg_launch_stats(const EvDesc* d, int nev, int nblk, double* out, hipStream
     hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
     hipLaunchKernelGGL(k_stats_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, out);
 }
+template <int N>
+static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, hipStream_t st) {
+    constexpr int C = ipgfft::cols_per_block(N);
+    const int nrow_fwd = nev * 2 * kc * N;
+    const int ncol = nev * 2 * kc * 4 * (N / C);
+    const int nrow_inv = nev * 2 * N;
+    hipLaunchKernelGGL(k_setv_rows_fwd<N>, dim3(nrow_fwd), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_setv_rows_inv<N>, dim3(nrow_inv), dim3(kInvBlock), 0, st, d, pl, k0, kc);
+}
 void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, hipStream_t st) {
-    const size_t lrow = sizeof(double2) * 4 * pl.N;
-    const size_t lcol = sizeof(double2) * pl.C * pl.N;
-    const int nrow_fwd = nev * 2 * kc * pl.N;
-    const int ncol = nev * 2 * kc * 4 * (pl.N / pl.C);
-    const int nrow_inv = nev * 2 * pl.N;
-    hipLaunchKernelGGL(k_setv_rows_fwd, dim3(nrow_fwd), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
-    hipLaunchKernelGGL(k_setv_cols, dim3(ncol), dim3(IPG_BLOCK), lcol, st, d, pl, k0, kc);
-    const int spt = (pl.N + IPG_BLOCK - 1) / IPG_BLOCK;
-    if (spt <= 1)
-        hipLaunchKernelGGL(k_setv_rows_inv<1>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
-    else if (spt == 2)
-        hipLaunchKernelGGL(k_setv_rows_inv<2>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
-    else
-        hipLaunchKernelGGL(k_setv_rows_inv<4>, dim3(nrow_inv), dim3(IPG_BLOCK), lrow, st, d, pl, k0, kc);
+    switch (pl.N) {
+        case 16: setv_chunk<16>(d, nev, pl, k0, kc, st); break;
+        case 32: setv_chunk<32>(d, nev, pl, k0, kc, st); break;
+        case 48: setv_chunk<48>(d, nev, pl, k0, kc, st); break;
+        case 64: setv_chunk<64>(d, nev, pl, k0, kc, st); break;
+        case 96: setv_chunk<96>(d, nev, pl, k0, kc, st); break;
+        case 128: setv_chunk<128>(d, nev, pl, k0, kc, st); break;
+        case 192: setv_chunk<192>(d, nev, pl, k0, kc, st); break;
+        case 256: setv_chunk<256>(d, nev, pl, k0, kc, st); break;
+        case 384: setv_chunk<384>(d, nev, pl, k0, kc, st); break;
+        case 512: setv_chunk<512>(d, nev, pl, k0, kc, st); break;
+        case 768: setv_chunk<768>(d, nev, pl, k0, kc, st); break;
+        case 1024: setv_chunk<1024>(d, nev, pl, k0, kc, st); break;
+        default: break;  // rejected by the engine set-up
+    }
 }
 void ipg_launch_unit_v(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_unit_v, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);

@@ -261,11 +261,13 @@ IPG_HD void expm_coeff(const double Q[8], cd out[9]) {
     const double x3 = c1 / 3.0;
     double c0max = fmax(1e-15, 2.0 * (x3 * sqrt(x3)));
     double th3 = acos(c0 / c0max) / 3.0;
-    double u = sqrt(c1 / 3.0) * cos(th3);
-    double w = sqrt(c1) * sin(th3);
-    double xi0 = sin(w) / w;
+    double sth, cth, sw, cw;
+    sincos(th3, &sth, &cth);
+    double u = sqrt(c1 / 3.0) * cth;
+    double w = sqrt(c1) * sth;
+    sincos(w, &sw, &cw);
+    double xi0 = sw / w;
     double den = 9.0 * u * u - w * w;
-    double cw = cos(w);
     double s2u, c2u, su, cu;
     sincos(2.0 * u, &s2u, &c2u);
     sincos(u, &su, &cu);

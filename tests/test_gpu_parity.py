@@ -96,7 +96,7 @@ def _init_to_fields(p, seed):
     return ev
 
 
-@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2)])
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2), ("AuAu", 48, 5), ("PbPb", 96, 7)])
 def test_wilson_lines(system, N, seed):
     p = K.params(system, N=N, Ny=3)
     ev = _init_to_fields(p, seed)
@@ -176,7 +176,7 @@ def _tmunu_check(Tg, To):
         assert err < 1e-6, (c, err)
 
 
-@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2)])
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("pPb", 32, 2), ("AuAu", 48, 5)])
 def test_evolution_tmunu_dedeta(system, N, seed):
     p = K.params(system, N=N, Ny=3)
     ev = _init_to_fields(p, seed)

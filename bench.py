@@ -67,6 +67,19 @@ def run_batched(I, p, seeds, use_graph=True):
     return time.perf_counter() - t0, out
 
 
+def measured_traffic(N):
+    """HBM bytes per k_step launch from the committed FETCH_SIZE/WRITE_SIZE
+    counter passes (profiles/k_step_pbpb_n512_hbm_traffic.md); None for
+    other sizes."""
+    f = os.path.join(ROOT, "profiles", "k_step_pbpb_n512_hbm_traffic.md")
+    if N != 512 or not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def roofline(I, p):
     ev = I.Event(p, seed=12345)
     while not ev.init():
@@ -81,7 +94,7 @@ def roofline(I, p):
     avg_ms = t["step_ms_total"] / max(1, t["nsteps"])
     achieved = BYTES_PER_SITE_STEP * S / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(p.size),
             "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5), "launches": t["nsteps"],
             "bytes_per_launch": BYTES_PER_SITE_STEP * S}
 

@@ -42,10 +42,24 @@ def pbpb_params(I, N=512, L=30.0, Ny=50, seed=1):
     return p
 
 
+def hetero_params(I):
+    """The heterogeneous batch: 16 p+Pb (N=256, L=10 fm, b in [0,5] fm),
+    16 O+O (N=256, L=15 fm, Woods-Saxon R=2.61 fm, a=0.513 fm), 24 Pb+Pb
+    (N=512, L=30 fm, b in [0,12] fm), 8 Pb+Pb (N=768, L=30 fm); Ny=50,
+    tau_max=0.4 fm/c, FP64, one seed per event."""
+    def mk(N, L, pr, tg, R, a, bmax):
+        p = pbpb_params(I, N=N, L=L)
+        p.set(Projectile=pr, Target=tg, R_WS=R, a_WS=a, bmin=0.0, bmax=bmax, samplebFromLinearDistribution=0)
+        return p
+    return ([mk(256, 10.0, "p", "Pb", 6.62, 0.546, 5.0)] * 16 + [mk(256, 15.0, "O", "O", 2.61, 0.513, 0.0)] * 16 +
+            [mk(512, 30.0, "Pb", "Pb", 6.62, 0.546, 12.0)] * 24 + [mk(768, 30.0, "Pb", "Pb", 6.62, 0.546, 0.0)] * 8)
+
+
 def run_plain(I, p, seeds):
     t0 = time.perf_counter()
     out = []
-    for s in seeds:
+    ps = p if isinstance(p, list) else [p] * len(seeds)
+    for p, s in zip(ps, seeds):
         ev = I.Event(p, seed=s)
         tries = 0
         while not ev.init():
@@ -60,7 +74,7 @@ def run_plain(I, p, seeds):
 
 def run_batched(I, p, seeds, use_graph=True):
     t0 = time.perf_counter()
-    b = I.Batch([p] * len(seeds), seeds)
+    b = I.Batch(p if isinstance(p, list) else [p] * len(seeds), seeds)
     b.run(use_graph=use_graph)
     out = [b.observables(e) for e in range(len(seeds))]
     b.close()
@@ -187,6 +201,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hetero", action="store_true",
+                    help="skip the heterogeneous 64-event batch (N=256/512/768)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -222,6 +238,21 @@ def main():
     ev_p = a.batch * a.steps / tp
     # batched and plain must agree event by event
     maxrel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(obs_b, obs_p))
+    hetero = None
+    if not a.no_hetero:
+        hp = hetero_params(I)
+        hs = list(range(5000, 5000 + len(hp)))
+        run_batched(I, hp, hs)
+        dtb, ob = run_batched(I, hp, hs)
+        print(f"[bench] heterogeneous batched: {dtb:.3f} s for {len(hp)} events", file=sys.stderr, flush=True)
+        dtp, op = run_plain(I, hp, hs)
+        print(f"[bench] heterogeneous plain: {dtp:.3f} s for {len(hp)} events", file=sys.stderr, flush=True)
+        hrel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(ob, op))
+        hetero = {"workload": "16 p+Pb N=256 L=10, 16 O+O N=256 L=15, 24 Pb+Pb N=512 L=30 b<12, "
+                              "8 Pb+Pb N=768 L=30; Ny=50, tau_max=0.4 fm/c",
+                  "events": len(hp), "batched_events_per_s": round(len(hp) / dtb, 3),
+                  "plain_events_per_s": round(len(hp) / dtp, 3), "batched_over_plain": round(dtp / dtb, 3),
+                  "batched_vs_plain_max_rel_dEdeta": hrel}
     rf = roofline(I, p)
     line = {
         "metric": "events/s: batched vs per-event launch, N=512 FP64, dE/deta rel err<=1e-9",
@@ -235,6 +266,7 @@ def main():
         "batched_vs_plain_max_rel_dEdeta": maxrel,
         "dEdeta_GeV": obs_b[0]["dEdeta"],
         "roofline": rf,
+        "heterogeneous_batch": hetero,
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p)

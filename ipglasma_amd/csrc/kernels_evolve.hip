@@ -223,7 +223,9 @@ __device__ __forceinline__ M3 tlquad(const M3& P1, const M3& P2, const M3& P3, c
 // barrier, so that only the matrices of one term are live at a time.
 #define IPG_FENCE() asm volatile("" ::: "memory")
 
-__global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ evs, int nev, int it) {
+// T^{mu nu} in three passes (register pressure): diagonal components,
+// T^{tau eta} and T^{xy}; the block sums of T^tautau
+__global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_a(const EvDesc* __restrict__ evs, int nev, int it) {
     const int e = find_event(evs, nev, blockIdx.x);
     const EvDesc& ev = evs[e];
     if (!ev.active || it != ev.itmax) return;
@@ -342,10 +344,47 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
             T[7 * S + s] = (2. / tau / tau * (0.25 * b)) / a4;
         }
         IPG_FENCE();
+    }
+    double bs = block_sum(ttsum);
+    if (threadIdx.x == 0) ev.partial[lb] = bs;
+}
+
+// clover parts of T^{tau x}, T^{tau y}, T^{x eta}, T^{y eta}
+__global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_b(const EvDesc* __restrict__ evs, int nev, int it) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active || it != ev.itmax) return;
+    const int lb = xcd_remap(blockIdx.x - ev.block0, ev.nblocks);
+    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
+    if (s < ev.S) {
+        const size_t S = ev.S;
+        const int N = ev.N;
+        const int p = (it - 1) & 1;
+        const double* Ux = ev.U[p][0];
+        const double* Uy = ev.U[p][1];
+        const double* phi = ev.phi[p];
+        const double* E1 = ev.E1;
+        const double* E2 = ev.E2;
+        const double* PI = ev.pi;
+        const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+        const int i = (int)(s / N), j = (int)(s - (s / N) * N);
+        auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
+        const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
+        const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
+        const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
+        double* T = ev.T;
+        const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+        // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
+        // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
+        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
+        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
+        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
+        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
+
         // T^{tau x}, T^{x eta}, T^{tau y}, T^{y eta} (Evolution.cpp:1047-1234).
         // The clover combinations enter with opposite signs, e.g. the T^{x eta}
         // clover (Pb - Pa)_TL + (Pd - Pc)_TL is -X1.
-        double ttx_im, ttx_re, tty_im, tty_re, xe_im, ye_im;
+        double ttx_im, tty_im, xe_im, ye_im;
         {
             // X1 = (Pa - Pb)_TL + (Pc - Pd)_TL,
             // Pa = Uy Ux(+y) Uy(+x)^+ Ux^+, Pb = Ux Uy(+x) Ux(+y)^+ Uy^+
@@ -398,8 +437,50 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
             tty_im += imtrprod(ldh(E1, S, pY), Y);
             ye_im -= imtrprod(Y, gXY1());
         }
-        IPG_FENCE();
+        // raw clover sums, finished by k_tmunu_c
+        T[4 * S + s] = ttx_im;
+        T[5 * S + s] = tty_im;
+        T[8 * S + s] = xe_im;
+        T[9 * S + s] = ye_im;
+    }
+}
+
+// pi grad-phi and E pi parts; finishes the four flux components
+__global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_c(const EvDesc* __restrict__ evs, int nev, int it) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active || it != ev.itmax) return;
+    const int lb = xcd_remap(blockIdx.x - ev.block0, ev.nblocks);
+    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
+    if (s < ev.S) {
+        const size_t S = ev.S;
+        const int N = ev.N;
+        const int p = (it - 1) & 1;
+        const double* Ux = ev.U[p][0];
+        const double* Uy = ev.U[p][1];
+        const double* phi = ev.phi[p];
+        const double* E1 = ev.E1;
+        const double* E2 = ev.E2;
+        const double* PI = ev.pi;
+        const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+        const int i = (int)(s / N), j = (int)(s - (s / N) * N);
+        auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
+        const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
+        const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
+        const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
+        double* T = ev.T;
+        const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+        // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
+        // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
+        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
+        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
+        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
+        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
+
         // pi phi-gradient parts of T^{tau x} and T^{tau y}
+        double ttx_re, tty_re;
+        const double ttx_im = T[4 * S + s], tty_im = T[5 * S + s], xe_im = T[8 * S + s], ye_im = T[9 * S + s];
+        IPG_FENCE();
         {
             M3 u = ldm(Ux, S, s), v = ldm(Ux, S, mX);
             ttx_re = retrprod(ldh(PI, S, s), m3(u, ldh(phi, S, pX), dag(u)) - m3(dag(v), ldh(phi, S, mX), v));
@@ -447,11 +528,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu(const EvDesc* __restrict__ 
             T[9 * S + s] = (-2. / tau / tau * (1. / 4. * g * r1 + 1. / 8. / g * ye_im)) / a5;
         }
     }
-    double bs = block_sum(ttsum);
-    if (threadIdx.x == 0) ev.partial[lb] = bs;
 }
 
-// Gauss-law violation max_x 1/2 sum |G|^2 (Evolution.cpp:295-364)
 __global__ __launch_bounds__(IPG_BLOCK) void k_gauss(const EvDesc* __restrict__ evs, int nev, int it) {
     const int e = find_event(evs, nev, blockIdx.x);
     const EvDesc& ev = evs[e];
@@ -508,7 +586,9 @@ void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t
     hipLaunchKernelGGL(k_step, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);
 }
 void ipg_launch_tmunu(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st) {
-    hipLaunchKernelGGL(k_tmunu, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);
+    hipLaunchKernelGGL(k_tmunu_a, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);
+    hipLaunchKernelGGL(k_tmunu_b, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);
+    hipLaunchKernelGGL(k_tmunu_c, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);
 }
 void ipg_launch_gauss(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st) {
     hipLaunchKernelGGL(k_gauss, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);

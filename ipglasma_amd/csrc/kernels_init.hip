@@ -577,152 +577,54 @@ __device__ __forceinline__ double residual(const M3& W, const M3& Us, const cd t
     return Fz;
 }
 
-// One Newton iteration of one link, Init.cpp:3386-3476 (the body of the
-// reference's while loop); returns the loop condition for the next round.
-struct NewtonLink {
-    M3 U0, W, Us;
-    double alpha[8], best_alpha[8];
-    double Fzero, Fmin;
-    int iter, nRestart, total, analytic, best_one;
-    uint32_t ndraw;
-};
+// Root finder, one link per group of NG = 8 lanes (Init.cpp:3386-3476).
+// Lane c of a group evaluates Jacobian column c (one SU(3) exponential) and
+// owns row c of the 8x9 augmented system during the pivoted elimination, so
+// an iteration costs one exponential of latency instead of nine.  Every lane
+// of a group carries the same link state and computes the residual, the
+// update and the next iterate redundantly; the per-element arithmetic (and
+// its order) is that of the serial algorithm.  A group takes its next link
+// from a wave-aggregated work counter as soon as its link is done.
+constexpr int NG = 8;
 
-__device__ __forceinline__ bool newton_iteration(NewtonLink& k, double* J, uint64_t seed, uint32_t link) {
-#define JA(a, b) J[((a) * 9 + (b)) * NWB]
-    const int maxIt = 2000, maxRe = 200;
-    k.iter++;
-    k.total++;
-    cd trc[8];
-    tr_gen(k.W - dag(k.W), trc);
-    double F[8];
-    k.Fzero = residual(k.W, k.Us, trc, F);
+__device__ __forceinline__ double pick8(const double (&v)[8], int c) {
+    double r = v[0];
 #pragma unroll
-    for (int a = 0; a < 8; a++) JA(a, 8) = F[a];
-    const M3 Ud = dag(k.Us);
-    bool good = true;
-    // forward-difference Jacobian, column by column
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const double d = fmax(0.001, fmin(10., 0.01 * fabs(k.alpha[b])));
-        const double ab = k.alpha[b];
-        k.alpha[b] = ab + d;
-        M3 Mp = k.W * (dag(exp_iQt(k.alpha) * k.U0) - Ud);
-        cd t[8];
-        tr_gen(Mp, t);
-        double chk = 0.;
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            double v = 2. * t[a].i / d;
-            JA(a, b) = v;
-            chk += fabs(v);
-            if (chk < 1e-15) good = false;
-        }
-        // the reference adds and then subtracts the step in place
-        k.alpha[b] = k.alpha[b] - d;
-    }
-    if (!good) {
-        k.analytic = 1;
-        for (int b = 0; b < 8; b++) {
-            M3 Mb = gen_mul(b, Ud);
-            for (int a = 0; a < 8; a++) JA(a, b) = -2. * retrprod(gen_mul(a, k.W), Mb);
-        }
-    }
-    // Gaussian elimination with partial pivoting (first maximal pivot); row
-    // swaps carry the right-hand side along
-    for (int j = 0; j < 7; j++) {
-        double mx = fabs(JA(j, j));
-        int ip = j;
-        for (int r = j + 1; r < 8; r++) {
-            double v = fabs(JA(r, j));
-            if (v > mx) {
-                mx = v;
-                ip = r;
-            }
-        }
-        if (ip != j) {
-            for (int c = 0; c < 9; c++) {
-                double t = JA(j, c);
-                JA(j, c) = JA(ip, c);
-                JA(ip, c) = t;
-            }
-        }
-        double ajj = JA(j, j);
-        if (ajj != 0.0) {
-            for (int r = j + 1; r < 8; r++) {
-                double arj = JA(r, j) / ajj;
-                JA(r, j) = arj;
-                for (int c = j + 1; c < 8; c++) JA(r, c) = JA(r, c) - arj * JA(j, c);
-            }
-        }
-    }
-    // unit-lower forward, upper back substitution on column 8
-    for (int r = 1; r < 8; r++) {
-        double t = JA(r, 8);
-        for (int c = 0; c < r; c++) t -= JA(r, c) * JA(c, 8);
-        JA(r, 8) = t;
-    }
-    JA(7, 8) = JA(7, 8) / JA(7, 7);
-    for (int r = 6; r >= 0; r--) {
-        double t = JA(r, 8);
-        for (int c = r + 1; c < 8; c++) t -= JA(r, c) * JA(c, 8);
-        JA(r, 8) = t / JA(r, r);
-    }
-    double y[8];
-    bool ok = true;
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        y[a] = JA(a, 8);
-        if (fabs(y[a]) > 1e5) ok = false;
-    }
-#undef JA
-    if (ok) {
-#pragma unroll
-        for (int a = 0; a < 8; a++) k.alpha[a] = k.alpha[a] + y[a];
-    } else {
-        restart_alpha(seed, link, k.ndraw++, k.alpha);
-    }
-    k.Us = exp_iQt(k.alpha) * k.U0;
-    if (k.Fzero < k.Fmin) {
-        // best = Us, kept as its exponent (Us is exp(i alpha t) U0)
-        k.Fmin = k.Fzero;
-#pragma unroll
-        for (int a = 0; a < 8; a++) k.best_alpha[a] = k.alpha[a];
-        k.best_one = 0;
-    }
-    if (k.iter == maxIt) {
-        restart_alpha(seed, link, k.ndraw++, k.alpha);
-        k.Us = exp_iQt(k.alpha) * k.U0;
-        k.nRestart++;
-        k.iter = 0;
-    }
-    return k.Fzero > 1e-6 && k.iter < maxIt && k.nRestart < maxRe;
+    for (int k = 1; k < 8; k++)
+        if (k == c) r = v[k];
+    return r;
 }
 
-// Persistent root finder: the iteration count per link is heavy-tailed
-// (median 2, p99 ~400, restarts after 2000), so each lane takes a new link
-// from a wave-aggregated work counter as soon as its link is done instead of
-// waiting for the slowest lane of a fixed wavefront of links.  Work items are
-// the link slots of all events of the launch (lblock0 * NWB + l).
 __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, int nev, int nitems,
                                                 int* __restrict__ counter) {
-    // augmented system [J | F] of every lane, column-interleaved across lanes
-    __shared__ double Jl[72 * NWB];
-    double* J = Jl + threadIdx.x;
+    __shared__ double Jt[NWB / NG][8][8];  // [group][row a][column c]
     const int lane = threadIdx.x;
-    int state = 0;  // 0: needs a link, 1: iterating, 2: no work left
+    const int c = lane & (NG - 1);
+    const int grp = lane / NG;
+    const int maxIt = 2000, maxRe = 200;
+    int state = 0;  // group-uniform: 0 needs a link, 1 iterating, 2 no work left
     int e = 0, dir = 0;
     long long l = 0, s = 0;
     uint32_t link = 0;
-    NewtonLink k;
+    M3 U0 = m_one(), W = m_one(), Us = m_one();
+    double alpha[8], best_alpha[8], trci[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) alpha[a] = best_alpha[a] = trci[a] = 0.;
+    double Fzero = 0., Fmin = 0.;
+    int iter = 0, nRestart = 0, total = 0, analytic = 0, best_one = 1;
+    uint32_t ndraw = 0;
     for (;;) {
-        const unsigned long long need = __ballot(state == 0);
+        // ---- one request per idle group
+        const bool lead_need = state == 0 && c == 0;
+        const unsigned long long need = __ballot(lead_need);
         if (need) {
             const int first = __ffsll((long long)need) - 1;
             int base = 0;
             if (lane == first) base = atomicAdd(counter, __popcll(need));
             base = __shfl(base, first, NWB);
+            int g = lead_need ? base + __popcll(need & ((1ull << lane) - 1ull)) : 0;
+            g = __shfl(g, 0, NG);
             if (state == 0) {
-                const int g = base + __popcll(need & ((1ull << lane) - 1ull));
                 if (g >= nitems) {
                     state = 2;
                 } else {
@@ -736,33 +638,174 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
                         link = (uint32_t)(2 * s + dir);
                         const M3 U1 = ldm(dir ? ev.L1y : ev.L1x, S, s);
                         const M3 U2 = ldm(dir ? ev.L2y : ev.L2x, S, s);
-                        k.U0 = U1 * U2;
-                        k.W = U1 + U2;
+                        U0 = U1 * U2;
+                        W = U1 + U2;
+                        cd trc[8];
+                        tr_gen(W - dag(W), trc);
 #pragma unroll
-                        for (int a = 0; a < 8; a++) k.alpha[a] = 0.;
-                        k.Us = exp_iQt(k.alpha) * k.U0;
-                        k.best_one = 1;
-                        k.Fzero = 10.;
-                        k.Fmin = 1e6;
-                        k.iter = k.nRestart = k.total = k.analytic = 0;
-                        k.ndraw = 0;
+                        for (int a = 0; a < 8; a++) {
+                            trci[a] = trc[a].i;
+                            alpha[a] = 0.;
+                        }
+                        Us = exp_iQt(alpha) * U0;
+                        best_one = 1;
+                        Fzero = 10.;
+                        Fmin = 1e6;
+                        iter = nRestart = total = analytic = 0;
+                        ndraw = 0;
                         state = 1;
                     }
-                    // padding slot or inactive event: fetch again
+                    // padding slot or inactive event: ask again
                 }
             }
         }
         if (__ballot(state != 2) == 0) break;
-        if (state == 1) {
-            const EvDesc& ev = evs[e];
-            if (!newton_iteration(k, J, ev.newton_seed, link)) {
-                const int conv = k.nRestart != 200;
-                M3 out = k.Us;
-                if (!conv) out = k.best_one ? m_one() : exp_iQt(k.best_alpha) * k.U0;
-                stm(ev.U[0][dir], ev.S, s, out);
-                ev.newton_stat[l] = k.total | (k.nRestart << 20) | ((!conv) << 28) | (k.analytic << 29);
-                state = 0;
+
+        // ---- one iteration; every group runs it (uniform control flow for
+        // the barriers), only groups in state 1 keep the result
+        iter++;
+        total++;
+        double F[8];
+        {
+            M3 Mt = mul_bd(W, Us) - mul_bd(Us, W);
+            cd t[8];
+            tr_gen(Mt, t);
+            Fzero = 0.;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                F[a] = -(trci[a] + t[a].i);
+                Fzero += fabs(F[a]);
             }
+        }
+        const M3 Ud = dag(Us);
+        // the serial loop adds the step to alpha[b] and subtracts it again in
+        // place, so column b sees (alpha[k] + d_k) - d_k for k < b
+        double dk[8], ar[8], p[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            dk[k] = fmax(0.001, fmin(10., 0.01 * fabs(alpha[k])));
+            ar[k] = (alpha[k] + dk[k]) - dk[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[k] = k < c ? ar[k] : (k == c ? alpha[k] + dk[k] : alpha[k]);
+        const double d = pick8(dk, c);
+        double col[8];
+        bool good = true;
+        {
+            M3 Mp = W * (dag(exp_iQt(p) * U0) - Ud);
+            cd t[8];
+            tr_gen(Mp, t);
+            double chk = 0.;
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                col[a] = 2. * t[a].i / d;
+                chk += fabs(col[a]);
+                if (chk < 1e-15) good = false;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) alpha[k] = ar[k];
+        const unsigned long long bad = __ballot(!good);
+        if ((bad >> (grp * NG)) & 0xFFull) {
+            analytic = 1;
+            const M3 Mb = gen_mul(c, Ud);
+#pragma unroll
+            for (int a = 0; a < 8; a++) col[a] = -2. * retrprod(gen_mul(a, W), Mb);
+        }
+        // ---- columns -> rows
+#pragma unroll
+        for (int a = 0; a < 8; a++) Jt[grp][a][c] = col[a];
+        __syncthreads();
+        double row[9];
+#pragma unroll
+        for (int k = 0; k < 8; k++) row[k] = Jt[grp][c][k];
+        row[8] = pick8(F, c);
+        __syncthreads();
+        // ---- Gaussian elimination with partial pivoting (first maximal
+        // pivot); row swaps carry the right-hand side along
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            double key = c >= j ? fabs(row[j]) : -1.;
+            int idx = c >= j ? c : NG;
+#pragma unroll
+            for (int m = 1; m < NG; m <<= 1) {
+                const double ok = __shfl_xor(key, m, NG);
+                const int oi = __shfl_xor(idx, m, NG);
+                if (ok > key || (ok == key && oi < idx)) {
+                    key = ok;
+                    idx = oi;
+                }
+            }
+            // no pivot above |a_jj| keeps row j (NaN entries are never taken)
+            const double ajj0 = __shfl(fabs(row[j]), j, NG);
+            const int ip = (idx < NG && key > ajj0) ? idx : j;
+            const int src = c == j ? ip : (c == ip ? j : c);
+#pragma unroll
+            for (int k = 0; k < 9; k++) row[k] = __shfl(row[k], src, NG);
+            double piv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) piv[k] = __shfl(row[k], j, NG);
+            const double ajj = piv[j];
+            if (c > j && ajj != 0.0) {
+                const double arj = row[j] / ajj;
+                row[j] = arj;
+#pragma unroll
+                for (int k = j + 1; k < 8; k++) row[k] = row[k] - arj * piv[k];
+            }
+        }
+        // ---- unit-lower forward, upper back substitution on column 8
+#pragma unroll
+        for (int cc = 0; cc < 7; cc++) {
+            const double yc = __shfl(row[8], cc, NG);
+            if (c > cc) row[8] -= row[cc] * yc;
+        }
+        double y[8];
+        if (c == 7) row[8] = row[8] / row[7];
+        y[7] = __shfl(row[8], 7, NG);
+#pragma unroll
+        for (int r = 6; r >= 0; r--) {
+            if (c == r) {
+                double t = row[8];
+#pragma unroll
+                for (int cc = r + 1; cc < 8; cc++) t -= row[cc] * y[cc];
+                row[8] = t / row[r];
+            }
+            y[r] = __shfl(row[8], r, NG);
+        }
+        bool okstep = true;
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+            if (fabs(y[a]) > 1e5) okstep = false;
+        if (okstep) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) alpha[a] = alpha[a] + y[a];
+        } else {
+            restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
+        }
+        Us = exp_iQt(alpha) * U0;
+        if (Fzero < Fmin) {
+            // best = Us, kept as its exponent (Us is exp(i alpha t) U0)
+            Fmin = Fzero;
+#pragma unroll
+            for (int a = 0; a < 8; a++) best_alpha[a] = alpha[a];
+            best_one = 0;
+        }
+        if (iter == maxIt) {
+            restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
+            Us = exp_iQt(alpha) * U0;
+            nRestart++;
+            iter = 0;
+        }
+        if (state == 1 && !(Fzero > 1e-6 && iter < maxIt && nRestart < maxRe)) {
+            const EvDesc& ev = evs[e];
+            const int conv = nRestart != maxRe;
+            if (c == 0) {
+                M3 out = Us;
+                if (!conv) out = best_one ? m_one() : exp_iQt(best_alpha) * U0;
+                stm(ev.U[0][dir], ev.S, s, out);
+                ev.newton_stat[l] = total | (nRestart << 20) | ((!conv) << 28) | (analytic << 29);
+            }
+            state = 0;
         }
     }
 }

@@ -6,8 +6,11 @@ correctly rounded operations, so the device reproduces the oracle's bit
 pattern for the same T^{mu nu}; the tests require agreement to 1e-13 of the
 component scale and report the exactly equal fraction.  Eccentricities are
 sums in a different order (and with z^n in place of atan/cos/sin): 1e-12
-relative.  End to end (device T^{mu nu} vs oracle T^{mu nu}) eps_2 is held to
-the dE/deta tolerance, 1e-9 relative.
+relative.  End to end (device chain vs oracle chain from the same seeds)
+eps_2 is held to 1e-8 relative: the tau=0 root finder stops at
+sum |F_a| <= 1e-6, so the two chains start from links that agree to ~1e-8
+(the root-finder test allows 1e-7), and eps_2 is a ratio of moments with
+cancellations; dE/deta, a sum of positive terms, stays within 1e-9.
 """
 import numpy as np
 import pytest
@@ -118,7 +121,8 @@ def test_landau_after_evolution_end_to_end():
     O.lib().orc_run(N, p.g, p.L, p.dtau, p.itmax, *[ref[k] for k in O.FIELDS], T.reshape(-1), obs)
     Lref, _, _ = O.landau(N, p.L, p.dtau, p.itmax, T)
     rr = O.eccentricity(N, p.L, Lref[0], Lref[1], gA, gB, p.QsmuRatio, p.QsmuRatio)
-    assert abs(o["eccentricity"][1] - rr[1]) <= 1e-9 * rr[1], (o["eccentricity"][1], rr[1])
+    assert abs(o["eccentricity"][1] - rr[1]) <= 1e-8 * rr[1], (o["eccentricity"][1], rr[1])
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-9 * obs[0], (o["dEdeta"], obs[0])
 
 
 def test_batch_landau_matches_plain():

@@ -267,6 +267,7 @@ def main():
         "dEdeta_GeV": obs_b[0]["dEdeta"],
         "roofline": rf,
         "heterogeneous_batch": hetero,
+        "autotune": I.autotune_report(),
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p)

@@ -144,6 +144,14 @@ int ipg_evolve_step(ipg_ctx *c, int it);
 
 int ipg_get_observables(ipg_ctx *c, ipg_observables *o);
 
+/* Autotuner state: one line per tuned kernel ("key: cand=best time ...
+ * chosen=k").  The variants of a key compute identical results (launch shape
+ * or register budget only); each is timed on the real workload until the
+ * fastest is known (IPG_AUTOTUNE=0 in the environment pins the first one).
+ * Writes at most len bytes (NUL-terminated) and returns the full length.
+ * No counterpart in the reference. */
+int ipg_autotune_report(char *buf, int len);
+
 /* Landau frame after ipg_evolve (MyEigen::flowVelocity4D, MyEigen.cpp:25-321,
  * called from Evolution.cpp:409 at itmax): out[15][S] = eps [fm^-4], u^tau,
  * u^x, u^y, u^eta [fm^-1], pi^{mu nu} in T's order (tautau, xx, yy, etaeta,

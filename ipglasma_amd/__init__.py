@@ -124,6 +124,7 @@ def lib():
             "ipg_set_field": (C.c_int, [vp, C.c_int, dp]),
             "ipg_get_tmunu": (C.c_int, [vp, dp]),
             "ipg_get_landau": (C.c_int, [vp, dp]),
+            "ipg_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
@@ -328,3 +329,19 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def autotune_report():
+    """The autotuner's table: {key: {"times_ms": [...], "chosen": k or None}}"""
+    n = lib().ipg_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().ipg_autotune_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        key, rest = line.split(":", 1)
+        parts = rest.split()
+        times = [float(p.split("=")[1].rstrip("ms")) for p in parts if p[0].isdigit()]
+        chosen = [int(p.split("=")[1]) for p in parts if p.startswith("chosen=")]
+        out[key] = {"times_ms": times, "chosen": chosen[0] if chosen else None}
+    return out
+

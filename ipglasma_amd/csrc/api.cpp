@@ -20,9 +20,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -66,6 +68,79 @@ int log2i(int n) {
     return l;
 }
 
+// ---- autotuner ------------------------------------------------------------------
+// Kernel variants that compute bit-identical results but differ in launch
+// shape or register budget are timed online on the real workload (HIP events
+// on the engine's stream), one candidate per launch until each has been seen
+// twice, then the fastest is kept for the rest of the process.  Keys carry the
+// lattice size.  IPG_AUTOTUNE=0 pins every key to its first candidate.
+struct Tuner {
+    struct Entry {
+        std::vector<float> best;  // best time per candidate (ms), < 0 unseen
+        std::vector<int> seen;
+        int chosen = -1;
+        int next = 0;
+    };
+    std::mutex mu;
+    std::map<std::string, Entry> tab;
+    bool enabled() {
+        const char* v = std::getenv("IPG_AUTOTUNE");
+        return !(v && v[0] == '0');
+    }
+    int pick(const std::string& key, int ncand, bool* measuring) {
+        std::lock_guard<std::mutex> g(mu);
+        Entry& e = tab[key];
+        if (e.best.empty()) {
+            e.best.assign(ncand, -1.f);
+            e.seen.assign(ncand, 0);
+        }
+        if (!enabled()) {
+            *measuring = false;
+            return 0;
+        }
+        *measuring = e.chosen < 0;
+        return e.chosen >= 0 ? e.chosen : e.next;
+    }
+    void report(const std::string& key, int cand, float ms) {
+        std::lock_guard<std::mutex> g(mu);
+        Entry& e = tab[key];
+        if (e.chosen >= 0 || cand >= (int)e.best.size()) return;
+        if (e.best[cand] < 0 || ms < e.best[cand]) e.best[cand] = ms;
+        e.seen[cand]++;
+        e.next = (e.next + 1) % (int)e.best.size();
+        bool done = true;
+        for (int n : e.seen) done = done && n >= 2;
+        if (done) {
+            int b = 0;
+            for (int k = 1; k < (int)e.best.size(); k++)
+                if (e.best[k] < e.best[b]) b = k;
+            e.chosen = b;
+        }
+    }
+    std::string text() {
+        std::lock_guard<std::mutex> g(mu);
+        std::string r;
+        for (auto& kv : tab) {
+            r += kv.first + ":";
+            for (size_t k = 0; k < kv.second.best.size(); k++) {
+                char b[64];
+                std::snprintf(b, sizeof(b), " %zu=%.3fms", k, kv.second.best[k]);
+                r += b;
+            }
+            r += kv.second.chosen >= 0 ? " chosen=" + std::to_string(kv.second.chosen) : std::string(" tuning");
+            r += "\n";
+        }
+        return r;
+    }
+};
+Tuner g_tuner;
+
+struct TimedLaunch {
+    std::string key;
+    int cand;
+    hipEvent_t a, b;
+};
+
 struct Plan {
     int N = 0, logN = 0, C = 0;
     double2* tw = nullptr;
@@ -104,13 +179,14 @@ struct Engine {
     bool have_table = false;
     int* d_flag = nullptr;
     int* d_work = nullptr;    // root-finder work counter
-    double* d_out = nullptr;  // 8 doubles per event
+    double* d_out = nullptr;  // 16 doubles per event (8 results + scratch)
     double* d_obs = nullptr;  // kObs doubles per event (Landau, eccentricities)
     int nblk = 0, nlinkblk = 0;
     bool timing = false;
     std::vector<hipEvent_t> tev;
     double t_steps_ms = 0., t_tmunu_ms = 0.;
     int t_nsteps = 0;
+    std::vector<TimedLaunch> timed;  // tuner measurements awaiting a sync
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_itm = -1;
@@ -135,6 +211,10 @@ struct Engine {
         if (d_out) hipFree(d_out);
         if (d_obs) hipFree(d_obs);
         for (hipEvent_t t : tev) hipEventDestroy(t);
+        for (TimedLaunch& t : timed) {
+            hipEventDestroy(t.a);
+            hipEventDestroy(t.b);
+        }
         if (st) hipStreamDestroy(st);
     }
 
@@ -270,7 +350,7 @@ struct Engine {
         HIPCHK(hipMalloc(&d_desc, sizeof(EvDesc) * nev));
         HIPCHK(hipMalloc(&d_flag, sizeof(int) * nev));
         HIPCHK(hipMalloc(&d_work, sizeof(int)));
-        HIPCHK(hipMalloc(&d_out, sizeof(double) * 8 * nev));
+        HIPCHK(hipMalloc(&d_out, sizeof(double) * 16 * nev));
         HIPCHK(hipMalloc(&d_obs, sizeof(double) * kObs * nev));
         HIPCHK(hipMemset(d_obs, 0, sizeof(double) * kObs * nev));
         for (int e = 0; e < nev; e++) desc[e].obs = d_obs + (size_t)kObs * e;
@@ -415,6 +495,26 @@ struct Engine {
         return upload_desc();
     }
 
+    // a launch of candidate `cand` of tuner key `key`: events bracket it
+    // when the key is still being measured
+    bool timed_begin(const std::string& key, int cand, bool measuring, hipEvent_t* a, hipEvent_t* b) {
+        *a = *b = nullptr;
+        if (!measuring) return false;
+        if (hipEventCreate(a) != hipSuccess || hipEventCreate(b) != hipSuccess) return false;
+        timed.push_back(TimedLaunch{key, cand, *a, *b});
+        return true;
+    }
+    // after a stream synchronisation: hand the measured times to the tuner
+    void timed_resolve() {
+        for (TimedLaunch& t : timed) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) g_tuner.report(t.key, t.cand, ms);
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        timed.clear();
+    }
+
     int wilson_lines() {
         int rc = prepare_normals();
         if (rc) return rc;
@@ -435,13 +535,24 @@ struct Engine {
                 maxNy = std::max(maxNy, desc[e].Ny);
                 kc = std::min(kc, desc[e].kc);
             }
+            const std::string key = "setv_rows_inv_block/N=" + std::to_string(kv.first) +
+                                    "/events=" + std::to_string(es.size());
+            int minNy = 1 << 30;
+            for (int e : es) minNy = std::min(minNy, desc[e].Ny);
+            auto chunk = [&](const EvDesc* dd, int n, int k0, int kcc, int ny) {
+                bool meas = false;
+                const int v = g_tuner.pick(key, 2, &meas);
+                hipEvent_t a, b;
+                // only full chunks are compared (the last one may be short)
+                timed_begin(key, v, meas && k0 + kcc <= ny, &a, &b);
+                ipg_launch_setv_chunk(dd, n, dp, k0, kcc, v, a, b, st);
+            };
             if (contiguous) {
-                for (int k0 = 0; k0 < maxNy; k0 += kc)
-                    ipg_launch_setv_chunk(d_desc + es.front(), (int)es.size(), dp, k0, kc, st);
+                for (int k0 = 0; k0 < maxNy; k0 += kc) chunk(d_desc + es.front(), (int)es.size(), k0, kc, minNy);
             } else {
                 for (int e : es)
                     for (int k0 = 0; k0 < desc[e].Ny; k0 += desc[e].kc)
-                        ipg_launch_setv_chunk(d_desc + e, 1, dp, k0, desc[e].kc, st);
+                        chunk(d_desc + e, 1, k0, desc[e].kc, desc[e].Ny);
             }
         }
         HIPCHK(hipGetLastError());
@@ -451,32 +562,38 @@ struct Engine {
     int fields() {
         const int nev = (int)ev.size();
         ipg_launch_links(d_desc, nev, nblk, st);
-        ipg_launch_newton(d_desc, nev, nlinkblk, d_work, st);
+        {
+            int maxN = 0;
+            for (const EvDesc& d : desc)
+                if (d.active) maxN = std::max(maxN, d.N);
+            int nact = 0;
+            for (const EvDesc& d : desc) nact += d.active;
+            const std::string key =
+                "newton_waves_per_simd/N=" + std::to_string(maxN) + "/events=" + std::to_string(nact);
+            bool meas = false;
+            const int v = g_tuner.pick(key, 3, &meas);
+            hipEvent_t a, b;
+            if (timed_begin(key, v, meas, &a, &b)) (void)hipEventRecord(a, st);
+            ipg_launch_newton(d_desc, nev, nlinkblk, d_work, v + 1, st);
+            if (b) (void)hipEventRecord(b, st);
+        }
+        ipg_launch_newton_stats(d_desc, nev, nlinkblk, d_out, st);
         ipg_launch_pi_init(d_desc, nev, nblk, st);
         ipg_launch_zero_fields(d_desc, nev, nblk, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        // root-finder statistics
+        timed_resolve();
+        // root-finder statistics (k_newton_stats)
+        std::vector<double> h(8 * (size_t)nev);
+        HIPCHK(hipMemcpy(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
         for (int e = 0; e < nev; e++) {
             if (!desc[e].active) continue;
-            std::vector<int> it(2 * (size_t)desc[e].S);
-            HIPCHK(hipMemcpy(it.data(), desc[e].newton_stat, sizeof(int) * it.size(), hipMemcpyDeviceToHost));
-            long long sum = 0;
-            int mx = 0, rs = 0, un = 0, an = 0;
-            for (int v : it) {
-                int n = v & 0xFFFFF;
-                sum += n;
-                mx = std::max(mx, n);
-                rs += (v >> 20) & 0xFF;
-                un += (v >> 28) & 1;
-                an += (v >> 29) & 1;
-            }
             ipg_observables& o = ev[e].obs;
-            o.newton_iterations = sum;
-            o.newton_max_iterations = mx;
-            o.newton_restarts = rs;
-            o.newton_unconverged = un;
-            o.newton_analytic_jacobian = an;
+            o.newton_iterations = (long long)h[8 * e + 0];
+            o.newton_max_iterations = (int)h[8 * e + 1];
+            o.newton_restarts = (int)h[8 * e + 2];
+            o.newton_unconverged = (int)h[8 * e + 3];
+            o.newton_analytic_jacobian = (int)h[8 * e + 4];
             ev[e].parity = 0;
             ev[e].last_it = -1;
         }
@@ -866,6 +983,15 @@ int ipg_get_tmunu(ipg_ctx* c, double* out) {
     if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
     HIPCHK(hipMemcpy(out, c->eng.desc[0].T, sizeof(double) * 10 * c->eng.desc[0].S, hipMemcpyDeviceToHost));
     return IPG_OK;
+}
+int ipg_autotune_report(char* buf, int len) {
+    const std::string t = g_tuner.text();
+    if (buf && len > 0) {
+        const size_t n = std::min((size_t)len - 1, t.size());
+        std::memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return (int)t.size();
 }
 int ipg_get_landau(ipg_ctx* c, double* out) {
     if (!c || !out) return fail(IPG_ERR_ARG, "null argument");

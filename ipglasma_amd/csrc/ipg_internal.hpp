@@ -29,12 +29,14 @@ struct FftPlanDev {
     const double* s2;      // 4 sin^2(pi q / N), q < N
 };
 // slices [k0, k0 + kc) of every event of the launch (one lattice size)
-void ipg_launch_setv_chunk(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k0, int kc, hipStream_t st);
+void ipg_launch_setv_chunk(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k0, int kc, int inv_variant,
+                           hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st);
 void ipg_launch_unit_v(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 
 // tau = 0 fields
 void ipg_launch_links(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
-void ipg_launch_newton(const EvDesc* d_evs, int nev, int nlinkblk, int* d_counter, hipStream_t st);
+void ipg_launch_newton(const EvDesc* d_evs, int nev, int nlinkblk, int* d_counter, int variant, hipStream_t st);
+void ipg_launch_newton_stats(const EvDesc* d_evs, int nev, int nlinkblk, double* d_out, hipStream_t st);
 void ipg_launch_pi_init(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_zero_fields(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_rho_normals(const EvDesc* d_evs, int nucleus, int k, double* d_out, hipStream_t st);

@@ -461,8 +461,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
 
 // rows inverse for every slice of the chunk in order; each thread keeps the
 // Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V
-constexpr int kInvBlock = 512;
-template <int N>
+template <int N, int kInvBlock>
 __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                             int kc) {
     constexpr int SPT = (N + kInvBlock - 1) / kInvBlock;
@@ -595,24 +594,53 @@ __device__ __forceinline__ double pick8(const double (&v)[8], int c) {
     return r;
 }
 
-__global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, int nev, int nitems,
-                                                int* __restrict__ counter) {
+__device__ __forceinline__ M3 lds_m3(const double* p) {
+    M3 m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k] = cd{p[2 * k], p[2 * k + 1]};
+    return m;
+}
+__device__ __forceinline__ void lds_st(double* p, const M3& m) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        p[2 * k] = m.e[k].r;
+        p[2 * k + 1] = m.e[k].i;
+    }
+}
+
+template <int OCC>
+__global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ evs, int nev, int nitems,
+                                                     int* __restrict__ counter) {
     __shared__ double Jt[NWB / NG][8][8];  // [group][row a][column c]
+    // per group: U0 = U1 U2, W = U1 + U2, the iterate Us (written by lane 0)
+    __shared__ double Ms[NWB / NG][3][18];
+    __shared__ double Bs[NWB / NG][16];    // best exponent, Im tr t^a (W - W^+)
     const int lane = threadIdx.x;
     const int c = lane & (NG - 1);
     const int grp = lane / NG;
     const int maxIt = 2000, maxRe = 200;
+    double* mU0 = Ms[grp][0];
+    double* mW = Ms[grp][1];
+    double* mUs = Ms[grp][2];
+    double* best_alpha = Bs[grp];
+    double* trci = Bs[grp] + 8;
     int state = 0;  // group-uniform: 0 needs a link, 1 iterating, 2 no work left
     int e = 0, dir = 0;
     long long l = 0, s = 0;
     uint32_t link = 0;
-    M3 U0 = m_one(), W = m_one(), Us = m_one();
-    double alpha[8], best_alpha[8], trci[8];
+    double alpha[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) alpha[a] = best_alpha[a] = trci[a] = 0.;
+    for (int a = 0; a < 8; a++) alpha[a] = 0.;
     double Fzero = 0., Fmin = 0.;
     int iter = 0, nRestart = 0, total = 0, analytic = 0, best_one = 1;
     uint32_t ndraw = 0;
+    if (c == 0) {
+        lds_st(mU0, m_one());
+        lds_st(mW, m_one());
+        lds_st(mUs, m_one());
+#pragma unroll
+        for (int a = 0; a < 16; a++) Bs[grp][a] = 0.;
+    }
     for (;;) {
         // ---- one request per idle group
         const bool lead_need = state == 0 && c == 0;
@@ -636,18 +664,21 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
                         dir = l >= S;
                         s = dir ? l - S : l;
                         link = (uint32_t)(2 * s + dir);
-                        const M3 U1 = ldm(dir ? ev.L1y : ev.L1x, S, s);
-                        const M3 U2 = ldm(dir ? ev.L2y : ev.L2x, S, s);
-                        U0 = U1 * U2;
-                        W = U1 + U2;
-                        cd trc[8];
-                        tr_gen(W - dag(W), trc);
 #pragma unroll
-                        for (int a = 0; a < 8; a++) {
-                            trci[a] = trc[a].i;
-                            alpha[a] = 0.;
+                        for (int a = 0; a < 8; a++) alpha[a] = 0.;
+                        if (c == 0) {
+                            const M3 U1 = ldm(dir ? ev.L1y : ev.L1x, S, s);
+                            const M3 U2 = ldm(dir ? ev.L2y : ev.L2x, S, s);
+                            const M3 U0 = U1 * U2;
+                            const M3 W = U1 + U2;
+                            cd trc[8];
+                            tr_gen(W - dag(W), trc);
+#pragma unroll
+                            for (int a = 0; a < 8; a++) trci[a] = trc[a].i;
+                            lds_st(mU0, U0);
+                            lds_st(mW, W);
+                            lds_st(mUs, exp_iQt(alpha) * U0);
                         }
-                        Us = exp_iQt(alpha) * U0;
                         best_one = 1;
                         Fzero = 10.;
                         Fmin = 1e6;
@@ -660,6 +691,7 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
             }
         }
         if (__ballot(state != 2) == 0) break;
+        __syncthreads();  // link state written by lane 0 of each group
 
         // ---- one iteration; every group runs it (uniform control flow for
         // the barriers), only groups in state 1 keep the result
@@ -667,6 +699,7 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
         total++;
         double F[8];
         {
+            const M3 W = lds_m3(mW), Us = lds_m3(mUs);
             M3 Mt = mul_bd(W, Us) - mul_bd(Us, W);
             cd t[8];
             tr_gen(Mt, t);
@@ -677,7 +710,6 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
                 Fzero += fabs(F[a]);
             }
         }
-        const M3 Ud = dag(Us);
         // the serial loop adds the step to alpha[b] and subtracts it again in
         // place, so column b sees (alpha[k] + d_k) - d_k for k < b
         double dk[8], ar[8], p[8];
@@ -692,7 +724,8 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
         double col[8];
         bool good = true;
         {
-            M3 Mp = W * (dag(exp_iQt(p) * U0) - Ud);
+            const M3 E = exp_iQt(p) * lds_m3(mU0);
+            M3 Mp = lds_m3(mW) * (dag(E) - dag(lds_m3(mUs)));
             cd t[8];
             tr_gen(Mp, t);
             double chk = 0.;
@@ -708,7 +741,8 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
         const unsigned long long bad = __ballot(!good);
         if ((bad >> (grp * NG)) & 0xFFull) {
             analytic = 1;
-            const M3 Mb = gen_mul(c, Ud);
+            const M3 Mb = gen_mul(c, dag(lds_m3(mUs)));
+            const M3 W = lds_m3(mW);
 #pragma unroll
             for (int a = 0; a < 8; a++) col[a] = -2. * retrprod(gen_mul(a, W), Mb);
         }
@@ -744,7 +778,7 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
             for (int k = 0; k < 9; k++) row[k] = __shfl(row[k], src, NG);
             double piv[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) piv[k] = __shfl(row[k], j, NG);
+            for (int k = j; k < 8; k++) piv[k] = __shfl(row[k], j, NG);
             const double ajj = piv[j];
             if (c > j && ajj != 0.0) {
                 const double arj = row[j] / ajj;
@@ -782,31 +816,39 @@ __global__ __launch_bounds__(NWB) void k_newton(const EvDesc* __restrict__ evs, 
         } else {
             restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
         }
-        Us = exp_iQt(alpha) * U0;
+        M3 Us = exp_iQt(alpha) * lds_m3(mU0);
         if (Fzero < Fmin) {
             // best = Us, kept as its exponent (Us is exp(i alpha t) U0)
             Fmin = Fzero;
+            if (c == 0)
 #pragma unroll
-            for (int a = 0; a < 8; a++) best_alpha[a] = alpha[a];
+                for (int a = 0; a < 8; a++) best_alpha[a] = alpha[a];
             best_one = 0;
         }
         if (iter == maxIt) {
             restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
-            Us = exp_iQt(alpha) * U0;
+            Us = exp_iQt(alpha) * lds_m3(mU0);
             nRestart++;
             iter = 0;
         }
+        if (c == 0) lds_st(mUs, Us);
         if (state == 1 && !(Fzero > 1e-6 && iter < maxIt && nRestart < maxRe)) {
             const EvDesc& ev = evs[e];
             const int conv = nRestart != maxRe;
             if (c == 0) {
                 M3 out = Us;
-                if (!conv) out = best_one ? m_one() : exp_iQt(best_alpha) * U0;
+                if (!conv) {
+                    double ba[8];
+#pragma unroll
+                    for (int a = 0; a < 8; a++) ba[a] = best_alpha[a];
+                    out = best_one ? m_one() : exp_iQt(ba) * lds_m3(mU0);
+                }
                 stm(ev.U[0][dir], ev.S, s, out);
                 ev.newton_stat[l] = total | (nRestart << 20) | ((!conv) << 28) | (analytic << 29);
             }
             state = 0;
         }
+        __syncthreads();  // Us, best exponent visible before the next round
     }
 }
 
@@ -873,32 +915,45 @@ void ipg_launch_stats(const EvDesc* d, int nev, int nblk, double* out, hipStream
     hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
     hipLaunchKernelGGL(k_stats_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, out);
 }
+// inv_variant: 0 = 256-thread blocks, 1 = 512-thread blocks for the inverse
+// row pass (the same arithmetic; chosen by the autotuner); the inverse row pass
+// is bracketed by ev_a / ev_b when they are given
 template <int N>
-static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, hipStream_t st) {
+static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, int inv_variant,
+                       hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st) {
     constexpr int C = ipgfft::cols_per_block(N);
     const int nrow_fwd = nev * 2 * kc * N;
     const int ncol = nev * 2 * kc * 4 * (N / C);
     const int nrow_inv = nev * 2 * N;
     hipLaunchKernelGGL(k_setv_rows_fwd<N>, dim3(nrow_fwd), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
     hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
-    hipLaunchKernelGGL(k_setv_rows_inv<N>, dim3(nrow_inv), dim3(kInvBlock), 0, st, d, pl, k0, kc);
+    if (ev_a) (void)hipEventRecord(ev_a, st);
+    if (inv_variant == 1)
+        hipLaunchKernelGGL((k_setv_rows_inv<N, 512>), dim3(nrow_inv), dim3(512), 0, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL((k_setv_rows_inv<N, 256>), dim3(nrow_inv), dim3(256), 0, st, d, pl, k0, kc);
+    if (ev_b) (void)hipEventRecord(ev_b, st);
 }
-void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, hipStream_t st) {
+void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, int inv_variant,
+                           hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st) {
+#define IPG_SETV_CASE(n) \
+    case n: setv_chunk<n>(d, nev, pl, k0, kc, inv_variant, ev_a, ev_b, st); break;
     switch (pl.N) {
-        case 16: setv_chunk<16>(d, nev, pl, k0, kc, st); break;
-        case 32: setv_chunk<32>(d, nev, pl, k0, kc, st); break;
-        case 48: setv_chunk<48>(d, nev, pl, k0, kc, st); break;
-        case 64: setv_chunk<64>(d, nev, pl, k0, kc, st); break;
-        case 96: setv_chunk<96>(d, nev, pl, k0, kc, st); break;
-        case 128: setv_chunk<128>(d, nev, pl, k0, kc, st); break;
-        case 192: setv_chunk<192>(d, nev, pl, k0, kc, st); break;
-        case 256: setv_chunk<256>(d, nev, pl, k0, kc, st); break;
-        case 384: setv_chunk<384>(d, nev, pl, k0, kc, st); break;
-        case 512: setv_chunk<512>(d, nev, pl, k0, kc, st); break;
-        case 768: setv_chunk<768>(d, nev, pl, k0, kc, st); break;
-        case 1024: setv_chunk<1024>(d, nev, pl, k0, kc, st); break;
+        IPG_SETV_CASE(16)
+        IPG_SETV_CASE(32)
+        IPG_SETV_CASE(48)
+        IPG_SETV_CASE(64)
+        IPG_SETV_CASE(96)
+        IPG_SETV_CASE(128)
+        IPG_SETV_CASE(192)
+        IPG_SETV_CASE(256)
+        IPG_SETV_CASE(384)
+        IPG_SETV_CASE(512)
+        IPG_SETV_CASE(768)
+        IPG_SETV_CASE(1024)
         default: break;  // rejected by the engine set-up
     }
+#undef IPG_SETV_CASE
 }
 void ipg_launch_unit_v(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_unit_v, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
@@ -908,12 +963,73 @@ void ipg_launch_links(const EvDesc* d, int nev, int nblk, hipStream_t st) {
 }
 __global__ void k_zero_counter(int* c) { *c = 0; }
 
-void ipg_launch_newton(const EvDesc* d, int nev, int nlinkblk, int* d_counter, hipStream_t st) {
-    // enough single-wave blocks to fill the machine (the kernel runs at one
-    // wave per SIMD: 256 CUs x 4); each loops until the work counter runs out
-    const int grid = nlinkblk < 2048 ? nlinkblk : 2048;
+// variant = waves per SIMD the kernel is compiled for (1, 2, 3: fewer
+// registers, more spill); every variant computes the same result
+void ipg_launch_newton(const EvDesc* d, int nev, int nlinkblk, int* d_counter, int variant, hipStream_t st) {
+    // persistent single-wave blocks, enough to fill the machine at the
+    // variant's occupancy (256 CUs x 4 SIMDs); each loops until the work
+    // counter runs out
+    const int occ = variant < 1 ? 1 : (variant > 3 ? 3 : variant);
+    int grid = 1024 * occ;
+    if (grid > nlinkblk) grid = nlinkblk;
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, d_counter);
-    hipLaunchKernelGGL(k_newton, dim3(grid), dim3(NWB), 0, st, d, nev, nlinkblk * NWB, d_counter);
+    if (occ == 3)
+        hipLaunchKernelGGL(k_newton<3>, dim3(grid), dim3(NWB), 0, st, d, nev, nlinkblk * NWB, d_counter);
+    else if (occ == 2)
+        hipLaunchKernelGGL(k_newton<2>, dim3(grid), dim3(NWB), 0, st, d, nev, nlinkblk * NWB, d_counter);
+    else
+        hipLaunchKernelGGL(k_newton<1>, dim3(grid), dim3(NWB), 0, st, d, nev, nlinkblk * NWB, d_counter);
+}
+
+// per event: sum, max of the link iteration counts, restarts, unconverged
+// and analytic-Jacobian links (out[8 e + 0..4]); one block per 64-link block
+// of the root finder, block sums combined with atomics (integers: exact)
+__global__ void k_newton_stats_zero(int nev, unsigned long long* __restrict__ acc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 5 * nev) acc[t] = 0ull;
+}
+__global__ __launch_bounds__(NWB) void k_newton_stats(const EvDesc* __restrict__ evs, int nev,
+                                                     unsigned long long* __restrict__ acc) {
+    const int e = find_event_links(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active) return;
+    const long long l = (long long)(blockIdx.x - ev.lblock0) * NWB + threadIdx.x;
+    unsigned long long v[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
+    if (l < 2 * ev.S) {
+        const int x = ev.newton_stat[l];
+        v[0] = (unsigned)(x & 0xFFFFF);
+        v[1] = v[0];
+        v[2] = (unsigned)((x >> 20) & 0xFF);
+        v[3] = (unsigned)((x >> 28) & 1);
+        v[4] = (unsigned)((x >> 29) & 1);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long t = __shfl_down(v[k], o, 64);
+            v[k] = k == 1 ? (t > v[k] ? t : v[k]) : v[k] + t;
+        }
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[5 * e + 0], v[0]);
+        atomicMax(&acc[5 * e + 1], v[1]);
+        atomicAdd(&acc[5 * e + 2], v[2]);
+        atomicAdd(&acc[5 * e + 3], v[3]);
+        atomicAdd(&acc[5 * e + 4], v[4]);
+    }
+}
+__global__ void k_newton_stats_out(int nev, const unsigned long long* __restrict__ acc, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 5 * nev) out[8 * (t / 5) + t % 5] = (double)acc[t];
+}
+// d_out holds 16 doubles per event: results in [8e, 8e+5), the integer
+// accumulators in the second half [8 nev, 8 nev + 5 nev)
+void ipg_launch_newton_stats(const EvDesc* d, int nev, int nlinkblk, double* d_out, hipStream_t st) {
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(d_out + 8 * (size_t)nev);
+    const int nb = (5 * nev + 255) / 256;
+    hipLaunchKernelGGL(k_newton_stats_zero, dim3(nb), dim3(256), 0, st, nev, acc);
+    hipLaunchKernelGGL(k_newton_stats, dim3(nlinkblk), dim3(NWB), 0, st, d, nev, acc);
+    hipLaunchKernelGGL(k_newton_stats_out, dim3(nb), dim3(256), 0, st, nev, acc, d_out);
 }
 void ipg_launch_pi_init(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_pi_init, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);

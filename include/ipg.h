@@ -144,6 +144,19 @@ int ipg_evolve_step(ipg_ctx *c, int it);
 
 int ipg_get_observables(ipg_ctx *c, ipg_observables *o);
 
+/* Reverse mode: dE/deta at tau_max and its gradient with respect to the
+ * tau = 0 fields that ipg_init leaves (the state Evolution::run starts from,
+ * Evolution.cpp:366-431).  Runs the tau = 0 kick and the itmax-1 leapfrog
+ * steps forward with checkpoints every ceil(sqrt(itmax-1)) steps, evaluates
+ * T^{mu nu} at itmax, and walks the steps back (adjoint of each update,
+ * forward states recomputed per segment).  grad (or NULL) receives
+ * [6][S][18]: the cotangents dL/dRe + i dL/dIm of U_x, U_y, E1, E2, phi, pi,
+ * the last four projected on Hermitian matrices.  Call after ipg_init in
+ * place of ipg_evolve: the state is consumed.  No counterpart in the
+ * reference (its evolution has no derivative); the chain from the tau = 0
+ * fields back to rho (root finder, Wilson lines) is not part of it. */
+int ipg_adjoint_dedeta(ipg_ctx *c, double *dedeta, double *grad);
+
 /* Autotuner state: one line per tuned kernel ("key: cand=best time ...
  * chosen=k").  The variants of a key compute identical results (launch shape
  * or register budget only); each is timed on the real workload until the

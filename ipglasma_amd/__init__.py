@@ -125,6 +125,7 @@ def lib():
             "ipg_get_tmunu": (C.c_int, [vp, dp]),
             "ipg_get_landau": (C.c_int, [vp, dp]),
             "ipg_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
+            "ipg_adjoint_dedeta": (C.c_int, [vp, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
@@ -247,6 +248,14 @@ class Event:
         out = np.zeros((10, self.S))
         _check(lib().ipg_get_tmunu(self._c, _dptr(out)))
         return out
+
+    def adjoint_dedeta(self):
+        """dE/deta and its gradient with respect to the tau = 0 fields
+        {name: [S, 18]} (consumes the state; call after init)"""
+        g = np.zeros((6, self.S, 18))
+        v = C.c_double()
+        _check(lib().ipg_adjoint_dedeta(self._c, C.byref(v), _dptr(g)))
+        return v.value, {k: g[i] for i, k in enumerate(("Ux", "Uy", "E1", "E2", "phi", "pi"))}
 
     def landau(self):
         """[15, S]: eps, u^tau, u^x, u^y, u^eta, pi^{mu nu} (T's order)"""

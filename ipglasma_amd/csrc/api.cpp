@@ -99,7 +99,10 @@ struct Tuner {
             return 0;
         }
         *measuring = e.chosen < 0;
-        return e.chosen >= 0 ? e.chosen : e.next;
+        if (e.chosen >= 0) return e.chosen;
+        const int c = e.next;  // round robin over the launches being measured
+        e.next = (e.next + 1) % ncand;
+        return c;
     }
     void report(const std::string& key, int cand, float ms) {
         std::lock_guard<std::mutex> g(mu);
@@ -107,7 +110,6 @@ struct Tuner {
         if (e.chosen >= 0 || cand >= (int)e.best.size()) return;
         if (e.best[cand] < 0 || ms < e.best[cand]) e.best[cand] = ms;
         e.seen[cand]++;
-        e.next = (e.next + 1) % (int)e.best.size();
         bool done = true;
         for (int n : e.seen) done = done && n >= 2;
         if (done) {
@@ -140,6 +142,15 @@ struct TimedLaunch {
     int cand;
     hipEvent_t a, b;
 };
+
+void hipFreeVoid(double* p) { (void)hipFree(p); }
+// [18][S] planes -> [S][18]
+__global__ void k_soa_to_aos_ext(const double* __restrict__ f, long long S, double* __restrict__ out) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 18 * S) return;
+    long long s = t / 18, c = t - s * 18;
+    out[t] = f[c * S + s];
+}
 
 struct Plan {
     int N = 0, logN = 0, C = 0;
@@ -750,6 +761,90 @@ struct Engine {
         return read_obs();
     }
 
+    // dE/deta at itmax and its gradient with respect to the state left by
+    // the initial conditions (U_x, U_y, E1, E2, phi, pi at tau = 0), by
+    // reverse mode through the tau = 0 kick and itmax-1 leapfrog steps with
+    // checkpoints every K = ceil(sqrt(itmax-1)) steps.  Consumes the state.
+    int adjoint_dedeta(double* dedeta, double* grad) {
+        if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
+        EvDesc& d = desc[0];
+        const int itm = d.itmax, T = itm - 1;
+        if (itm < 1) return fail(IPG_ERR_ARG, "itmax must be at least 1");
+        const long long S = d.S;
+        const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
+        const int nck = T / K + 1;
+        const size_t slot = 72 * (size_t)S, cset = 108 * (size_t)S;
+        const size_t nslot = 1 + (size_t)nck + (size_t)K + 1 + 1;
+        double* mem = nullptr;
+        HIPCHK(hipMalloc(&mem, sizeof(double) * (nslot * slot + 2 * cset)));
+        std::unique_ptr<double, decltype(&hipFreeVoid)> hold(mem, &hipFreeVoid);
+        double* s_init = mem;
+        double* s_ck = s_init + slot;                 // nck checkpoints
+        double* s_seg = s_ck + (size_t)nck * slot;    // K + 1 segment states
+        double* s_T = s_seg + (size_t)(K + 1) * slot;
+        double* cA = s_T + slot;
+        double* cB = cA + cset;
+        int rc = upload_desc();
+        if (rc) return rc;
+        // forward with checkpoints
+        ipg_adj_state_save(d_desc, S, 0, s_init, st);
+        ipg_launch_step0(d_desc, 1, nblk, st);
+        ipg_adj_state_save(d_desc, S, 0, s_ck, st);
+        for (int it = 1; it <= T; it++) {
+            ipg_launch_step(d_desc, 1, nblk, it, st);
+            if (it % K == 0) ipg_adj_state_save(d_desc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
+        }
+        ipg_adj_state_save(d_desc, S, T & 1, s_T, st);
+        // the loss: T^{mu nu} at itmax from the state after step itmax-1
+        ipg_launch_tmunu(d_desc, 1, nblk, itm, st);
+        ipg_launch_finish(d_desc, 1, itm, d_out, st);
+        HIPCHK(hipGetLastError());
+        double tt = 0.;
+        HIPCHK(hipMemcpy(&tt, d_out, sizeof(double), hipMemcpyDeviceToHost));
+        const double tau_fm = itm * d.dtau * d.a;
+        const double L = tau_fm * d.a * d.a * ipg::HBARC * tt;
+        // reverse
+        ipg_adj_zero(cA, (long long)cset, st);
+        ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
+        double *cin = cA, *cout = cB;
+        for (int k = nck - 1; k >= 0; k--) {
+            const int s0 = k * K, s1 = std::min(s0 + K, T);
+            if (s1 <= s0) continue;
+            const double* x0 = s_ck + (size_t)k * slot;
+            ipg_adj_state_load(d_desc, S, s0 & 1, x0, st);
+            for (int it = s0 + 1; it <= s1; it++) {
+                ipg_launch_step(d_desc, 1, nblk, it, st);
+                ipg_adj_state_save(d_desc, S, it & 1, s_seg + (size_t)(it - s0) * slot, st);
+            }
+            for (int it = s1; it > s0; it--) {
+                const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
+                const double* xa = s_seg + (size_t)(it - s0) * slot;
+                ipg_adj_zero(cout, (long long)cset, st);
+                ipg_adj_step(d_desc, S, xb, xa, it, cin, cout, st);
+                std::swap(cin, cout);
+            }
+        }
+        ipg_adj_kick(d_desc, S, s_init, cin, cout, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (grad) {
+            // [6][S][18]: U_x, U_y, E1, E2, phi, pi
+            double* tmp = (double*)d.fftbuf;
+            for (int f = 0; f < 6; f++) {
+                hipLaunchKernelGGL(k_soa_to_aos_ext, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, st,
+                                   cout + (size_t)f * 18 * S, S, tmp);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(grad + (size_t)f * 18 * S, tmp, sizeof(double) * 18 * S,
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+        }
+        if (dedeta) *dedeta = L;
+        ev[0].obs.dEdeta = L;
+        ev[0].obs.tau_fm = tau_fm;
+        return IPG_OK;
+    }
+
     int single_step(int e, int it) {
         // one schedule piece for a single-event engine
         int rc = upload_desc();
@@ -983,6 +1078,10 @@ int ipg_get_tmunu(ipg_ctx* c, double* out) {
     if (!c || !out) return fail(IPG_ERR_ARG, "null argument");
     HIPCHK(hipMemcpy(out, c->eng.desc[0].T, sizeof(double) * 10 * c->eng.desc[0].S, hipMemcpyDeviceToHost));
     return IPG_OK;
+}
+int ipg_adjoint_dedeta(ipg_ctx* c, double* dedeta, double* grad) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    return c->eng.adjoint_dedeta(dedeta, grad);
 }
 int ipg_autotune_report(char* buf, int len) {
     const std::string t = g_tuner.text();

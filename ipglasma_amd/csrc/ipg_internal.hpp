@@ -40,3 +40,15 @@ void ipg_launch_newton_stats(const EvDesc* d_evs, int nev, int nlinkblk, double*
 void ipg_launch_pi_init(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_zero_fields(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_rho_normals(const EvDesc* d_evs, int nucleus, int k, double* d_out, hipStream_t st);
+
+// reverse mode of dE/deta through the leapfrog (one event, evs[0])
+void ipg_adj_state_save(const EvDesc* d_evs, long long S, int parity, double* slot, hipStream_t st);
+void ipg_adj_state_load(const EvDesc* d_evs, long long S, int parity, const double* slot, hipStream_t st);
+void ipg_adj_zero(double* p, long long n, hipStream_t st);
+void ipg_adj_loss(const EvDesc* d_evs, long long S, const double* xs, int it, double yb, double* cout,
+                  hipStream_t st);
+void ipg_adj_step(const EvDesc* d_evs, long long S, const double* xb, const double* xa, int it, const double* cin,
+                  double* cout, hipStream_t st);
+void ipg_adj_kick(const EvDesc* d_evs, long long S, const double* x0, const double* cin, double* cout,
+                  hipStream_t st);
+

@@ -1,0 +1,75 @@
+// Host build of ipglasma_amd/csrc/adjoint.hpp: the per-site forward and
+// reverse functions over flat arrays (matrices as 18 doubles, row-major,
+// interleaved re/im), for finite-difference checks.
+#include "../../ipglasma_amd/csrc/adjoint.hpp"
+
+using namespace ipgad;
+
+static M3 ld(const double* p) {
+    M3 m;
+    for (int k = 0; k < 9; k++) m.e[k] = cd{p[2 * k], p[2 * k + 1]};
+    return m;
+}
+static void st(double* p, const M3& m) {
+    for (int k = 0; k < 9; k++) {
+        p[2 * k] = m.e[k].r;
+        p[2 * k + 1] = m.e[k].i;
+    }
+}
+template <class T>
+static void unpack(const double* x, T& q, int n) {
+    M3* m = reinterpret_cast<M3*>(&q);
+    for (int k = 0; k < n; k++) m[k] = ld(x + 18 * k);
+}
+template <class T>
+static void pack(double* x, const T& q, int n) {
+    const M3* m = reinterpret_cast<const M3*>(&q);
+    for (int k = 0; k < n; k++) st(x + 18 * k, m[k]);
+}
+
+extern "C" {
+// pi: 10 input matrices, 1 output
+void adjh_pi(const double* x, double cp, const double* yb, double* y, double* xb) {
+    PiIn q;
+    unpack(x, q, 10);
+    st(y, pi_fwd(q, cp));
+    PiBar b;
+    for (int k = 0; k < 10; k++) reinterpret_cast<M3*>(&b)[k] = m_zero();
+    pi_adj(q, cp, ld(yb), b);
+    pack(xb, b, 10);
+}
+// E: 15 inputs, 2 outputs
+void adjh_e(const double* x, double tau, double dtau, double g, const double* yb, double* y, double* xb) {
+    EIn q;
+    unpack(x, q, 15);
+    M3 a, c;
+    e_fwd(q, tau, dtau, g, a, c);
+    st(y, a);
+    st(y + 18, c);
+    EBar b;
+    for (int k = 0; k < 15; k++) reinterpret_cast<M3*>(&b)[k] = m_zero();
+    e_adj(q, tau, dtau, g, ld(yb), ld(yb + 18), b);
+    pack(xb, b, 15);
+}
+// U update: inputs U, E; output U'
+void adjh_u(const double* x, double kappa, const double* yb, double* y, double* xb) {
+    const M3 U = ld(x), E = ld(x + 18);
+    st(y, u_fwd(U, E, kappa));
+    M3 Ub = m_zero(), Eb = m_zero();
+    u_adj(U, E, kappa, ld(yb), Ub, Eb);
+    st(xb, Ub);
+    st(xb + 18, Eb);
+}
+// T^tautau: 16 inputs, scalar output
+double adjh_tt(const double* x, int plaq, double tau, double g, double yb, double* xb) {
+    TIn q;
+    unpack(x, q, 16);
+    q.plaq = plaq;
+    const double y = tt_fwd(q, tau, g);
+    TBar b;
+    for (int k = 0; k < 16; k++) reinterpret_cast<M3*>(&b)[k] = m_zero();
+    tt_adj(q, tau, g, yb, b);
+    pack(xb, b, 16);
+    return y;
+}
+}

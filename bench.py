@@ -55,6 +55,36 @@ def hetero_params(I):
             [mk(512, 30.0, "Pb", "Pb", 6.62, 0.546, 12.0)] * 24 + [mk(768, 30.0, "Pb", "Pb", 6.62, 0.546, 0.0)] * 8)
 
 
+def adjoint_timing(I, seed=77):
+    """Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c: one forward event
+    (initial conditions + evolution) against the reverse mode to rho
+    (initial conditions + ipg_adjoint_rho, whose forward pass is included)."""
+    p = pbpb_params(I, N=256, L=30.0)
+    p.set(Projectile="Au", Target="Au", R_WS=6.38, a_WS=0.535, SigmaNN=42.0, roots=200.0, bmin=5.0, bmax=5.0)
+    def timed(adj):
+        ev = I.Event(p, seed=seed)
+        t0 = time.perf_counter()
+        while not ev.init():
+            pass
+        t1 = time.perf_counter()
+        if adj:
+            L, g = ev.adjoint_rho()
+        else:
+            ev.run()
+            L = ev.observables()["dEdeta"]
+        t2 = time.perf_counter()
+        ev.close()
+        return t1 - t0, t2 - t1, L
+    timed(False)
+    timed(True)
+    i_f, e_f, Lf = timed(False)
+    i_a, e_a, La = timed(True)
+    return {"workload": "Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c, d(dE/deta)/d rho",
+            "forward_ms": round(1e3 * (i_f + e_f), 2), "adjoint_ms": round(1e3 * (i_a + e_a), 2),
+            "adjoint_over_forward": round((i_a + e_a) / (i_f + e_f), 2),
+            "dEdeta_rel_diff": abs(La - Lf) / abs(Lf)}
+
+
 def run_plain(I, p, seeds):
     t0 = time.perf_counter()
     out = []
@@ -203,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hetero", action="store_true",
                     help="skip the heterogeneous 64-event batch (N=256/512/768)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode timing")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -253,6 +284,10 @@ def main():
                   "events": len(hp), "batched_events_per_s": round(len(hp) / dtb, 3),
                   "plain_events_per_s": round(len(hp) / dtp, 3), "batched_over_plain": round(dtp / dtb, 3),
                   "batched_vs_plain_max_rel_dEdeta": hrel}
+    adj = None
+    if not a.no_adjoint:
+        adj = adjoint_timing(I)
+        print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
     rf = roofline(I, p)
     line = {
         "metric": "events/s: batched vs per-event launch, N=512 FP64, dE/deta rel err<=1e-9",
@@ -267,6 +302,7 @@ def main():
         "dEdeta_GeV": obs_b[0]["dEdeta"],
         "roofline": rf,
         "heterogeneous_batch": hetero,
+        "adjoint": adj,
         "autotune": I.autotune_report(),
     }
     if not a.no_cpu:

@@ -157,6 +157,15 @@ int ipg_get_observables(ipg_ctx *c, ipg_observables *o);
  * fields back to rho (root finder, Wilson lines) is not part of it. */
 int ipg_adjoint_dedeta(ipg_ctx *c, double *dedeta, double *grad);
 
+/* Reverse mode to the colour charges: d(dE/deta)/d rho^a_k(x) for both
+ * nuclei, grad[2][Ny][S][8] (nucleus, slice, site, colour), through the
+ * evolution (as ipg_adjoint_dedeta), pi at tau = 0, the root finder by
+ * implicit differentiation (Init.cpp:3344-3481), the links, the ordered
+ * product of slice exponentials and the Poisson solve (Init.cpp:1678-1850).
+ * rho = g sqrt(g2mu2/Ny) z with z the normals.  Call after ipg_init in place
+ * of ipg_evolve: the state is consumed.  No counterpart in the reference. */
+int ipg_adjoint_rho(ipg_ctx *c, double *dedeta, double *grad);
+
 /* Autotuner state: one line per tuned kernel ("key: cand=best time ...
  * chosen=k").  The variants of a key compute identical results (launch shape
  * or register budget only); each is timed on the real workload until the

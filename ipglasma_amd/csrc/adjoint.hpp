@@ -238,4 +238,127 @@ IPG_HD void tt_adj(const TIn& q, double tau, double g, double yb, TBar& b) {
     }
 }
 
+// ---- tau = 0 root finder by implicit differentiation (Init.cpp:3344-3481) ----
+// The solution U of F(U; W) = 0, F_a = -Im tr t^a [(W - W^+) + (W U^+ - U W^+)],
+// lies on the coset SU(3) U0 (U = exp(i alpha t) U0, U0 = U1 U2, W = U1 + U2).
+// Near it, U(beta) = exp(i beta t) E* U0 with E* = U U0^{-1}; then
+//   dF_a/dbeta_b = Re tr(t^a W U^+ t^b) + Re tr(t^a t^b U W^+),
+//   dU = i (dbeta . t) U + E* dU0,
+// and the reverse pass is  betab_b = Re tr(Ub^+ i t^b U),
+// lambda = -J^{-T} betab,  T = sum_a lambda_a t^a,
+//   Wb = -2i T (1 + U) ... (from F_lambda = Re tr(i T M)),  Usb = 2i T W,
+//   U0b = E*^+ (Ub + Usb),  U1b = Wb + U0b U2^+,  U2b = Wb + U1^+ U0b.
+// E* = U U0^+ (U0 is unitary: products of Wilson lines).
+IPG_HD void tgen(int a, M3& t) {
+    t = m_zero();
+    const double s3 = 1.0 / (2.0 * sqrt(3.0));
+    switch (a) {
+        case 0: t.e[1] = cd{0.5, 0.}; t.e[3] = cd{0.5, 0.}; break;
+        case 1: t.e[1] = cd{0., -0.5}; t.e[3] = cd{0., 0.5}; break;
+        case 2: t.e[0] = cd{0.5, 0.}; t.e[4] = cd{-0.5, 0.}; break;
+        case 3: t.e[2] = cd{0.5, 0.}; t.e[6] = cd{0.5, 0.}; break;
+        case 4: t.e[2] = cd{0., -0.5}; t.e[6] = cd{0., 0.5}; break;
+        case 5: t.e[5] = cd{0.5, 0.}; t.e[7] = cd{0.5, 0.}; break;
+        case 6: t.e[5] = cd{0., -0.5}; t.e[7] = cd{0., 0.5}; break;
+        default: t.e[0] = cd{s3, 0.}; t.e[4] = cd{s3, 0.}; t.e[8] = cd{-2. * s3, 0.}; break;
+    }
+}
+// solve A x = b (8x8, partial pivoting), A row-major, overwritten
+IPG_HD void solve8(double* A, double* b) {
+    for (int j = 0; j < 8; j++) {
+        int p = j;
+        double mx = fabs(A[8 * j + j]);
+        for (int r = j + 1; r < 8; r++)
+            if (fabs(A[8 * r + j]) > mx) {
+                mx = fabs(A[8 * r + j]);
+                p = r;
+            }
+        if (p != j) {
+            for (int c = 0; c < 8; c++) {
+                const double t = A[8 * j + c];
+                A[8 * j + c] = A[8 * p + c];
+                A[8 * p + c] = t;
+            }
+            const double t = b[j];
+            b[j] = b[p];
+            b[p] = t;
+        }
+        for (int r = j + 1; r < 8; r++) {
+            const double f = A[8 * r + j] / A[8 * j + j];
+            for (int c = j; c < 8; c++) A[8 * r + c] -= f * A[8 * j + c];
+            b[r] -= f * b[j];
+        }
+    }
+    for (int r = 7; r >= 0; r--) {
+        double t = b[r];
+        for (int c = r + 1; c < 8; c++) t -= A[8 * r + c] * b[c];
+        b[r] = t / A[8 * r + r];
+    }
+}
+IPG_HD void newton_adj(const M3& U1, const M3& U2, const M3& U, const M3& Ub, M3& U1b, M3& U2b) {
+    const M3 W = U1 + U2, U0 = U1 * U2;
+    const M3 A = mul_bd(W, U), B = mul_bd(U, W);  // W U^+, U W^+
+    M3 t[8];
+    for (int a = 0; a < 8; a++) tgen(a, t[a]);
+    double J[64], rhs[8];
+    for (int a = 0; a < 8; a++) {
+        const M3 ta_A = t[a] * A;
+        for (int b = 0; b < 8; b++) J[8 * a + b] = retrprod(ta_A, t[b]) + retrprod(t[a] * t[b], B);
+    }
+    // betab_b = Re tr(Ub^+ i t^b U); solve J^T lambda = -betab
+    double JT[64];
+    for (int a = 0; a < 8; a++)
+        for (int b = 0; b < 8; b++) JT[8 * a + b] = J[8 * b + a];
+    for (int b = 0; b < 8; b++) {
+        const M3 d = mul_i(1., t[b] * U);
+        rhs[b] = -retrprod(dag(Ub), d);
+    }
+    solve8(JT, rhs);
+    M3 T = m_zero();
+    for (int a = 0; a < 8; a++) T = T + rhs[a] * t[a];
+    const M3 iT = mul_i(1., T);
+    const M3 Wb = (-2.) * (iT * (m_one() + U));
+    const M3 Usb = 2. * (iT * W);
+    const M3 Es = mul_bd(U, U0);  // E* = U U0^+
+    const M3 U0b = mul_ad(Es, Ub + Usb);
+    U1b = U1b + Wb + mul_bd(U0b, U2);
+    U2b = U2b + Wb + mul_ad(U1, U0b);
+}
+
+// ---- pi at tau = 0: pi = -(i / 4g) acc (Init.cpp:2521-2584, 2669-2673) --------------
+// x part of acc at site s from D = L1x(s) - L2x(s), U = Ux(s) and the site s-x:
+//   (D U^+ - D - U D^+ + D^+) + (-U'^+ D' + D' + D'^+ U' - D'^+)
+IPG_HD M3 acc_dir(const M3& D, const M3& U, const M3& Dm, const M3& Um) {
+    const M3 Dd = dag(D), Dmd = dag(Dm);
+    M3 acc = ((mul_bd(D, U) - D) - U * Dd) + Dd;
+    return (((acc - mul_ad(Um, Dm)) + Dm) + Dmd * Um) - Dmd;
+}
+IPG_HD void acc_dir_adj(const M3& D, const M3& U, const M3& Dm, const M3& Um, const M3& ab, M3& Db, M3& Ub,
+                        M3& Dmb, M3& Umb) {
+    const M3 abd = dag(ab);
+    Db = Db + ab * U - ab - abd * U + abd;
+    Ub = Ub + abd * D - ab * D;
+    Dmb = Dmb - Um * ab + ab + Um * abd - abd;
+    Umb = Umb - Dm * abd + Dm * ab;
+}
+
+// ---- E = exp(i Q^a t^a) (Matrix.cpp:310-426): Qb_a = Re tr(Eb^+ dE/dQ_a) -------------
+// by central differences of the closed form with step h (error O(h^2))
+IPG_HD void expq_adj(const double (&Q)[8], const M3& Eb, double (&Qb)[8]) {
+    const double h = 1e-5;
+    double q[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) q[a] = Q[a];
+    for (int a = 0; a < 8; a++) {
+        const double q0 = q[a];
+        q[a] = q0 + h;
+        const M3 Ep = exp_iQt(q);
+        q[a] = q0 - h;
+        const M3 Em = exp_iQt(q);
+        q[a] = q0;
+        Qb[a] = retrprod(dag(Eb), Ep - Em) / (2. * h);
+    }
+}
+
 }  // namespace ipgad
+

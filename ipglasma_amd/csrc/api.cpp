@@ -766,6 +766,61 @@ struct Engine {
     // reverse mode through the tau = 0 kick and itmax-1 leapfrog steps with
     // checkpoints every K = ceil(sqrt(itmax-1)) steps.  Consumes the state.
     int adjoint_dedeta(double* dedeta, double* grad) {
+        double* mem = nullptr;
+        const double *s_init = nullptr, *cs0 = nullptr;
+        int rc = adjoint_evolution(dedeta, grad, &mem, &s_init, &cs0);
+        if (mem) (void)hipFree(mem);
+        return rc;
+    }
+
+    // d dE/deta / d rho^a_k(x): the evolution adjoint, then pi at tau = 0, the
+    // root finder (implicit), the links and setV in reverse.  grad receives
+    // [2][Ny][S][8] (nucleus, slice, site, colour).  Call after ipg_init: the
+    // Wilson lines of that call are still in the parity-1 link buffers.
+    int adjoint_rho(double* dedeta, double* grad) {
+        if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
+        EvDesc& d = desc[0];
+        const long long S = d.S;
+        const size_t F = 18 * (size_t)S;
+        double* keep = nullptr;
+        HIPCHK(hipMalloc(&keep, sizeof(double) * (6 * F + 2 * (size_t)d.Ny * S * 8)));
+        std::unique_ptr<double, decltype(&hipFreeVoid)> hold(keep, &hipFreeVoid);
+        double* vab = keep;           // V_A, V_B of the initial conditions
+        double* ulb = vab + 2 * F;    // cotangents of the tau = 0 links (x, y)
+        double* dbb = ulb + 2 * F;    // cotangents of D = L1 - L2 (x, y)
+        double* gr = dbb + 2 * F;     // d/d rho
+        HIPCHK(hipMemcpyAsync(vab, d.VA, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(vab + F, d.VB, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
+        double* mem = nullptr;
+        const double *s_init = nullptr, *cs0 = nullptr;
+        int rc = adjoint_evolution(dedeta, nullptr, &mem, &s_init, &cs0);
+        std::unique_ptr<double, decltype(&hipFreeVoid)> hold2(mem, &hipFreeVoid);
+        if (rc) return rc;
+        // tau = 0 links: total cotangent = from the evolution + from pi
+        HIPCHK(hipMemcpyAsync(ulb, cs0, sizeof(double) * 2 * F, hipMemcpyDeviceToDevice, st));
+        ipg_adj_zero(dbb, (long long)(2 * F), st);
+        ipg_adj_pi_init(d_desc, S, s_init, vab, cs0, ulb, dbb, st);
+        // Wilson-line cotangents (reuse the evolution's cotangent buffer)
+        double* vb = const_cast<double*>(cs0);
+        ipg_adj_zero(vb, (long long)(2 * F), st);
+        ipg_adj_links_newton(d_desc, S, s_init, vab, ulb, dbb, vb, st);
+        // setV in reverse, chunk by chunk (the chunk's A_k are recomputed)
+        const Plan& pl = plans[d.N];
+        FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
+        const int kc = d.kc;
+        const int nch = (d.Ny + kc - 1) / kc;
+        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_desc, dp, c * kc, kc, vab, vb, gr, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (grad) HIPCHK(hipMemcpy(grad, gr, sizeof(double) * 2 * (size_t)d.Ny * S * 8, hipMemcpyDeviceToHost));
+        return IPG_OK;
+    }
+
+    // the evolution part; on success *mem owns the device buffers, *s_init is
+    // the tau = 0 state slot and *cs0 its cotangent set (U_x, U_y, E1, E2,
+    // phi, pi, 18 planes each)
+    int adjoint_evolution(double* dedeta, double* grad, double** mem_out, const double** s_init_out,
+                          const double** cs0_out) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
         EvDesc& d = desc[0];
         const int itm = d.itmax, T = itm - 1;
@@ -777,7 +832,7 @@ struct Engine {
         const size_t nslot = 1 + (size_t)nck + (size_t)K + 1 + 1;
         double* mem = nullptr;
         HIPCHK(hipMalloc(&mem, sizeof(double) * (nslot * slot + 2 * cset)));
-        std::unique_ptr<double, decltype(&hipFreeVoid)> hold(mem, &hipFreeVoid);
+        *mem_out = mem;
         double* s_init = mem;
         double* s_ck = s_init + slot;                 // nck checkpoints
         double* s_seg = s_ck + (size_t)nck * slot;    // K + 1 segment states
@@ -842,6 +897,8 @@ struct Engine {
         if (dedeta) *dedeta = L;
         ev[0].obs.dEdeta = L;
         ev[0].obs.tau_fm = tau_fm;
+        *s_init_out = s_init;
+        *cs0_out = cout;
         return IPG_OK;
     }
 
@@ -1082,6 +1139,10 @@ int ipg_get_tmunu(ipg_ctx* c, double* out) {
 int ipg_adjoint_dedeta(ipg_ctx* c, double* dedeta, double* grad) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     return c->eng.adjoint_dedeta(dedeta, grad);
+}
+int ipg_adjoint_rho(ipg_ctx* c, double* dedeta, double* grad) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    return c->eng.adjoint_rho(dedeta, grad);
 }
 int ipg_autotune_report(char* buf, int len) {
     const std::string t = g_tuner.text();

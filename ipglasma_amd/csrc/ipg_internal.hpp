@@ -51,4 +51,13 @@ void ipg_adj_step(const EvDesc* d_evs, long long S, const double* xb, const doub
                   double* cout, hipStream_t st);
 void ipg_adj_kick(const EvDesc* d_evs, long long S, const double* x0, const double* cin, double* cout,
                   hipStream_t st);
+void ipg_adj_pi_init(const EvDesc* d_evs, long long S, const double* s0, const double* vab, const double* cs0,
+                     double* ulb, double* db, hipStream_t st);
+void ipg_adj_links_newton(const EvDesc* d_evs, long long S, const double* s0, const double* vab, const double* ulb,
+                          const double* db, double* vb, hipStream_t st);
+// setV in reverse for slices [k0, k0+kc) of event 0: vv/vbar are the current
+// Wilson lines and their cotangents (2 nuclei x 18 planes), updated in place;
+// grad[2][Ny][S][8] receives d/d rho
+void ipg_adj_setv_chunk(const EvDesc* d_evs, const FftPlanDev& plan, int k0, int kc, double* vv, double* vbar,
+                        double* grad, hipStream_t st);
 

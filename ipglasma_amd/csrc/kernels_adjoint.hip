@@ -285,7 +285,99 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_kick(const EvDesc* __restrict
     stm(cot(cout, S, 5), S, s, hproj(pib + ((tau + dt / 2.) * dt) * phb));
 }
 
+// ---- back from the tau = 0 fields to the Wilson lines ------------------------------
+// links from the saved Wilson lines (k_links, clamped neighbours)
+__device__ __forceinline__ M3 link_of(const double* V, const Lat& L, int i, int j, int dir) {
+    const size_t s = (size_t)i * L.N + j;
+    const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
+    return mul_bd(ldm(V, L.S, s), ldm(V, L.S, n));
+}
+
+// pi = -(i/4g) acc at tau = 0 (k_pi_init) in reverse: accb = conj(-i/4g) pib;
+// adds to the cotangents of the tau = 0 links U (ulb, 2 x 18 planes) and of
+// D = L1 - L2 (db, 2 x 18 planes)
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_pi_init(const EvDesc* __restrict__ evs,
+                                                          const double* __restrict__ s0,
+                                                          const double* __restrict__ vab,
+                                                          const double* __restrict__ cs0, double* __restrict__ ulb,
+                                                          double* __restrict__ db) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const Slot X = slot_of(s0, S);
+    const double* VA = vab;
+    const double* VB = vab + 18 * S;
+    const double c = -1. / (4. * ev.g);
+    const M3 accb = cscale(cd{0., -c}, hproj(ldm(cot(const_cast<double*>(cs0), S, 5), S, s)));
+    for (int dir = 0; dir < 2; dir++) {
+        const int im = dir == 0 ? i - 1 : i, jm = dir == 0 ? j : j - 1;
+        const size_t sm = L.at(im, jm);
+        const int imc = (int)(sm / N), jmc = (int)(sm - (size_t)imc * N);
+        const M3 D = link_of(VA, L, i, j, dir) - link_of(VB, L, i, j, dir);
+        const M3 Dm = link_of(VA, L, imc, jmc, dir) - link_of(VB, L, imc, jmc, dir);
+        const double* Uf = dir == 0 ? X.Ux : X.Uy;
+        M3 Db = m_zero(), Ub = m_zero(), Dmb = m_zero(), Umb = m_zero();
+        acc_dir_adj(D, ldm(Uf, S, s), Dm, ldm(Uf, S, sm), accb, Db, Ub, Dmb, Umb);
+        double* ul = ulb + (size_t)dir * 18 * S;
+        double* dd = db + (size_t)dir * 18 * S;
+        atomic_add_m3(ul, S, s, Ub);
+        atomic_add_m3(ul, S, sm, Umb);
+        atomic_add_m3(dd, S, s, Db);
+        atomic_add_m3(dd, S, sm, Dmb);
+    }
+}
+
+// root finder (implicit) and links in reverse: for each link, U1b, U2b from
+// the total cotangent of its solution, plus Db (L1b += Db, L2b -= Db); then
+// L = V(s) V(s+d)^+ adds to the Wilson-line cotangents (vb, 2 x 18 planes)
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __restrict__ evs,
+                                                               const double* __restrict__ s0,
+                                                               const double* __restrict__ vab,
+                                                               const double* __restrict__ ulb,
+                                                               const double* __restrict__ db,
+                                                               double* __restrict__ vb) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const Slot X = slot_of(s0, S);
+    const double* VA = vab;
+    const double* VB = vab + 18 * S;
+    for (int dir = 0; dir < 2; dir++) {
+        const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
+        const M3 va = ldm(VA, S, s), van = ldm(VA, S, n), vbs = ldm(VB, S, s), vbn = ldm(VB, S, n);
+        const M3 U1 = mul_bd(va, van), U2 = mul_bd(vbs, vbn);
+        const M3 U = ldm(dir == 0 ? X.Ux : X.Uy, S, s);
+        M3 U1b = m_zero(), U2b = m_zero();
+        newton_adj(U1, U2, U, ldm(ulb + (size_t)dir * 18 * S, S, s), U1b, U2b);
+        const M3 Db = ldm(db + (size_t)dir * 18 * S, S, s);
+        U1b = U1b + Db;
+        U2b = U2b - Db;
+        // L = V(s) V(n)^+:  V(s) b += Lb V(n),  V(n) b += Lb^+ V(s)
+        atomic_add_m3(vb, S, s, U1b * van);
+        atomic_add_m3(vb, S, n, mul_ad(U1b, va));
+        atomic_add_m3(vb + 18 * S, S, s, U2b * vbn);
+        atomic_add_m3(vb + 18 * S, S, n, mul_ad(U2b, vbs));
+    }
+}
+
 }  // namespace
+
+void ipg_adj_pi_init(const EvDesc* d, long long S, const double* s0, const double* vab, const double* cs0,
+                     double* ulb, double* db, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_pi_init, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0, st, d,
+                       s0, vab, cs0, ulb, db);
+}
+void ipg_adj_links_newton(const EvDesc* d, long long S, const double* s0, const double* vab, const double* ulb,
+                          const double* db, double* vb, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_links_newton, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0,
+                       st, d, s0, vab, ulb, db, vb);
+}
 
 static unsigned nb_of(long long n) { return (unsigned)((n + IPG_BLOCK - 1) / IPG_BLOCK); }
 

@@ -14,6 +14,7 @@
 #include "ipg_device.hpp"
 #include "ipg_internal.hpp"
 #include "fft_lds.hpp"
+#include "adjoint.hpp"
 
 namespace {
 
@@ -511,6 +512,131 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
     }
 }
 
+// ---- reverse mode of setV (one event) ---------------------------------------------
+// V = E_{Ny-1} ... E_0, E_k = exp(-i A_k^a t^a).  Walking the slices of a chunk
+// backwards with the current V and its cotangent Vb in registers:
+//   V_k = E_k^+ V,  Eb_k = Vb V_k^+,  Vb <- E_k^+ Vb,  Ab_k = -Qb(Q = -A_k, Eb_k).
+// The chunk's A_k are recomputed by the forward row and column passes; Ab_k
+// replaces A_k in the transform buffer, row by row.
+template <int N, int BLK>
+__global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0, int kc,
+                                                      double* __restrict__ vv, double* __restrict__ vbar) {
+    constexpr int SPT = (N + BLK - 1) / BLK;
+    __shared__ double2 lds[4 * N];
+    const int n = blockIdx.x / N;
+    const int i = blockIdx.x % N;
+    const EvDesc& ev = evs[0];
+    if (!ev.active || k0 >= ev.Ny) return;
+    const size_t S = ev.S;
+    double* Vg = vv + (size_t)n * 18 * S;
+    double* Bg = vbar + (size_t)n * 18 * S;
+    M3 V[SPT], Vb[SPT];
+#pragma unroll
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * BLK;
+        V[t] = j < N ? ldm(Vg, S, (size_t)i * N + j) : m_one();
+        Vb[t] = j < N ? ldm(Bg, S, (size_t)i * N + j) : m_zero();
+    }
+    const int kend = min(kc, ev.Ny - k0);
+    for (int kk = kend - 1; kk >= 0; kk--) {
+        double2* io = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
+        for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
+            const int p = q / N, j = q - p * N;
+            lds[q] = io[(size_t)p * S + (size_t)i * N + j];
+        }
+        __syncthreads();
+        ipgfft::lds_fft<N, 4, BLK, true>(lds, pl.tw);
+#pragma unroll
+        for (int t = 0; t < SPT; t++) {
+            const int j = threadIdx.x + t * BLK;
+            if (j < N) {
+                double Q[8];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    const double2 v = lds[p * N + j];
+                    Q[2 * p] = -v.x;
+                    Q[2 * p + 1] = -v.y;
+                }
+                const M3 E = exp_iQt(Q);
+                const M3 Vk = mul_ad(E, V[t]);
+                const M3 Eb = mul_bd(Vb[t], Vk);
+                Vb[t] = mul_ad(E, Vb[t]);
+                V[t] = Vk;
+                double Qb[8];
+                ipgad::expq_adj(Q, Eb, Qb);
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+                    io[(size_t)p * S + (size_t)i * N + j] = make_double2(-Qb[2 * p], -Qb[2 * p + 1]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * BLK;
+        if (j < N) {
+            stm(Vg, S, (size_t)i * N + j, V[t]);
+            stm(Bg, S, (size_t)i * N + j, Vb[t]);
+        }
+    }
+}
+
+// the Poisson filter is a real, even convolution (self-adjoint): Ab -> rhob
+// by the same row / column / row passes; rows forward from the buffer
+template <int N>
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
+                                                                  int k0, int kc) {
+    __shared__ double2 lds[4 * N];
+    const int per_n = kc * N;
+    const int n = blockIdx.x / per_n;
+    const int kk = (blockIdx.x / N) % kc;
+    const int i = blockIdx.x % N;
+    const EvDesc& ev = evs[0];
+    if (!ev.active || k0 + kk >= ev.Ny) return;
+    const size_t S = ev.S;
+    double2* io = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
+    for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
+        const int p = q / N, j = q - p * N;
+        lds[q] = io[(size_t)p * S + (size_t)i * N + j];
+    }
+    __syncthreads();
+    ipgfft::lds_fft<N, 4, IPG_BLOCK, false>(lds, pl.tw);
+    for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
+        const int p = q / N, j = q - p * N;
+        io[(size_t)p * S + (size_t)i * N + j] = lds[q];
+    }
+}
+// rows inverse: rhob for colours (2p, 2p+1) = (Re, Im) of packed field p into
+// grad[n][k][s][8]
+template <int N>
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl,
+                                                                  int k0, int kc, double* __restrict__ grad) {
+    __shared__ double2 lds[4 * N];
+    const int per_n = kc * N;
+    const int n = blockIdx.x / per_n;
+    const int kk = (blockIdx.x / N) % kc;
+    const int i = blockIdx.x % N;
+    const EvDesc& ev = evs[0];
+    const int k = k0 + kk;
+    if (!ev.active || k >= ev.Ny) return;
+    const size_t S = ev.S;
+    const double2* in = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
+    for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
+        const int p = q / N, j = q - p * N;
+        lds[q] = in[(size_t)p * S + (size_t)i * N + j];
+    }
+    __syncthreads();
+    ipgfft::lds_fft<N, 4, IPG_BLOCK, true>(lds, pl.tw);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        double* g = grad + (((size_t)n * ev.Ny + k) * S + (size_t)i * N + j) * 8;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            g[2 * p] = lds[p * N + j].x;
+            g[2 * p + 1] = lds[p * N + j].y;
+        }
+    }
+}
+
 __global__ __launch_bounds__(IPG_BLOCK) void k_unit_v(const EvDesc* __restrict__ evs, int nev) {
     const int e = find_event(evs, nev, blockIdx.x);
     const EvDesc& ev = evs[e];
@@ -954,6 +1080,41 @@ void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k
         default: break;  // rejected by the engine set-up
     }
 #undef IPG_SETV_CASE
+}
+template <int N>
+static void adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
+                           double* grad, hipStream_t st) {
+    constexpr int C = ipgfft::cols_per_block(N);
+    const int nrow = 2 * kc * N;
+    const int ncol = 2 * kc * 4 * (N / C);
+    // recompute the chunk's A_k (forward row and column passes)
+    hipLaunchKernelGGL(k_setv_rows_fwd<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL((k_adj_setv_rows<N, 256>), dim3(2 * N), dim3(256), 0, st, d, pl, k0, kc, vv, vbar);
+    hipLaunchKernelGGL(k_adj_filter_rows_fwd<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    hipLaunchKernelGGL(k_adj_filter_rows_inv<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc, grad);
+}
+void ipg_adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
+                        double* grad, hipStream_t st) {
+#define IPG_ADJ_CASE(n) \
+    case n: adj_setv_chunk<n>(d, pl, k0, kc, vv, vbar, grad, st); break;
+    switch (pl.N) {
+        IPG_ADJ_CASE(16)
+        IPG_ADJ_CASE(32)
+        IPG_ADJ_CASE(48)
+        IPG_ADJ_CASE(64)
+        IPG_ADJ_CASE(96)
+        IPG_ADJ_CASE(128)
+        IPG_ADJ_CASE(192)
+        IPG_ADJ_CASE(256)
+        IPG_ADJ_CASE(384)
+        IPG_ADJ_CASE(512)
+        IPG_ADJ_CASE(768)
+        IPG_ADJ_CASE(1024)
+        default: break;
+    }
+#undef IPG_ADJ_CASE
 }
 void ipg_launch_unit_v(const EvDesc* d, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_unit_v, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);

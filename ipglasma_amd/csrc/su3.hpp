@@ -304,7 +304,7 @@ IPG_HD void expm_coeff(const double Q[8], cd out[9]) {
     }
 #pragma unroll
     for (int k = 0; k < 9; k++)
-        if (isnan(out[k].r) || isnan(out[k].i)) out[k] = cd{0., 0.};
+        if (out[k].r != out[k].r || out[k].i != out[k].i) out[k] = cd{0., 0.};
 }
 
 // exp(i Q^a t^a); the identity when |Re u0| < 1e-15 (Init.cpp:3488-3489)

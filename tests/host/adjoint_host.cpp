@@ -72,4 +72,27 @@ double adjh_tt(const double* x, int plaq, double tau, double g, double yb, doubl
     pack(xb, b, 16);
     return y;
 }
+// root finder: inputs U1, U2 and the solution U; Ub -> U1b, U2b
+void adjh_newton(const double* u1, const double* u2, const double* u, const double* ub, double* u1b,
+                 double* u2b) {
+    M3 b1 = m_zero(), b2 = m_zero();
+    newton_adj(ld(u1), ld(u2), ld(u), ld(ub), b1, b2);
+    st(u1b, b1);
+    st(u2b, b2);
+}
+// one direction of the pi assembly: inputs D, U, Dm, Um; output acc
+void adjh_acc(const double* x, const double* ab, double* y, double* xb) {
+    const M3 D = ld(x), U = ld(x + 18), Dm = ld(x + 36), Um = ld(x + 54);
+    st(y, acc_dir(D, U, Dm, Um));
+    M3 b[4] = {m_zero(), m_zero(), m_zero(), m_zero()};
+    acc_dir_adj(D, U, Dm, Um, ld(ab), b[0], b[1], b[2], b[3]);
+    for (int k = 0; k < 4; k++) st(xb + 18 * k, b[k]);
+}
+void adjh_expq(const double* q, const double* eb, double* e, double* qb) {
+    double Q[8], Qb[8];
+    for (int a = 0; a < 8; a++) Q[a] = q[a];
+    st(e, exp_iQt(Q));
+    expq_adj(Q, ld(eb), Qb);
+    for (int a = 0; a < 8; a++) qb[a] = Qb[a];
+}
 }

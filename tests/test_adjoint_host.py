@@ -32,6 +32,9 @@ def lib():
         L.adjh_u.argtypes = [dp, C.c_double, dp, dp, dp]
         L.adjh_tt.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, dp]
         L.adjh_tt.restype = C.c_double
+        L.adjh_newton.argtypes = [dp] * 6
+        L.adjh_acc.argtypes = [dp] * 4
+        L.adjh_expq.argtypes = [dp] * 4
         _lib = L
     return _lib
 
@@ -143,3 +146,78 @@ def test_tautau_adjoint(plaq):
         fm = L.adjh_tt(np.ascontiguousarray(x - h * d), plaq, 1.3, 2.0, yb, scratch)
         fd = yb * (fp - fm) / (2 * h)
         assert abs(np.dot(xb, d) - fd) <= 1e-6 * max(1.0, abs(fd)), (np.dot(xb, d), fd)
+
+
+def test_pi_assembly_adjoint():
+    rng = np.random.default_rng(7)
+    L = lib()
+    x = cm(np.concatenate([rand_link(rng, 4)]))
+
+    def f(x, yb):
+        y = np.zeros(18)
+        xb = np.zeros(72)
+        L.adjh_acc(np.ascontiguousarray(x), np.ascontiguousarray(yb), y, xb)
+        return y, xb
+    fd_check(f, x, [0, 0, 0, 0], 1, rng)
+
+
+def test_root_finder_implicit_adjoint():
+    """against central differences of the oracle's root finder (an
+    independent forward implementation), at unitary U1, U2"""
+    import scipy.linalg as sl
+    import _oracle as O
+    rng = np.random.default_rng(8)
+    L = lib()
+
+    def su3(scale):
+        a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        h = scale * (a + a.conj().T) / 2
+        return sl.expm(1j * (h - np.trace(h) / 3 * np.eye(3)))
+
+    for trial in range(5):
+        U1, U2 = su3(0.6), su3(0.6)
+        U, st = O.newton_link(U1, U2)
+        assert st.converged
+        Ub = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        b1 = np.zeros(18)
+        b2 = np.zeros(18)
+        L.adjh_newton(cm(U1), cm(U2), cm(U), cm(Ub), b1, b2)
+        d1 = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        d2 = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        h = 1e-6
+
+        def loss(a, b):
+            Us, s2 = O.newton_link(a, b)
+            assert s2.converged
+            return float(np.real(np.trace(Ub.conj().T @ Us)))
+        fd = (loss(U1 + h * d1, U2 + h * d2) - loss(U1 - h * d1, U2 - h * d2)) / (2 * h)
+        pred = float(np.dot(b1, cm(d1)) + np.dot(b2, cm(d2)))
+        assert abs(pred - fd) <= 1e-5 * max(1.0, abs(fd)), (trial, pred, fd)
+
+
+def test_su3_exponential_adjoint_against_frechet():
+    """Qb_a = Re tr(Eb^+ dE/dQ_a) against scipy's Frechet derivative of expm"""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(9)
+    L = lib()
+    lam = np.zeros((8, 3, 3), complex)
+    lam[0][0, 1] = lam[0][1, 0] = 0.5
+    lam[1][0, 1], lam[1][1, 0] = -0.5j, 0.5j
+    lam[2][0, 0], lam[2][1, 1] = 0.5, -0.5
+    lam[3][0, 2] = lam[3][2, 0] = 0.5
+    lam[4][0, 2], lam[4][2, 0] = -0.5j, 0.5j
+    lam[5][1, 2] = lam[5][2, 1] = 0.5
+    lam[6][1, 2], lam[6][2, 1] = -0.5j, 0.5j
+    lam[7] = np.diag([1.0, 1.0, -2.0]) / (2 * np.sqrt(3))
+    for scale in (0.05, 0.7, 3.0):
+        Q = rng.normal(size=8) * scale
+        Eb = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        e = np.zeros(18)
+        qb = np.zeros(8)
+        L.adjh_expq(Q, cm(Eb), e, qb)
+        H = 1j * np.einsum("a,aij->ij", Q, lam)
+        E = sl.expm(H)
+        assert np.allclose(e[0::2] + 1j * e[1::2], E.reshape(-1), atol=1e-13)
+        ref = np.array([np.real(np.trace(Eb.conj().T @ sl.expm_frechet(H, 1j * lam[a], compute_expm=False)))
+                        for a in range(8)])
+        assert np.allclose(qb, ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max()), (scale, qb, ref)

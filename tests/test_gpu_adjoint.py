@@ -52,3 +52,54 @@ def test_adjoint_dedeta_matches_finite_differences(N, maxtime):
             xm[k] = xm[k] - h * d[k]
         fd = (_loss_at(p, xp) - _loss_at(p, xm)) / (2 * h)
         assert abs(pred - fd) <= 1e-6 * abs(fd), (trial, pred, fd)
+
+
+def _loss_z(p, seed, z):
+    import ipglasma_amd as I
+    ev = I.Event(p, seed=seed)
+    ev.set_rho_normals(z)
+    assert ev.init()
+    ev.run()
+    v = ev.observables()["dEdeta"]
+    ev.close()
+    return v
+
+
+@pytest.mark.parametrize("N,Ny", [(32, 3), (64, 2)])
+def test_adjoint_rho_matches_finite_differences(N, Ny):
+    """d(dE/deta)/d rho through evolution, pi, root finder, links and setV,
+    against central differences in the colour-charge normals z
+    (rho = g sqrt(g2mu2/Ny) z).  The forward root finder stops at
+    sum |F| <= 1e-6 with a forward-difference Jacobian (linear convergence),
+    so its link solutions carry ~1e-9 errors that move with every
+    perturbation: the step is 2e-4 (noise ~5e-6, truncation ~1e-7) and the
+    bound 1e-4 relative.  Coarser lattices (N=16 at L=8 fm, a=0.5 fm) are not
+    used: there the root finder restarts and changes branch under such steps,
+    so the forward map is not smooth at any usable step."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=N, Ny=Ny, maxtime=0.4, L=8.0)
+    seed = 21
+    probe = I.Event(p, seed=seed)
+    probe.init_nucleons()
+    ok = probe.init_color_charge()
+    assert ok
+    z = np.stack([np.stack([probe.rho_normals(n, k) for k in range(Ny)]) for n in range(2)])
+    probe.close()
+    ev = I.Event(p, seed=seed)
+    ev.set_rho_normals(z)
+    assert ev.init()
+    gA, gB = ev.field("g2mu2A"), ev.field("g2mu2B")
+    L0, grho = ev.adjoint_rho()
+    ev.close()
+    assert L0 == pytest.approx(_loss_z(p, seed, z), rel=1e-12)
+    amp = np.stack([p.g * np.sqrt(gA / Ny), p.g * np.sqrt(gB / Ny)])  # [2, S]
+    gz = grho * amp[:, None, :, None]
+    rng = np.random.default_rng(N + Ny)
+    for trial in range(3):
+        d = rng.normal(size=z.shape)
+        h = 2e-4
+        fd = (_loss_z(p, seed, z + h * d) - _loss_z(p, seed, z - h * d)) / (2 * h)
+        pred = float(np.sum(gz * d))
+        print(f"rho gradient N={N} trial {trial}: adjoint {pred:.10g} central difference {fd:.10g} "
+              f"rel {abs(pred - fd) / abs(fd):.2e}")
+        assert abs(pred - fd) <= 1e-4 * abs(fd), (trial, pred, fd)

@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -68,20 +70,27 @@ def adjoint_timing(I, seed=77):
             pass
         t1 = time.perf_counter()
         if adj:
-            L, g = ev.adjoint_rho()
+            L, _ = ev.adjoint_rho(fetch=False)
         else:
             ev.run()
             L = ev.observables()["dEdeta"]
         t2 = time.perf_counter()
+        t3 = t2
+        if adj:
+            g = ev.rho_grad()
+            t3 = time.perf_counter()
+            if not np.all(np.isfinite(g)):
+                raise RuntimeError("non-finite rho gradient")
         ev.close()
-        return t1 - t0, t2 - t1, L
+        return t1 - t0, t2 - t1, t3 - t2, L
     timed(False)
     timed(True)
-    i_f, e_f, Lf = timed(False)
-    i_a, e_a, La = timed(True)
+    i_f, e_f, _, Lf = timed(False)
+    i_a, e_a, f_a, La = timed(True)
     return {"workload": "Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c, d(dE/deta)/d rho",
             "forward_ms": round(1e3 * (i_f + e_f), 2), "adjoint_ms": round(1e3 * (i_a + e_a), 2),
             "adjoint_over_forward": round((i_a + e_a) / (i_f + e_f), 2),
+            "gradient_to_host_ms": round(1e3 * f_a, 2),
             "dEdeta_rel_diff": abs(La - Lf) / abs(Lf)}
 
 
@@ -236,6 +245,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode timing")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the reverse-mode timing (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (the HBM counter calibration)")
     a = ap.parse_args()
@@ -249,6 +260,9 @@ def main():
         copy_probe(a.copy_probe)
     if a.roofline_only:
         print(json.dumps(roofline(I, p)))
+        return
+    if a.adjoint_only:
+        print(json.dumps(adjoint_timing(I)))
         return
     seeds = list(range(1000, 1000 + a.batch))
     for w in range(a.warmup):

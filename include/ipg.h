@@ -163,8 +163,15 @@ int ipg_adjoint_dedeta(ipg_ctx *c, double *dedeta, double *grad);
  * implicit differentiation (Init.cpp:3344-3481), the links, the ordered
  * product of slice exponentials and the Poisson solve (Init.cpp:1678-1850).
  * rho = g sqrt(g2mu2/Ny) z with z the normals.  Call after ipg_init in place
- * of ipg_evolve: the state is consumed.  No counterpart in the reference. */
+ * of ipg_evolve: the state is consumed.  With grad NULL the gradient stays in
+ * device memory owned by the context (ipg_get_rho_grad copies it out); the
+ * context keeps its reverse-mode workspace (checkpoints and cotangents) for
+ * the next call.  No counterpart in the reference. */
 int ipg_adjoint_rho(ipg_ctx *c, double *dedeta, double *grad);
+
+/* Host copy of the last ipg_adjoint_rho gradient, grad[2][Ny][S][8].
+ * IPG_ERR_STATE when there is none.  No counterpart in the reference. */
+int ipg_get_rho_grad(ipg_ctx *c, double *grad);
 
 /* Autotuner state: one line per tuned kernel ("key: cand=best time ...
  * chosen=k").  The variants of a key compute identical results (launch shape

@@ -127,6 +127,7 @@ def lib():
             "ipg_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
             "ipg_adjoint_dedeta": (C.c_int, [vp, dp, dp]),
             "ipg_adjoint_rho": (C.c_int, [vp, dp, dp]),
+            "ipg_get_rho_grad": (C.c_int, [vp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
@@ -258,13 +259,19 @@ class Event:
         _check(lib().ipg_adjoint_dedeta(self._c, C.byref(v), _dptr(g)))
         return v.value, {k: g[i] for i, k in enumerate(("Ux", "Uy", "E1", "E2", "phi", "pi"))}
 
-    def adjoint_rho(self):
+    def adjoint_rho(self, fetch=True):
         """dE/deta and d(dE/deta)/d rho as [2, Ny, S, 8] (consumes the state;
-        call after init)"""
-        g = np.zeros((2, self.p.Ny, self.S, 8))
+        call after init).  fetch=False leaves the gradient on the device and
+        returns None in its place (rho_grad() copies it later)."""
         v = C.c_double()
-        _check(lib().ipg_adjoint_rho(self._c, C.byref(v), _dptr(g)))
-        return v.value, g
+        _check(lib().ipg_adjoint_rho(self._c, C.byref(v), None))
+        return v.value, (self.rho_grad() if fetch else None)
+
+    def rho_grad(self):
+        """the last adjoint_rho gradient, [2, Ny, S, 8]"""
+        g = np.empty((2, self.p.Ny, self.S, 8))
+        _check(lib().ipg_get_rho_grad(self._c, _dptr(g)))
+        return g
 
     def landau(self):
         """[15, S]: eps, u^tau, u^x, u^y, u^eta, pi^{mu nu} (T's order)"""

@@ -43,15 +43,16 @@ IPG_HD void adj_conjd(const M3& V, const M3& G, const M3& Yb, M3& Vb, M3& Gb) {
     Vb = Vb + (G * V) * (Yb + dag(Yb));
     Gb = Gb + mul_bd(V * Yb, V);
 }
-// Y = M1 M2 M3 M4: cotangents of the four factors
+// Y = M1 M2 M3 M4: cotangents of the four factors (each product is formed
+// just before its last use)
 IPG_HD void adj_prod4(const M3& M1, const M3& M2, const M3& M3_, const M3& M4, const M3& Yb, M3& B1, M3& B2,
                       M3& B3, M3& B4) {
-    const M3 P12 = M1 * M2, P34 = M3_ * M4;
-    const M3 P234 = M2 * P34, P123 = P12 * M3_;
-    B1 = mul_bd(Yb, P234);                   // Yb (M2 M3 M4)^+
+    const M3 P34 = M3_ * M4;
+    B1 = mul_bd(Yb, M2 * P34);               // Yb (M2 M3 M4)^+
     B2 = mul_bd(mul_ad(M1, Yb), P34);        // M1^+ Yb (M3 M4)^+
+    const M3 P12 = M1 * M2;
     B3 = mul_bd(mul_ad(P12, Yb), M4);        // (M1 M2)^+ Yb M4^+
-    B4 = mul_ad(P123, Yb);                   // (M1 M2 M3)^+ Yb
+    B4 = mul_ad(P12 * M3_, Yb);              // (M1 M2 M3)^+ Yb
 }
 
 // ---- pi update -------------------------------------------------------------------
@@ -102,17 +103,32 @@ IPG_HD void e_fwd(const EIn& q, double tau, double dtau, double g, M3& e1n, M3& 
     t2 = ty * q.ph - q.ph * ty;
     e2n = traceless(q.e2 + (cscale(c1, t1) + cscale(c2, t2)));
 }
+// the E' update in reverse, in pieces: Z1, Z2 = TL(E1'b), TL(E2'b) give the
+// cotangents of the three plaquette products U12, U1m2, U2m1 ...
+IPG_HD void plaq_bars(const M3& Z1, const M3& Z2, double tau, double dtau, double g, M3& U12b, M3& U1m2b,
+                      M3& U2m1b) {
+    const cd c1c{0., -tau * dtau / (2. * g * g)};  // conj(c1)
+    const M3 t1b = traceless(cscale(c1c, Z1)), t1pb = traceless(cscale(c1c, Z2));
+    // t1 = U12 + U1m2 - U12^+ - U1m2^+, t1' = U12^+ + U2m1 - U12 - U2m1^+
+    U12b = (t1b - dag(t1b)) + (dag(t1pb) - t1pb);
+    U1m2b = t1b - dag(t1b);
+    U2m1b = t1pb - dag(t1pb);
+}
+// ... and, with t2b = conj(c2) Z, the commutator t2 = tx phi - phi tx,
+// tx = U phn U^+ (tx, phi Hermitian): cotangents of phi, U and phn
+IPG_HD void comm_adj(const M3& U, const M3& phn, const M3& ph, const M3& t2b, M3& phb, M3& Ub, M3& phnb) {
+    const M3 tx = U * mul_bd(phn, U);
+    const M3 txb = t2b * ph - ph * t2b;
+    phb = phb + (tx * t2b - t2b * tx);
+    adj_conj(U, phn, txb, Ub, phnb);
+}
 IPG_HD void e_adj(const EIn& q, double tau, double dtau, double g, const M3& Y1b, const M3& Y2b, EBar& b) {
-    const cd c1c{0., -tau * dtau / (2. * g * g)}, c2c{0., -dtau / tau};  // conjugates
+    const cd c2c{0., -dtau / tau};  // conj(c2)
     const M3 Z1 = traceless(Y1b), Z2 = traceless(Y2b);
     b.e1 = b.e1 + Z1;
     b.e2 = b.e2 + Z2;
-    const M3 t1b = traceless(cscale(c1c, Z1)), t1pb = traceless(cscale(c1c, Z2));
-    const M3 t2b = cscale(c2c, Z1), t2pb = cscale(c2c, Z2);
-    // plaquettes: t1 = U12 + U1m2 - U12^+ - U1m2^+, t1' = U12^+ + U2m1 - U12 - U2m1^+
-    const M3 U12b = (t1b - dag(t1b)) + (dag(t1pb) - t1pb);
-    const M3 U1m2b = t1b - dag(t1b);
-    const M3 U2m1b = t1pb - dag(t1pb);
+    M3 U12b, U1m2b, U2m1b;
+    plaq_bars(Z1, Z2, tau, dtau, g, U12b, U1m2b, U2m1b);
     M3 B1, B2, B3, B4;
     // U12 = ux uy_px ux_py^+ uy^+
     adj_prod4(q.ux, q.uy_px, dag(q.ux_py), dag(q.uy), U12b, B1, B2, B3, B4);
@@ -132,14 +148,8 @@ IPG_HD void e_adj(const EIn& q, double tau, double dtau, double g, const M3& Y1b
     b.ux_mxpy = b.ux_mxpy + dag(B2);
     b.uy_mx = b.uy_mx + dag(B3);
     b.ux_mx = b.ux_mx + B4;
-    // commutators t2 = tx phi - phi tx (tx, phi Hermitian)
-    const M3 tx = q.ux * mul_bd(q.phpx, q.ux), ty = q.uy * mul_bd(q.phpy, q.uy);
-    M3 txb = t2b * q.ph - q.ph * t2b;
-    b.ph = b.ph + (tx * t2b - t2b * tx);
-    adj_conj(q.ux, q.phpx, txb, b.ux, b.phpx);
-    M3 tyb = t2pb * q.ph - q.ph * t2pb;
-    b.ph = b.ph + (ty * t2pb - t2pb * ty);
-    adj_conj(q.uy, q.phpy, tyb, b.uy, b.phpy);
+    comm_adj(q.ux, q.phpx, q.ph, cscale(c2c, Z1), b.ph, b.ux, b.phpx);
+    comm_adj(q.uy, q.phpy, q.ph, cscale(c2c, Z2), b.ph, b.uy, b.phpy);
 }
 
 // ---- link update U' = (1 + X + X^2/2) U, X = i kappa E' ---------------------------

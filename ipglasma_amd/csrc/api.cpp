@@ -201,6 +201,12 @@ struct Engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_itm = -1;
+    // reverse-mode workspace (checkpoints, cotangents, the rho gradient),
+    // kept between calls; grown on demand
+    double* adj_ws = nullptr;
+    size_t adj_ws_n = 0;
+    double* rho_grad = nullptr;  // last ipg_adjoint_rho result, in adj_ws
+    size_t rho_grad_n = 0;
 
     ~Engine() {
         if (gexec) hipGraphExecDestroy(gexec);
@@ -221,6 +227,7 @@ struct Engine {
         if (d_work) hipFree(d_work);
         if (d_out) hipFree(d_out);
         if (d_obs) hipFree(d_obs);
+        if (adj_ws) hipFree(adj_ws);
         for (hipEvent_t t : tev) hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
             hipEventDestroy(t.a);
@@ -766,35 +773,58 @@ struct Engine {
     // reverse mode through the tau = 0 kick and itmax-1 leapfrog steps with
     // checkpoints every K = ceil(sqrt(itmax-1)) steps.  Consumes the state.
     int adjoint_dedeta(double* dedeta, double* grad) {
-        double* mem = nullptr;
         const double *s_init = nullptr, *cs0 = nullptr;
-        int rc = adjoint_evolution(dedeta, grad, &mem, &s_init, &cs0);
-        if (mem) (void)hipFree(mem);
-        return rc;
+        return adjoint_evolution(dedeta, grad, &s_init, &cs0);
+    }
+
+    // doubles of the evolution part of the workspace
+    size_t adj_evolution_doubles(const EvDesc& d) const {
+        const int T = d.itmax - 1;
+        const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
+        const size_t nslot = 1 + (size_t)(T / K + 1) + (size_t)K + 1 + 1;
+        return nslot * 72 * (size_t)d.S + (2 * IPG_PEND_PLANES + 108) * (size_t)d.S;
+    }
+
+    // grows the workspace to at least n doubles (contents not kept)
+    int adj_reserve(size_t n) {
+        if (n <= adj_ws_n) return IPG_OK;
+        if (adj_ws) HIPCHK(hipFree(adj_ws));
+        adj_ws = nullptr;
+        adj_ws_n = 0;
+        rho_grad = nullptr;
+        rho_grad_n = 0;
+        HIPCHK(hipMalloc(&adj_ws, sizeof(double) * n));
+        adj_ws_n = n;
+        return IPG_OK;
     }
 
     // d dE/deta / d rho^a_k(x): the evolution adjoint, then pi at tau = 0, the
-    // root finder (implicit), the links and setV in reverse.  grad receives
-    // [2][Ny][S][8] (nucleus, slice, site, colour).  Call after ipg_init: the
-    // Wilson lines of that call are still in the parity-1 link buffers.
+    // root finder (implicit), the links and setV in reverse.  The gradient
+    // [2][Ny][S][8] (nucleus, slice, site, colour) stays in the workspace
+    // (rho_grad) and is copied to grad when grad is not null.  Call after
+    // ipg_init: the Wilson lines of that call are still in the V buffers.
     int adjoint_rho(double* dedeta, double* grad) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
         EvDesc& d = desc[0];
         const long long S = d.S;
         const size_t F = 18 * (size_t)S;
-        double* keep = nullptr;
-        HIPCHK(hipMalloc(&keep, sizeof(double) * (6 * F + 2 * (size_t)d.Ny * S * 8)));
-        std::unique_ptr<double, decltype(&hipFreeVoid)> hold(keep, &hipFreeVoid);
+        const size_t nev_part = adj_evolution_doubles(d);
+        const size_t ng = 2 * (size_t)d.Ny * S * 8;
+        int rc = adj_reserve(nev_part + 6 * F + ng);
+        if (rc) return rc;
+        rho_grad = nullptr;
+        rho_grad_n = 0;
+        // [evolution part (offset 0) | V_A, V_B | link cotangents | D cotangents | gradient]
+        if (nev_part + 6 * F + ng > adj_ws_n) return fail(IPG_ERR_STATE, "adjoint workspace too small");
+        double* keep = adj_ws + nev_part;
         double* vab = keep;           // V_A, V_B of the initial conditions
         double* ulb = vab + 2 * F;    // cotangents of the tau = 0 links (x, y)
         double* dbb = ulb + 2 * F;    // cotangents of D = L1 - L2 (x, y)
         double* gr = dbb + 2 * F;     // d/d rho
         HIPCHK(hipMemcpyAsync(vab, d.VA, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(vab + F, d.VB, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
-        double* mem = nullptr;
         const double *s_init = nullptr, *cs0 = nullptr;
-        int rc = adjoint_evolution(dedeta, nullptr, &mem, &s_init, &cs0);
-        std::unique_ptr<double, decltype(&hipFreeVoid)> hold2(mem, &hipFreeVoid);
+        rc = adjoint_evolution(dedeta, nullptr, &s_init, &cs0);
         if (rc) return rc;
         // tau = 0 links: total cotangent = from the evolution + from pi
         HIPCHK(hipMemcpyAsync(ulb, cs0, sizeof(double) * 2 * F, hipMemcpyDeviceToDevice, st));
@@ -812,15 +842,23 @@ struct Engine {
         for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_desc, dp, c * kc, kc, vab, vb, gr, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        if (grad) HIPCHK(hipMemcpy(grad, gr, sizeof(double) * 2 * (size_t)d.Ny * S * 8, hipMemcpyDeviceToHost));
+        rho_grad = gr;
+        rho_grad_n = ng;
+        if (grad) HIPCHK(hipMemcpy(grad, gr, sizeof(double) * ng, hipMemcpyDeviceToHost));
         return IPG_OK;
     }
 
-    // the evolution part; on success *mem owns the device buffers, *s_init is
-    // the tau = 0 state slot and *cs0 its cotangent set (U_x, U_y, E1, E2,
-    // phi, pi, 18 planes each)
-    int adjoint_evolution(double* dedeta, double* grad, double** mem_out, const double** s_init_out,
-                          const double** cs0_out) {
+    int get_rho_grad(double* grad) {
+        if (!rho_grad) return fail(IPG_ERR_STATE, "no gradient: call ipg_adjoint_rho first");
+        HIPCHK(hipMemcpy(grad, rho_grad, sizeof(double) * rho_grad_n, hipMemcpyDeviceToHost));
+        return IPG_OK;
+    }
+
+    // the evolution part, at the start of the workspace (grown if needed;
+    // adjoint_rho reserves its own buffers behind it first); *s_init is the
+    // tau = 0 state slot and *cs0 its cotangent set (U_x, U_y, E1, E2, phi,
+    // pi, 18 planes each)
+    int adjoint_evolution(double* dedeta, double* grad, const double** s_init_out, const double** cs0_out) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
         EvDesc& d = desc[0];
         const int itm = d.itmax, T = itm - 1;
@@ -828,17 +866,24 @@ struct Engine {
         const long long S = d.S;
         const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
         const int nck = T / K + 1;
-        const size_t slot = 72 * (size_t)S, cset = 108 * (size_t)S;
-        const size_t nslot = 1 + (size_t)nck + (size_t)K + 1 + 1;
-        double* mem = nullptr;
-        HIPCHK(hipMalloc(&mem, sizeof(double) * (nslot * slot + 2 * cset)));
-        *mem_out = mem;
+        const size_t slot = 72 * (size_t)S, pend = IPG_PEND_PLANES * (size_t)S, cset = 108 * (size_t)S;
+        rho_grad = nullptr;
+        rho_grad_n = 0;
+        const size_t need = adj_evolution_doubles(d);
+        {
+            int rc = adj_reserve(need);
+            if (rc) return rc;
+        }
+        if ((size_t)(nck + K + 3) * slot + 2 * pend + cset > need)
+            return fail(IPG_ERR_STATE, "adjoint workspace layout");
+        double* mem = adj_ws;
         double* s_init = mem;
         double* s_ck = s_init + slot;                 // nck checkpoints
         double* s_seg = s_ck + (size_t)nck * slot;    // K + 1 segment states
         double* s_T = s_seg + (size_t)(K + 1) * slot;
-        double* cA = s_T + slot;
-        double* cB = cA + cset;
+        double* cA = s_T + slot;     // pending cotangent sets
+        double* cB = cA + pend;
+        double* c0 = cB + pend;      // compact cotangents of the tau = 0 fields
         int rc = upload_desc();
         if (rc) return rc;
         // forward with checkpoints
@@ -859,7 +904,7 @@ struct Engine {
         const double tau_fm = itm * d.dtau * d.a;
         const double L = tau_fm * d.a * d.a * ipg::HBARC * tt;
         // reverse
-        ipg_adj_zero(cA, (long long)cset, st);
+        ipg_adj_zero(cA, (long long)pend, st);
         ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
         double *cin = cA, *cout = cB;
         for (int k = nck - 1; k >= 0; k--) {
@@ -874,12 +919,11 @@ struct Engine {
             for (int it = s1; it > s0; it--) {
                 const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
                 const double* xa = s_seg + (size_t)(it - s0) * slot;
-                ipg_adj_zero(cout, (long long)cset, st);
                 ipg_adj_step(d_desc, S, xb, xa, it, cin, cout, st);
                 std::swap(cin, cout);
             }
         }
-        ipg_adj_kick(d_desc, S, s_init, cin, cout, st);
+        ipg_adj_kick(d_desc, S, s_init, cin, c0, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         if (grad) {
@@ -887,7 +931,7 @@ struct Engine {
             double* tmp = (double*)d.fftbuf;
             for (int f = 0; f < 6; f++) {
                 hipLaunchKernelGGL(k_soa_to_aos_ext, dim3((unsigned)((18 * S + 255) / 256)), dim3(256), 0, st,
-                                   cout + (size_t)f * 18 * S, S, tmp);
+                                   c0 + (size_t)f * 18 * S, S, tmp);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(grad + (size_t)f * 18 * S, tmp, sizeof(double) * 18 * S,
                                       hipMemcpyDeviceToHost, st));
@@ -898,7 +942,7 @@ struct Engine {
         ev[0].obs.dEdeta = L;
         ev[0].obs.tau_fm = tau_fm;
         *s_init_out = s_init;
-        *cs0_out = cout;
+        *cs0_out = c0;
         return IPG_OK;
     }
 
@@ -1143,6 +1187,10 @@ int ipg_adjoint_dedeta(ipg_ctx* c, double* dedeta, double* grad) {
 int ipg_adjoint_rho(ipg_ctx* c, double* dedeta, double* grad) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     return c->eng.adjoint_rho(dedeta, grad);
+}
+int ipg_get_rho_grad(ipg_ctx* c, double* grad) {
+    if (!c || !grad) return fail(IPG_ERR_ARG, "null argument");
+    return c->eng.get_rho_grad(grad);
 }
 int ipg_autotune_report(char* buf, int len) {
     const std::string t = g_tuner.text();

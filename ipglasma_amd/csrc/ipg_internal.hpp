@@ -41,7 +41,12 @@ void ipg_launch_pi_init(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_zero_fields(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 void ipg_launch_rho_normals(const EvDesc* d_evs, int nucleus, int k, double* d_out, hipStream_t st);
 
-// reverse mode of dE/deta through the leapfrog (one event, evs[0])
+// reverse mode of dE/deta through the leapfrog (one event, evs[0]); the
+// step's cotangents travel in pending sets of IPG_PEND_PLANES planes (own
+// cotangents plus what each site sends to its neighbours), the loss fills
+// one (zeroed by the caller), the kick turns the last into a compact set of
+// 108 planes
+#define IPG_PEND_PLANES 252
 void ipg_adj_state_save(const EvDesc* d_evs, long long S, int parity, double* slot, hipStream_t st);
 void ipg_adj_state_load(const EvDesc* d_evs, long long S, int parity, const double* slot, hipStream_t st);
 void ipg_adj_zero(double* p, long long n, hipStream_t st);

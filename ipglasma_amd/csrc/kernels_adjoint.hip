@@ -5,13 +5,25 @@
 // from the tau = 0 fields.  The backward pass walks the steps in reverse: for
 // step it it needs the state before and after the step, which the host
 // recomputes segment by segment from checkpoints (every K = ceil(sqrt(T))
-// steps).  Every site evaluates the per-site reverse functions of adjoint.hpp
-// and adds the cotangents of the neighbour fields it read with FP64 atomics
-// (gathers would need each site's stencil cotangents twice).
+// steps).
+//
+// No atomics in the step: every site writes the cotangents of its own fields
+// ("self") and, per neighbour offset, the cotangent it sends to that
+// neighbour ("outbox"); the next reverse step gathers the outboxes of the
+// sites whose clamped neighbour it is.  One step runs as three site-parallel
+// passes (local updates; the three plaquettes of E'; the commutators of E'
+// and the pi' stencil) so that few matrices are live at a time.
 //
 // A state slot is 72 planes of S doubles: U_x, U_y (18 each, full complex),
-// E1, E2, phi, pi (9 each, Hermitian storage).  A cotangent set is 108
-// planes: U_x, U_y, E1, E2, phi, pi, all full complex (18 each).
+// E1, E2, phi, pi (9 each, Hermitian storage).  A compact cotangent set is
+// 108 planes: U_x, U_y, E1, E2, phi, pi, all full complex (18 each).  A
+// pending set is kPend planes:
+//   self     U_x, U_y (18 each), E1, E2, phi, pi (9 each, Hermitian)
+//   outbox   U_x to (0,+1) (0,-1) (-1,+1) (-1,0)   4 x 18
+//            U_y to (+1,0) (+1,-1) (0,-1) (-1,0)   4 x 18
+//            phi to (+1,0) (0,+1) (-1,0) (0,-1)    4 x 9 (Hermitian)
+// The cotangents of E, phi and pi are used only through their Hermitian
+// part, so self and outbox keep that part.
 #include "adjoint.hpp"
 #include "ipg_device.hpp"
 #include "ipg_internal.hpp"
@@ -34,6 +46,11 @@ __device__ __forceinline__ Slot slot_of(const double* base, size_t S) {
 // cotangent set: field f (0 Ux, 1 Uy, 2 E1, 3 E2, 4 phi, 5 pi)
 __device__ __forceinline__ double* cot(double* base, size_t S, int f) { return base + (size_t)f * 18 * S; }
 
+// pending set planes
+constexpr int kPU1 = 0, kPU2 = 18, kPE1 = 36, kPE2 = 45, kPPh = 54, kPPi = 63;
+constexpr int kPOx = 72, kPOy = 144, kPOph = 216;
+static_assert(kPOph + 36 == IPG_PEND_PLANES, "pending set layout");
+
 __device__ __forceinline__ void atomic_add_m3(double* f, size_t S, size_t s, const M3& m) {
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -47,6 +64,74 @@ __device__ __forceinline__ void add_m3(double* f, size_t S, size_t s, const M3& 
         f[(2 * k) * S + s] += m.e[k].r;
         f[(2 * k + 1) * S + s] += m.e[k].i;
     }
+}
+
+// adds hproj(m) to a Hermitian-storage plane group
+__device__ __forceinline__ void atomic_add_h(double* f, size_t S, size_t s, const M3& m) {
+    const M3 h = hproj(m);
+    atomicAdd(f + s, h.e[0].r);
+    atomicAdd(f + S + s, h.e[4].r);
+    atomicAdd(f + 2 * S + s, h.e[8].r);
+    atomicAdd(f + 3 * S + s, h.e[1].r);
+    atomicAdd(f + 4 * S + s, h.e[1].i);
+    atomicAdd(f + 5 * S + s, h.e[2].r);
+    atomicAdd(f + 6 * S + s, h.e[2].i);
+    atomicAdd(f + 7 * S + s, h.e[5].r);
+    atomicAdd(f + 8 * S + s, h.e[5].i);
+}
+__device__ __forceinline__ void add_h(double* f, size_t S, size_t s, const M3& h) {
+    sth(f, S, s, ldh(f, S, s) + h);
+}
+
+// The sites u along one axis whose clamped neighbour clamp(u + d) is t: one
+// in the interior, two or none at the edges.
+struct Src {
+    int a, b, n;
+};
+__device__ __forceinline__ Src srcs(int t, int d, int N) {
+    if (d == 0) return Src{t, t, 1};
+    if (d > 0) return t == N - 1 ? Src{t - 1, t, 2} : (t >= 1 ? Src{t - 1, t - 1, 1} : Src{0, 0, 0});
+    return t == 0 ? Src{1, 0, 2} : (t <= N - 2 ? Src{t + 1, t + 1, 1} : Src{0, 0, 0});
+}
+// what site (i, j) receives through one outbox offset (di, dj)
+template <bool HERM>
+__device__ __forceinline__ M3 gather(const double* f, size_t S, int N, int i, int j, int di, int dj) {
+    const Src a = srcs(i, di, N), b = srcs(j, dj, N);
+    M3 r = m_zero();
+    for (int p = 0; p < a.n; p++) {
+        const int si = p ? a.b : a.a;
+        for (int q = 0; q < b.n; q++) {
+            const int sj = q ? b.b : b.a;
+            const size_t u = (size_t)si * N + sj;
+            r = r + (HERM ? ldh(f, S, u) : ldm(f, S, u));
+        }
+    }
+    return r;
+}
+// total cotangents of U_x, U_y and phi at (i, j) = s in a pending set
+__device__ __forceinline__ M3 cin_ux(const double* P, size_t S, int N, int i, int j, size_t s) {
+    M3 r = ldm(P + kPU1 * S, S, s);
+    r = r + gather<false>(P + (kPOx + 0) * S, S, N, i, j, 0, 1);
+    r = r + gather<false>(P + (kPOx + 18) * S, S, N, i, j, 0, -1);
+    r = r + gather<false>(P + (kPOx + 36) * S, S, N, i, j, -1, 1);
+    r = r + gather<false>(P + (kPOx + 54) * S, S, N, i, j, -1, 0);
+    return r;
+}
+__device__ __forceinline__ M3 cin_uy(const double* P, size_t S, int N, int i, int j, size_t s) {
+    M3 r = ldm(P + kPU2 * S, S, s);
+    r = r + gather<false>(P + (kPOy + 0) * S, S, N, i, j, 1, 0);
+    r = r + gather<false>(P + (kPOy + 18) * S, S, N, i, j, 1, -1);
+    r = r + gather<false>(P + (kPOy + 36) * S, S, N, i, j, 0, -1);
+    r = r + gather<false>(P + (kPOy + 54) * S, S, N, i, j, -1, 0);
+    return r;
+}
+__device__ __forceinline__ M3 cin_ph(const double* P, size_t S, int N, int i, int j, size_t s) {
+    M3 r = ldh(P + kPPh * S, S, s);
+    r = r + gather<true>(P + (kPOph + 0) * S, S, N, i, j, 1, 0);
+    r = r + gather<true>(P + (kPOph + 9) * S, S, N, i, j, 0, 1);
+    r = r + gather<true>(P + (kPOph + 18) * S, S, N, i, j, -1, 0);
+    r = r + gather<true>(P + (kPOph + 27) * S, S, N, i, j, 0, -1);
+    return r;
 }
 
 struct Lat {
@@ -93,6 +178,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_zero(double* __restrict__ p, long
 }
 
 // ---- loss: dL/dT^tautau(s) = tau_fm hbarc / a^2 for every site --------------------
+// into the own-field planes of a zeroed pending set
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
                                                        int it, double yb, double* __restrict__ cout) {
     const EvDesc& ev = evs[0];
@@ -125,37 +211,71 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict
     M3* bb = reinterpret_cast<M3*>(&b);
     for (int k = 0; k < 16; k++) bb[k] = m_zero();
     tt_adj(q, it * ev.dtau, ev.g, yb, b);
-    double* cU1 = cot(cout, S, 0);
-    double* cU2 = cot(cout, S, 1);
-    double* cE1 = cot(cout, S, 2);
-    double* cE2 = cot(cout, S, 3);
-    double* cPh = cot(cout, S, 4);
-    double* cPi = cot(cout, S, 5);
-    atomic_add_m3(cE1, S, s, b.e1);
-    atomic_add_m3(cE1, S, pY, b.e1py);
-    atomic_add_m3(cE2, S, s, b.e2);
-    atomic_add_m3(cE2, S, pX, b.e2px);
-    atomic_add_m3(cPi, S, s, b.p);
-    atomic_add_m3(cPi, S, pX, b.ppx);
-    atomic_add_m3(cPi, S, pY, b.ppy);
-    atomic_add_m3(cPi, S, pXY, b.ppxy);
+    double* cU1 = cout + kPU1 * S;
+    double* cU2 = cout + kPU2 * S;
+    double* cE1 = cout + kPE1 * S;
+    double* cE2 = cout + kPE2 * S;
+    double* cPh = cout + kPPh * S;
+    double* cPi = cout + kPPi * S;
+    atomic_add_h(cE1, S, s, b.e1);
+    atomic_add_h(cE1, S, pY, b.e1py);
+    atomic_add_h(cE2, S, s, b.e2);
+    atomic_add_h(cE2, S, pX, b.e2px);
+    atomic_add_h(cPi, S, s, b.p);
+    atomic_add_h(cPi, S, pX, b.ppx);
+    atomic_add_h(cPi, S, pY, b.ppy);
+    atomic_add_h(cPi, S, pXY, b.ppxy);
     atomic_add_m3(cU1, S, s, b.ux);
     atomic_add_m3(cU2, S, s, b.uy);
     atomic_add_m3(cU2, S, pX, b.uy_px);
     atomic_add_m3(cU1, S, pY, b.ux_py);
-    atomic_add_m3(cPh, S, s, b.ph);
-    atomic_add_m3(cPh, S, pX, b.phpx);
-    atomic_add_m3(cPh, S, pY, b.phpy);
-    atomic_add_m3(cPh, S, pXY, b.phpxy);
+    atomic_add_h(cPh, S, s, b.ph);
+    atomic_add_h(cPh, S, pX, b.phpx);
+    atomic_add_h(cPh, S, pY, b.phpy);
+    atomic_add_h(cPh, S, pXY, b.phpxy);
 }
 
 // ---- one leapfrog step in reverse ------------------------------------------------------
-// xb: state before the step, xa: state after; cin: cotangent of the state after
-// (E, phi, pi taken on their Hermitian part), cout: cotangent of the state
-// before (zeroed by the caller)
-__global__ __launch_bounds__(IPG_BLOCK) void k_adj_step(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
-                                                       const double* __restrict__ xa, int it,
-                                                       const double* __restrict__ cin, double* __restrict__ cout) {
+// xb: state before the step, xa: state after; pin: pending cotangents of the
+// state after; pout: pending cotangents of the state before (every plane is
+// written: pass 1 the own-field planes, pass 2 the link outboxes, pass 3 the
+// phi outboxes)
+
+// pass 1, site-local: U' = M(E') U, phi' = phi + (tau + dtau/2) dtau pi',
+// and the traceless part taken by E'.  Leaves in pout: the U cotangents from
+// the link update, Z_i = TL(E_i'b) (the own E_i cotangent), phi'b and pi'b.
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_local(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                        const double* __restrict__ xa, int it,
+                                                        const double* __restrict__ pin, double* __restrict__ pout) {
+    const EvDesc& ev = evs[0];
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= ev.S) return;
+    const size_t S = ev.S;
+    const int N = ev.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const double dt = ev.dtau, tau = it * dt, g = ev.g;
+    const Slot B = slot_of(xb, S), A = slot_of(xa, S);
+    const double kappa = g * g * dt / (tau + dt / 2.);
+    {
+        M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
+        u_adj(ldm(B.Ux, S, s), ldh(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+        stm(pout + kPU1 * S, S, s, ub);
+        sth(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
+    }
+    {
+        M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
+        u_adj(ldm(B.Uy, S, s), ldh(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+        stm(pout + kPU2 * S, S, s, ub);
+        sth(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
+    }
+    const M3 phb = cin_ph(pin, S, N, i, j, s);
+    sth(pout + kPPh * S, S, s, phb);
+    sth(pout + kPPi * S, S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
+}
+
+// pass 2: the plaquettes of E' (U12 = ux uy_px ux_py^+ uy^+,
+// U1m2 = ux uy_pxmy^+ ux_my^+ uy_my, U2m1 = uy ux_mxpy^+ uy_mx^+ ux_mx)
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_plaq(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                       int it, double* __restrict__ pout) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
     const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
@@ -163,126 +283,112 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_step(const EvDesc* __restrict
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
     const double dt = ev.dtau, tau = it * dt, g = ev.g;
-    const Slot B = slot_of(xb, S), A = slot_of(xa, S);
-    double* cU1 = cot(cout, S, 0);
-    double* cU2 = cot(cout, S, 1);
-    double* cE1 = cot(cout, S, 2);
-    double* cE2 = cot(cout, S, 3);
-    double* cPh = cot(cout, S, 4);
-    double* cPi = cot(cout, S, 5);
-    const double* iU1 = cot(const_cast<double*>(cin), S, 0);
-    const double* iU2 = cot(const_cast<double*>(cin), S, 1);
-    // local: U' = M(E') U and phi' = phi + c pi'
-    const double kappa = g * g * dt / (tau + dt / 2.);
-    M3 e1b = hproj(ldm(cot(const_cast<double*>(cin), S, 2), S, s));
-    M3 e2b = hproj(ldm(cot(const_cast<double*>(cin), S, 3), S, s));
-    const M3 phb = hproj(ldm(cot(const_cast<double*>(cin), S, 4), S, s));
-    M3 pib = hproj(ldm(cot(const_cast<double*>(cin), S, 5), S, s));
+    const Slot B = slot_of(xb, S);
+    M3 U12b, U1m2b, U2m1b;
+    plaq_bars(ldh(pout + kPE1 * S, S, s), ldh(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
+    M3 B1, B2, B3, B4, uxs, uys;
     {
-        M3 ub = m_zero();
-        u_adj(ldm(B.Ux, S, s), ldh(A.E1, S, s), kappa, ldm(iU1, S, s), ub, e1b);
-        atomic_add_m3(cU1, S, s, ub);
-        ub = m_zero();
-        u_adj(ldm(B.Uy, S, s), ldh(A.E2, S, s), kappa, ldm(iU2, S, s), ub, e2b);
-        atomic_add_m3(cU2, S, s, ub);
-        // E' is stored Hermitian
-        e1b = hproj(e1b);
-        e2b = hproj(e2b);
+        const M3 ux = ldm(B.Ux, S, s);
+        adj_prod4(ux, ldm(B.Uy, S, L.at(i + 1, j)), dag(ldm(B.Ux, S, L.at(i, j + 1))), dag(ldm(B.Uy, S, s)), U12b,
+                  B1, B2, B3, B4);
+        uxs = B1;
+        stm(pout + (kPOy + 0) * S, S, s, B2);
+        stm(pout + (kPOx + 0) * S, S, s, dag(B3));
+        uys = dag(B4);
+        adj_prod4(ux, dag(ldm(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm(B.Ux, S, L.at(i, j - 1))),
+                  ldm(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
     }
-    atomic_add_m3(cPh, S, s, phb);
-    pib = pib + ((tau + dt / 2.) * dt) * phb;
-    // E' stencil
+    uxs = uxs + B1;
+    stm(pout + (kPOy + 18) * S, S, s, dag(B2));
+    stm(pout + (kPOx + 18) * S, S, s, dag(B3));
+    stm(pout + (kPOy + 36) * S, S, s, B4);
+    adj_prod4(ldm(B.Uy, S, s), dag(ldm(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm(B.Uy, S, L.at(i - 1, j))),
+              ldm(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
+    uys = uys + B1;
+    stm(pout + (kPOx + 36) * S, S, s, dag(B2));
+    stm(pout + (kPOy + 54) * S, S, s, dag(B3));
+    stm(pout + (kPOx + 54) * S, S, s, B4);
+    add_m3(pout + kPU1 * S, S, s, uxs);
+    add_m3(pout + kPU2 * S, S, s, uys);
+}
+
+// pass 3: the commutators of E' and the pi' stencil
+//   pi' = pi + (dtau/tau) [Ux phi(x+x) Ux^+ + Ux(x-x)^+ phi(x-x) Ux(x-x) + (y) - 4 phi]
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_phi(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                      int it, double* __restrict__ pout) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const double dt = ev.dtau, tau = it * dt;
+    const Slot B = slot_of(xb, S);
+    const cd c2c{0., -dt / tau};  // conj(i dtau / tau)
+    const M3 Bb = (dt / tau) * ldh(pout + kPPi * S, S, s);
+    const M3 ph = ldh(B.phi, S, s);
+    M3 phs = -4. * Bb;
     {
-        EIn q;
-        q.ux = ldm(B.Ux, S, s);
-        q.uy = ldm(B.Uy, S, s);
-        q.uy_px = ldm(B.Uy, S, L.at(i + 1, j));
-        q.ux_py = ldm(B.Ux, S, L.at(i, j + 1));
-        q.uy_pxmy = ldm(B.Uy, S, L.at(i + 1, j - 1));
-        q.ux_my = ldm(B.Ux, S, L.at(i, j - 1));
-        q.uy_my = ldm(B.Uy, S, L.at(i, j - 1));
-        q.ux_mxpy = ldm(B.Ux, S, L.at(i - 1, j + 1));
-        q.uy_mx = ldm(B.Uy, S, L.at(i - 1, j));
-        q.ux_mx = ldm(B.Ux, S, L.at(i - 1, j));
-        q.ph = ldh(B.phi, S, s);
-        q.phpx = ldh(B.phi, S, L.at(i + 1, j));
-        q.phpy = ldh(B.phi, S, L.at(i, j + 1));
-        q.e1 = ldh(B.E1, S, s);
-        q.e2 = ldh(B.E2, S, s);
-        EBar b;
-        M3* bb = reinterpret_cast<M3*>(&b);
-        for (int k = 0; k < 15; k++) bb[k] = m_zero();
-        e_adj(q, tau, dt, g, e1b, e2b, b);
-        add_m3(cE1, S, s, b.e1);
-        add_m3(cE2, S, s, b.e2);
-        atomic_add_m3(cU1, S, s, b.ux);
-        atomic_add_m3(cU2, S, s, b.uy);
-        atomic_add_m3(cU2, S, L.at(i + 1, j), b.uy_px);
-        atomic_add_m3(cU1, S, L.at(i, j + 1), b.ux_py);
-        atomic_add_m3(cU2, S, L.at(i + 1, j - 1), b.uy_pxmy);
-        atomic_add_m3(cU1, S, L.at(i, j - 1), b.ux_my);
-        atomic_add_m3(cU2, S, L.at(i, j - 1), b.uy_my);
-        atomic_add_m3(cU1, S, L.at(i - 1, j + 1), b.ux_mxpy);
-        atomic_add_m3(cU2, S, L.at(i - 1, j), b.uy_mx);
-        atomic_add_m3(cU1, S, L.at(i - 1, j), b.ux_mx);
-        atomic_add_m3(cPh, S, s, b.ph);
-        atomic_add_m3(cPh, S, L.at(i + 1, j), b.phpx);
-        atomic_add_m3(cPh, S, L.at(i, j + 1), b.phpy);
+        const M3 u = ldm(B.Ux, S, s), pn = ldh(B.phi, S, L.at(i + 1, j));
+        M3 ub = m_zero(), pnb = m_zero();
+        comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE1 * S, S, s)), phs, ub, pnb);
+        adj_conj(u, pn, Bb, ub, pnb);
+        add_m3(pout + kPU1 * S, S, s, ub);
+        sth(pout + (kPOph + 0) * S, S, s, hproj(pnb));
     }
-    // pi' stencil
     {
-        PiIn q;
-        q.ux = ldm(B.Ux, S, s);
-        q.uy = ldm(B.Uy, S, s);
-        q.uxm = ldm(B.Ux, S, L.at(i - 1, j));
-        q.uym = ldm(B.Uy, S, L.at(i, j - 1));
-        q.ph = ldh(B.phi, S, s);
-        q.phpx = ldh(B.phi, S, L.at(i + 1, j));
-        q.phmx = ldh(B.phi, S, L.at(i - 1, j));
-        q.phpy = ldh(B.phi, S, L.at(i, j + 1));
-        q.phmy = ldh(B.phi, S, L.at(i, j - 1));
-        q.pi = ldh(B.pi, S, s);
-        PiBar b;
-        M3* bb = reinterpret_cast<M3*>(&b);
-        for (int k = 0; k < 10; k++) bb[k] = m_zero();
-        pi_adj(q, dt / tau, pib, b);
-        add_m3(cPi, S, s, b.pi);
-        atomic_add_m3(cU1, S, s, b.ux);
-        atomic_add_m3(cU2, S, s, b.uy);
-        atomic_add_m3(cU1, S, L.at(i - 1, j), b.uxm);
-        atomic_add_m3(cU2, S, L.at(i, j - 1), b.uym);
-        atomic_add_m3(cPh, S, s, b.ph);
-        atomic_add_m3(cPh, S, L.at(i + 1, j), b.phpx);
-        atomic_add_m3(cPh, S, L.at(i - 1, j), b.phmx);
-        atomic_add_m3(cPh, S, L.at(i, j + 1), b.phpy);
-        atomic_add_m3(cPh, S, L.at(i, j - 1), b.phmy);
+        const M3 u = ldm(B.Uy, S, s), pn = ldh(B.phi, S, L.at(i, j + 1));
+        M3 ub = m_zero(), pnb = m_zero();
+        comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE2 * S, S, s)), phs, ub, pnb);
+        adj_conj(u, pn, Bb, ub, pnb);
+        add_m3(pout + kPU2 * S, S, s, ub);
+        sth(pout + (kPOph + 9) * S, S, s, hproj(pnb));
     }
+    {
+        const size_t sm = L.at(i - 1, j);
+        M3 ub = m_zero(), pnb = m_zero();
+        adj_conjd(ldm(B.Ux, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+        add_m3(pout + (kPOx + 54) * S, S, s, ub);
+        sth(pout + (kPOph + 18) * S, S, s, hproj(pnb));
+    }
+    {
+        const size_t sm = L.at(i, j - 1);
+        M3 ub = m_zero(), pnb = m_zero();
+        adj_conjd(ldm(B.Uy, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+        add_m3(pout + (kPOy + 36) * S, S, s, ub);
+        sth(pout + (kPOph + 27) * S, S, s, hproj(pnb));
+    }
+    add_h(pout + kPPh * S, S, s, hproj(phs));
 }
 
 // ---- the tau = 0 kick (phi += dtau^2/2 pi, U = M(E) U) in reverse ----------------
+// pin: pending cotangents after the kick; cout: compact cotangents of the
+// tau = 0 fields
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_kick(const EvDesc* __restrict__ evs, const double* __restrict__ x0,
-                                                       const double* __restrict__ cin, double* __restrict__ cout) {
+                                                       const double* __restrict__ pin, double* __restrict__ cout) {
     const EvDesc& ev = evs[0];
     const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
     if (s >= ev.S) return;
     const size_t S = ev.S;
+    const int N = ev.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
     const double dt = ev.dtau, tau = 0., g = ev.g;
     const Slot X = slot_of(x0, S);
     const double kappa = g * g * dt / (tau + dt / 2.);
-    M3 e1b = hproj(ldm(cot(const_cast<double*>(cin), S, 2), S, s));
-    M3 e2b = hproj(ldm(cot(const_cast<double*>(cin), S, 3), S, s));
-    const M3 phb = hproj(ldm(cot(const_cast<double*>(cin), S, 4), S, s));
-    const M3 pib = hproj(ldm(cot(const_cast<double*>(cin), S, 5), S, s));
-    M3 ub = m_zero();
-    u_adj(ldm(X.Ux, S, s), ldh(X.E1, S, s), kappa, ldm(cot(const_cast<double*>(cin), S, 0), S, s), ub, e1b);
-    stm(cot(cout, S, 0), S, s, ub);
-    ub = m_zero();
-    u_adj(ldm(X.Uy, S, s), ldh(X.E2, S, s), kappa, ldm(cot(const_cast<double*>(cin), S, 1), S, s), ub, e2b);
-    stm(cot(cout, S, 1), S, s, ub);
-    stm(cot(cout, S, 2), S, s, hproj(e1b));
-    stm(cot(cout, S, 3), S, s, hproj(e2b));
+    {
+        M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
+        u_adj(ldm(X.Ux, S, s), ldh(X.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+        stm(cot(cout, S, 0), S, s, ub);
+        stm(cot(cout, S, 2), S, s, hproj(e1b));
+    }
+    {
+        M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
+        u_adj(ldm(X.Uy, S, s), ldh(X.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+        stm(cot(cout, S, 1), S, s, ub);
+        stm(cot(cout, S, 3), S, s, hproj(e2b));
+    }
+    const M3 phb = cin_ph(pin, S, N, i, j, s);
     stm(cot(cout, S, 4), S, s, phb);
-    stm(cot(cout, S, 5), S, s, hproj(pib + ((tau + dt / 2.) * dt) * phb));
+    stm(cot(cout, S, 5), S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
 }
 
 // ---- back from the tau = 0 fields to the Wilson lines ------------------------------
@@ -395,7 +501,9 @@ void ipg_adj_loss(const EvDesc* d, long long S, const double* xs, int it, double
 }
 void ipg_adj_step(const EvDesc* d, long long S, const double* xb, const double* xa, int it, const double* cin,
                   double* cout, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_step, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
+    hipLaunchKernelGGL(k_adj_local, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
+    hipLaunchKernelGGL(k_adj_plaq, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
+    hipLaunchKernelGGL(k_adj_phi, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
 }
 void ipg_adj_kick(const EvDesc* d, long long S, const double* x0, const double* cin, double* cout, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_kick, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, x0, cin, cout);

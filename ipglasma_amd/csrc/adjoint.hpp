@@ -353,21 +353,165 @@ IPG_HD void acc_dir_adj(const M3& D, const M3& U, const M3& Dm, const M3& Um, co
 }
 
 // ---- E = exp(i Q^a t^a) (Matrix.cpp:310-426): Qb_a = Re tr(Eb^+ dE/dQ_a) -------------
-// by central differences of the closed form with step h (error O(h^2))
-IPG_HD void expq_adj(const double (&Q)[8], const M3& Eb, double (&Qb)[8]) {
-    const double h = 1e-5;
-    double q[8];
-#pragma unroll
-    for (int a = 0; a < 8; a++) q[a] = Q[a];
-    for (int a = 0; a < 8; a++) {
-        const double q0 = q[a];
-        q[a] = q0 + h;
-        const M3 Ep = exp_iQt(q);
-        q[a] = q0 - h;
-        const M3 Em = exp_iQt(q);
-        q[a] = q0;
-        Qb[a] = retrprod(dag(Eb), Ep - Em) / (2. * h);
+// By Cayley-Hamilton, exp(iQ) = f0 + f1 Q + f2 Q^2 for traceless Hermitian Q,
+// the f_j being functions of c0 = det Q and c1 = tr(Q^2)/2 only
+// (Morningstar and Peardon, Phys. Rev. D 69 (2004) 054501: eigenvalues 2u,
+// -u+w, -u-w).  Then
+//   Qb_a = Re tr(G t^a),  G = Re(b0) Q^2 + Re(b1) Q + f1 Eb^+ + f2 (Q Eb^+ + Eb^+ Q),
+//   b_i = sum_j (d f_j / d c_i) tr(Eb^+ Q^j),
+// from d c0 / d Q_a = tr(Q^2 t^a) and d c1 / d Q_a = tr(Q t^a).
+
+// xi0(w) = sin(w)/w and xi1(w) = (cos w - xi0)/w^2 (Taylor series near 0)
+IPG_HD void expq_xi(double w, double& x0, double& x1) {
+    const double w2 = w * w;
+    if (fabs(w) < 0.05) {
+        x0 = 1. - w2 / 6. * (1. - w2 / 20. * (1. - w2 / 42.));
+        x1 = -1. / 3. + w2 * (1. / 30. - w2 * (1. / 840. - w2 / 45360.));
+    } else {
+        double sw, cw;
+        sincos(w, &sw, &cw);
+        x0 = sw / w;
+        x1 = (cw - x0) / w2;
     }
+}
+
+// f_j(c0, c1) and their partial derivatives fa_j = d/dc0, fb_j = d/dc1.
+// Small c1 (all |eigenvalues| <= sqrt(4 c1 / 3) < 0.82): the exponential
+// series reduced with Q^3 = c1 Q + c0; otherwise the closed form at |c0|,
+// with f_j(-c0, c1) = (-1)^j conj(f_j(c0, c1)) for negative c0, which keeps
+// 9u^2 - w^2 >= 2 c1 away from zero.
+IPG_HD void expq_fcoef(double c0, double c1, cd (&f)[3], cd (&fa)[3], cd (&fb)[3]) {
+    if (c1 < 0.5) {
+        // Q^n = a I + b Q + d Q^2; (a, b, d) <- (c0 d, a + c1 d, b)
+        double a = 1., b = 0., d = 0.;
+        double a0 = 0., b0 = 0., d0 = 0., a1 = 0., b1 = 0., d1 = 0.;  // d/dc0, d/dc1
+        double fr[3] = {0., 0., 0.}, fi[3] = {0., 0., 0.};
+        double ar[3] = {0., 0., 0.}, ai[3] = {0., 0., 0.}, br[3] = {0., 0., 0.}, bi[3] = {0., 0., 0.};
+        double coef = 1.;  // 1/n!
+        for (int n = 0; n < 26; n++) {
+            // i^n / n! = coef * (1, i, -1, -i)[n mod 4]
+            const int m = n & 3;
+            const double cr = m == 0 ? coef : (m == 2 ? -coef : 0.);
+            const double ci = m == 1 ? coef : (m == 3 ? -coef : 0.);
+            const double v[3] = {a, b, d}, v0[3] = {a0, b0, d0}, v1[3] = {a1, b1, d1};
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                fr[j] += cr * v[j];
+                fi[j] += ci * v[j];
+                ar[j] += cr * v0[j];
+                ai[j] += ci * v0[j];
+                br[j] += cr * v1[j];
+                bi[j] += ci * v1[j];
+            }
+            const double na = c0 * d, nb = a + c1 * d, nd = b;
+            const double na0 = d + c0 * d0, nb0 = a0 + c1 * d0, nd0 = b0;
+            const double na1 = c0 * d1, nb1 = a1 + d + c1 * d1, nd1 = b1;
+            a = na, b = nb, d = nd;
+            a0 = na0, b0 = nb0, d0 = nd0;
+            a1 = na1, b1 = nb1, d1 = nd1;
+            coef /= (double)(n + 1);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            f[j] = cd{fr[j], fi[j]};
+            fa[j] = cd{ar[j], ai[j]};
+            fb[j] = cd{br[j], bi[j]};
+        }
+        return;
+    }
+    const bool neg = c0 < 0.;
+    const double c0p = fabs(c0);
+    const double x3 = c1 / 3.;
+    const double c0max = 2. * x3 * sqrt(x3);
+    const double th = acos(fmin(c0p / c0max, 1.)) / 3.;
+    double sth, cth;
+    sincos(th, &sth, &cth);
+    const double u = sqrt(x3) * cth, w = sqrt(c1) * sth;
+    double x0, x1;
+    expq_xi(w, x0, x1);
+    const double cw = cos(w), u2 = u * u, w2 = w * w;
+    double s2u, c2u, su, cu;
+    sincos(2. * u, &s2u, &c2u);
+    sincos(u, &su, &cu);
+    const cd e2{c2u, s2u}, em{cu, -su};
+    // h_j = D f_j, D = 9u^2 - w^2; hu = dh/du; hw = (1/w) dh/dw
+    cd h[3], hu[3], hw[3];
+    h[0] = e2 * (u2 - w2) + em * cd{8. * u2 * cw, 2. * u * (3. * u2 + w2) * x0};
+    h[1] = e2 * (2. * u) - em * cd{2. * u * cw, -(3. * u2 - w2) * x0};
+    h[2] = e2 - em * cd{cw, 3. * u * x0};
+    hu[0] = e2 * cd{2. * u, 2. * (u2 - w2)} +
+            em * cd{16. * u * cw + 2. * u * (3. * u2 + w2) * x0, -8. * u2 * cw + 2. * (9. * u2 + w2) * x0};
+    hw[0] = e2 * (-2.) + em * cd{-8. * u2 * x0, 2. * u * (2. * x0 + (3. * u2 + w2) * x1)};
+    hu[1] = e2 * cd{2., 4. * u} + em * cd{(3. * u2 - w2) * x0 - 2. * cw, 2. * u * cw + 6. * u * x0};
+    hw[1] = em * cd{2. * u * x0, -2. * x0 + (3. * u2 - w2) * x1};
+    hu[2] = e2 * cd{0., 2.} + em * cd{-3. * u * x0, cw - 3. * x0};
+    hw[2] = em * cd{x0, -3. * u * x1};
+    const double D = 9. * u2 - w2;
+    const double iD = 1. / D;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        f[j] = h[j] * iD;
+        const cd fu = (hu[j] - f[j] * (18. * u)) * iD;
+        const cd fw = (hw[j] + f[j] * 2.) * iD;
+        fa[j] = (fu - fw * (3. * u)) * (0.5 * iD);
+        fb[j] = fu * (u * iD) + fw * ((3. * u2 - w2) * 0.5 * iD);
+    }
+    if (neg) {
+        // f_j(-c0) = (-1)^j conj f_j(c0); d/dc0 picks up one more sign
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double sj = (j & 1) ? -1. : 1.;
+            f[j] = cd{sj * f[j].r, -sj * f[j].i};
+            fa[j] = cd{-sj * fa[j].r, sj * fa[j].i};
+            fb[j] = cd{sj * fb[j].r, -sj * fb[j].i};
+        }
+    }
+}
+
+// Q = Q^a t^a, t^a = lambda^a / 2 (Gell-Mann order)
+IPG_HD M3 gen_matrix(const double (&Q)[8]) {
+    const double s3 = 1. / (2. * sqrt(3.));
+    M3 m;
+    m.e[0] = cd{0.5 * Q[2] + s3 * Q[7], 0.};
+    m.e[4] = cd{-0.5 * Q[2] + s3 * Q[7], 0.};
+    m.e[8] = cd{-2. * s3 * Q[7], 0.};
+    m.e[1] = cd{0.5 * Q[0], -0.5 * Q[1]};
+    m.e[3] = cd{0.5 * Q[0], 0.5 * Q[1]};
+    m.e[2] = cd{0.5 * Q[3], -0.5 * Q[4]};
+    m.e[6] = cd{0.5 * Q[3], 0.5 * Q[4]};
+    m.e[5] = cd{0.5 * Q[5], -0.5 * Q[6]};
+    m.e[7] = cd{0.5 * Q[5], 0.5 * Q[6]};
+    return m;
+}
+// Re tr(G t^a), a = 0..7
+IPG_HD void gen_project(const M3& G, double (&out)[8]) {
+    const double s3 = 1. / (2. * sqrt(3.));
+    out[0] = 0.5 * (G.e[1].r + G.e[3].r);
+    out[1] = 0.5 * (G.e[3].i - G.e[1].i);
+    out[2] = 0.5 * (G.e[0].r - G.e[4].r);
+    out[3] = 0.5 * (G.e[2].r + G.e[6].r);
+    out[4] = 0.5 * (G.e[6].i - G.e[2].i);
+    out[5] = 0.5 * (G.e[5].r + G.e[7].r);
+    out[6] = 0.5 * (G.e[7].i - G.e[5].i);
+    out[7] = s3 * (G.e[0].r + G.e[4].r - 2. * G.e[8].r);
+}
+
+IPG_HD void expq_adj(const double (&Q)[8], const M3& Eb, double (&Qb)[8]) {
+    const M3 Qm = gen_matrix(Q);
+    const M3 Q2 = Qm * Qm;
+    double c1 = 0.;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
+    c1 *= 0.25;
+    const double c0 = retrprod(Q2, Qm) / 3.;  // det Q = tr(Q^3) / 3
+    cd f[3], fa[3], fb[3];
+    expq_fcoef(c0, c1, f, fa, fb);
+    const M3 Ebd = dag(Eb);
+    const cd t0 = tr(Ebd), t1 = trprod(Ebd, Qm), t2 = trprod(Ebd, Q2);
+    const double b0 = (fa[0] * t0 + fa[1] * t1 + fa[2] * t2).r;
+    const double b1 = (fb[0] * t0 + fb[1] * t1 + fb[2] * t2).r;
+    const M3 G = (b0 * Q2 + b1 * Qm) + (cscale(f[1], Ebd) + cscale(f[2], Qm * Ebd + Ebd * Qm));
+    gen_project(G, Qb);
 }
 
 }  // namespace ipgad

@@ -18,15 +18,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <sys/stat.h>
 #include <map>
 #include <mutex>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host.hpp"
@@ -104,14 +107,16 @@ struct Tuner {
         e.next = (e.next + 1) % ncand;
         return c;
     }
+    // the first sample of a candidate is discarded (code-object load, clock
+    // ramp); the best of the next two decides
     void report(const std::string& key, int cand, float ms) {
         std::lock_guard<std::mutex> g(mu);
         Entry& e = tab[key];
         if (e.chosen >= 0 || cand >= (int)e.best.size()) return;
+        if (e.seen[cand]++ == 0) return;
         if (e.best[cand] < 0 || ms < e.best[cand]) e.best[cand] = ms;
-        e.seen[cand]++;
         bool done = true;
-        for (int n : e.seen) done = done && n >= 2;
+        for (int n : e.seen) done = done && n >= 3;
         if (done) {
             int b = 0;
             for (int k = 1; k < (int)e.best.size(); k++)
@@ -169,6 +174,7 @@ struct Event {
     ipg_observables obs;
     double* mem = nullptr;   // one device allocation per event
     double* d_nuc = nullptr;
+    size_t nuc_cap = 0;      // doubles allocated at d_nuc
     double* d_z = nullptr;   // injected or MT normals
     std::vector<double> hz;  // host copy of injected normals
     int success = 0;
@@ -178,6 +184,48 @@ struct Event {
 };
 
 }  // namespace
+
+// The Q_s table (200 T_p rows x 44 rapidities, Init.cpp:441-480), parsed once
+// per process and file version (path, size, modification time): a context
+// per event would otherwise parse 8800 lines per event.
+int parsed_table(const char* path, std::vector<double>& Tl, std::vector<double>& Q) {
+    struct Parsed {
+        long long size = -1, mtime = 0;
+        std::vector<double> Tl, Q;
+    };
+    static std::mutex mu;
+    static std::map<std::string, Parsed> cache;
+    struct stat sb;
+    if (stat(path, &sb) != 0) return fail(IPG_ERR_IO, std::string("Q_s table ") + path + " does not exist");
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(path);
+        if (it != cache.end() && it->second.size == (long long)sb.st_size &&
+            it->second.mtime == (long long)sb.st_mtime) {
+            Tl = it->second.Tl;
+            Q = it->second.Q;
+            return IPG_OK;
+        }
+    }
+    std::ifstream in(path);
+    if (!in) return fail(IPG_ERR_IO, std::string("Q_s table ") + path + " does not exist");
+    Tl.assign(200, 0.);
+    Q.assign(200 * 44, 0.);
+    for (int iT = 0; iT < 200; iT++)
+        for (int iy = 0; iy < 44; iy++) {
+            std::string y, T, q;
+            if (!(in >> y >> T >> q)) return fail(IPG_ERR_IO, "Q_s table ended prematurely");
+            Tl[iT] = std::atof(T.c_str());
+            Q[iT * 44 + iy] = std::atof(q.c_str());
+        }
+    std::lock_guard<std::mutex> lk(mu);
+    Parsed& e = cache[path];
+    e.size = (long long)sb.st_size;
+    e.mtime = (long long)sb.st_mtime;
+    e.Tl = Tl;
+    e.Q = Q;
+    return IPG_OK;
+}
 
 struct Engine {
     std::vector<Event> ev;
@@ -198,9 +246,10 @@ struct Engine {
     double t_steps_ms = 0., t_tmunu_ms = 0.;
     int t_nsteps = 0;
     std::vector<TimedLaunch> timed;  // tuner measurements awaiting a sync
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_itm = -1;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};  // one per evolution schedule
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    int graph_itm[3] = {-1, -1, -1};
+    std::vector<int> graph_act[3];  // active flags the graph was captured with
     // reverse-mode workspace (checkpoints, cotangents, the rho gradient),
     // kept between calls; grown on demand
     double* adj_ws = nullptr;
@@ -209,8 +258,10 @@ struct Engine {
     size_t rho_grad_n = 0;
 
     ~Engine() {
-        if (gexec) hipGraphExecDestroy(gexec);
-        if (graph) hipGraphDestroy(graph);
+        for (int k = 0; k < 3; k++) {
+            if (gexec[k]) hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) hipGraphDestroy(graph[k]);
+        }
         for (Event& e : ev) {
             if (e.mem) hipFree(e.mem);
             if (e.d_nuc) hipFree(e.d_nuc);
@@ -382,16 +433,9 @@ struct Engine {
     }
 
     int load_table(const char* path) {
-        std::ifstream in(path);
-        if (!in) return fail(IPG_ERR_IO, std::string("Q_s table ") + path + " does not exist");
-        std::vector<double> Tl(200), Q(200 * 44);
-        for (int iT = 0; iT < 200; iT++)
-            for (int iy = 0; iy < 44; iy++) {
-                std::string y, T, q;
-                if (!(in >> y >> T >> q)) return fail(IPG_ERR_IO, "Q_s table ended prematurely");
-                Tl[iT] = std::atof(T.c_str());
-                Q[iT * 44 + iy] = std::atof(q.c_str());
-            }
+        std::vector<double> Tl, Q;
+        int rc = parsed_table(path, Tl, Q);
+        if (rc) return rc;
         if (!d_Tl) HIPCHK(hipMalloc(&d_Tl, sizeof(double) * 200));
         if (!d_Q) HIPCHK(hipMalloc(&d_Q, sizeof(double) * 200 * 44));
         HIPCHK(hipMemcpy(d_Tl, Tl.data(), sizeof(double) * 200, hipMemcpyHostToDevice));
@@ -400,14 +444,49 @@ struct Engine {
         return IPG_OK;
     }
 
+    // host nucleon sampling for event e (no HIP calls: events sample in
+    // parallel, each from its own generator)
+    int sample(int e, std::string& err) {
+        Event& E = ev[e];
+        int rc = ipg::sample_event(E.rng, E.p, E.proj, E.targ, E.A, E.B, E.info, err);
+        if (rc) return rc;
+        E.attempt++;
+        return IPG_OK;
+    }
+
     // host nucleon sampling for event e, upload of the nucleon list
     int nucleons(int e) {
-        Event& E = ev[e];
         std::string err;
-        int rc = ipg::sample_event(E.rng, E.p, E.proj, E.targ, E.A, E.B, E.info, err);
+        int rc = sample(e, err);
         if (rc) return fail(rc, err);
-        E.attempt++;
         return upload_nucleons(e);
+    }
+
+    // the same for several events: sampled on up to 16 host threads, then
+    // uploaded in order
+    int nucleons_many(const std::vector<int>& which) {
+        const int n = (int)which.size();
+        std::vector<int> rcs(n, IPG_OK);
+        std::vector<std::string> errs(n);
+        const int nt = std::max(1, std::min(n, 16));
+        if (nt == 1) {
+            for (int k = 0; k < n; k++) rcs[k] = sample(which[k], errs[k]);
+        } else {
+            std::atomic<int> next{0};
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; t++)
+                th.emplace_back([&]() {
+                    for (int k = next++; k < n; k = next++) rcs[k] = sample(which[k], errs[k]);
+                });
+            for (std::thread& x : th) x.join();
+        }
+        for (int k = 0; k < n; k++)
+            if (rcs[k]) return fail(rcs[k], errs[k]);
+        for (int k = 0; k < n; k++) {
+            int rc = upload_nucleons(which[k]);
+            if (rc) return rc;
+        }
+        return IPG_OK;
     }
 
     int upload_nucleons(int e) {
@@ -423,9 +502,14 @@ struct Engine {
                 h[4 * k + 3] = q.collided;
                 k++;
             }
-        if (E.d_nuc) hipFree(E.d_nuc);
-        E.d_nuc = nullptr;
-        HIPCHK(hipMalloc(&E.d_nuc, sizeof(double) * std::max(4, 4 * n)));
+        const size_t need = (size_t)std::max(4, 4 * n);
+        if (E.nuc_cap < need) {
+            if (E.d_nuc) HIPCHK(hipFree(E.d_nuc));
+            E.d_nuc = nullptr;
+            E.nuc_cap = 0;
+            HIPCHK(hipMalloc(&E.d_nuc, sizeof(double) * need));
+            E.nuc_cap = need;
+        }
         HIPCHK(hipMemcpy(E.d_nuc, h.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
         desc[e].nuc = E.d_nuc;
         desc[e].A1 = (int)E.A.size();
@@ -651,27 +735,98 @@ struct Engine {
         }
     }
 
+    // the whole evolution.  Schedule 0: every launch covers all events (one
+    // leapfrog launch per step).  Schedules 1 and 2: between the steps at
+    // which an event ends, the events go in groups of consecutive events of
+    // at least kGroupBlocks[sched] blocks, each group taking its run of steps
+    // before the next group starts, so that a group's fields are still in
+    // the caches at its next step.  The events are independent: all
+    // schedules give the same fields.
+    static constexpr int kGroupBlocks[3] = {0, 1024, 4096};
+    void launch_evolution(int sched) {
+        const int itm = max_itmax();
+        const int nev = (int)ev.size();
+        if (sched == 0) {
+            for (int it = 0; it <= itm; it++) launch_iteration(it);
+            return;
+        }
+        // groups [first, last] of active events
+        std::vector<std::pair<int, int>> groups;
+        for (int e = 0; e < nev;) {
+            if (!desc[e].active) {
+                e++;
+                continue;
+            }
+            int f = e, nb = 0;
+            while (e < nev && (nb < kGroupBlocks[sched] || e == f)) nb += desc[e++].nblocks;
+            groups.push_back({f, e - 1});
+        }
+        launch_iteration(0);
+        int it = 1;
+        while (it <= itm) {
+            if (any_final(it)) {
+                launch_iteration(it);
+                it++;
+                continue;
+            }
+            int end = it;
+            while (end + 1 <= itm && !any_final(end + 1)) end++;
+            for (const auto& gr : groups) {
+                int last_it = 0;
+                for (int e = gr.first; e <= gr.second; e++)
+                    if (desc[e].active) last_it = std::max(last_it, desc[e].itmax - 1);
+                const int b0 = desc[gr.first].block0;
+                const int nb = desc[gr.second].block0 + desc[gr.second].nblocks - b0;
+                for (int k = it; k <= std::min(end, last_it); k++) ipg_launch_step_event(d_desc, nev, b0, nb, k, st);
+            }
+            it = end + 1;
+        }
+    }
+
     int evolve(bool use_graph) {
         int rc = upload_desc();
         if (rc) return rc;
         const int itm = max_itmax();
         const int nev = (int)ev.size();
+        int nact = 0, maxN = 0;
+        for (const EvDesc& d : desc)
+            if (d.active) {
+                nact++;
+                maxN = std::max(maxN, d.N);
+            }
+        // schedule: a tuner choice when several events share the launches
+        const std::string key =
+            "evolve_schedule/N=" + std::to_string(maxN) + "/events=" + std::to_string(nact);
+        bool meas = false;
+        int sched = 0;
+        if (nact > 1 && !(timing && !use_graph)) sched = g_tuner.pick(key, 3, &meas);
+        hipEvent_t ta = nullptr, tb = nullptr;
+        std::vector<int> act(desc.size());
+        for (size_t e = 0; e < desc.size(); e++) act[e] = desc[e].active;
         if (use_graph) {
-            if (gexec && graph_itm != itm) {
-                hipGraphExecDestroy(gexec);
-                hipGraphDestroy(graph);
-                gexec = nullptr;
-                graph = nullptr;
+            // the launch sequence depends on which events are active
+            if (gexec[sched] && (graph_itm[sched] != itm || graph_act[sched] != act)) {
+                hipGraphExecDestroy(gexec[sched]);
+                hipGraphDestroy(graph[sched]);
+                gexec[sched] = nullptr;
+                graph[sched] = nullptr;
             }
-            if (!gexec) {
+            if (!gexec[sched]) {
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                for (int it = 0; it <= itm; it++) launch_iteration(it);
-                HIPCHK(hipStreamEndCapture(st, &graph));
-                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-                graph_itm = itm;
+                launch_evolution(sched);
+                HIPCHK(hipStreamEndCapture(st, &graph[sched]));
+                HIPCHK(hipGraphInstantiate(&gexec[sched], graph[sched], nullptr, nullptr, 0));
+                graph_itm[sched] = itm;
+                graph_act[sched] = act;
             }
-            HIPCHK(hipGraphLaunch(gexec, st));
+            // timed from the launch (capture and instantiation are host work)
+            if (timed_begin(key, sched, meas, &ta, &tb)) (void)hipEventRecord(ta, st);
+            HIPCHK(hipGraphLaunch(gexec[sched], st));
+        } else if (sched > 0) {
+            if (timed_begin(key, sched, meas, &ta, &tb)) (void)hipEventRecord(ta, st);
+            launch_evolution(sched);
         } else {
+            if (timed_begin(key, sched, meas, &ta, &tb)) (void)hipEventRecord(ta, st);
             if (timing) {
                 for (hipEvent_t t : tev) hipEventDestroy(t);
                 tev.assign(3 * (size_t)(itm + 1), nullptr);
@@ -692,8 +847,10 @@ struct Engine {
                 }
             }
         }
+        if (tb) (void)hipEventRecord(tb, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        timed_resolve();
         if (timing && !use_graph) {
             t_steps_ms = 0.;
             t_tmunu_ms = 0.;
@@ -1319,16 +1476,15 @@ int ipg_batch_run(ipg_batch* b, int use_graph, int max_retries) {
     const int nev = (int)g.ev.size();
     std::vector<int> pending(nev, 1);
     for (int attempt = 0; attempt <= max_retries; attempt++) {
-        bool any = false;
+        std::vector<int> which;
         for (int e = 0; e < nev; e++) {
             g.desc[e].active = pending[e];
-            if (!pending[e]) continue;
-            any = true;
-            int rc = g.nucleons(e);
-            if (rc) return rc;
+            if (pending[e]) which.push_back(e);
         }
-        if (!any) break;
-        int rc = g.color_charge();
+        if (which.empty()) break;
+        int rc = g.nucleons_many(which);
+        if (rc) return rc;
+        rc = g.color_charge();
         if (rc) return rc;
         for (int e = 0; e < nev; e++)
             if (pending[e] && g.ev[e].success) pending[e] = 0;

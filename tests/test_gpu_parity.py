@@ -276,7 +276,11 @@ def test_batch_matches_plain():
         ev.run()
         plain.append(ev.observables())
         ev.close()
-    for graph in (False, True):
+    # six runs: while it measures them, the tuner takes the three evolution
+    # schedules (all events per launch; groups of events taking their runs of
+    # steps one group after the other, two group sizes) in turn, so every
+    # pairing of schedule and graph replay is run once
+    for graph in (False, True, True, True, False, False):
         b = I.Batch(ps, seeds)
         b.run(use_graph=graph)
         for e in range(len(ps)):
@@ -284,3 +288,4 @@ def test_batch_matches_plain():
             assert o["Ncoll"] == plain[e]["Ncoll"]
             assert o["dEdeta"] == pytest.approx(plain[e]["dEdeta"], rel=1e-12)
         b.close()
+    assert "evolve_schedule/N=32/events=3" in I.autotune_report()

@@ -134,6 +134,12 @@ def measured_traffic(N):
 
 
 def roofline(I, p):
+    # one untimed event first: the autotuner settles the step kernel's build
+    w = I.Event(p, seed=12344)
+    while not w.init():
+        pass
+    w.run()
+    w.close()
     ev = I.Event(p, seed=12345)
     while not ev.init():
         pass

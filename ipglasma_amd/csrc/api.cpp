@@ -124,6 +124,13 @@ struct Tuner {
             e.chosen = b;
         }
     }
+    // the decided candidate of a key, or -1 (also -1 while tuning is off)
+    int chosen(const std::string& key) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!enabled()) return -1;
+        auto it = tab.find(key);
+        return it == tab.end() ? -1 : it->second.chosen;
+    }
     std::string text() {
         std::lock_guard<std::mutex> g(mu);
         std::string r;
@@ -250,6 +257,8 @@ struct Engine {
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     int graph_itm[3] = {-1, -1, -1};
     std::vector<int> graph_act[3];  // active flags the graph was captured with
+    int graph_var[3] = {-1, -1, -1};  // k_step build the graph was captured with
+    int step_var = 0;                 // k_step build for launches not being timed
     // reverse-mode workspace (checkpoints, cotangents, the rho gradient),
     // kept between calls; grown on demand
     double* adj_ws = nullptr;
@@ -716,6 +725,18 @@ struct Engine {
         return false;
     }
 
+    // tuner key of the k_step build (waves per SIMD) for the active events
+    std::string step_key() const {
+        int nact = 0, maxN = 0;
+        for (const EvDesc& d : desc)
+            if (d.active) {
+                nact++;
+                maxN = std::max(maxN, d.N);
+            }
+        return "step_waves_per_simd/N=" + std::to_string(maxN) + "/events=" + std::to_string(nact);
+    }
+    int step_decided() const { return std::max(0, g_tuner.chosen(step_key())); }
+
     // launches of leapfrog iteration it for all events (it = 0: tau=0 kick);
     // the T^{mu nu}, Gauss-law and reduction kernels are launched only at
     // iterations where some event takes its final half step
@@ -727,7 +748,7 @@ struct Engine {
         }
         const bool fin = any_final(it);
         if (fin) ipg_launch_tmunu(d_desc, nev, nblk, it, st);
-        ipg_launch_step(d_desc, nev, nblk, it, st);
+        ipg_launch_step(d_desc, nev, nblk, it, st, step_var);
         if (fin) {
             ipg_launch_gauss(d_desc, nev, nblk, it, st);
             ipg_launch_finish(d_desc, nev, it, d_out, st);
@@ -777,7 +798,8 @@ struct Engine {
                     if (desc[e].active) last_it = std::max(last_it, desc[e].itmax - 1);
                 const int b0 = desc[gr.first].block0;
                 const int nb = desc[gr.second].block0 + desc[gr.second].nblocks - b0;
-                for (int k = it; k <= std::min(end, last_it); k++) ipg_launch_step_event(d_desc, nev, b0, nb, k, st);
+                for (int k = it; k <= std::min(end, last_it); k++)
+                    ipg_launch_step_event(d_desc, nev, b0, nb, k, st, step_var);
             }
             it = end + 1;
         }
@@ -803,9 +825,11 @@ struct Engine {
         hipEvent_t ta = nullptr, tb = nullptr;
         std::vector<int> act(desc.size());
         for (size_t e = 0; e < desc.size(); e++) act[e] = desc[e].active;
+        step_var = step_decided();
         if (use_graph) {
             // the launch sequence depends on which events are active
-            if (gexec[sched] && (graph_itm[sched] != itm || graph_act[sched] != act)) {
+            if (gexec[sched] &&
+                (graph_itm[sched] != itm || graph_act[sched] != act || graph_var[sched] != step_var)) {
                 hipGraphExecDestroy(gexec[sched]);
                 hipGraphDestroy(graph[sched]);
                 gexec[sched] = nullptr;
@@ -818,6 +842,7 @@ struct Engine {
                 HIPCHK(hipGraphInstantiate(&gexec[sched], graph[sched], nullptr, nullptr, 0));
                 graph_itm[sched] = itm;
                 graph_act[sched] = act;
+                graph_var[sched] = step_var;
             }
             // timed from the launch (capture and instantiation are host work)
             if (timed_begin(key, sched, meas, &ta, &tb)) (void)hipEventRecord(ta, st);
@@ -838,7 +863,16 @@ struct Engine {
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it], st));
                 if (fin) ipg_launch_tmunu(d_desc, nev, nblk, it, st);
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it + 1], st));
-                ipg_launch_step(d_desc, nev, nblk, it, st);
+                {
+                    // full steps of every active event are timed for the tuner
+                    const std::string sk = step_key();
+                    bool sm = false;
+                    const int v = timing ? step_decided() : g_tuner.pick(sk, 2, &sm);
+                    hipEvent_t a = nullptr, b = nullptr;
+                    if (timed_begin(sk, v, sm && !fin && !timing, &a, &b)) (void)hipEventRecord(a, st);
+                    ipg_launch_step(d_desc, nev, nblk, it, st, v);
+                    if (b) (void)hipEventRecord(b, st);
+                }
                 if (timing) HIPCHK(hipEventRecord(tev[3 * it + 2], st));
                 if (fin) {
                     ipg_launch_gauss(d_desc, nev, nblk, it, st);
@@ -1043,12 +1077,13 @@ struct Engine {
         double* c0 = cB + pend;      // compact cotangents of the tau = 0 fields
         int rc = upload_desc();
         if (rc) return rc;
+        const int sv = step_decided();
         // forward with checkpoints
         ipg_adj_state_save(d_desc, S, 0, s_init, st);
         ipg_launch_step0(d_desc, 1, nblk, st);
         ipg_adj_state_save(d_desc, S, 0, s_ck, st);
         for (int it = 1; it <= T; it++) {
-            ipg_launch_step(d_desc, 1, nblk, it, st);
+            ipg_launch_step(d_desc, 1, nblk, it, st, sv);
             if (it % K == 0) ipg_adj_state_save(d_desc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
         }
         ipg_adj_state_save(d_desc, S, T & 1, s_T, st);
@@ -1070,7 +1105,7 @@ struct Engine {
             const double* x0 = s_ck + (size_t)k * slot;
             ipg_adj_state_load(d_desc, S, s0 & 1, x0, st);
             for (int it = s0 + 1; it <= s1; it++) {
-                ipg_launch_step(d_desc, 1, nblk, it, st);
+                ipg_launch_step(d_desc, 1, nblk, it, st, sv);
                 ipg_adj_state_save(d_desc, S, it & 1, s_seg + (size_t)(it - s0) * slot, st);
             }
             for (int it = s1; it > s0; it--) {
@@ -1107,6 +1142,7 @@ struct Engine {
         // one schedule piece for a single-event engine
         int rc = upload_desc();
         if (rc) return rc;
+        step_var = step_decided();
         launch_iteration(it);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));

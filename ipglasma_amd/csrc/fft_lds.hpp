@@ -176,8 +176,24 @@ IPGF_HD int out_index(int j) {
     return (j / NS) * NS * R + j % NS;
 }
 
+// ---- LDS layout -------------------------------------------------------------------
+// Element i of row `row` sits at row * row_stride(N) + pad(i) (double2 slots).
+// pad() skips one slot in every 8, so that the first Stockham stage's writes
+// at stride R = 8 or 4 (and the second stage's runs of 8 that are 64 apart)
+// fall in distinct banks; the row stride is 9 mod 16 slots, so that the
+// column pass's transposed accesses (one lane per row) do too.  Unpadded,
+// about half of the LDS cycles of the transform kernels were bank conflicts.
+IPGF_HD constexpr int pad(int i) { return i + (i >> 3); }
+IPGF_HD constexpr int row_stride(int N) {
+    const int b = pad(N - 1) + 1;
+    return b + ((9 - b % 16) + 16) % 16;
+}
+// LDS slots of NROW rows
+IPGF_HD constexpr int lds_slots(int N, int NROW) { return NROW * row_stride(N); }
+IPGF_HD constexpr int at(int N, int row, int i) { return row * row_stride(N) + pad(i); }
+
 // ---- block-cooperative transform of NROW rows of length N in LDS ---------------
-// (device only; BLK threads; rows at x + row * N)
+// (device only; BLK threads; element i of row `row` at x[at(N, row, i)])
 template <int N, int NROW, int BLK, bool INV, int S>
 __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ tw) {
     if constexpr (S < nstages(N)) {
@@ -186,6 +202,7 @@ __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ t
         constexpr int NB = N / R;
         constexpr int TOT = NROW * NB;
         constexpr int K = (TOT + BLK - 1) / BLK;
+        constexpr int RS = row_stride(N);
         double2 v[K][R];
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -193,7 +210,7 @@ __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ t
             if (q < TOT) {
                 const int row = q / NB, j = q - row * NB;
 #pragma unroll
-                for (int r = 0; r < R; r++) v[k][r] = x[row * N + j + r * NB];
+                for (int r = 0; r < R; r++) v[k][r] = x[row * RS + pad(j + r * NB)];
             }
         }
         __syncthreads();
@@ -203,9 +220,9 @@ __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ t
             if (q < TOT) {
                 const int row = q / NB, j = q - row * NB;
                 butterfly<N, R, NS, INV>(v[k], j, tw);
-                const int o = row * N + out_index<N, R, NS>(j);
+                const int o = out_index<N, R, NS>(j);
 #pragma unroll
-                for (int r = 0; r < R; r++) x[o + r * NS] = v[k][r];
+                for (int r = 0; r < R; r++) x[row * RS + pad(o + r * NS)] = v[k][r];
             }
         }
         __syncthreads();

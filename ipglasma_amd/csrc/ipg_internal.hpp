@@ -6,9 +6,11 @@
 
 // site-parallel helpers: nblk = total blocks over all events of the launch
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
-void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
+// variant 0: register budget for 1 wave per SIMD, 1: for 2 (same results)
+void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant = 0);
 // the step for the blocks [block0, block0 + nblocks) of one event only
-void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st);
+void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st,
+                           int variant = 0);
 void ipg_launch_tmunu(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
 void ipg_launch_gauss(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st);
 void ipg_launch_finish(const EvDesc* d_evs, int nev, int it, double* d_out, hipStream_t st);

@@ -92,7 +92,10 @@ __device__ __forceinline__ M3 taylor2(double c, const M3& E) {
 
 #define IPG_PHASE() __builtin_amdgcn_sched_barrier(0)
 
-__global__ __launch_bounds__(IPG_BLOCK) void k_step(const EvDesc* __restrict__ evs, int nev, int it, int b0) {
+// OCC: waves per SIMD the register budget is sized for (1: no spill, 2:
+// half the registers); the autotuner picks one, the results are identical
+template <int OCC>
+__global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restrict__ evs, int nev, int it, int b0) {
     // b0: first block of a launch that covers one event's blocks only (a
     // multiple of 8, so the XCD of a block is unchanged)
     const int bg = (int)blockIdx.x + b0;
@@ -585,11 +588,18 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_finish(const EvDesc* __restrict__
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_step0, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev);
 }
-void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st) {
-    hipLaunchKernelGGL(k_step, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
+void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant) {
+    if (variant == 1)
+        hipLaunchKernelGGL(k_step<2>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
+    else
+        hipLaunchKernelGGL(k_step<1>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
 }
-void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st) {
-    hipLaunchKernelGGL(k_step, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st,
+                           int variant) {
+    if (variant == 1)
+        hipLaunchKernelGGL(k_step<2>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+    else
+        hipLaunchKernelGGL(k_step<1>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
 }
 void ipg_launch_tmunu(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st) {
     hipLaunchKernelGGL(k_tmunu_a, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it);

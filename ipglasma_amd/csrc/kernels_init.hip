@@ -378,7 +378,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_stats_finish(const EvDesc* __rest
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
                                                             int k0, int kc) {
-    __shared__ double2 lds[4 * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int per_ev = 2 * kc * N;
     const int e = blockIdx.x / per_ev;
     const int rem = blockIdx.x - e * per_ev;
@@ -402,14 +402,14 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
         }
         const double amp = ev.g * sqrt(g2[s] / (double)ev.Ny);
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + j] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[ipgfft::at(N, p, j)] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
     }
     __syncthreads();
     ipgfft::lds_fft<N, 4, IPG_BLOCK, false>(lds, pl.tw);
     double2* out = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
-        out[(size_t)p * S + (size_t)i * N + j] = lds[q];
+        out[(size_t)p * S + (size_t)i * N + j] = lds[ipgfft::at(N, p, j)];
     }
 }
 
@@ -419,7 +419,7 @@ template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                         int kc) {
     constexpr int C = ipgfft::cols_per_block(N);
-    __shared__ double2 lds[C * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, C)];
     constexpr int per_pack = N / C;
     const int per_ev = 2 * kc * 4 * per_pack;
     const int e = blockIdx.x / per_ev;
@@ -432,10 +432,10 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
     if (!ev.active || k >= ev.Ny) return;
     const size_t S = ev.S;
     double2* buf = ev.fftbuf + ((size_t)nk * 4 + p) * S;
-    // load C columns: lds[c * N + r]
+    // load C columns: column c of the block is LDS row c
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
-        lds[c * N + r] = buf[(size_t)r * N + j0 + c];
+        lds[ipgfft::at(N, c, r)] = buf[(size_t)r * N + j0 + c];
     }
     __syncthreads();
     ipgfft::lds_fft<N, C, IPG_BLOCK, false>(lds, pl.tw);
@@ -450,13 +450,14 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
         else
             f = (1. / (kt2 + m2)) * exp(-sqrt(kt2) * uv);
         f *= invN2;
-        lds[q] = make_double2(lds[q].x * f, lds[q].y * f);
+        const int a = ipgfft::at(N, c, r);
+        lds[a] = make_double2(lds[a].x * f, lds[a].y * f);
     }
     __syncthreads();
     ipgfft::lds_fft<N, C, IPG_BLOCK, true>(lds, pl.tw);
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
-        buf[(size_t)r * N + j0 + c] = lds[c * N + r];
+        buf[(size_t)r * N + j0 + c] = lds[ipgfft::at(N, c, r)];
     }
 }
 
@@ -466,7 +467,7 @@ template <int N, int kInvBlock>
 __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                             int kc) {
     constexpr int SPT = (N + kInvBlock - 1) / kInvBlock;
-    __shared__ double2 lds[4 * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int e = blockIdx.x / (2 * N);
     const int n = (blockIdx.x / N) & 1;
     const int i = blockIdx.x % N;
@@ -485,7 +486,7 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
         const double2* in = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
         for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
             const int p = q / N, j = q - p * N;
-            lds[q] = in[(size_t)p * S + (size_t)i * N + j];
+            lds[ipgfft::at(N, p, j)] = in[(size_t)p * S + (size_t)i * N + j];
         }
         __syncthreads();
         ipgfft::lds_fft<N, 4, kInvBlock, true>(lds, pl.tw);
@@ -496,7 +497,7 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
                 double Q[8];
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
-                    double2 v = lds[p * N + j];
+                    double2 v = lds[ipgfft::at(N, p, j)];
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
@@ -522,7 +523,7 @@ template <int N, int BLK>
 __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0, int kc,
                                                       double* __restrict__ vv, double* __restrict__ vbar) {
     constexpr int SPT = (N + BLK - 1) / BLK;
-    __shared__ double2 lds[4 * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int n = blockIdx.x / N;
     const int i = blockIdx.x % N;
     const EvDesc& ev = evs[0];
@@ -542,7 +543,7 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
         double2* io = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
         for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
             const int p = q / N, j = q - p * N;
-            lds[q] = io[(size_t)p * S + (size_t)i * N + j];
+            lds[ipgfft::at(N, p, j)] = io[(size_t)p * S + (size_t)i * N + j];
         }
         __syncthreads();
         ipgfft::lds_fft<N, 4, BLK, true>(lds, pl.tw);
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
                 double Q[8];
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
-                    const double2 v = lds[p * N + j];
+                    const double2 v = lds[ipgfft::at(N, p, j)];
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
                                                                   int k0, int kc) {
-    __shared__ double2 lds[4 * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int per_n = kc * N;
     const int n = blockIdx.x / per_n;
     const int kk = (blockIdx.x / N) % kc;
@@ -597,13 +598,13 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_fwd(const EvDesc*
     double2* io = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
-        lds[q] = io[(size_t)p * S + (size_t)i * N + j];
+        lds[ipgfft::at(N, p, j)] = io[(size_t)p * S + (size_t)i * N + j];
     }
     __syncthreads();
     ipgfft::lds_fft<N, 4, IPG_BLOCK, false>(lds, pl.tw);
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
-        io[(size_t)p * S + (size_t)i * N + j] = lds[q];
+        io[(size_t)p * S + (size_t)i * N + j] = lds[ipgfft::at(N, p, j)];
     }
 }
 // rows inverse: rhob for colours (2p, 2p+1) = (Re, Im) of packed field p into
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_fwd(const EvDesc*
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl,
                                                                   int k0, int kc, double* __restrict__ grad) {
-    __shared__ double2 lds[4 * N];
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int per_n = kc * N;
     const int n = blockIdx.x / per_n;
     const int kk = (blockIdx.x / N) % kc;
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_inv(const EvDesc*
     const double2* in = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
-        lds[q] = in[(size_t)p * S + (size_t)i * N + j];
+        lds[ipgfft::at(N, p, j)] = in[(size_t)p * S + (size_t)i * N + j];
     }
     __syncthreads();
     ipgfft::lds_fft<N, 4, IPG_BLOCK, true>(lds, pl.tw);
@@ -631,8 +632,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_filter_rows_inv(const EvDesc*
         double* g = grad + (((size_t)n * ev.Ny + k) * S + (size_t)i * N + j) * 8;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            g[2 * p] = lds[p * N + j].x;
-            g[2 * p + 1] = lds[p * N + j].y;
+            g[2 * p] = lds[ipgfft::at(N, p, j)].x;
+            g[2 * p + 1] = lds[ipgfft::at(N, p, j)].y;
         }
     }
 }
@@ -1149,20 +1150,25 @@ __global__ void k_newton_stats_zero(int nev, unsigned long long* __restrict__ ac
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 5 * nev) acc[t] = 0ull;
 }
-__global__ __launch_bounds__(NWB) void k_newton_stats(const EvDesc* __restrict__ evs, int nev,
+// kStatBlocks blocks of 256 threads per event stride over its 2S link
+// counters; one set of atomics per block (one per 64-link wave made the
+// atomics on the five accumulators the cost of the kernel)
+constexpr int kStatBlocks = 64;
+__global__ __launch_bounds__(256) void k_newton_stats(const EvDesc* __restrict__ evs, int nev,
                                                      unsigned long long* __restrict__ acc) {
-    const int e = find_event_links(evs, nev, blockIdx.x);
+    const int e = blockIdx.x / kStatBlocks;
+    const int part = blockIdx.x - e * kStatBlocks;
     const EvDesc& ev = evs[e];
     if (!ev.active) return;
-    const long long l = (long long)(blockIdx.x - ev.lblock0) * NWB + threadIdx.x;
     unsigned long long v[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
-    if (l < 2 * ev.S) {
+    for (long long l = (long long)part * 256 + threadIdx.x; l < 2 * ev.S; l += (long long)kStatBlocks * 256) {
         const int x = ev.newton_stat[l];
-        v[0] = (unsigned)(x & 0xFFFFF);
-        v[1] = v[0];
-        v[2] = (unsigned)((x >> 20) & 0xFF);
-        v[3] = (unsigned)((x >> 28) & 1);
-        v[4] = (unsigned)((x >> 29) & 1);
+        const unsigned long long it = (unsigned)(x & 0xFFFFF);
+        v[0] += it;
+        v[1] = it > v[1] ? it : v[1];
+        v[2] += (unsigned)((x >> 20) & 0xFF);
+        v[3] += (unsigned)((x >> 28) & 1);
+        v[4] += (unsigned)((x >> 29) & 1);
     }
 #pragma unroll
     for (int k = 0; k < 5; k++)
@@ -1171,7 +1177,16 @@ __global__ __launch_bounds__(NWB) void k_newton_stats(const EvDesc* __restrict__
             const unsigned long long t = __shfl_down(v[k], o, 64);
             v[k] = k == 1 ? (t > v[k] ? t : v[k]) : v[k] + t;
         }
+    __shared__ unsigned long long w[4][5];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 5; k++) w[wave][k] = v[k];
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 5; k++) v[k] = k == 1 ? (w[q][k] > v[k] ? w[q][k] : v[k]) : v[k] + w[q][k];
         atomicAdd(&acc[5 * e + 0], v[0]);
         atomicMax(&acc[5 * e + 1], v[1]);
         atomicAdd(&acc[5 * e + 2], v[2]);
@@ -1189,7 +1204,8 @@ void ipg_launch_newton_stats(const EvDesc* d, int nev, int nlinkblk, double* d_o
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(d_out + 8 * (size_t)nev);
     const int nb = (5 * nev + 255) / 256;
     hipLaunchKernelGGL(k_newton_stats_zero, dim3(nb), dim3(256), 0, st, nev, acc);
-    hipLaunchKernelGGL(k_newton_stats, dim3(nlinkblk), dim3(NWB), 0, st, d, nev, acc);
+    (void)nlinkblk;
+    hipLaunchKernelGGL(k_newton_stats, dim3(kStatBlocks * nev), dim3(256), 0, st, d, nev, acc);
     hipLaunchKernelGGL(k_newton_stats_out, dim3(nb), dim3(256), 0, st, nev, acc, d_out);
 }
 void ipg_launch_pi_init(const EvDesc* d, int nev, int nblk, hipStream_t st) {

@@ -1,6 +1,8 @@
 """The LDS transform of the colour-charge solve (ipglasma_amd/csrc/fft_lds.hpp),
 built for the host, against numpy.fft: every supported length, both signs,
-and the stage count of the plan."""
+three rows in the padded LDS layout (every slot distinct, inside the
+allocation, for the row counts the kernels use), and the stage count of the
+plan."""
 import ctypes as C
 import os
 import shutil
@@ -28,7 +30,7 @@ def lib():
         subprocess.run([HIPCC if os.path.exists(HIPCC) else "hipcc", "-x", "hip", "--offload-host-only", "-O2",
                         "-std=c++17", "-fPIC", "-shared", "-o", OUT, SRC], check=True, capture_output=True)
         L = C.CDLL(OUT)
-        L.ipgfft_host.argtypes = [C.c_int, np.ctypeslib.ndpointer(np.float64, flags="C"), C.c_int]
+        L.ipgfft_host.argtypes = [C.c_int, np.ctypeslib.ndpointer(np.float64, flags="C"), C.c_int, C.c_int]
         L.ipgfft_host.restype = C.c_int
         _lib = L
     return _lib
@@ -37,11 +39,11 @@ def lib():
 @pytest.mark.parametrize("N", SIZES)
 def test_lds_fft_matches_numpy(N):
     rng = np.random.default_rng(N)
-    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    x = rng.normal(size=(3, N)) + 1j * rng.normal(size=(3, N))
     for inverse in (0, 1):
         buf = np.ascontiguousarray(np.stack([x.real, x.imag], -1).reshape(-1))
-        ns = lib().ipgfft_host(N, buf, inverse)
+        ns = lib().ipgfft_host(N, buf, 3, inverse)
         assert ns == STAGES[N]
-        y = buf[0::2] + 1j * buf[1::2]
-        ref = np.fft.fft(x) if not inverse else np.fft.ifft(x) * N
+        y = (buf[0::2] + 1j * buf[1::2]).reshape(3, N)
+        ref = np.fft.fft(x, axis=1) if not inverse else np.fft.ifft(x, axis=1) * N
         assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max(), (N, inverse)

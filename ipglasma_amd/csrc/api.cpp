@@ -265,6 +265,8 @@ struct Engine {
     size_t adj_ws_n = 0;
     double* rho_grad = nullptr;  // last ipg_adjoint_rho result, in adj_ws
     size_t rho_grad_n = 0;
+    EvDesc* d_rdesc = nullptr;   // descriptors of the recomputed steps
+    size_t rdesc_n = 0;
 
     ~Engine() {
         for (int k = 0; k < 3; k++) {
@@ -288,6 +290,7 @@ struct Engine {
         if (d_out) hipFree(d_out);
         if (d_obs) hipFree(d_obs);
         if (adj_ws) hipFree(adj_ws);
+        if (d_rdesc) hipFree(d_rdesc);
         for (hipEvent_t t : tev) hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
             hipEventDestroy(t.a);
@@ -380,6 +383,9 @@ struct Engine {
             d.E1 = take(H);
             d.E2 = take(H);
             d.pi = take(H);
+            d.E1o = d.E1;
+            d.E2o = d.E2;
+            d.pio = d.pi;
             d.VA = d.U[1][0];
             d.VB = d.U[1][1];
             d.TpA = take(S);
@@ -1095,6 +1101,42 @@ struct Engine {
         HIPCHK(hipMemcpy(&tt, d_out, sizeof(double), hipMemcpyDeviceToHost));
         const double tau_fm = itm * d.dtau * d.a;
         const double L = tau_fm * d.a * d.a * ipg::HBARC * tt;
+        // The recomputed steps go from one stored state to the next (step it
+        // reads the slot of state it-1 and writes the slot of state it), each
+        // with a descriptor of its own: no copies through the live fields.
+        if (T > 0) {
+            std::vector<EvDesc> rd((size_t)T);
+            for (int k = 0; k < nck; k++) {
+                const int s0 = k * K, s1 = std::min(s0 + K, T);
+                for (int it = s0 + 1; it <= s1; it++) {
+                    double* in = it - 1 == s0 ? s_ck + (size_t)k * slot : s_seg + (size_t)(it - 1 - s0) * slot;
+                    double* out = s_seg + (size_t)(it - s0) * slot;
+                    EvDesc q = d;
+                    const int pp = (it - 1) & 1;
+                    q.U[pp][0] = in;
+                    q.U[pp][1] = in + 18 * (size_t)S;
+                    q.E1 = in + 36 * (size_t)S;
+                    q.E2 = in + 45 * (size_t)S;
+                    q.phi[pp] = in + 54 * (size_t)S;
+                    q.pi = in + 63 * (size_t)S;
+                    q.U[pp ^ 1][0] = out;
+                    q.U[pp ^ 1][1] = out + 18 * (size_t)S;
+                    q.E1o = out + 36 * (size_t)S;
+                    q.E2o = out + 45 * (size_t)S;
+                    q.phi[pp ^ 1] = out + 54 * (size_t)S;
+                    q.pio = out + 63 * (size_t)S;
+                    rd[(size_t)it - 1] = q;
+                }
+            }
+            if (rdesc_n < (size_t)T) {
+                if (d_rdesc) HIPCHK(hipFree(d_rdesc));
+                d_rdesc = nullptr;
+                rdesc_n = 0;
+                HIPCHK(hipMalloc(&d_rdesc, sizeof(EvDesc) * (size_t)T));
+                rdesc_n = (size_t)T;
+            }
+            HIPCHK(hipMemcpy(d_rdesc, rd.data(), sizeof(EvDesc) * (size_t)T, hipMemcpyHostToDevice));
+        }
         // reverse
         ipg_adj_zero(cA, (long long)pend, st);
         ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
@@ -1103,11 +1145,7 @@ struct Engine {
             const int s0 = k * K, s1 = std::min(s0 + K, T);
             if (s1 <= s0) continue;
             const double* x0 = s_ck + (size_t)k * slot;
-            ipg_adj_state_load(d_desc, S, s0 & 1, x0, st);
-            for (int it = s0 + 1; it <= s1; it++) {
-                ipg_launch_step(d_desc, 1, nblk, it, st, sv);
-                ipg_adj_state_save(d_desc, S, it & 1, s_seg + (size_t)(it - s0) * slot, st);
-            }
+            for (int it = s0 + 1; it <= s1; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nblk, it, st, sv);
             for (int it = s1; it > s0; it--) {
                 const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
                 const double* xa = s_seg + (size_t)(it - s0) * slot;

@@ -31,6 +31,12 @@ struct EvDesc {
     double* E1;
     double* E2;
     double* pi;
+    // where k_step writes the updated E1, E2, pi: the same planes (in place),
+    // except for the reverse mode's recomputed steps, which go from one
+    // stored state to the next
+    double* E1o;
+    double* E2o;
+    double* pio;
     // Wilson lines V_A, V_B (alias the parity-1 link buffers during init)
     double* VA;
     double* VB;

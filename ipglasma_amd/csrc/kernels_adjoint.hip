@@ -438,7 +438,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_pi_init(const EvDesc* __restr
 
 // root finder (implicit) and links in reverse: for each link, U1b, U2b from
 // the total cotangent of its solution, plus Db (L1b += Db, L2b -= Db); then
-// L = V(s) V(s+d)^+ adds to the Wilson-line cotangents (vb, 2 x 18 planes)
+// L = V(s) V(s+d)^+ adds to the Wilson-line cotangents (vb, 2 x 18 planes).
+// One launch per direction DIR (fewer live registers than both in one).
+template <int DIR>
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __restrict__ evs,
                                                                const double* __restrict__ s0,
                                                                const double* __restrict__ vab,
@@ -454,7 +456,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __
     const Slot X = slot_of(s0, S);
     const double* VA = vab;
     const double* VB = vab + 18 * S;
-    for (int dir = 0; dir < 2; dir++) {
+    {
+        constexpr int dir = DIR;
         const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
         const M3 va = ldm(VA, S, s), van = ldm(VA, S, n), vbs = ldm(VB, S, s), vbn = ldm(VB, S, n);
         const M3 U1 = mul_bd(va, van), U2 = mul_bd(vbs, vbn);
@@ -481,8 +484,9 @@ void ipg_adj_pi_init(const EvDesc* d, long long S, const double* s0, const doubl
 }
 void ipg_adj_links_newton(const EvDesc* d, long long S, const double* s0, const double* vab, const double* ulb,
                           const double* db, double* vb, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_links_newton, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0,
-                       st, d, s0, vab, ulb, db, vb);
+    const dim3 g((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK));
+    hipLaunchKernelGGL(k_adj_links_newton<0>, g, dim3(IPG_BLOCK), 0, st, d, s0, vab, ulb, db, vb);
+    hipLaunchKernelGGL(k_adj_links_newton<1>, g, dim3(IPG_BLOCK), 0, st, d, s0, vab, ulb, db, vb);
 }
 
 static unsigned nb_of(long long n) { return (unsigned)((n + IPG_BLOCK - 1) / IPG_BLOCK); }

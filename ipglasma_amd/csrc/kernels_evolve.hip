@@ -138,7 +138,7 @@ __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restric
         }
         br = br - 4. * ph;
         pin = herm_part(ldh(ev.pi, S, s) + (dt / tau) * br);
-        sth(ev.pi, S, s, pin);
+        sth(ev.pio, S, s, pin);
         if (!half) sth(ev.phi[p ^ 1], S, s, ph + ((tau + dt / 2.) * dt) * pin);  // evolvePhi
     }
     IPG_PHASE();
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restric
         M3 P = mul_bd(ux, ldm(Uy, S, n.pXmY)) * mul_ad(ldm(Ux, S, n.mY), ldm(Uy, S, n.mY));
         const AH t1 = ah_tl(ah_of(U12 + P));
         const M3 e1n = e_update(ldh(ev.E1, S, s), a1, t1, a2, c1);
-        sth(ev.E1, S, s, e1n);
+        sth(ev.E1o, S, s, e1n);
         if (!half) stm(ev.U[p ^ 1][0], S, s, taylor2(cu, e1n) * ux);  // evolveU
     }
     IPG_PHASE();
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restric
         M3 P = mul_bd(uy, ldm(Ux, S, n.mXpY)) * mul_ad(ldm(Uy, S, n.mX), ldm(Ux, S, n.mX));
         const AH t1 = ah_tl(ah_of(dag(U12) + P));
         const M3 e2n = e_update(ldh(ev.E2, S, s), a1, t1, a2, c2);
-        sth(ev.E2, S, s, e2n);
+        sth(ev.E2o, S, s, e2n);
         if (!half) stm(ev.U[p ^ 1][1], S, s, taylor2(cu, e2n) * uy);
     }
 }

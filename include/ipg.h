@@ -169,6 +169,30 @@ int ipg_adjoint_dedeta(ipg_ctx *c, double *dedeta, double *grad);
  * the next call.  No counterpart in the reference. */
 int ipg_adjoint_rho(ipg_ctx *c, double *dedeta, double *grad);
 
+/* Reverse mode with a choice of loss (all at tau_max, from T^{mu nu} at
+ * itmax as Evolution.cpp:406-431 takes it):
+ *   IPG_LOSS_DEDETA  dE/deta (as ipg_adjoint_dedeta / ipg_adjoint_rho)
+ *   IPG_LOSS_ECC2    eps_2 of the Landau-frame weights eps u^tau
+ *                    (Evolution.cpp:1280-1660 with MyEigen.cpp:25-321)
+ *   IPG_LOSS_TMUNU   sum_s sum_k weights[k] T_k(s) over the ten stored
+ *                    components (T's order: tautau, xx, yy, etaeta, taux,
+ *                    tauy, taueta, xy, xeta, yeta; weights has 10 entries)
+ * wrt IPG_WRT_FIELDS: grad[6][S][18] as ipg_adjoint_dedeta; IPG_WRT_RHO:
+ * grad[2][Ny][S][8] as ipg_adjoint_rho (NULL keeps it on the device);
+ * IPG_WRT_TMUNU: grad[10][S], the loss's gradient with respect to the ten
+ * stored T^{mu nu} components at itmax (no reverse evolution).
+ * value receives the loss.  Call after ipg_init in place of ipg_evolve.
+ * Landau-frame branches that set a constant (rest frame, eps = T^tautau
+ * fallbacks) are differentiated as constants.  No counterpart in the
+ * reference. */
+#define IPG_LOSS_DEDETA 0
+#define IPG_LOSS_ECC2 1
+#define IPG_LOSS_TMUNU 2
+#define IPG_WRT_FIELDS 0
+#define IPG_WRT_RHO 1
+#define IPG_WRT_TMUNU 2
+int ipg_adjoint_loss(ipg_ctx *c, int loss, const double *weights, int wrt, double *value, double *grad);
+
 /* Host copy of the last ipg_adjoint_rho gradient, grad[2][Ny][S][8].
  * IPG_ERR_STATE when there is none.  No counterpart in the reference. */
 int ipg_get_rho_grad(ipg_ctx *c, double *grad);

@@ -128,6 +128,7 @@ def lib():
             "ipg_adjoint_dedeta": (C.c_int, [vp, dp, dp]),
             "ipg_adjoint_rho": (C.c_int, [vp, dp, dp]),
             "ipg_get_rho_grad": (C.c_int, [vp, dp]),
+            "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
@@ -258,6 +259,38 @@ class Event:
         v = C.c_double()
         _check(lib().ipg_adjoint_dedeta(self._c, C.byref(v), _dptr(g)))
         return v.value, {k: g[i] for i, k in enumerate(("Ux", "Uy", "E1", "E2", "phi", "pi"))}
+
+    LOSSES = {"dEdeta": 0, "eps2": 1, "tmunu": 2}
+
+    def adjoint(self, loss="dEdeta", weights=None, wrt="fields", fetch=True):
+        """Reverse mode with a choice of loss at tau_max: "dEdeta", "eps2"
+        (eps_2 of the Landau-frame weights) or "tmunu" (sum over sites of
+        weights[k] T_k, ten weights in T's order).  wrt "fields": gradient
+        {name: [S, 18]} with respect to the tau = 0 fields; wrt "rho":
+        [2, Ny, S, 8] (fetch=False leaves it on the device); wrt "tmunu":
+        [10, S], the gradient with respect to T^{mu nu} at itmax.  Consumes
+        the state; call after init."""
+        v = C.c_double()
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            if w.shape != (10,):
+                raise ValueError("weights: ten entries in T's order")
+        if wrt == "fields":
+            g = np.zeros((6, self.S, 18))
+            _check(lib().ipg_adjoint_loss(self._c, self.LOSSES[loss], None if w is None else _dptr(w), 0,
+                                          C.byref(v), _dptr(g)))
+            return v.value, {k: g[i] for i, k in enumerate(("Ux", "Uy", "E1", "E2", "phi", "pi"))}
+        if wrt == "rho":
+            _check(lib().ipg_adjoint_loss(self._c, self.LOSSES[loss], None if w is None else _dptr(w), 1,
+                                          C.byref(v), None))
+            return v.value, (self.rho_grad() if fetch else None)
+        if wrt == "tmunu":
+            g = np.zeros((10, self.S))
+            _check(lib().ipg_adjoint_loss(self._c, self.LOSSES[loss], None if w is None else _dptr(w), 2,
+                                          C.byref(v), _dptr(g)))
+            return v.value, g
+        raise ValueError("wrt: 'fields', 'rho' or 'tmunu'")
 
     def adjoint_rho(self, fetch=True):
         """dE/deta and d(dE/deta)/d rho as [2, Ny, S, 8] (consumes the state;

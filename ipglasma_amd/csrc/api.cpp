@@ -969,9 +969,17 @@ struct Engine {
     // the initial conditions (U_x, U_y, E1, E2, phi, pi at tau = 0), by
     // reverse mode through the tau = 0 kick and itmax-1 leapfrog steps with
     // checkpoints every K = ceil(sqrt(itmax-1)) steps.  Consumes the state.
-    int adjoint_dedeta(double* dedeta, double* grad) {
+    int adjoint_dedeta(double* dedeta, double* grad) { return adjoint_fields(IPG_LOSS_DEDETA, nullptr, dedeta, grad); }
+
+    // the loss's gradient with respect to the tau = 0 fields
+    int adjoint_fields(int loss, const double* w10, double* value, double* grad) {
         const double *s_init = nullptr, *cs0 = nullptr;
-        return adjoint_evolution(dedeta, grad, &s_init, &cs0);
+        return adjoint_evolution(loss, w10, value, grad, &s_init, &cs0);
+    }
+    // the loss's gradient with respect to the ten T^{mu nu} planes at itmax
+    int adjoint_tmunu(int loss, const double* w10, double* value, double* grad) {
+        const double *s_init = nullptr, *cs0 = nullptr;
+        return adjoint_evolution(loss, w10, value, grad, &s_init, &cs0, true);
     }
 
     // doubles of the evolution part of the workspace
@@ -979,7 +987,7 @@ struct Engine {
         const int T = d.itmax - 1;
         const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
         const size_t nslot = 1 + (size_t)(T / K + 1) + (size_t)K + 1 + 1;
-        return nslot * 72 * (size_t)d.S + (2 * IPG_PEND_PLANES + 108) * (size_t)d.S;
+        return nslot * 72 * (size_t)d.S + (2 * IPG_PEND_PLANES + 108 + 10) * (size_t)d.S + 16;
     }
 
     // grows the workspace to at least n doubles (contents not kept)
@@ -1000,7 +1008,9 @@ struct Engine {
     // [2][Ny][S][8] (nucleus, slice, site, colour) stays in the workspace
     // (rho_grad) and is copied to grad when grad is not null.  Call after
     // ipg_init: the Wilson lines of that call are still in the V buffers.
-    int adjoint_rho(double* dedeta, double* grad) {
+    int adjoint_rho(double* dedeta, double* grad) { return adjoint_rho_loss(IPG_LOSS_DEDETA, nullptr, dedeta, grad); }
+
+    int adjoint_rho_loss(int loss, const double* w10, double* dedeta, double* grad) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
         EvDesc& d = desc[0];
         const long long S = d.S;
@@ -1021,7 +1031,7 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(vab, d.VA, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(vab + F, d.VB, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
         const double *s_init = nullptr, *cs0 = nullptr;
-        rc = adjoint_evolution(dedeta, nullptr, &s_init, &cs0);
+        rc = adjoint_evolution(loss, w10, dedeta, nullptr, &s_init, &cs0);
         if (rc) return rc;
         // tau = 0 links: total cotangent = from the evolution + from pi
         HIPCHK(hipMemcpyAsync(ulb, cs0, sizeof(double) * 2 * F, hipMemcpyDeviceToDevice, st));
@@ -1055,8 +1065,11 @@ struct Engine {
     // adjoint_rho reserves its own buffers behind it first); *s_init is the
     // tau = 0 state slot and *cs0 its cotangent set (U_x, U_y, E1, E2, phi,
     // pi, 18 planes each)
-    int adjoint_evolution(double* dedeta, double* grad, const double** s_init_out, const double** cs0_out) {
+    int adjoint_evolution(int loss, const double* w10, double* dedeta, double* grad, const double** s_init_out,
+                          const double** cs0_out, bool tb_only = false) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
+        if (loss < IPG_LOSS_DEDETA || loss > IPG_LOSS_TMUNU) return fail(IPG_ERR_ARG, "unknown loss");
+        if (loss == IPG_LOSS_TMUNU && !w10) return fail(IPG_ERR_ARG, "the T^{mu nu} loss needs 10 weights");
         EvDesc& d = desc[0];
         const int itm = d.itmax, T = itm - 1;
         if (itm < 1) return fail(IPG_ERR_ARG, "itmax must be at least 1");
@@ -1071,7 +1084,7 @@ struct Engine {
             int rc = adj_reserve(need);
             if (rc) return rc;
         }
-        if ((size_t)(nck + K + 3) * slot + 2 * pend + cset > need)
+        if ((size_t)(nck + K + 3) * slot + 2 * pend + cset + 10 * (size_t)S + 16 > need)
             return fail(IPG_ERR_STATE, "adjoint workspace layout");
         double* mem = adj_ws;
         double* s_init = mem;
@@ -1081,6 +1094,8 @@ struct Engine {
         double* cA = s_T + slot;     // pending cotangent sets
         double* cB = cA + pend;
         double* c0 = cB + pend;      // compact cotangents of the tau = 0 fields
+        double* tb = c0 + cset;      // cotangents of the ten T^{mu nu} planes at itmax
+        double* wdev = tb + 10 * (size_t)S;
         int rc = upload_desc();
         if (rc) return rc;
         const int sv = step_decided();
@@ -1100,7 +1115,24 @@ struct Engine {
         double tt = 0.;
         HIPCHK(hipMemcpy(&tt, d_out, sizeof(double), hipMemcpyDeviceToHost));
         const double tau_fm = itm * d.dtau * d.a;
-        const double L = tau_fm * d.a * d.a * ipg::HBARC * tt;
+        const double dEdeta = tau_fm * d.a * d.a * ipg::HBARC * tt;
+        double L = dEdeta;
+        if (loss == IPG_LOSS_ECC2) {
+            // Landau frame and eccentricities at itmax (k_landau ... k_ecc_finish)
+            ipg_launch_landau(d_desc, 1, nblk, itm, 0, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpy(&L, d_obs + 1, sizeof(double), hipMemcpyDeviceToHost));
+        } else if (loss == IPG_LOSS_TMUNU) {
+            std::vector<double> hT(10 * (size_t)S);
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipMemcpy(hT.data(), d.T, sizeof(double) * hT.size(), hipMemcpyDeviceToHost));
+            L = 0.;
+            for (int k = 0; k < 10; k++) {
+                double sk = 0.;
+                for (long long q = 0; q < S; q++) sk += hT[(size_t)k * S + q];
+                L += w10[k] * sk;
+            }
+        }
         // The recomputed steps go from one stored state to the next (step it
         // reads the slot of state it-1 and writes the slot of state it), each
         // with a descriptor of its own: no copies through the live fields.
@@ -1137,9 +1169,40 @@ struct Engine {
             }
             HIPCHK(hipMemcpy(d_rdesc, rd.data(), sizeof(EvDesc) * (size_t)T, hipMemcpyHostToDevice));
         }
-        // reverse
+        if (tb_only) {
+            // dL/dT^{mu nu} at itmax only
+            if (loss == IPG_LOSS_ECC2) {
+                ipg_adj_ecc2(d_desc, S, itm, tb, st);
+            } else {
+                double w[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+                if (loss == IPG_LOSS_TMUNU)
+                    for (int k = 0; k < 10; k++) w[k] = w10[k];
+                else
+                    w[0] = tau_fm * d.a * d.a * ipg::HBARC;
+                HIPCHK(hipMemcpy(wdev, w, sizeof(double) * 10, hipMemcpyHostToDevice));
+                ipg_adj_fill_tb(S, wdev, tb, st);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));
+            if (grad) HIPCHK(hipMemcpy(grad, tb, sizeof(double) * 10 * (size_t)S, hipMemcpyDeviceToHost));
+            if (dedeta) *dedeta = L;
+            *s_init_out = s_init;
+            *cs0_out = nullptr;
+            return IPG_OK;
+        }
+        // reverse: the loss's cotangents of the state after step itmax-1
         ipg_adj_zero(cA, (long long)pend, st);
-        ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
+        if (loss == IPG_LOSS_DEDETA) {
+            ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
+        } else {
+            if (loss == IPG_LOSS_ECC2) {
+                ipg_adj_ecc2(d_desc, S, itm, tb, st);
+            } else {
+                HIPCHK(hipMemcpy(wdev, w10, sizeof(double) * 10, hipMemcpyHostToDevice));
+                ipg_adj_fill_tb(S, wdev, tb, st);
+            }
+            ipg_adj_tmunu(d_desc, S, s_T, itm, tb, cA, st);
+        }
         double *cin = cA, *cout = cB;
         for (int k = nck - 1; k >= 0; k--) {
             const int s0 = k * K, s1 = std::min(s0 + K, T);
@@ -1169,7 +1232,7 @@ struct Engine {
             }
         }
         if (dedeta) *dedeta = L;
-        ev[0].obs.dEdeta = L;
+        ev[0].obs.dEdeta = dEdeta;
         ev[0].obs.tau_fm = tau_fm;
         *s_init_out = s_init;
         *cs0_out = c0;
@@ -1418,6 +1481,13 @@ int ipg_adjoint_dedeta(ipg_ctx* c, double* dedeta, double* grad) {
 int ipg_adjoint_rho(ipg_ctx* c, double* dedeta, double* grad) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     return c->eng.adjoint_rho(dedeta, grad);
+}
+int ipg_adjoint_loss(ipg_ctx* c, int loss, const double* weights, int wrt, double* value, double* grad) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    if (wrt == IPG_WRT_FIELDS) return c->eng.adjoint_fields(loss, weights, value, grad);
+    if (wrt == IPG_WRT_RHO) return c->eng.adjoint_rho_loss(loss, weights, value, grad);
+    if (wrt == IPG_WRT_TMUNU) return c->eng.adjoint_tmunu(loss, weights, value, grad);
+    return fail(IPG_ERR_ARG, "wrt must be IPG_WRT_FIELDS, IPG_WRT_RHO or IPG_WRT_TMUNU");
 }
 int ipg_get_rho_grad(ipg_ctx* c, double* grad) {
     if (!c || !grad) return fail(IPG_ERR_ARG, "null argument");

@@ -58,6 +58,14 @@ void ipg_adj_loss(const EvDesc* d_evs, long long S, const double* xs, int it, do
                   hipStream_t st);
 void ipg_adj_step(const EvDesc* d_evs, long long S, const double* xb, const double* xa, int it, const double* cin,
                   double* cout, hipStream_t st);
+// T^{mu nu} at it in reverse: tb[10][S] cotangents of the stored components,
+// field cotangents of the state xs into a zeroed pending set
+void ipg_adj_tmunu(const EvDesc* d_evs, long long S, const double* xs, int it, const double* tb, double* pout,
+                   hipStream_t st);
+// d eps_2 / dT at it (after ipg_launch_landau of the same it) into tb[10][S]
+void ipg_adj_ecc2(const EvDesc* d_evs, long long S, int it, double* tb, hipStream_t st);
+// tb[k][s] = w[k] (device array of 10 weights)
+void ipg_adj_fill_tb(long long S, const double* w, double* tb, hipStream_t st);
 void ipg_adj_kick(const EvDesc* d_evs, long long S, const double* x0, const double* cin, double* cout,
                   hipStream_t st);
 void ipg_adj_pi_init(const EvDesc* d_evs, long long S, const double* s0, const double* vab, const double* cs0,

@@ -235,6 +235,328 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict
     atomic_add_h(cPh, S, pXY, b.phpxy);
 }
 
+// ---- T^{mu nu} at itmax in reverse (Evolution.cpp:659-1246) -----------------------
+// tb: cotangents of the ten stored components (T's order, after the a^-4,
+// a^-5, a^-6 scalings) at every site; the cotangents of the fields of the
+// state xs go into the own-field planes of a zeroed pending set (atomics:
+// each site's terms reach up to two sites away).  Three passes as the
+// forward's: diagonal components with T^{tau eta} and T^{xy}; the clovers of
+// T^{tau x}, T^{tau y}, T^{x eta}, T^{y eta}; their pi and E pi parts.
+// f = Re tr(A B): Ab += c B^+, Bb += c A^+;  f = Im tr(A B): Ab += c i B^+, Bb += c i A^+.
+
+struct TScat {
+    double* P;
+    size_t S;
+    __device__ void u1(size_t q, const M3& m) const { atomic_add_m3(P + kPU1 * S, S, q, m); }
+    __device__ void u2(size_t q, const M3& m) const { atomic_add_m3(P + kPU2 * S, S, q, m); }
+    __device__ void e1(size_t q, const M3& m) const { atomic_add_h(P + kPE1 * S, S, q, m); }
+    __device__ void e2(size_t q, const M3& m) const { atomic_add_h(P + kPE2 * S, S, q, m); }
+    __device__ void ph(size_t q, const M3& m) const { atomic_add_h(P + kPPh * S, S, q, m); }
+    __device__ void pi(size_t q, const M3& m) const { atomic_add_h(P + kPPi * S, S, q, m); }
+};
+
+// the phi gradient G = U phi(n) U^+ - phi(m) in reverse: U at uq (field f:
+// 1 = U_x, 2 = U_y), given Gb
+__device__ __forceinline__ void grad_adj(const TScat& sc, const Slot& X, size_t S, int f, size_t uq, size_t n,
+                                         size_t m, const M3& Gb) {
+    const M3 u = ldm(f == 1 ? X.Ux : X.Uy, S, uq);
+    M3 ub = m_zero(), pb = m_zero();
+    adj_conj(u, ldh(X.phi, S, n), Gb, ub, pb);
+    if (f == 1)
+        sc.u1(uq, ub);
+    else
+        sc.u2(uq, ub);
+    sc.ph(n, pb);
+    sc.ph(m, -1. * Gb);
+}
+__device__ __forceinline__ M3 grad_fwd(const Slot& X, size_t S, int f, size_t uq, size_t n, size_t m) {
+    const M3 u = ldm(f == 1 ? X.Ux : X.Uy, S, uq);
+    return (u * ldh(X.phi, S, n)) * dag(u) - ldh(X.phi, S, m);
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_a(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
+                                                          int it, const double* __restrict__ tb,
+                                                          double* __restrict__ pout) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const Slot X = slot_of(xs, S);
+    const TScat sc{pout, S};
+    const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+    const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+    const size_t pX = L.at(i + 1, j), pY = L.at(i, j + 1), pXY = L.at(i + 1, j + 1);
+    const double b0 = tb[s] / a4, b1 = tb[S + s] / a4, b2 = tb[2 * S + s] / a4, b3 = tb[3 * S + s] / a6;
+    const double gt = g * g / tau / tau, it2 = 1. / tau / tau, h = 0.5 / tau / tau;
+    const double t1b = gt / 2. * (b0 - b1 + b2 + it2 * b3), t2b = gt / 2. * (b0 + b1 - b2 + it2 * b3);
+    const double lb = b0 + b1 + b2 - it2 * b3;  // the pi sum and the plaquette term
+    const double Ab = h * (b0 + b1 - b2 + it2 * b3), Cb = h * (b0 - b1 + b2 + it2 * b3);
+    const double reb = tb[6 * S + s] * g / (tau * tau * tau) / a5;  // T^{tau eta}
+    const double bb = tb[7 * S + s] * 0.5 / (tau * tau) / a4;       // T^{xy}
+    // pi: sum Re tr(pi^2) / 4
+    sc.pi(s, (lb / 2.) * ldh(X.pi, S, s));
+    sc.pi(pX, (lb / 2.) * ldh(X.pi, S, pX));
+    sc.pi(pY, (lb / 2.) * ldh(X.pi, S, pY));
+    sc.pi(pXY, (lb / 2.) * ldh(X.pi, S, pXY));
+    // plaquette: 2/g^2 (3 - Re tr(ux uy(+x) ux(+y)^+ uy^+)), 1 on the last row
+    if (i != N - 1) {
+        M3 B1, B2, B3, B4;
+        adj_prod4(ldm(X.Ux, S, s), ldm(X.Uy, S, pX), dag(ldm(X.Ux, S, pY)), dag(ldm(X.Uy, S, s)),
+                  (-2. / (g * g) * lb) * m_one(), B1, B2, B3, B4);
+        sc.u1(s, B1);
+        sc.u2(pX, B2);
+        sc.u1(pY, dag(B3));
+        sc.u2(s, dag(B4));
+    }
+    const M3 e1 = ldh(X.E1, S, s), e1p = ldh(X.E1, S, pY), e2 = ldh(X.E2, S, s), e2p = ldh(X.E2, S, pX);
+    M3 e1b = (2. * t1b) * e1, e1pb = (2. * t1b) * e1p, e2b = (2. * t2b) * e2, e2pb = (2. * t2b) * e2p;
+    const M3 gx = grad_fwd(X, S, 1, s, pX, s), gy = grad_fwd(X, S, 2, s, pY, s);
+    const M3 gxy1 = grad_fwd(X, S, 1, pY, pXY, pY), gxy2 = grad_fwd(X, S, 2, pX, pXY, pX);
+    M3 gxb = (2. * Ab) * gx, gxy1b = (2. * Ab) * gxy1, gyb = (2. * Cb) * gy, gxy2b = (2. * Cb) * gxy2;
+    // T^{tau eta}: Re tr(E1 gX) + Re tr(E1(+y) gXY1) + Re tr(E2 gY) + Re tr(E2(+x) gXY2)
+    e1b = e1b + reb * dag(gx);
+    gxb = gxb + reb * dag(e1);
+    e1pb = e1pb + reb * dag(gxy1);
+    gxy1b = gxy1b + reb * dag(e1p);
+    e2b = e2b + reb * dag(gy);
+    gyb = gyb + reb * dag(e2);
+    e2pb = e2pb + reb * dag(gxy2);
+    gxy2b = gxy2b + reb * dag(e2p);
+    // T^{xy}: 4 Re tr(c P Q) + Re tr(gx gy) + Re tr(yl gy) + Re tr(gx xl) + Re tr(yl xl)
+    {
+        const M3 ux = ldm(X.Ux, S, s), uy = ldm(X.Uy, S, s);
+        M3 uxb = m_zero(), uyb = m_zero();
+        const double c = -0.25 * g * g;
+        const M3 Pm = e1 + (uy * e1p) * dag(uy), Qm = e2 + (ux * e2p) * dag(ux);
+        const M3 Pb = (4. * bb * c) * dag(Qm), Qb = (4. * bb * c) * dag(Pm);
+        e1b = e1b + Pb;
+        adj_conj(uy, e1p, Pb, uyb, e1pb);
+        e2b = e2b + Qb;
+        adj_conj(ux, e2p, Qb, uxb, e2pb);
+        const M3 yl = (uy * gxy1) * dag(uy), xl = (ux * gxy2) * dag(ux);
+        M3 ylb = bb * (dag(gy) + dag(xl));
+        M3 xlb = bb * (dag(gx) + dag(yl));
+        gxb = gxb + bb * (dag(gy) + dag(xl));
+        gyb = gyb + bb * (dag(gx) + dag(yl));
+        adj_conj(uy, gxy1, ylb, uyb, gxy1b);
+        adj_conj(ux, gxy2, xlb, uxb, gxy2b);
+        sc.u1(s, uxb);
+        sc.u2(s, uyb);
+    }
+    sc.e1(s, e1b);
+    sc.e1(pY, e1pb);
+    sc.e2(s, e2b);
+    sc.e2(pX, e2pb);
+    grad_adj(sc, X, S, 1, s, pX, s, gxb);
+    grad_adj(sc, X, S, 2, s, pY, s, gyb);
+    grad_adj(sc, X, S, 1, pY, pXY, pY, gxy1b);
+    grad_adj(sc, X, S, 2, pX, pXY, pX, gxy2b);
+}
+
+// one clover product m4(f1 q1 [d1], ...) in reverse: factor k is field fk
+// (1 = U_x, 2 = U_y) at site qk, daggered when dk
+struct Fac {
+    int f;
+    size_t q;
+    bool d;
+};
+__device__ __forceinline__ M3 fac_ld(const Slot& X, size_t S, const Fac& a) {
+    const M3 m = ldm(a.f == 1 ? X.Ux : X.Uy, S, a.q);
+    return a.d ? dag(m) : m;
+}
+__device__ __forceinline__ void fac_st(const TScat& sc, const Fac& a, const M3& b) {
+    const M3 m = a.d ? dag(b) : b;
+    if (a.f == 1)
+        sc.u1(a.q, m);
+    else
+        sc.u2(a.q, m);
+}
+__device__ __forceinline__ M3 prod4(const Slot& X, size_t S, const Fac (&f)[4]) {
+    return ((fac_ld(X, S, f[0]) * fac_ld(X, S, f[1])) * fac_ld(X, S, f[2])) * fac_ld(X, S, f[3]);
+}
+__device__ __forceinline__ void prod4_adj(const TScat& sc, const Slot& X, size_t S, const Fac (&f)[4], const M3& Yb) {
+    M3 B1, B2, B3, B4;
+    adj_prod4(fac_ld(X, S, f[0]), fac_ld(X, S, f[1]), fac_ld(X, S, f[2]), fac_ld(X, S, f[3]), Yb, B1, B2, B3, B4);
+    fac_st(sc, f[0], B1);
+    fac_st(sc, f[1], B2);
+    fac_st(sc, f[2], B3);
+    fac_st(sc, f[3], B4);
+}
+// K = TL(P1 - P2) + TL(P3 - P4) and its reverse given Kb
+__device__ __forceinline__ M3 clover(const Slot& X, size_t S, const Fac (&p1)[4], const Fac (&p2)[4],
+                                     const Fac (&p3)[4], const Fac (&p4)[4]) {
+    return traceless(prod4(X, S, p1) - prod4(X, S, p2)) + traceless(prod4(X, S, p3) - prod4(X, S, p4));
+}
+__device__ __forceinline__ void clover_adj(const TScat& sc, const Slot& X, size_t S, const Fac (&p1)[4],
+                                           const Fac (&p2)[4], const Fac (&p3)[4], const Fac (&p4)[4], const M3& Kb) {
+    const M3 t = traceless(Kb);
+    prod4_adj(sc, X, S, p1, t);
+    prod4_adj(sc, X, S, p2, -1. * t);
+    prod4_adj(sc, X, S, p3, t);
+    prod4_adj(sc, X, S, p4, -1. * t);
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
+                                                          int it, const double* __restrict__ tb,
+                                                          double* __restrict__ pout) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int i = (int)(s / L.N), j = (int)(s - (long long)i * L.N);
+    const Slot X = slot_of(xs, S);
+    const TScat sc{pout, S};
+    const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+    const double a4 = a * a * a * a, a5 = a4 * a;
+    const size_t s0 = (size_t)s;
+    const size_t pX = L.at(i + 1, j), pY = L.at(i, j + 1), pXY = L.at(i + 1, j + 1), mX = L.at(i - 1, j);
+    const size_t mY = L.at(i, j - 1), mXpY = L.at(i - 1, j + 1), pXmY = L.at(i + 1, j - 1);
+    const size_t p2X = L.at(i + 2, j), p2Y = L.at(i, j + 2);
+    const double txb = tb[4 * S + s] / (4. * tau * a4), tyb = tb[5 * S + s] / (4. * tau * a4);
+    const double xeb = -tb[8 * S + s] / (4. * g * tau * tau * a5), yeb = -tb[9 * S + s] / (4. * g * tau * tau * a5);
+    // X1 (tau x, x eta with E2(s), gY)
+    {
+        const Fac pa[4] = {{2, s0, false}, {1, pY, false}, {2, pX, true}, {1, s0, true}};
+        const Fac pb[4] = {{1, s0, false}, {2, pX, false}, {1, pY, true}, {2, s0, true}};
+        const Fac pc[4] = {{1, mX, true}, {2, mX, false}, {1, mXpY, false}, {2, s0, true}};
+        const Fac pd[4] = {{2, s0, false}, {1, mXpY, true}, {2, mX, true}, {1, mX, false}};
+        const M3 K = clover(X, S, pa, pb, pc, pd);
+        const M3 e2 = ldh(X.E2, S, s0), gy = grad_fwd(X, S, 2, s0, pY, s0);
+        // ttx_im = Im tr(E2 K), xe_im = -Im tr(K gY)
+        const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gy));
+        sc.e2(s0, mul_i(txb, dag(K)));
+        grad_adj(sc, X, S, 2, s0, pY, s0, mul_i(-xeb, dag(K)));
+        clover_adj(sc, X, S, pa, pb, pc, pd, Kb);
+    }
+    // Y1 (tau y, y eta with E1(s), gX)
+    {
+        const Fac pa[4] = {{1, s0, false}, {2, pX, false}, {1, pY, true}, {2, s0, true}};
+        const Fac pb[4] = {{2, s0, false}, {1, pY, false}, {2, pX, true}, {1, s0, true}};
+        const Fac pc[4] = {{2, mY, true}, {1, mY, false}, {2, pXmY, false}, {1, s0, true}};
+        const Fac pd[4] = {{1, s0, false}, {2, pXmY, true}, {1, mY, true}, {2, mY, false}};
+        const M3 K = clover(X, S, pa, pb, pc, pd);
+        const M3 e1 = ldh(X.E1, S, s0), gx = grad_fwd(X, S, 1, s0, pX, s0);
+        const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gx));
+        sc.e1(s0, mul_i(tyb, dag(K)));
+        grad_adj(sc, X, S, 1, s0, pX, s0, mul_i(-yeb, dag(K)));
+        clover_adj(sc, X, S, pa, pb, pc, pd, Kb);
+    }
+    // X2 (tau x, x eta with E2(+x), gXY2)
+    {
+        const Fac pe[4] = {{2, pX, false}, {1, pXY, false}, {2, p2X, true}, {1, pX, true}};
+        const Fac pf[4] = {{1, pX, false}, {2, p2X, false}, {1, pXY, true}, {2, pX, true}};
+        const Fac pg[4] = {{1, s0, true}, {2, s0, false}, {1, pY, false}, {2, pX, true}};
+        const Fac ph[4] = {{2, pX, false}, {1, pY, true}, {2, s0, true}, {1, s0, false}};
+        const M3 K = clover(X, S, pe, pf, pg, ph);
+        const M3 e2 = ldh(X.E2, S, pX), gg = grad_fwd(X, S, 2, pX, pXY, pX);
+        const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gg));
+        sc.e2(pX, mul_i(txb, dag(K)));
+        grad_adj(sc, X, S, 2, pX, pXY, pX, mul_i(-xeb, dag(K)));
+        clover_adj(sc, X, S, pe, pf, pg, ph, Kb);
+    }
+    // Y2 (tau y, y eta with E1(+y), gXY1)
+    {
+        const Fac qe[4] = {{1, pY, false}, {2, pXY, false}, {1, p2Y, true}, {2, pY, true}};
+        const Fac qf[4] = {{2, pY, false}, {1, p2Y, false}, {2, pXY, true}, {1, pY, true}};
+        const Fac qg[4] = {{2, s0, true}, {1, s0, false}, {2, pX, false}, {1, pY, true}};
+        const Fac qh[4] = {{1, pY, false}, {2, pX, true}, {1, s0, true}, {2, s0, false}};
+        const M3 K = clover(X, S, qe, qf, qg, qh);
+        const M3 e1 = ldh(X.E1, S, pY), gg = grad_fwd(X, S, 1, pY, pXY, pY);
+        const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gg));
+        sc.e1(pY, mul_i(tyb, dag(K)));
+        grad_adj(sc, X, S, 1, pY, pXY, pY, mul_i(-yeb, dag(K)));
+        clover_adj(sc, X, S, qe, qf, qg, qh, Kb);
+    }
+}
+
+// Re tr(PI(w) (U phi(n) U^+ - V^+ phi(m) V)) in reverse with weight c; U at uq, V at vq
+__device__ __forceinline__ void pigrad_adj(const TScat& sc, const Slot& X, size_t S, int f, double c, size_t w,
+                                           size_t uq, size_t n, size_t vq, size_t m) {
+    const double* U = f == 1 ? X.Ux : X.Uy;
+    const M3 u = ldm(U, S, uq), v = ldm(U, S, vq), pw = ldh(X.pi, S, w);
+    const M3 G = (u * ldh(X.phi, S, n)) * dag(u) - (dag(v) * ldh(X.phi, S, m)) * v;
+    sc.pi(w, c * dag(G));
+    const M3 Yb = c * dag(pw);
+    M3 ub = m_zero(), vb = m_zero(), nb = m_zero(), mb = m_zero();
+    adj_conj(u, ldh(X.phi, S, n), Yb, ub, nb);
+    adj_conjd(v, ldh(X.phi, S, m), -1. * Yb, vb, mb);
+    if (f == 1) {
+        sc.u1(uq, ub);
+        sc.u1(vq, vb);
+    } else {
+        sc.u2(uq, ub);
+        sc.u2(vq, vb);
+    }
+    sc.ph(n, nb);
+    sc.ph(m, mb);
+}
+// Re tr(E(w) (PI(w) + U PI(n) U^+)) in reverse with weight c; E field fe
+__device__ __forceinline__ void epi_adj(const TScat& sc, const Slot& X, size_t S, int fe, double c, size_t w,
+                                        size_t uq, size_t n) {
+    const double* U = fe == 1 ? X.Ux : X.Uy;
+    const M3 u = ldm(U, S, uq), e = ldh(fe == 1 ? X.E1 : X.E2, S, w);
+    const M3 Z = ldh(X.pi, S, w) + (u * ldh(X.pi, S, n)) * dag(u);
+    if (fe == 1)
+        sc.e1(w, c * dag(Z));
+    else
+        sc.e2(w, c * dag(Z));
+    const M3 Zb = c * dag(e);
+    sc.pi(w, Zb);
+    M3 ub = m_zero(), nb = m_zero();
+    adj_conj(u, ldh(X.pi, S, n), Zb, ub, nb);
+    if (fe == 1)
+        sc.u1(uq, ub);
+    else
+        sc.u2(uq, ub);
+    sc.pi(n, nb);
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_c(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
+                                                          int it, const double* __restrict__ tb,
+                                                          double* __restrict__ pout) {
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int i = (int)(s / L.N), j = (int)(s - (long long)i * L.N);
+    const Slot X = slot_of(xs, S);
+    const TScat sc{pout, S};
+    const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+    const double a4 = a * a * a * a, a5 = a4 * a;
+    const size_t s0 = (size_t)s;
+    const size_t pX = L.at(i + 1, j), pY = L.at(i, j + 1), pXY = L.at(i + 1, j + 1), mX = L.at(i - 1, j);
+    const size_t mY = L.at(i, j - 1), mXpY = L.at(i - 1, j + 1), pXmY = L.at(i + 1, j - 1);
+    const size_t p2X = L.at(i + 2, j), p2Y = L.at(i, j + 2), p2XY = L.at(i + 2, j + 1), pX2Y = L.at(i + 1, j + 2);
+    // ttx_re, tty_re enter T^{tau x}, T^{tau y} with -2/(8 tau) a^-4
+    const double cx = -tb[4 * S + s] / (4. * tau * a4), cy = -tb[5 * S + s] / (4. * tau * a4);
+    pigrad_adj(sc, X, S, 1, cx, s0, s0, pX, mX, mX);
+    pigrad_adj(sc, X, S, 1, cx, pY, pY, pXY, mXpY, mXpY);
+    pigrad_adj(sc, X, S, 1, cx, pX, pX, p2X, s0, s0);
+    pigrad_adj(sc, X, S, 1, cx, pXY, pXY, p2XY, pY, pY);
+    pigrad_adj(sc, X, S, 2, cy, s0, s0, pY, mY, mY);
+    pigrad_adj(sc, X, S, 2, cy, pX, pX, pXY, pXmY, pXmY);
+    pigrad_adj(sc, X, S, 2, cy, pY, pY, p2Y, s0, s0);
+    pigrad_adj(sc, X, S, 2, cy, pXY, pXY, pX2Y, pX, pX);
+    // r1 enters T^{x eta}, T^{y eta} with -2/tau^2 g/4 a^-5
+    const double rx = -tb[8 * S + s] * g / (2. * tau * tau * a5), ry = -tb[9 * S + s] * g / (2. * tau * tau * a5);
+    epi_adj(sc, X, S, 1, rx, s0, s0, pX);
+    epi_adj(sc, X, S, 1, rx, pY, pY, pXY);
+    epi_adj(sc, X, S, 2, ry, s0, s0, pY);
+    epi_adj(sc, X, S, 2, ry, pX, pX, pXY);
+}
+
+// uniform weights of the ten components (the linear loss sum_s sum_k w_k T_k(s))
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_fill_tb(long long S, const double* __restrict__ w,
+                                                          double* __restrict__ tb) {
+    const long long t = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    if (t >= 10 * S) return;
+    tb[t] = w[t / S];
+}
+
 // ---- one leapfrog step in reverse ------------------------------------------------------
 // xb: state before the step, xa: state after; pin: pending cotangents of the
 // state after; pout: pending cotangents of the state before (every plane is
@@ -511,4 +833,14 @@ void ipg_adj_step(const EvDesc* d, long long S, const double* xb, const double* 
 }
 void ipg_adj_kick(const EvDesc* d, long long S, const double* x0, const double* cin, double* cout, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_kick, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, x0, cin, cout);
+}
+
+void ipg_adj_tmunu(const EvDesc* d, long long S, const double* xs, int it, const double* tb, double* pout,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_tmunu_a, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+    hipLaunchKernelGGL(k_adj_tmunu_b, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+    hipLaunchKernelGGL(k_adj_tmunu_c, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+}
+void ipg_adj_fill_tb(long long S, const double* w, double* tb, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_fill_tb, dim3(nb_of(10 * S)), dim3(IPG_BLOCK), 0, st, S, w, tb);
 }

@@ -272,6 +272,66 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_ecc_finish(const EvDesc* __restri
     }
 }
 
+// ---- eps_2 at itmax in reverse ------------------------------------------------------
+// Pass 1: block sums of C, Sn, R, Mx, My (partial slots 0-4, one block per 256
+// sites, after the forward eccentricity kernels have run).  Pass 2: per site,
+// dL/dw -> (dL/d eps, dL/d u^tau) -> dL/dT (ipgl::landau_site_adj) into tb[10][S].
+__global__ __launch_bounds__(IPG_BLOCK) void k_e2a_sums(const EvDesc* __restrict__ evs) {
+    const EvDesc& ev = evs[0];
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long S = ev.S;
+    double acc[5] = {0., 0., 0., 0., 0.};
+    if (s < S) {
+        const int N = ev.N;
+        const int si = (int)(s / N), sj = (int)(s - (long long)si * N);
+        if (si >= 2 && si < N - 2 && sj >= 2 && sj < N - 2) {
+            const double dx = -ev.L / 2. + ev.a * si - ev.obs[17], dy = -ev.L / 2. + ev.a * sj - ev.obs[18];
+            const double eps = ev.land[s], ut = ev.land[S + s];
+            const double w = eps < kCutoff ? 0. : eps * ut;
+            acc[0] = (dx * dx - dy * dy) * w;
+            acc[1] = 2. * dx * dy * w;
+            acc[2] = (dx * dx + dy * dy) * w;
+            acc[3] = dx * w;
+            acc[4] = dy * w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const double r = bsum<IPG_BLOCK>(acc[k]);
+        if (threadIdx.x == 0) ev.partial[(size_t)k * ev.nblocks + blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(IPG_BLOCK) void k_e2a_site(const EvDesc* __restrict__ evs, int it,
+                                                       double* __restrict__ tb) {
+    const EvDesc& ev = evs[0];
+    // slot_sum is valid in thread 0: broadcast the five sums through LDS
+    __shared__ double qs[5];
+    for (int k = 0; k < 5; k++) {
+        const double v = slot_sum(ev, k);
+        if (threadIdx.x == 0) qs[k] = v;
+    }
+    __syncthreads();
+    double q[5];
+    for (int k = 0; k < 5; k++) q[k] = qs[k];
+    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long S = ev.S;
+    if (s >= S) return;
+    const int N = ev.N;
+    const int si = (int)(s / N), sj = (int)(s - (long long)si * N);
+    const double dx = -ev.L / 2. + ev.a * si - ev.obs[17], dy = -ev.L / 2. + ev.a * sj - ev.obs[18];
+    const bool in = si >= 2 && si < N - 2 && sj >= 2 && sj < N - 2;
+    const double wb = ipgl::ecc2_wbar(q[0], q[1], q[2], q[3], q[4], ev.obs[25], dx, dy, in, 1.);
+    const double eps = ev.land[s], ut = ev.land[S + s];
+    double T[10], Tb[10];
+    load_T(ev, s, T);
+#pragma unroll
+    for (int k = 0; k < 10; k++) Tb[k] = 0.;
+    if (!(eps < kCutoff)) ipgl::landau_site_adj(T, N, ev.a, ev.dtau, it, si, sj, wb * ut, wb * eps, Tb);
+#pragma unroll
+    for (int k = 0; k < 10; k++) tb[(size_t)k * S + s] = Tb[k];
+}
+
 }  // namespace
 
 void ipg_launch_landau(const EvDesc* d, int nev, int nblk, int it, int all, hipStream_t st) {
@@ -280,4 +340,10 @@ void ipg_launch_landau(const EvDesc* d, int nev, int nblk, int it, int all, hipS
     hipLaunchKernelGGL(k_ecc_mid, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
     hipLaunchKernelGGL(k_ecc, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
     hipLaunchKernelGGL(k_ecc_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, it, all);
+}
+
+void ipg_adj_ecc2(const EvDesc* d, long long S, int it, double* tb, hipStream_t st) {
+    const unsigned nb = (unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK);
+    hipLaunchKernelGGL(k_e2a_sums, dim3(nb), dim3(IPG_BLOCK), 0, st, d);
+    hipLaunchKernelGGL(k_e2a_site, dim3(nb), dim3(IPG_BLOCK), 0, st, d, it, tb);
 }

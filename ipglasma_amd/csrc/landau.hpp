@@ -563,6 +563,257 @@ IPGL_HD void landau_site(const double (&T)[10], int N, double a, double dtau, in
     o.eps_var = eps;
 }
 
+// ---- reverse mode of the cell values (eps, u^tau) --------------------------------
+// x = M^T-solve of the 5x5 bordered system (partial pivoting); b overwritten
+IPGL_HD void solve5(double (&M)[5][5], double (&b)[5]) {
+    for (int c = 0; c < 5; c++) {
+        int pr = c;
+        for (int r = c + 1; r < 5; r++)
+            if (fabs(M[r][c]) > fabs(M[pr][c])) pr = r;
+        if (pr != c) {
+            for (int k = 0; k < 5; k++) {
+                const double t = M[c][k];
+                M[c][k] = M[pr][k];
+                M[pr][k] = t;
+            }
+            const double t = b[c];
+            b[c] = b[pr];
+            b[pr] = t;
+        }
+        const double d = M[c][c];
+        if (d == 0.) continue;
+        for (int r = c + 1; r < 5; r++) {
+            const double f = M[r][c] / d;
+            for (int k = c; k < 5; k++) M[r][k] -= f * M[c][k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = 4; c >= 0; c--) {
+        double v = b[c];
+        for (int k = c + 1; k < 5; k++) v -= M[c][k] * b[k];
+        b[c] = M[c][c] != 0. ? v / M[c][c] : 0.;
+    }
+}
+
+// Given eps_b = dL/d eps and ut_b = dL/d u^tau of the cell values of one site
+// (LandauSite::eps, u[0]), adds dL/dT[k] to Tb (T's order).  The branches of
+// landau_site are replayed: a value set to a constant (the rest frame, u^tau
+// = 1) carries no derivative, eps = T^tautau (the fallbacks and sites without
+// a timelike vector) carries dL/dT^tautau, and a real eigenpair (lambda, u)
+// of A = T g, g = diag(1, -1, -1, -tau^2), u^T g u = 1, gives
+//   d lambda = (g u)^T dA u
+//   du       from [[A - lambda, u], [(g u)^T, 0]] [du; s] = [d lambda u - dA u; 0],
+// so that with M^T [l; m] = [ut_b e_tau; 0] and rho = (eps_b + l.u) g u - l,
+// dL = rho^T dA u.  Where the reference keeps eps = Re(lambda) of a complex
+// eigenvalue (near the lattice edges) its derivative is taken; u^tau from the
+// real part of a complex eigenvector is differentiated by central differences.
+IPGL_HD void landau_site_adj(const double (&T)[10], int N, double a, double dtau, int it, int si, int sj,
+                            double eps_b, double ut_b, double (&Tb)[10]) {
+    const double Ttt = T[0], Txx = T[1], Tyy = T[2], Tee = T[3], Ttx = T[4], Tty = T[5];
+    const double Tte = T[6], Txy = T[7], Txe = T[8], Tye = T[9];
+    const double tt = (it * dtau * a) * (it * dtau * a);
+    const double A[4][4] = {{Ttt, -Ttx, -Tty, -tt * Tte},
+                            {Ttx, -Txx, -Txy, -tt * Txe},
+                            {Tty, -Txy, -Tyy, -tt * Tye},
+                            {Tte, -Txe, -Tye, -tt * Tee}};
+    Eig4 r;
+    if (!eig4(A, r)) {
+        Tb[0] += eps_b;
+        return;
+    }
+    const double tau2 = a * it * dtau * a * it * dtau;
+    // replay: sources of the running eps and of the cell values
+    // (-1: constant, 0..3: eigenpair, 4: T^tautau)
+    double eps = 0.;
+    int e_src = -1, cell_e = 4, cell_u = -1;
+    double cell_flip = 1.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double sqr = 0., eur = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double zr = r.vr[j][i], zi = r.vi[j][i];
+            const double pr = zr * zr - zi * zi;
+            eur += pr;
+            if (j == 0)
+                sqr += pr;
+            else if (j < 3)
+                sqr -= pr;
+            else
+                sqr -= (tau2 * zr) * zr - (tau2 * zi) * zi;
+        }
+        const double factor = sqrt(fabs(eur / sqr));
+        if (sqr > 0.) {
+            eps = r.wr[i];
+            e_src = i;
+            if (fabs(r.wi[i]) > 0.001 && si > 0 && sj > 0 && si < N - 5 && sj < N - 5) {
+                eps = Ttt;
+                e_src = 4;
+            }
+        }
+        double s2 = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double zr = r.vr[j][i] * factor, zi = r.vi[j][i] * factor;
+            const double pr = zr * zr - zi * zi;
+            if (j == 0)
+                s2 += pr;
+            else if (j < 3)
+                s2 -= pr;
+            else
+                s2 -= (tau2 * zr) * zr - (tau2 * zi) * zi;
+        }
+        if (s2 > 0.) {
+            const double flip = r.vr[0][i] * factor < 0. ? -1. : 1.;
+            int u_src = eps > 0.1 ? i : -1;
+            bool fb = false;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const double zi = r.vi[j][i] * factor;
+                if (fabs(zi) > 0.001 && eps > 0.001 && si > 10 && sj > 10 && si < N - 10 && sj < N - 10) fb = true;
+            }
+            if (fb) {
+                u_src = -1;
+                eps = Ttt;
+                e_src = 4;
+            }
+            cell_e = e_src;
+            cell_u = u_src;
+            cell_flip = flip;
+        }
+    }
+    // eps from T^tautau
+    if (cell_e == 4) Tb[0] += eps_b;
+    const double g[4] = {1., -1., -1., -tt};
+    // the normalised real eigenvector of pair k (u^tau > 0 when flipped as the
+    // cell's), and its lambda
+    auto vec = [&](int k, double fl, double (&u)[4]) {
+        double sqr = 0.;
+        for (int j = 0; j < 4; j++) sqr += g[j] * r.vr[j][k] * r.vr[j][k];
+        const double f = fl / sqrt(fabs(sqr));
+        for (int j = 0; j < 4; j++) u[j] = r.vr[j][k] * f;
+    };
+    auto add = [&](const double (&rho)[4], const double (&u)[4]) {
+        // Tb[k] += rho^T (dA / dT[k]) u, A[m][n] = T^{mn} g_n
+        const int ia[10] = {0, 1, 2, 3, 0, 0, 0, 1, 1, 2};
+        const int ib[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+        for (int k = 0; k < 10; k++) {
+            const int al = ia[k], be = ib[k];
+            double v = rho[al] * g[be] * u[be];
+            if (al != be) v += rho[be] * g[al] * u[al];
+            Tb[k] += v;
+        }
+    };
+    // eps = Re(lambda) of a complex eigenvalue (kept at the lattice edges):
+    // d lambda = (g v)^T dA v / (v^T g v) with the complex eigenvector v
+    if (cell_e >= 0 && cell_e < 4 && r.wi[cell_e] != 0.) {
+        double dr = 0., di = 0.;
+        for (int j = 0; j < 4; j++) {
+            const double vr = r.vr[j][cell_e], vi = r.vi[j][cell_e];
+            dr += g[j] * (vr * vr - vi * vi);
+            di += g[j] * 2. * vr * vi;
+        }
+        const double dd = dr * dr + di * di;
+        if (dd > 0.) {
+            // rho = eps_b g v / (v^T g v)
+            double pr[4], pi_[4];
+            for (int j = 0; j < 4; j++) {
+                const double gr = g[j] * r.vr[j][cell_e], gi = g[j] * r.vi[j][cell_e];
+                pr[j] = eps_b * (gr * dr + gi * di) / dd;
+                pi_[j] = eps_b * (gi * dr - gr * di) / dd;
+            }
+            const int ia[10] = {0, 1, 2, 3, 0, 0, 0, 1, 1, 2};
+            const int ib[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+            for (int k = 0; k < 10; k++) {
+                const int al = ia[k], be = ib[k];
+                const double vbr = g[be] * r.vr[be][cell_e], vbi = g[be] * r.vi[be][cell_e];
+                double v = pr[al] * vbr - pi_[al] * vbi;
+                if (al != be) {
+                    const double var = g[al] * r.vr[al][cell_e], vai = g[al] * r.vi[al][cell_e];
+                    v += pr[be] * var - pi_[be] * vai;
+                }
+                Tb[k] += v;
+            }
+        }
+    }
+    // u^mu from the real part of a complex eigenvector (the reference's
+    // u outside the central region): its phase follows the solver's
+    // back-substitution convention, so u^tau is differentiated by central
+    // differences of landau_site in each of the ten components
+    if (cell_u >= 0 && r.wi[cell_u] != 0. && ut_b != 0.) {
+        // fourth-order stencil (8 (f(h) - f(-h)) - (f(2h) - f(-2h))) / 12 h
+        for (int k = 0; k < 10; k++) {
+            const double hk = 1e-5 * (fabs(T[k]) + 1e-4 * fabs(T[0])) + 1e-300;
+            double f[4];
+            const double off[4] = {hk, -hk, 2. * hk, -2. * hk};
+            for (int m = 0; m < 4; m++) {
+                double tq[10];
+                for (int q = 0; q < 10; q++) tq[q] = T[q];
+                tq[k] += off[m];
+                LandauSite o;
+                landau_site(tq, N, a, dtau, it, si, sj, o);
+                f[m] = o.u[0];
+            }
+            Tb[k] += ut_b * (8. * (f[0] - f[1]) - (f[2] - f[3])) / (12. * hk);
+        }
+    }
+    const bool e_pair = cell_e >= 0 && cell_e < 4 && r.wi[cell_e] == 0.;
+    const bool u_pair = cell_u >= 0 && r.wi[cell_u] == 0. && ut_b != 0.;
+    if (u_pair) {
+        double u[4];
+        vec(cell_u, cell_flip, u);
+        const double lam = r.wr[cell_u];
+        double gu[4];
+        for (int j = 0; j < 4; j++) gu[j] = g[j] * u[j];
+        // M^T = [[A^T - lam, g u], [u^T, 0]]
+        double M[5][5], b[5] = {ut_b, 0., 0., 0., 0.};
+        for (int i = 0; i < 4; i++) {
+            for (int j = 0; j < 4; j++) M[i][j] = A[j][i] - (i == j ? lam : 0.);
+            M[i][4] = gu[i];
+            M[4][i] = u[i];
+        }
+        M[4][4] = 0.;
+        solve5(M, b);
+        double lu = 0.;
+        for (int j = 0; j < 4; j++) lu += b[j] * u[j];
+        const double eb = (e_pair && cell_e == cell_u) ? eps_b : 0.;
+        double rho[4];
+        for (int j = 0; j < 4; j++) rho[j] = (eb + lu) * gu[j] - b[j];
+        add(rho, u);
+        if (e_pair && cell_e != cell_u) {
+            double v[4], gv[4];
+            vec(cell_e, 1., v);
+            for (int j = 0; j < 4; j++) gv[j] = g[j] * v[j] * eps_b;
+            add(gv, v);
+        }
+    } else if (e_pair) {
+        double v[4], gv[4];
+        vec(cell_e, 1., v);
+        for (int j = 0; j < 4; j++) gv[j] = g[j] * v[j] * eps_b;
+        add(gv, v);
+    }
+}
+
+// ---- eps_2 in reverse (Evolution.cpp:1280-1660) ------------------------------
+// eps_2 = |C + i Sn| / R over the sites 2 <= ix, iy < N-2, relative to the
+// centroid (xb, yb) = sum_all w (x, y) / W of the weights w = eps u^tau:
+//   C = sum w (dx^2 - dy^2), Sn = sum 2 w dx dy, R = sum w (dx^2 + dy^2)
+// (the reference's r^2 cos 2phi, r^2 sin 2phi, r^2).  Mx, My = sum w dx, w dy
+// over the same sites.  Returns dL/dw at a site (dx, dy from the centroid)
+// for dL/d eps_2 = e2b, through the sums and the centroid.
+IPGL_HD double ecc2_wbar(double C, double Sn, double R, double Mx, double My, double W, double dx, double dy,
+                        bool interior, double e2b) {
+    const double m = sqrt(C * C + Sn * Sn);
+    if (m == 0. || R == 0. || W == 0.) return 0.;
+    const double e2 = m / R;
+    const double aC = e2b * C / (m * R), aS = e2b * Sn / (m * R), aR = -e2b * e2 / R;
+    const double xbb = -2. * Mx * (aC + aR) - 2. * My * aS;
+    const double ybb = 2. * My * aC - 2. * Mx * aS - 2. * My * aR;
+    double v = (xbb * dx + ybb * dy) / W;
+    if (interior) v += aC * (dx * dx - dy * dy) + aS * 2. * dx * dy + aR * (dx * dx + dy * dy);
+    return v;
+}
+
 // pi^{mu nu} from T and the running (eps, u^mu) (MyEigen.cpp:267-309)
 IPGL_HD void landau_pi(const double (&T)[10], double eps, double ut, double ux, double uy, double ue, int it,
                       double dtau, double a, double (&pi)[10]) {

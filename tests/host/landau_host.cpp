@@ -55,3 +55,59 @@ extern "C" void ipgl_landau_host(int N, double L, double dtau, int it, const dou
         for (int k = 0; k < 10; k++) out[(5 + k) * S + s] = pi[k];
     }
 }
+
+// cell values (eps, u^tau) of one site and the reverse mode of
+// eps_b eps + ut_b u^tau
+extern "C" void ipgl_landau_cell_host(const double* T, int N, double a, double dtau, int it, int si, int sj,
+                                      double* eps, double* ut) {
+    double t[10];
+    for (int k = 0; k < 10; k++) t[k] = T[k];
+    ipgl::LandauSite o;
+    ipgl::landau_site(t, N, a, dtau, it, si, sj, o);
+    *eps = o.eps;
+    *ut = o.u[0];
+}
+extern "C" void ipgl_landau_adj_host(const double* T, int N, double a, double dtau, int it, int si, int sj,
+                                     double eps_b, double ut_b, double* Tb) {
+    double t[10], tb[10] = {0.};
+    for (int k = 0; k < 10; k++) t[k] = T[k];
+    ipgl::landau_site_adj(t, N, a, dtau, it, si, sj, eps_b, ut_b, tb);
+    for (int k = 0; k < 10; k++) Tb[k] = tb[k];
+}
+
+// d eps_2 / d eps and d eps_2 / d u^tau at every site (the sums of the
+// reverse kernels done serially; weights eps u^tau where eps >= cutoff)
+extern "C" void ipgl_ecc2_adj_host(int N, double L, double cutoff, const double* eps, const double* ut,
+                                   double* eps_b, double* ut_b) {
+    const double a = L / N;
+    const long long S = (long long)N * N;
+    double W = 0., X = 0., Y = 0.;
+    for (long long s = 0; s < S; s++) {
+        const int i = (int)(s / N), j = (int)(s % N);
+        const double w = eps[s] < cutoff ? 0. : eps[s] * ut[s];
+        W += w;
+        X += (-L / 2. + a * i) * w;
+        Y += (-L / 2. + a * j) * w;
+    }
+    const double xb = X / W, yb = Y / W;
+    double q[5] = {0., 0., 0., 0., 0.};
+    for (long long s = 0; s < S; s++) {
+        const int i = (int)(s / N), j = (int)(s % N);
+        if (i < 2 || i >= N - 2 || j < 2 || j >= N - 2) continue;
+        const double w = eps[s] < cutoff ? 0. : eps[s] * ut[s];
+        const double dx = -L / 2. + a * i - xb, dy = -L / 2. + a * j - yb;
+        q[0] += (dx * dx - dy * dy) * w;
+        q[1] += 2. * dx * dy * w;
+        q[2] += (dx * dx + dy * dy) * w;
+        q[3] += dx * w;
+        q[4] += dy * w;
+    }
+    for (long long s = 0; s < S; s++) {
+        const int i = (int)(s / N), j = (int)(s % N);
+        const double dx = -L / 2. + a * i - xb, dy = -L / 2. + a * j - yb;
+        const bool in = i >= 2 && i < N - 2 && j >= 2 && j < N - 2;
+        const double wb = ipgl::ecc2_wbar(q[0], q[1], q[2], q[3], q[4], W, dx, dy, in, 1.);
+        eps_b[s] = eps[s] < cutoff ? 0. : wb * ut[s];
+        ut_b[s] = eps[s] < cutoff ? 0. : wb * eps[s];
+    }
+}

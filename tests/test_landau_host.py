@@ -38,6 +38,11 @@ def lib():
         L.ipgl_eig4_host.restype = C.c_int
         L.ipgl_landau_host.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, dp, dp,
                                        C.POINTER(C.c_int)]
+        L.ipgl_landau_cell_host.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ipgl_landau_adj_host.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, dp]
+        L.ipgl_ecc2_adj_host.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -98,3 +103,131 @@ def test_landau_device_code_bit_identical(tau):
     assert nf1 == nf2.value == 0
     assert np.array_equal(o1, o2.reshape(15, S))
     np.testing.assert_allclose(avg2, avg1, rtol=1e-13)
+
+
+def _fluid(rng, tau, eps, P, boost):
+    """T^{mu nu} (tautau, xx, yy, etaeta, taux, tauy, taueta, xy, xeta, yeta) of
+    an anisotropic fluid: eps u u + P_x X X + P_y Y Y + P_e Z Z with an
+    orthonormal frame (u, X, Y, Z) of g = diag(1, -1, -1, -tau^2)"""
+    g = np.diag([1.0, -1.0, -1.0, -tau * tau])
+    # boosted frame: Lorentz boost of the rest frame
+    v = np.array(boost)
+    b2 = v @ v
+    gam = 1.0 / np.sqrt(1.0 - b2)
+    Lm = np.eye(4)
+    Lm[0, 0] = gam
+    Lm[0, 1:] = Lm[1:, 0] = gam * v
+    Lm[1:, 1:] += (gam - 1.0) * np.outer(v, v) / b2
+    # rest-frame basis in the (t, x, y, tau*eta) orthonormal frame, then eta
+    # components divided by tau
+    E = Lm @ np.eye(4)
+    E[3, :] /= tau
+    pres = [eps, P[0], P[1], P[2]]
+    T = sum(pres[k] * np.outer(E[:, k], E[:, k]) for k in range(4))
+    T += 1e-3 * eps * (lambda M: M + M.T)(rng.normal(size=(4, 4))) * np.outer([1, 1, 1, 1 / tau], [1, 1, 1, 1 / tau])
+    idx = [(0, 0), (1, 1), (2, 2), (3, 3), (0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    return np.array([T[i, j] for i, j in idx])
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_landau_cell_reverse_mode_matches_finite_differences(case):
+    """d(eps_b eps + ut_b u^tau)/dT of the cell values against central
+    differences of landau_site in random symmetric directions: boosted
+    anisotropic fluids (eigenvector branch), a cold cell (eps <= 0.1: rest
+    frame, only eps varies) and an edge site"""
+    rng = np.random.default_rng(100 + case)
+    N, a, dtau, it = 64, 0.1, 0.1, 20
+    tau = it * dtau * a
+    eps = [30.0, 12.0, 5.0, 50.0, 0.05, 8.0][case]
+    boost = [(0.3, 0.1, 0.05), (0.5, -0.2, 0.1), (0.1, 0.4, -0.2), (0.6, 0.3, 0.2), (0.2, 0.1, 0.0),
+             (0.4, 0.2, 0.1)][case]
+    P = np.array([0.3, 0.25, 0.2]) * eps * (1 + 0.3 * rng.random(3))
+    T = _fluid(rng, tau, eps, P, boost)
+    si, sj = (30, 30) if case < 5 else (0, 7)
+    eb, ub = rng.normal(), rng.normal()
+    Tb = np.zeros(10)
+    lib().ipgl_landau_adj_host(T, N, a, dtau, it, si, sj, eb, ub, Tb)
+
+    def loss(Tx):
+        e, u = C.c_double(), C.c_double()
+        lib().ipgl_landau_cell_host(np.ascontiguousarray(Tx), N, a, dtau, it, si, sj, C.byref(e), C.byref(u))
+        return eb * e.value + ub * u.value
+
+    for trial in range(4):
+        d = rng.normal(size=10) * np.abs(T).max()
+        h = 1e-6
+        fd = (loss(T + h * d) - loss(T - h * d)) / (2 * h)
+        pred = float(Tb @ d)
+        assert abs(pred - fd) <= 1e-6 * max(1.0, abs(fd)), (case, trial, pred, fd)
+
+
+def test_eps2_reverse_mode_matches_oracle_finite_differences():
+    """d eps_2 / d(eps, u^tau) (centroid included, cutoff at 0 drops the
+    negative-eps sites) against central differences of the oracle's
+    orc_eccentricity on an off-centre elliptic profile with noise"""
+    import _oracle as O
+    rng = np.random.default_rng(5)
+    N, L = 24, 12.0
+    a = L / N
+    x = -L / 2 + a * np.arange(N)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    eps = 20 * np.exp(-((X - 0.7) ** 2 / 6 + (Y + 0.4) ** 2 / 3)) + 0.5 * rng.normal(size=(N, N))
+    ut = 1 + 0.1 * rng.random((N, N))
+    eps, ut = eps.reshape(-1), ut.reshape(-1)
+    assert (eps < 0).any()
+    eb, ub = np.zeros(N * N), np.zeros(N * N)
+    lib().ipgl_ecc2_adj_host(N, L, 0.0, np.ascontiguousarray(eps), np.ascontiguousarray(ut), eb, ub)
+    z = np.zeros(N * N)
+
+    def e2(e, u):
+        return O.eccentricity(N, L, e, u, z, z, 0.8, 0.8, 0.0)[1]
+
+    for trial in range(4):
+        de = rng.normal(size=N * N) * (eps > 0.05)  # keep every site on its side of the cutoff
+        du = rng.normal(size=N * N) * 0.1
+        h = 1e-6
+        fd = (e2(eps + h * de, ut + h * du) - e2(eps - h * de, ut - h * du)) / (2 * h)
+        pred = float(eb @ de + ub @ du)
+        assert abs(pred - fd) <= 1e-7 * max(abs(fd), 1e-3), (trial, pred, fd)
+
+
+def test_landau_cell_reverse_complex_eigenvalue_at_the_edge():
+    """a site on the lattice edge whose timelike eigenvalue is complex keeps
+    eps = Re(lambda) (no fallback there): its derivative against central
+    differences"""
+    rng = np.random.default_rng(77)
+    N, a, dtau, it = 64, 0.25, 0.094, 17
+    tau = it * dtau * a
+    ok = 0
+    for trial in range(40):
+        T = np.array([10.0, 4.0, 3.0, 2.0 / tau ** 2, 8.0, 1.0, 0.2 / tau, 0.5, 0.1 / tau, 0.1 / tau])
+        T = T * (1 + 0.3 * rng.normal(size=10))
+        wr, wi = np.zeros(4), np.zeros(4)
+        vr, vi = np.zeros(16), np.zeros(16)
+        g = np.diag([1.0, -1.0, -1.0, -tau * tau])
+        idx = [(0, 0), (1, 1), (2, 2), (3, 3), (0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+        M = np.zeros((4, 4))
+        for k, (i, j) in enumerate(idx):
+            M[i, j] = M[j, i] = T[k]
+        A = np.ascontiguousarray((M @ g).reshape(-1))
+        lib().ipgl_eig4_host(A, wr, wi, vr, vi)
+        if not (np.abs(wi) > 1e-3).any():
+            continue
+        e0, u0 = C.c_double(), C.c_double()
+        lib().ipgl_landau_cell_host(T, N, a, dtau, it, 0, 7, C.byref(e0), C.byref(u0))
+        if not any(abs(e0.value - wr[k]) == 0 and wi[k] != 0 for k in range(4)):
+            continue
+        ok += 1
+        Tb = np.zeros(10)
+        lib().ipgl_landau_adj_host(T, N, a, dtau, it, 0, 7, 1.0, 0.0, Tb)
+
+        def eps(Tx):
+            e, u = C.c_double(), C.c_double()
+            lib().ipgl_landau_cell_host(np.ascontiguousarray(Tx), N, a, dtau, it, 0, 7, C.byref(e), C.byref(u))
+            return e.value
+
+        d = rng.normal(size=10) * np.abs(T)
+        h = 1e-7
+        fd = (eps(T + h * d) - eps(T - h * d)) / (2 * h)
+        assert abs(float(Tb @ d) - fd) <= 1e-6 * max(1.0, abs(fd)), (trial, float(Tb @ d), fd)
+    assert ok >= 3

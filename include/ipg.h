@@ -193,6 +193,12 @@ int ipg_adjoint_rho(ipg_ctx *c, double *dedeta, double *grad);
 #define IPG_WRT_TMUNU 2
 int ipg_adjoint_loss(ipg_ctx *c, int loss, const double *weights, int wrt, double *value, double *grad);
 
+/* Steps between the reverse mode's checkpoints: 0 (the default) takes
+ * ceil(sqrt(itmax-1)), storing about 2 sqrt(T) states and recomputing each
+ * segment once; 1 stores every state (the full tape, nothing recomputed).
+ * Gradients do not depend on it.  No counterpart in the reference. */
+int ipg_set_checkpoint_interval(ipg_ctx *c, int K);
+
 /* Host copy of the last ipg_adjoint_rho gradient, grad[2][Ny][S][8].
  * IPG_ERR_STATE when there is none.  No counterpart in the reference. */
 int ipg_get_rho_grad(ipg_ctx *c, double *grad);

@@ -128,6 +128,7 @@ def lib():
             "ipg_adjoint_dedeta": (C.c_int, [vp, dp, dp]),
             "ipg_adjoint_rho": (C.c_int, [vp, dp, dp]),
             "ipg_get_rho_grad": (C.c_int, [vp, dp]),
+            "ipg_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
             "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
@@ -261,6 +262,11 @@ class Event:
         return v.value, {k: g[i] for i, k in enumerate(("Ux", "Uy", "E1", "E2", "phi", "pi"))}
 
     LOSSES = {"dEdeta": 0, "eps2": 1, "tmunu": 2}
+
+    def set_checkpoint_interval(self, K):
+        """steps between the reverse mode's checkpoints (0: ceil(sqrt(T));
+        1: every state stored, nothing recomputed)"""
+        _check(lib().ipg_set_checkpoint_interval(self._c, int(K)))
 
     def adjoint(self, loss="dEdeta", weights=None, wrt="fields", fetch=True):
         """Reverse mode with a choice of loss at tau_max: "dEdeta", "eps2"

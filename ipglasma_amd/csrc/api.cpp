@@ -267,6 +267,14 @@ struct Engine {
     size_t rho_grad_n = 0;
     EvDesc* d_rdesc = nullptr;   // descriptors of the recomputed steps
     size_t rdesc_n = 0;
+    int adj_K = 0;               // checkpoint interval (0: ceil(sqrt(T)))
+
+    // steps between checkpoints of the reverse mode for T leapfrog steps;
+    // 1 stores every state (the full tape: nothing is recomputed)
+    int ckpt_interval(int T) const {
+        if (adj_K > 0) return std::max(1, std::min(adj_K, std::max(T, 1)));
+        return std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
+    }
 
     ~Engine() {
         for (int k = 0; k < 3; k++) {
@@ -985,7 +993,7 @@ struct Engine {
     // doubles of the evolution part of the workspace
     size_t adj_evolution_doubles(const EvDesc& d) const {
         const int T = d.itmax - 1;
-        const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
+        const int K = ckpt_interval(T);
         const size_t nslot = 1 + (size_t)(T / K + 1) + (size_t)K + 1 + 1;
         return nslot * 72 * (size_t)d.S + (2 * IPG_PEND_PLANES + 108 + 10) * (size_t)d.S + 16;
     }
@@ -1074,7 +1082,7 @@ struct Engine {
         const int itm = d.itmax, T = itm - 1;
         if (itm < 1) return fail(IPG_ERR_ARG, "itmax must be at least 1");
         const long long S = d.S;
-        const int K = std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
+        const int K = ckpt_interval(T);
         const int nck = T / K + 1;
         const size_t slot = 72 * (size_t)S, pend = IPG_PEND_PLANES * (size_t)S, cset = 108 * (size_t)S;
         rho_grad = nullptr;
@@ -1208,10 +1216,12 @@ struct Engine {
             const int s0 = k * K, s1 = std::min(s0 + K, T);
             if (s1 <= s0) continue;
             const double* x0 = s_ck + (size_t)k * slot;
-            for (int it = s0 + 1; it <= s1; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nblk, it, st, sv);
+            if (K > 1)
+                for (int it = s0 + 1; it <= s1; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nblk, it, st, sv);
             for (int it = s1; it > s0; it--) {
                 const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
-                const double* xa = s_seg + (size_t)(it - s0) * slot;
+                // K = 1: the state after the step is the next stored state
+                const double* xa = K > 1 ? s_seg + (size_t)(it - s0) * slot : s_ck + (size_t)it * slot;
                 ipg_adj_step(d_desc, S, xb, xa, it, cin, cout, st);
                 std::swap(cin, cout);
             }
@@ -1488,6 +1498,11 @@ int ipg_adjoint_loss(ipg_ctx* c, int loss, const double* weights, int wrt, doubl
     if (wrt == IPG_WRT_RHO) return c->eng.adjoint_rho_loss(loss, weights, value, grad);
     if (wrt == IPG_WRT_TMUNU) return c->eng.adjoint_tmunu(loss, weights, value, grad);
     return fail(IPG_ERR_ARG, "wrt must be IPG_WRT_FIELDS, IPG_WRT_RHO or IPG_WRT_TMUNU");
+}
+int ipg_set_checkpoint_interval(ipg_ctx* c, int K) {
+    if (!c || K < 0) return fail(IPG_ERR_ARG, "bad argument");
+    c->eng.adj_K = K;
+    return IPG_OK;
 }
 int ipg_get_rho_grad(ipg_ctx* c, double* grad) {
     if (!c || !grad) return fail(IPG_ERR_ARG, "null argument");

@@ -179,3 +179,89 @@ def test_adjoint_eps2_fields_match_finite_differences(N, maxtime):
     # 1e-8 of the field scale (at 1e-5 the second direction is still 4e-4 off)
     h = 1e-7 * max(1.0, np.abs(x0["pi"]).max())
     _fd_fields(p, x0, g, "eps2", None, rng, 3, h, 1e-5)
+
+
+def _rho_setup(p, seed):
+    import ipglasma_amd as I
+    probe = I.Event(p, seed=seed)
+    probe.init_nucleons()
+    assert probe.init_color_charge()
+    z = np.stack([np.stack([probe.rho_normals(n, k) for k in range(p.Ny)]) for n in range(2)])
+    probe.close()
+    return z
+
+
+def test_full_tape_gradient_matches_checkpointed():
+    """the reverse mode with every forward state stored (checkpoint interval
+    1, nothing recomputed) against the default sqrt(T) checkpoints at N=64:
+    the recomputed states are bit-identical, so the gradients agree to the
+    round-off of the atomic sums (the loss, T^{mu nu}, pi and link reverse
+    kernels add in a run-dependent order): 1e-12 of the largest entry
+    (dE/deta with respect to rho, eps_2 with respect to the tau = 0 fields)"""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=64, Ny=2, maxtime=0.4, L=8.0)
+    p.set(bmin=3.0, bmax=3.0)
+    z = _rho_setup(p, 21)
+    out = {}
+    for k in (0, 1):
+        ev = I.Event(p, seed=21)
+        ev.set_rho_normals(z)
+        assert ev.init()
+        ev.set_checkpoint_interval(k)
+        out[("rho", k)] = ev.adjoint(loss="dEdeta", wrt="rho")
+        ev.close()
+        ev = I.Event(p, seed=21)
+        ev.set_rho_normals(z)
+        assert ev.init()
+        ev.set_checkpoint_interval(k)
+        out[("eps2", k)] = ev.adjoint(loss="eps2", wrt="fields")
+        ev.close()
+    assert out[("rho", 0)][0] == out[("rho", 1)][0]
+    a, b = out[("rho", 0)][1], out[("rho", 1)][1]
+    print("rho gradient, full tape vs checkpoints: max diff / max", np.abs(a - b).max() / np.abs(a).max())
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    assert out[("eps2", 0)][0] == out[("eps2", 1)][0]
+    scale = max(np.abs(v).max() for v in out[("eps2", 0)][1].values())
+    for f in out[("eps2", 0)][1]:
+        a, b = out[("eps2", 0)][1][f], out[("eps2", 1)][1][f]
+        assert np.abs(a - b).max() <= 1e-12 * scale, f
+
+
+def test_adjoint_eps2_rho_matches_finite_differences():
+    """d eps_2 / d rho (normals z) through the whole chain, against central
+    differences.  The root finder's stopping noise (~1e-9 in the links) and
+    the curvature of the boosted edge cells bound the usable steps from
+    both sides; the step 2e-5 and the bound 1e-3 sit between them."""
+    import ipglasma_amd as I
+    N, Ny, seed = 32, 3, 21
+    p = K.params("AuAu", N=N, Ny=Ny, maxtime=0.4, L=8.0)
+    p.set(bmin=3.0, bmax=3.0)
+    z = _rho_setup(p, seed)
+    ev = I.Event(p, seed=seed)
+    ev.set_rho_normals(z)
+    assert ev.init()
+    gA, gB = ev.field("g2mu2A"), ev.field("g2mu2B")
+    L0, grho = ev.adjoint(loss="eps2", wrt="rho")
+    ev.close()
+
+    def e2(zz):
+        e = I.Event(p, seed=seed)
+        e.set_rho_normals(zz)
+        assert e.init()
+        e.run()
+        v = e.observables()["eccentricity"][1]
+        e.close()
+        return v
+
+    assert L0 == pytest.approx(e2(z), rel=1e-11)
+    amp = np.stack([p.g * np.sqrt(gA / Ny), p.g * np.sqrt(gB / Ny)])
+    gz = grho * amp[:, None, :, None]
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        d = rng.normal(size=z.shape)
+        h = 2e-5
+        fd = (e2(z + h * d) - e2(z - h * d)) / (2 * h)
+        pred = float(np.sum(gz * d))
+        print(f"eps2 rho gradient trial {trial}: adjoint {pred:.10g} central difference {fd:.10g} "
+              f"rel {abs(pred - fd) / abs(fd):.2e}")
+        assert abs(pred - fd) <= 1e-3 * abs(fd), (trial, pred, fd)

@@ -69,7 +69,9 @@ def adjoint_timing(I, seed=77):
         while not ev.init():
             pass
         t1 = time.perf_counter()
-        if adj:
+        if adj == "eps2":
+            L, _ = ev.adjoint(loss="eps2", wrt="rho", fetch=False)
+        elif adj:
             L, _ = ev.adjoint_rho(fetch=False)
         else:
             ev.run()
@@ -85,13 +87,17 @@ def adjoint_timing(I, seed=77):
         return t1 - t0, t2 - t1, t3 - t2, L
     timed(False)
     timed(True)
+    timed("eps2")
     i_f, e_f, _, Lf = timed(False)
     i_a, e_a, f_a, La = timed(True)
+    i_e, e_e, _, Le = timed("eps2")
     return {"workload": "Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c, d(dE/deta)/d rho",
             "forward_ms": round(1e3 * (i_f + e_f), 2), "adjoint_ms": round(1e3 * (i_a + e_a), 2),
             "adjoint_over_forward": round((i_a + e_a) / (i_f + e_f), 2),
             "gradient_to_host_ms": round(1e3 * f_a, 2),
-            "dEdeta_rel_diff": abs(La - Lf) / abs(Lf)}
+            "dEdeta_rel_diff": abs(La - Lf) / abs(Lf),
+            "eps2_adjoint_ms": round(1e3 * (i_e + e_e), 2),
+            "eps2_adjoint_over_forward": round((i_e + e_e) / (i_f + e_f), 2), "eps2": Le}
 
 
 def run_plain(I, p, seeds):

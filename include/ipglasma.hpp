@@ -180,6 +180,17 @@ class Evolution {
         check(ipg_get_observables(lat->ctx(), &o));
         param->setObservables(o);
     }
+    // In place of run(): the loss at tau_max (IPG_LOSS_DEDETA, IPG_LOSS_ECC2,
+    // or IPG_LOSS_TMUNU with ten weights) and its gradient with respect to
+    // rho ([2][Ny][S][8]), the tau = 0 fields ([6][S][18]) or T^{mu nu} at
+    // itmax ([10][S]).  No counterpart in the reference.
+    double adjoint(Lattice* lat, int loss, int wrt, std::vector<double>& grad, size_t S, int Ny,
+                   const double* weights = nullptr) {
+        grad.assign(wrt == IPG_WRT_RHO ? 2 * (size_t)Ny * S * 8 : (wrt == IPG_WRT_TMUNU ? 10 * S : 6 * S * 18), 0.);
+        double v = 0.;
+        check(ipg_adjoint_loss(lat->ctx(), loss, weights, wrt, &v, grad.data()));
+        return v;
+    }
 };
 
 }  // namespace ipglasma

@@ -660,25 +660,40 @@ struct Engine {
                 maxNy = std::max(maxNy, desc[e].Ny);
                 kc = std::min(kc, desc[e].kc);
             }
-            const std::string key = "setv_rows_inv_block/N=" + std::to_string(kv.first) +
-                                    "/events=" + std::to_string(es.size());
             int minNy = 1 << 30;
             for (int e : es) minNy = std::min(minNy, desc[e].Ny);
             auto chunk = [&](const EvDesc* dd, int n, int k0, int kcc, int ny) {
+                // keyed by the launch's own event count and chunk length
+                const std::string ck = "setv_rows_inv_block/N=" + std::to_string(kv.first) +
+                                       "/events=" + std::to_string(n) + "/kc=" + std::to_string(kcc);
                 bool meas = false;
-                const int v = g_tuner.pick(key, 2, &meas);
+                const int v = g_tuner.pick(ck, 2, &meas);
                 hipEvent_t a, b;
                 // only full chunks are compared (the last one may be short)
-                timed_begin(key, v, meas && k0 + kcc <= ny, &a, &b);
+                timed_begin(ck, v, meas && k0 + kcc <= ny, &a, &b);
                 ipg_launch_setv_chunk(dd, n, dp, k0, kcc, v, a, b, st);
             };
-            if (contiguous) {
+            // schedule (a tuner choice when several events share the size):
+            // 0: each chunk launch covers all events; 1: one event at a
+            // time; 2: one event at a time in chunks of half the slices, so
+            // that a chunk's transform buffer stays in L2/MALL across its
+            // row, column and inverse-row passes
+            int sched = contiguous ? 0 : 1;
+            bool smeas = false;
+            const std::string skey = "setv_schedule/N=" + std::to_string(kv.first) +
+                                     "/events=" + std::to_string(es.size());
+            if (contiguous && es.size() > 1) sched = g_tuner.pick(skey, 3, &smeas);
+            hipEvent_t sa = nullptr, sb = nullptr;
+            if (timed_begin(skey, sched, smeas, &sa, &sb)) (void)hipEventRecord(sa, st);
+            if (sched == 0) {
                 for (int k0 = 0; k0 < maxNy; k0 += kc) chunk(d_desc + es.front(), (int)es.size(), k0, kc, minNy);
             } else {
-                for (int e : es)
-                    for (int k0 = 0; k0 < desc[e].Ny; k0 += desc[e].kc)
-                        chunk(d_desc + e, 1, k0, desc[e].kc, desc[e].Ny);
+                for (int e : es) {
+                    const int kce = sched == 2 ? std::max(1, desc[e].kc / 2) : desc[e].kc;
+                    for (int k0 = 0; k0 < desc[e].Ny; k0 += kce) chunk(d_desc + e, 1, k0, kce, desc[e].Ny);
+                }
             }
+            if (sb) (void)hipEventRecord(sb, st);
         }
         HIPCHK(hipGetLastError());
         return IPG_OK;

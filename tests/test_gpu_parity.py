@@ -289,3 +289,30 @@ def test_batch_matches_plain():
             assert o["dEdeta"] == pytest.approx(plain[e]["dEdeta"], rel=1e-12)
         b.close()
     assert "evolve_schedule/N=32/events=3" in I.autotune_report()
+
+
+def test_batch_setv_schedules_match_plain():
+    """three same-size events: while the tuner measures the three setV
+    schedules (one launch per chunk for all events; one event at a time;
+    one event at a time in half-length chunks) every run must give the plain
+    path's results"""
+    import ipglasma_amd as I
+    ps = [K.params("AuAu", N=32, Ny=5)] * 3
+    seeds = [31, 32, 33]
+    plain = []
+    for p, s in zip(ps, seeds):
+        ev = I.Event(p, seed=s)
+        tries = 0
+        while not ev.init() and tries < 20:
+            tries += 1
+        ev.run()
+        plain.append(ev.observables())
+        ev.close()
+    for run in range(9):
+        b = I.Batch(ps, seeds)
+        b.run(use_graph=True)
+        for e in range(3):
+            o = b.observables(e)
+            assert o["dEdeta"] == pytest.approx(plain[e]["dEdeta"], rel=1e-12), (run, e)
+        b.close()
+    assert "setv_schedule/N=32/events=3" in I.autotune_report()

@@ -278,33 +278,33 @@ struct Engine {
 
     ~Engine() {
         for (int k = 0; k < 3; k++) {
-            if (gexec[k]) hipGraphExecDestroy(gexec[k]);
-            if (graph[k]) hipGraphDestroy(graph[k]);
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
         }
         for (Event& e : ev) {
-            if (e.mem) hipFree(e.mem);
-            if (e.d_nuc) hipFree(e.d_nuc);
-            if (e.d_z) hipFree(e.d_z);
+            if (e.mem) (void)hipFree(e.mem);
+            if (e.d_nuc) (void)hipFree(e.d_nuc);
+            if (e.d_z) (void)hipFree(e.d_z);
         }
         for (auto& kv : plans) {
-            hipFree(kv.second.tw);
-            hipFree(kv.second.s2);
+            (void)hipFree(kv.second.tw);
+            (void)hipFree(kv.second.s2);
         }
-        if (d_desc) hipFree(d_desc);
-        if (d_Tl) hipFree(d_Tl);
-        if (d_Q) hipFree(d_Q);
-        if (d_flag) hipFree(d_flag);
-        if (d_work) hipFree(d_work);
-        if (d_out) hipFree(d_out);
-        if (d_obs) hipFree(d_obs);
-        if (adj_ws) hipFree(adj_ws);
-        if (d_rdesc) hipFree(d_rdesc);
-        for (hipEvent_t t : tev) hipEventDestroy(t);
+        if (d_desc) (void)hipFree(d_desc);
+        if (d_Tl) (void)hipFree(d_Tl);
+        if (d_Q) (void)hipFree(d_Q);
+        if (d_flag) (void)hipFree(d_flag);
+        if (d_work) (void)hipFree(d_work);
+        if (d_out) (void)hipFree(d_out);
+        if (d_obs) (void)hipFree(d_obs);
+        if (adj_ws) (void)hipFree(adj_ws);
+        if (d_rdesc) (void)hipFree(d_rdesc);
+        for (hipEvent_t t : tev) (void)hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
-            hipEventDestroy(t.a);
-            hipEventDestroy(t.b);
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
         }
-        if (st) hipStreamDestroy(st);
+        if (st) (void)hipStreamDestroy(st);
     }
 
     int setup(const ipg_params* ps, int nev, const uint64_t* seeds) {
@@ -859,8 +859,8 @@ struct Engine {
             // the launch sequence depends on which events are active
             if (gexec[sched] &&
                 (graph_itm[sched] != itm || graph_act[sched] != act || graph_var[sched] != step_var)) {
-                hipGraphExecDestroy(gexec[sched]);
-                hipGraphDestroy(graph[sched]);
+                (void)hipGraphExecDestroy(gexec[sched]);
+                (void)hipGraphDestroy(graph[sched]);
                 gexec[sched] = nullptr;
                 graph[sched] = nullptr;
             }
@@ -882,7 +882,7 @@ struct Engine {
         } else {
             if (timed_begin(key, sched, meas, &ta, &tb)) (void)hipEventRecord(ta, st);
             if (timing) {
-                for (hipEvent_t t : tev) hipEventDestroy(t);
+                for (hipEvent_t t : tev) (void)hipEventDestroy(t);
                 tev.assign(3 * (size_t)(itm + 1), nullptr);
                 for (hipEvent_t& t : tev) HIPCHK(hipEventCreate(&t));
             }

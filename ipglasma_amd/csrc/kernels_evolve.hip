@@ -247,15 +247,16 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_a(const EvDesc* __restrict_
         const double* phi = ev.phi[p];
         const double* E1 = ev.E1;
         const double* E2 = ev.E2;
-        const double* PI = ev.pi;
-        const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+        [[maybe_unused]] const double* PI = ev.pi;
+        [[maybe_unused]] const double tau = it * ev.dtau, g = ev.g, a = ev.a;
         const int i = (int)(s / N), j = (int)(s - (s / N) * N);
         auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
-        const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
-        const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
-        const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
+        // the neighbours of the whole T^{mu nu} stencil (each pass uses a subset)
+        [[maybe_unused]] const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
+        [[maybe_unused]] const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
+        [[maybe_unused]] const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
         double* T = ev.T;
-        const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+        [[maybe_unused]] const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
         // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
         // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
         auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
@@ -371,15 +372,16 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_b(const EvDesc* __restrict_
         const double* phi = ev.phi[p];
         const double* E1 = ev.E1;
         const double* E2 = ev.E2;
-        const double* PI = ev.pi;
-        const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+        [[maybe_unused]] const double* PI = ev.pi;
+        [[maybe_unused]] const double tau = it * ev.dtau, g = ev.g, a = ev.a;
         const int i = (int)(s / N), j = (int)(s - (s / N) * N);
         auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
-        const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
-        const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
-        const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
+        // the neighbours of the whole T^{mu nu} stencil (each pass uses a subset)
+        [[maybe_unused]] const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
+        [[maybe_unused]] const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
+        [[maybe_unused]] const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
         double* T = ev.T;
-        const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
+        [[maybe_unused]] const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
         // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
         // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
         auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
@@ -467,22 +469,16 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_c(const EvDesc* __restrict_
         const double* phi = ev.phi[p];
         const double* E1 = ev.E1;
         const double* E2 = ev.E2;
-        const double* PI = ev.pi;
-        const double tau = it * ev.dtau, g = ev.g, a = ev.a;
+        [[maybe_unused]] const double* PI = ev.pi;
+        [[maybe_unused]] const double tau = it * ev.dtau, g = ev.g, a = ev.a;
         const int i = (int)(s / N), j = (int)(s - (s / N) * N);
         auto at = [&](int di, int dj) { return (size_t)clampi(i + di, N) * N + clampi(j + dj, N); };
-        const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
-        const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
-        const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
+        // the neighbours of the whole T^{mu nu} stencil (each pass uses a subset)
+        [[maybe_unused]] const size_t pX = at(1, 0), pY = at(0, 1), pXY = at(1, 1), mX = at(-1, 0), mY = at(0, -1);
+        [[maybe_unused]] const size_t mXpY = at(-1, 1), pXmY = at(1, -1), p2X = at(2, 0), p2Y = at(0, 2);
+        [[maybe_unused]] const size_t p2XY = at(2, 1), pX2Y = at(1, 2);
         double* T = ev.T;
-        const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
-        // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
-        // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
-        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
-        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
-        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
-        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
-
+        [[maybe_unused]] const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
         // pi phi-gradient parts of T^{tau x} and T^{tau y}
         double ttx_re, tty_re;
         const double ttx_im = T[4 * S + s], tty_im = T[5 * S + s], xe_im = T[8 * S + s], ye_im = T[9 * S + s];

@@ -1217,7 +1217,7 @@ void ipg_launch_zero_fields(const EvDesc* d, int nev, int nblk, hipStream_t st) 
 void ipg_launch_rho_normals(const EvDesc* d, int nucleus, int k, double* out, hipStream_t st) {
     // single-event helper for parity tests: S sites from the first descriptor
     EvDesc h;
-    hipMemcpy(&h, d, sizeof(EvDesc), hipMemcpyDeviceToHost);
+    if (hipMemcpy(&h, d, sizeof(EvDesc), hipMemcpyDeviceToHost) != hipSuccess) return;
     int nb = (int)((h.S + 255) / 256);
     hipLaunchKernelGGL(k_rho_normals, dim3(nb), dim3(256), 0, st, d, nucleus, k, out);
 }

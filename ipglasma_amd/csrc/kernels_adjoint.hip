@@ -384,18 +384,17 @@ __device__ __forceinline__ void prod4_adj(const TScat& sc, const Slot& X, size_t
     fac_st(sc, f[2], B3);
     fac_st(sc, f[3], B4);
 }
-// K = TL(P1 - P2) + TL(P3 - P4) and its reverse given Kb
-__device__ __forceinline__ M3 clover(const Slot& X, size_t S, const Fac (&p1)[4], const Fac (&p2)[4],
-                                     const Fac (&p3)[4], const Fac (&p4)[4]) {
-    return traceless(prod4(X, S, p1) - prod4(X, S, p2)) + traceless(prod4(X, S, p3) - prod4(X, S, p4));
-}
-__device__ __forceinline__ void clover_adj(const TScat& sc, const Slot& X, size_t S, const Fac (&p1)[4],
-                                           const Fac (&p2)[4], const Fac (&p3)[4], const Fac (&p4)[4], const M3& Kb) {
-    const M3 t = traceless(Kb);
-    prod4_adj(sc, X, S, p1, t);
-    prod4_adj(sc, X, S, p2, -1. * t);
-    prod4_adj(sc, X, S, p3, t);
-    prod4_adj(sc, X, S, p4, -1. * t);
+// Each clover of the forward, K = TL(P1 - P2) + TL(P3 - P4), pairs a
+// plaquette product with its conjugate (P2 = P1^+, P4 = P3^+), so
+// K = ah(P1) + ah(P3) with ah(P) = TL(P - P^+).  Reverse of ah given Kb:
+// t = TL(Kb), Pb = t - t^+ (the cotangent of P^+ is the conjugate of that
+// of P).  The T^{tau x} and T^{tau y} clovers at s share the plaquette at s
+// in the same cyclic order (with opposite signs), so the four clovers need
+// seven products and seven product reverses.
+__device__ __forceinline__ M3 ah(const M3& p) { return traceless(p - dag(p)); }
+__device__ __forceinline__ M3 ah_bar(const M3& kb) {
+    const M3 t = traceless(kb);
+    return t - dag(t);
 }
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
@@ -417,63 +416,67 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
     const size_t p2X = L.at(i + 2, j), p2Y = L.at(i, j + 2);
     const double txb = tb[4 * S + s] / (4. * tau * a4), tyb = tb[5 * S + s] / (4. * tau * a4);
     const double xeb = -tb[8 * S + s] / (4. * g * tau * tau * a5), yeb = -tb[9 * S + s] / (4. * g * tau * tau * a5);
-    // X1 (tau x, x eta with E2(s), gY)
+    // the plaquette at s from U_y(s): shared by the clovers X1 and Y1
+    const Fac pa[4] = {{2, s0, false}, {1, pY, false}, {2, pX, true}, {1, s0, true}};
+    const M3 ta = ah(prod4(X, S, pa));
+    M3 tab;
+    // X1 (tau x, x eta with E2(s), gY): K = ah(pa) + ah(pc)
     {
-        const Fac pa[4] = {{2, s0, false}, {1, pY, false}, {2, pX, true}, {1, s0, true}};
-        const Fac pb[4] = {{1, s0, false}, {2, pX, false}, {1, pY, true}, {2, s0, true}};
         const Fac pc[4] = {{1, mX, true}, {2, mX, false}, {1, mXpY, false}, {2, s0, true}};
-        const Fac pd[4] = {{2, s0, false}, {1, mXpY, true}, {2, mX, true}, {1, mX, false}};
-        const M3 K = clover(X, S, pa, pb, pc, pd);
+        const M3 K = ta + ah(prod4(X, S, pc));
         const M3 e2 = ldh(X.E2, S, s0), gy = grad_fwd(X, S, 2, s0, pY, s0);
         // ttx_im = Im tr(E2 K), xe_im = -Im tr(K gY)
         const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gy));
         sc.e2(s0, mul_i(txb, dag(K)));
         grad_adj(sc, X, S, 2, s0, pY, s0, mul_i(-xeb, dag(K)));
-        clover_adj(sc, X, S, pa, pb, pc, pd, Kb);
+        tab = ah_bar(Kb);
+        prod4_adj(sc, X, S, pc, tab);
     }
-    // Y1 (tau y, y eta with E1(s), gX)
+    // Y1 (tau y, y eta with E1(s), gX): K = ah(pa^+) + ah(pc) = -ah(pa) + ah(pc)
     {
-        const Fac pa[4] = {{1, s0, false}, {2, pX, false}, {1, pY, true}, {2, s0, true}};
-        const Fac pb[4] = {{2, s0, false}, {1, pY, false}, {2, pX, true}, {1, s0, true}};
         const Fac pc[4] = {{2, mY, true}, {1, mY, false}, {2, pXmY, false}, {1, s0, true}};
-        const Fac pd[4] = {{1, s0, false}, {2, pXmY, true}, {1, mY, true}, {2, mY, false}};
-        const M3 K = clover(X, S, pa, pb, pc, pd);
+        const M3 K = ah(prod4(X, S, pc)) - ta;
         const M3 e1 = ldh(X.E1, S, s0), gx = grad_fwd(X, S, 1, s0, pX, s0);
         const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gx));
         sc.e1(s0, mul_i(tyb, dag(K)));
         grad_adj(sc, X, S, 1, s0, pX, s0, mul_i(-yeb, dag(K)));
-        clover_adj(sc, X, S, pa, pb, pc, pd, Kb);
+        const M3 tb2 = ah_bar(Kb);
+        prod4_adj(sc, X, S, pc, tb2);
+        tab = tab - tb2;
     }
-    // X2 (tau x, x eta with E2(+x), gXY2)
+    prod4_adj(sc, X, S, pa, tab);
+    // X2 (tau x, x eta with E2(+x), gXY2): K = ah(pe) + ah(pg)
     {
         const Fac pe[4] = {{2, pX, false}, {1, pXY, false}, {2, p2X, true}, {1, pX, true}};
-        const Fac pf[4] = {{1, pX, false}, {2, p2X, false}, {1, pXY, true}, {2, pX, true}};
         const Fac pg[4] = {{1, s0, true}, {2, s0, false}, {1, pY, false}, {2, pX, true}};
-        const Fac ph[4] = {{2, pX, false}, {1, pY, true}, {2, s0, true}, {1, s0, false}};
-        const M3 K = clover(X, S, pe, pf, pg, ph);
+        const M3 K = ah(prod4(X, S, pe)) + ah(prod4(X, S, pg));
         const M3 e2 = ldh(X.E2, S, pX), gg = grad_fwd(X, S, 2, pX, pXY, pX);
         const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gg));
         sc.e2(pX, mul_i(txb, dag(K)));
         grad_adj(sc, X, S, 2, pX, pXY, pX, mul_i(-xeb, dag(K)));
-        clover_adj(sc, X, S, pe, pf, pg, ph, Kb);
+        const M3 t = ah_bar(Kb);
+        prod4_adj(sc, X, S, pe, t);
+        prod4_adj(sc, X, S, pg, t);
     }
-    // Y2 (tau y, y eta with E1(+y), gXY1)
+    // Y2 (tau y, y eta with E1(+y), gXY1): K = ah(qe) + ah(qg)
     {
         const Fac qe[4] = {{1, pY, false}, {2, pXY, false}, {1, p2Y, true}, {2, pY, true}};
-        const Fac qf[4] = {{2, pY, false}, {1, p2Y, false}, {2, pXY, true}, {1, pY, true}};
         const Fac qg[4] = {{2, s0, true}, {1, s0, false}, {2, pX, false}, {1, pY, true}};
-        const Fac qh[4] = {{1, pY, false}, {2, pX, true}, {1, s0, true}, {2, s0, false}};
-        const M3 K = clover(X, S, qe, qf, qg, qh);
+        const M3 K = ah(prod4(X, S, qe)) + ah(prod4(X, S, qg));
         const M3 e1 = ldh(X.E1, S, pY), gg = grad_fwd(X, S, 1, pY, pXY, pY);
         const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gg));
         sc.e1(pY, mul_i(tyb, dag(K)));
         grad_adj(sc, X, S, 1, pY, pXY, pY, mul_i(-yeb, dag(K)));
-        clover_adj(sc, X, S, qe, qf, qg, qh, Kb);
+        const M3 t = ah_bar(Kb);
+        prod4_adj(sc, X, S, qe, t);
+        prod4_adj(sc, X, S, qg, t);
     }
 }
 
-// Re tr(PI(w) (U phi(n) U^+ - V^+ phi(m) V)) in reverse with weight c; U at uq, V at vq
-__device__ __forceinline__ void pigrad_adj(const TScat& sc, const Slot& X, size_t S, int f, double c, size_t w,
+// Re tr(PI(w) (U phi(n) U^+ - V^+ phi(m) V)) in reverse with weight c; U at uq, V at vq.
+// This helper and epi_adj stay out of line: inlined, their twelve calls in
+// k_adj_tmunu_c keep ~1.3 KB per lane live and spill it to scratch.
+__device__ __attribute__((noinline)) void pigrad_adj(const TScat& sc, const Slot& X, size_t S, int f, double c, size_t w,
                                            size_t uq, size_t n, size_t vq, size_t m) {
     const double* U = f == 1 ? X.Ux : X.Uy;
     const M3 u = ldm(U, S, uq), v = ldm(U, S, vq), pw = ldh(X.pi, S, w);
@@ -494,7 +497,7 @@ __device__ __forceinline__ void pigrad_adj(const TScat& sc, const Slot& X, size_
     sc.ph(m, mb);
 }
 // Re tr(E(w) (PI(w) + U PI(n) U^+)) in reverse with weight c; E field fe
-__device__ __forceinline__ void epi_adj(const TScat& sc, const Slot& X, size_t S, int fe, double c, size_t w,
+__device__ __attribute__((noinline)) void epi_adj(const TScat& sc, const Slot& X, size_t S, int fe, double c, size_t w,
                                         size_t uq, size_t n) {
     const double* U = fe == 1 ? X.Ux : X.Uy;
     const M3 u = ldm(U, S, uq), e = ldh(fe == 1 ? X.E1 : X.E2, S, w);

@@ -316,3 +316,54 @@ def test_batch_setv_schedules_match_plain():
             assert o["dEdeta"] == pytest.approx(plain[e]["dEdeta"], rel=1e-12), (run, e)
         b.close()
     assert "setv_schedule/N=32/events=3" in I.autotune_report()
+
+
+def test_full_size_pbpb_event_invariants():
+    """The throughput workload at its full size (Pb+Pb b=0, N=512, L=30 fm,
+    Ny=50, tau_max=0.4 fm/c, itmax=69), where an oracle event takes minutes,
+    checked through properties that hold at any size:
+    - links at tau = 0 in SU(3) to round-off (U = exp(i alpha^a t^a) U1 U2,
+      Init.cpp:3344-3481);
+    - E1, E2 traceless after the last step (Evolution.cpp:245-290);
+    - T^tt - T^xx - T^yy - tau^2 T^ee = 0 at every site (the electric and
+      magnetic parts are traceless separately, Evolution.cpp:754-905);
+    - T^tt >= 0 up to the unitarity drift of the links: the second-order
+      link update (Evolution.cpp:35-88) is not exactly unitary, so after 69
+      steps Re tr U_p exceeds 3 by ~1e-9 where the plaquette is nearly 1
+      (observed: min T^tt = -4e-12 of its maximum, in the vacuum rows);
+    - dE/deta = tau a^2 hbarc sum_s T^tt(s), re-summed on the host;
+    - a second event of the same seed reproduces every T^{mu nu} value."""
+    import math
+
+    import ipglasma_amd as I
+    p = K.params("PbPb", N=512, Ny=50, L=30.0, bmin=0.0, bmax=0.0, samplebFromLinearDistribution=1)
+    N, S = 512, 512 * 512
+    a = p.L / N
+    ev = I.Event(p, seed=1)
+    assert ev.init()
+    assert p.itmax == 69
+    for name in ("Ux", "Uy"):
+        U = O.from_c(ev.field(name))
+        dev = np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max()
+        assert dev < 1e-12, (name, dev)
+        assert np.abs(np.linalg.det(U) - 1).max() < 1e-12, name
+    ev.run()
+    o = ev.observables()
+    for name in ("E1", "E2"):
+        E = O.from_c(ev.field(name))
+        tr = np.abs(np.trace(E, axis1=1, axis2=2)).max()
+        assert tr <= 1e-12 * np.abs(E).max(), (name, tr)
+    T = ev.tmunu()
+    tau = p.itmax * p.dtau * a
+    trace = T[0] - T[1] - T[2] - tau ** 2 * T[3]
+    assert np.abs(trace).max() < 1e-11 * np.abs(T[0]).max()
+    assert T[0].min() > -1e-10 * T[0].max()
+    host = tau * a * a * 0.1973269718 * math.fsum(T[0])
+    assert o["dEdeta"] == pytest.approx(host, rel=1e-12)
+    ev2 = I.Event(p, seed=1)
+    assert ev2.init()
+    ev2.run()
+    T2 = ev2.tmunu()
+    for k in range(10):
+        np.testing.assert_allclose(T2[k], T[k], rtol=0, atol=1e-12 * np.abs(T[k]).max(), err_msg=str(k))
+    assert ev2.observables()["dEdeta"] == pytest.approx(o["dEdeta"], rel=1e-14)

@@ -7,7 +7,7 @@ tau=0 root finder -> 69 leapfrog steps -> T^{mu nu} -> dE/deta, for every
 event of the batch, timed from the host.  Inputs are generated on the device
 (counter-based colour-charge normals); the Q_s table is resident.
 
-Printed JSON fields beyond the driver contract:
+Printed JSON fields beyond metric, value, unit, steps, config and the timing keys:
   plain_events_per_s  one launch sequence per event (same events, same code)
   batched_over_plain  the ratio
   roofline            the leapfrog step kernel: 1728 algorithmic bytes per
@@ -263,8 +263,9 @@ def main():
                     help="also run one device copy of MIB MiB (the HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: events are independent and this build has no multi-GPU "
-              "path (see DESIGN.md for how it would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: events are independent and there is no multi-GPU "
+              "path (events would be dealt out to one process per GPU, with no collectives in the "
+              "step)", file=sys.stderr)
         sys.exit(2)
     import ipglasma_amd as I
     p = pbpb_params(I, N=a.size)

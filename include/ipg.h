@@ -258,7 +258,7 @@ int ipg_get_rho_normals(ipg_ctx *c, int nucleus, int k, double *out);
 int ipg_enable_timing(ipg_ctx *c, int on);
 int ipg_get_timing(ipg_ctx *c, double *out);
 
-/* ---- many events in one launch sequence (north_star batched form) ------ */
+/* ---- many events in one launch sequence (batched form) ----------------- */
 /* nev events, each with its own parameters and seed; lattice sizes and
  * collision systems may differ.  Every kernel launch covers all events. */
 ipg_batch *ipg_batch_create(const ipg_params *p, int nev, const uint64_t *seeds);

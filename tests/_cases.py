@@ -1,7 +1,7 @@
 """Shared parity cases: parameter sets for the product and the matching
 oracle configuration.  The nucleus constants are Glauber.cpp:19-245; the
-collision setups follow the north-star configurations, scaled down in N and
-Ny so that the oracle finishes in seconds."""
+collision setups are the Pb+Pb, Au+Au, O+O and p+Pb throughput cases,
+scaled down in N and Ny so that the oracle finishes in seconds."""
 import ctypes as C
 
 import numpy as np

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C ABI against the C oracle on the same
 seeded inputs, stage by stage and end to end.
 
-Tolerances (north star): dE/deta 1e-9 relative, pointwise T^{mu nu} 1e-6
+Tolerances: dE/deta 1e-9 relative, pointwise T^{mu nu} 1e-6
 relative (of the largest |T| of each component), bit-exact nucleon positions
 and collision counts.  Intermediate fields are held to FP64 round-off scaled
 by the depth of the computation.

@@ -13,6 +13,11 @@ Printed JSON fields beyond metric, value, unit, steps, config and the timing key
   roofline            the leapfrog step kernel: 1728 algorithmic bytes per
                       site-step (read+write Ux, Uy, E1, E2, phi, pi once),
                       HIP-event timed on the library's stream
+  heterogeneous_batch 64 events of four collision systems and three lattice
+                      sizes, batched against plain
+  small_events        64 p+Pb events at N=128, batched against plain: where one
+                      event leaves most of the GPU idle and the plain path is
+                      bound by launch latency
   cpu_baseline        the C oracle (OpenMP) on a bounded sample of the same
                       event, extrapolated to one event
 """
@@ -55,6 +60,30 @@ def hetero_params(I):
         return p
     return ([mk(256, 10.0, "p", "Pb", 6.62, 0.546, 5.0)] * 16 + [mk(256, 15.0, "O", "O", 2.61, 0.513, 0.0)] * 16 +
             [mk(512, 30.0, "Pb", "Pb", 6.62, 0.546, 12.0)] * 24 + [mk(768, 30.0, "Pb", "Pb", 6.62, 0.546, 0.0)] * 8)
+
+
+def small_params(I):
+    """64 p+Pb events, N=128, L=10 fm, b in [0,5] fm, Ny=50, tau_max=0.4 fm/c."""
+    p = pbpb_params(I, N=128, L=10.0)
+    p.set(Projectile="p", Target="Pb", bmin=0.0, bmax=5.0, samplebFromLinearDistribution=0)
+    return [p] * 64
+
+
+def batch_vs_plain(I, ps, seeds, label):
+    """Batched against plain over one set of events, after untimed batches
+    until the autotuner has decided this batch's variants."""
+    for _ in range(10):
+        run_batched(I, ps, seeds)
+        if all(v["chosen"] is not None for v in I.autotune_report().values()):
+            break
+    dtb, ob = run_batched(I, ps, seeds)
+    print(f"[bench] {label} batched: {dtb:.3f} s for {len(ps)} events", file=sys.stderr, flush=True)
+    dtp, op = run_plain(I, ps, seeds)
+    print(f"[bench] {label} plain: {dtp:.3f} s for {len(ps)} events", file=sys.stderr, flush=True)
+    rel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(ob, op))
+    return {"events": len(ps), "batched_events_per_s": round(len(ps) / dtb, 3),
+            "plain_events_per_s": round(len(ps) / dtp, 3), "batched_over_plain": round(dtp / dtb, 3),
+            "batched_vs_plain_max_rel_dEdeta": rel}
 
 
 def adjoint_timing(I, seed=77):
@@ -253,7 +282,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hetero", action="store_true",
-                    help="skip the heterogeneous 64-event batch (N=256/512/768)")
+                    help="skip the heterogeneous 64-event batch (N=256/512/768) and the 64 small events (N=128)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode timing")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
@@ -306,24 +335,15 @@ def main():
     ev_p = a.batch * a.steps / tp
     # batched and plain must agree event by event
     maxrel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(obs_b, obs_p))
-    hetero = None
+    hetero = small = None
     if not a.no_hetero:
         hp = hetero_params(I)
-        hs = list(range(5000, 5000 + len(hp)))
-        for _ in range(10):  # untimed, until the tuner has decided this batch's variants
-            run_batched(I, hp, hs)
-            if all(v["chosen"] is not None for v in I.autotune_report().values()):
-                break
-        dtb, ob = run_batched(I, hp, hs)
-        print(f"[bench] heterogeneous batched: {dtb:.3f} s for {len(hp)} events", file=sys.stderr, flush=True)
-        dtp, op = run_plain(I, hp, hs)
-        print(f"[bench] heterogeneous plain: {dtp:.3f} s for {len(hp)} events", file=sys.stderr, flush=True)
-        hrel = max(abs(x["dEdeta"] - y["dEdeta"]) / abs(y["dEdeta"]) for x, y in zip(ob, op))
         hetero = {"workload": "16 p+Pb N=256 L=10, 16 O+O N=256 L=15, 24 Pb+Pb N=512 L=30 b<12, "
-                              "8 Pb+Pb N=768 L=30; Ny=50, tau_max=0.4 fm/c",
-                  "events": len(hp), "batched_events_per_s": round(len(hp) / dtb, 3),
-                  "plain_events_per_s": round(len(hp) / dtp, 3), "batched_over_plain": round(dtp / dtb, 3),
-                  "batched_vs_plain_max_rel_dEdeta": hrel}
+                              "8 Pb+Pb N=768 L=30; Ny=50, tau_max=0.4 fm/c"}
+        hetero.update(batch_vs_plain(I, hp, list(range(5000, 5000 + len(hp))), "heterogeneous"))
+        sp = small_params(I)
+        small = {"workload": "64 p+Pb N=128 L=10 fm b<5 fm; Ny=50, tau_max=0.4 fm/c"}
+        small.update(batch_vs_plain(I, sp, list(range(7000, 7000 + len(sp))), "small events"))
     adj = None
     if not a.no_adjoint:
         adj = adjoint_timing(I)
@@ -343,6 +363,7 @@ def main():
         "dEdeta_GeV": obs_b[0]["dEdeta"],
         "roofline": rf,
         "heterogeneous_batch": hetero,
+        "small_events": small,
         "adjoint": adj,
         "autotune": I.autotune_report(),
     }

@@ -127,6 +127,32 @@ int ipg_seed(ipg_ctx *c, uint64_t seed);
  * test, Wilson lines, links, tau=0 root finder, pi.  *success receives the
  * reference's success flag; on 0 the caller retries (main.cpp:339-360). */
 int ipg_init(ipg_ctx *c, int *success);
+
+/* ---- Wilson-line files: the writeInitialWilsonLines output of setV
+ * (Init.cpp:1857-1980) and Init::readV (Init.cpp:1988-2301) ------------
+ * V is [N*N][18]: site s = i*N + j (i along x), entry r*3 + c as (re, im).
+ * format 1, text: "ix iy" and the matrix column by column (%.15g), one site
+ * per line, an empty line after each x row.  format 2, binary: int N,
+ * int Nc, double L, double a, double rapidity, then (re, im) doubles with
+ * the site index transposed (file position p*N + q holds site (q, p)).
+ * A text file read back gives the transposed matrices (the reader takes
+ * them row by row, as the reference's does); binary files round-trip
+ * exactly.  The reference names the files V-<eventId + 2 seed mpiSize>
+ * (nucleus A) and V-<eventId + (1 + 2 seed) mpiSize> (B), ".txt" for text. */
+int ipg_write_wilson_line_file(const char *path, int N, double L, double rapidity, const double *V,
+                               int format);
+/* nucleus 0 (A: rows shifted by -b/2) or 1 (B: +b/2); sites the file does
+ * not reach keep what V holds.  Binary: N must match, |L - L_file| <= 1e-5. */
+int ipg_read_wilson_line_file(const char *path, int N, double L, double b, int nucleus, int format,
+                              double *V);
+/* the event's V_A and V_B after ipg_init_wilson_lines */
+int ipg_write_wilson_lines(ipg_ctx *c, const char *path_a, const char *path_b, int format);
+/* Init::init with READFROMFILE > 0 (Init.cpp:2319-2369), in place of
+ * ipg_init_nucleons, the colour charge and setV: b from the event's first
+ * draw, no nucleons, V_A and V_B from the files (sites the shifted rows do
+ * not reach hold the identity); then ipg_init_fields.  ipg_init refuses a
+ * context whose parameters set readInitialWilsonLines > 0. */
+int ipg_read_wilson_lines(ipg_ctx *c, const char *path_a, const char *path_b, int format);
 /* Init::init split into its stages (same order, for drop-in callers and
  * parity checks) */
 int ipg_init_nucleons(ipg_ctx *c);      /* b, sampleTA, reaction plane, wounding */

@@ -48,6 +48,14 @@ class Parameters {
     unsigned long long getSeed() const { return p_.seed; }
     int getUseTimeForSeed() const { return p_.useTimeForSeed; }
     std::string getNucleusQsTableFileName() const { return p_.NucleusQsTableFileName; }
+    int getReadInitialWilsonLines() const { return p_.readInitialWilsonLines; }
+    int getWriteInitialWilsonLines() const { return p_.writeInitialWilsonLines; }
+    // set by the driver per event (main.cpp:71,148); they name the
+    // Wilson-line files
+    int getEventId() const { return event_id_; }
+    void setEventId(int id) { event_id_ = id; }
+    int getMPISize() const { return mpi_size_; }
+    void setMPISize(int n) { mpi_size_ = n; }
     // filled by Init::init (Init.cpp:1614-1644)
     int getSuccess() const { return success_; }
     void setSuccess(int s) { success_ = s; }
@@ -63,6 +71,7 @@ class Parameters {
 
    private:
     ipg_params p_;
+    int event_id_ = 0, mpi_size_ = 1;
     int success_ = 0;
     ipg_observables obs_{};
 };
@@ -155,17 +164,44 @@ class Init {
               int READFROMFILE) {
         (void)group;
         (void)glauber;
-        if (READFROMFILE != 0) throw Error("reading Wilson lines from file is not supported");
         if (!lat->seeded) {
             check(ipg_seed(lat->ctx(), random->seed()));
             lat->seeded = true;
         }
         int ok = 0;
-        check(ipg_init(lat->ctx(), &ok));
+        if (READFROMFILE > 0) {
+            readV(lat, param, READFROMFILE);
+            ok = 1;
+        } else if (param->getWriteInitialWilsonLines() > 0) {
+            // setV writes V_A, V_B before the links (Init.cpp:1857-1980)
+            check(ipg_init_nucleons(lat->ctx()));
+            check(ipg_init_color_charge(lat->ctx(), &ok));
+            if (ok) {
+                check(ipg_init_wilson_lines(lat->ctx()));
+                const int f = param->getWriteInitialWilsonLines();
+                check(ipg_write_wilson_lines(lat->ctx(), vname(param, 0, f).c_str(), vname(param, 1, f).c_str(), f));
+                check(ipg_init_fields(lat->ctx()));
+            }
+        } else {
+            check(ipg_init(lat->ctx(), &ok));
+        }
         ipg_observables o;
         check(ipg_get_observables(lat->ctx(), &o));
         param->setObservables(o);
         param->setSuccess(ok);
+    }
+    // Init::readV (Init.cpp:1988-2301) with the b draw of Init::init before it
+    void readV(Lattice* lat, Parameters* param, int format) {
+        check(ipg_read_wilson_lines(lat->ctx(), vname(param, 0, format).c_str(), vname(param, 1, format).c_str(),
+                                    format));
+        check(ipg_init_fields(lat->ctx()));
+    }
+    // V-<eventId + 2 seed mpiSize> (A), V-<eventId + (1 + 2 seed) mpiSize> (B),
+    // ".txt" for text (Init.cpp:1863-1877)
+    static std::string vname(const Parameters* param, int nucleus, int format) {
+        const unsigned long long n = (unsigned long long)param->getEventId() +
+                                     (nucleus + 2ULL * param->getSeed()) * (unsigned long long)param->getMPISize();
+        return "V-" + std::to_string(n) + (format == 1 ? ".txt" : "");
     }
 };
 

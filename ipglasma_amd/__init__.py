@@ -117,6 +117,11 @@ def lib():
             "ipg_init_color_charge": (C.c_int, [vp, ip]),
             "ipg_init_wilson_lines": (C.c_int, [vp]),
             "ipg_init_fields": (C.c_int, [vp]),
+            "ipg_write_wilson_line_file": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, dp, C.c_int]),
+            "ipg_read_wilson_line_file": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                                    dp]),
+            "ipg_write_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
+            "ipg_read_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
             "ipg_evolve": (C.c_int, [vp]),
             "ipg_evolve_step": (C.c_int, [vp, C.c_int]),
             "ipg_get_observables": (C.c_int, [vp, C.POINTER(Observables)]),
@@ -222,6 +227,16 @@ class Event:
 
     def init_wilson_lines(self):
         _check(lib().ipg_init_wilson_lines(self._c))
+
+    # Init.cpp:1857-1980: the writeInitialWilsonLines output of setV
+    def write_wilson_lines(self, path_a, path_b, fmt=2):
+        _check(lib().ipg_write_wilson_lines(self._c, os.fsencode(path_a), os.fsencode(path_b), fmt))
+
+    # Init::init with READFROMFILE > 0 (Init.cpp:2319-2369, readV 1988-2301),
+    # in place of init_nucleons, init_color_charge and init_wilson_lines: b
+    # from the first draw, no nucleons; then init_fields
+    def read_wilson_lines(self, path_a, path_b, fmt=2):
+        _check(lib().ipg_read_wilson_lines(self._c, os.fsencode(path_a), os.fsencode(path_b), fmt))
 
     def init_fields(self):
         _check(lib().ipg_init_fields(self._c))
@@ -393,6 +408,32 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def wilson_line_file_names(seed, event_id=0, mpi_size=1, fmt=2):
+    """The reference's names of the two Wilson-line files (Init.cpp:1863-1877)."""
+    ext = ".txt" if fmt == 1 else ""
+    return (f"V-{event_id + 2 * seed * mpi_size}{ext}", f"V-{event_id + (1 + 2 * seed) * mpi_size}{ext}")
+
+
+def write_wilson_line_file(path, V, L, rapidity=0.0, fmt=2):
+    """One nucleus's Wilson lines V [N*N, 18] to a file (format 1 text, 2 binary)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    N = int(round(np.sqrt(V.shape[0])))
+    if V.shape != (N * N, 18):
+        raise IPGError(f"V must be [N*N, 18], got {V.shape}")
+    _check(lib().ipg_write_wilson_line_file(os.fsencode(path), N, L, rapidity, _dptr(V), fmt))
+
+
+def read_wilson_line_file(path, N, L, b=0.0, nucleus=0, fmt=2, V=None):
+    """Init::readV for one nucleus: V [N*N, 18] (the identity where the file,
+    shifted by -+b/2 along x, does not reach, unless V is given)."""
+    if V is None:
+        V = np.zeros((N * N, 18))
+        V[:, [0, 8, 16]] = 1.0
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    _check(lib().ipg_read_wilson_line_file(os.fsencode(path), N, L, b, nucleus, fmt, _dptr(V)))
+    return V
 
 
 def autotune_report():

@@ -324,10 +324,13 @@ struct Engine {
             if (!ipgfft::supported(N))
                 return fail(IPG_ERR_ARG, "lattice size must be 2^a or 3 * 2^a in [16, 1024]");
             if (p.useNucleus != 1 || p.useSmoothNucleus != 0 || p.usePseudoRapidity != 0 || p.runningCoupling != 0 ||
-                p.readInitialWilsonLines != 0 || p.useFixedNpart != 0)
+                p.useFixedNpart != 0)
                 return fail(IPG_ERR_ARG, "option outside the supported hot path (useNucleus=1, useSmoothNucleus=0, "
-                                         "usePseudoRapidity=0, runningCoupling=0, readInitialWilsonLines=0, "
-                                         "useFixedNpart=0)");
+                                         "usePseudoRapidity=0, runningCoupling=0, useFixedNpart=0)");
+            if (p.readInitialWilsonLines < 0 || p.readInitialWilsonLines > 2 || p.writeInitialWilsonLines < 0 ||
+                p.writeInitialWilsonLines > 2)
+                return fail(IPG_ERR_ARG, "readInitialWilsonLines and writeInitialWilsonLines take 0, 1 (text) or 2 "
+                                         "(binary)");
             if (!ipg::find_nucleus(p.Projectile, p, &E.proj)) return fail(IPG_ERR_ARG, std::string("unknown nucleus ") + p.Projectile);
             if (!ipg::find_nucleus(p.Target, p, &E.targ)) return fail(IPG_ERR_ARG, std::string("unknown nucleus ") + p.Target);
             E.seed = seeds ? seeds[e] : p.seed;
@@ -475,6 +478,19 @@ struct Engine {
         if (rc) return rc;
         E.attempt++;
         return IPG_OK;
+    }
+
+    // Init::init with READFROMFILE > 0 (Init.cpp:2319-2369): the first draw
+    // gives b and no nucleons are sampled (the Wilson lines come from files)
+    int draw_b_only() {
+        Event& E = ev[0];
+        E.A.clear();
+        E.B.clear();
+        E.info = ipg::EventInfo{};
+        E.info.b = ipg::sample_b(E.rng, E.p);
+        E.attempt++;
+        desc[0].active = 1;
+        return upload_nucleons(0);
     }
 
     // host nucleon sampling for event e, upload of the nucleon list
@@ -1423,8 +1439,57 @@ int ipg_init_fields(ipg_ctx* c) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     return c->eng.fields();
 }
+int ipg_write_wilson_line_file(const char* path, int N, double L, double rapidity, const double* V, int format) {
+    if (!path || !V || N <= 0) return fail(IPG_ERR_ARG, "null argument or N <= 0");
+    std::string err;
+    const int rc = ipg::write_wilson_lines(path, N, L, rapidity, V, format, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
+int ipg_read_wilson_line_file(const char* path, int N, double L, double b, int nucleus, int format, double* V) {
+    if (!path || !V || N <= 0) return fail(IPG_ERR_ARG, "null argument or N <= 0");
+    std::string err;
+    const int rc = ipg::read_wilson_lines(path, N, L, b, nucleus, format, V, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
+int ipg_write_wilson_lines(ipg_ctx* c, const char* path_a, const char* path_b, int format) {
+    if (!c || !path_a || !path_b) return fail(IPG_ERR_ARG, "null argument");
+    const ipg_params& p = c->eng.ev[0].p;
+    std::vector<double> V(18 * (size_t)c->eng.desc[0].S);
+    for (int n = 0; n < 2; n++) {
+        int rc = ipg_get_field(c, n ? IPG_F_VB : IPG_F_VA, V.data());
+        if (rc) return rc;
+        rc = ipg_write_wilson_line_file(n ? path_b : path_a, p.size, p.L, p.Rapidity, V.data(), format);
+        if (rc) return rc;
+    }
+    return IPG_OK;
+}
+int ipg_read_wilson_lines(ipg_ctx* c, const char* path_a, const char* path_b, int format) {
+    if (!c || !path_a || !path_b) return fail(IPG_ERR_ARG, "null argument");
+    Engine& g = c->eng;
+    int rc = g.draw_b_only();
+    if (rc) return rc;
+    const ipg_params& p = g.ev[0].p;
+    const size_t S = (size_t)g.desc[0].S;
+    std::vector<double> V(18 * S);
+    for (int n = 0; n < 2; n++) {
+        // a fresh lattice holds the identity (Cell.cpp:5-6) where the shifted
+        // rows of the file do not land
+        std::fill(V.begin(), V.end(), 0.);
+        for (size_t s = 0; s < S; s++) V[18 * s] = V[18 * s + 8] = V[18 * s + 16] = 1.;
+        rc = ipg_read_wilson_line_file(n ? path_b : path_a, p.size, p.L, g.ev[0].info.b, n, format, V.data());
+        if (rc) return rc;
+        rc = ipg_set_field(c, n ? IPG_F_VB : IPG_F_VA, V.data());
+        if (rc) return rc;
+    }
+    g.ev[0].success = 1;
+    g.ev[0].obs.success = 1;
+    return g.upload_desc();
+}
 int ipg_init(ipg_ctx* c, int* success) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
+    if (c->eng.ev[0].p.readInitialWilsonLines > 0)
+        return fail(IPG_ERR_STATE, "readInitialWilsonLines > 0: start the event with ipg_read_wilson_lines (the file "
+                                   "names depend on the event id), then ipg_init_fields");
     int rc = ipg_init_nucleons(c);
     if (rc) return rc;
     int ok = 0;

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -278,6 +279,13 @@ void rotate_3d(MT64& rng, std::vector<Nucleon>& n) {
 
 }  // namespace
 
+// the first draw of Init::init (Init.cpp:2320-2347)
+double sample_b(MT64& rng, const ipg_params& p) {
+    const double xb = rng.real1();
+    return p.samplebFromLinearDistribution == 1 ? std::sqrt((p.bmax * p.bmax - p.bmin * p.bmin) * xb + p.bmin * p.bmin)
+                                                : (p.bmax - p.bmin) * xb + p.bmin;
+}
+
 int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
                  std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err) {
     if (p.nucleonPositionsFromFile != 0) {
@@ -292,10 +300,7 @@ int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const 
         err = "deuteron and He3 samplers are not supported";
         return IPG_ERR_ARG;
     }
-    const double xb = rng.real1();
-    const double b = p.samplebFromLinearDistribution == 1
-                         ? std::sqrt((p.bmax * p.bmax - p.bmin * p.bmin) * xb + p.bmin * p.bmin)
-                         : (p.bmax - p.bmin) * xb + p.bmin;
+    const double b = sample_b(rng, p);
     A.clear();
     B.clear();
     if (proj.A == 1)
@@ -518,6 +523,145 @@ int read_input(const char* path, ipg_params* p, std::string& err) {
     I("SubNucleonParamType", p->SubNucleonParamType);
     I("rho_rng", p->rho_rng);
     derive(p);
+    return IPG_OK;
+}
+
+
+// ---- Wilson-line files (Init.cpp:1857-1980 writer, 1988-2301 reader) ----------
+//
+// Text (format 1): one line per site in (x, y) order, "ix iy" and the 18
+// numbers of the matrix COLUMN by column (Matrix::MatrixToString,
+// Matrix.cpp:606-627), printed as an ostream with precision 15 prints them
+// (%.15g); an empty line after each x row.  The text reader takes the 18
+// numbers ROW by row, so a text file written and read back gives V^T, in the
+// reference as here.
+// Binary (format 2): int N, int Nc, double L, double a, double rapidity,
+// then for x-major file position (p, q) = (slow, fast) the site with
+// i = q, j = p, entries row by row as (re, im) doubles.
+
+int write_wilson_lines(const char* path, int N, double L, double rapidity, const double* V, int format,
+                       std::string& err) {
+    if (format != 1 && format != 2) {
+        err = "unknown Wilson-line format " + std::to_string(format) + " (1 = text, 2 = binary)";
+        return IPG_ERR_ARG;
+    }
+    FILE* f = std::fopen(path, format == 1 ? "w" : "wb");
+    if (!f) {
+        err = std::string("cannot open ") + path + " for writing";
+        return IPG_ERR_IO;
+    }
+    bool ok = true;
+    if (format == 1) {
+        for (int i = 0; i < N && ok; i++) {
+            for (int j = 0; j < N && ok; j++) {
+                const double* m = V + 18 * ((size_t)i * N + j);
+                ok = std::fprintf(f, "%d %d", i, j) > 0;
+                for (int c = 0; c < 3; c++)
+                    for (int r = 0; r < 3; r++)
+                        ok = ok && std::fprintf(f, " %.15g %.15g", m[2 * (3 * r + c)], m[2 * (3 * r + c) + 1]) > 0;
+                ok = ok && std::fputc('\n', f) != EOF;
+            }
+            ok = ok && std::fputc('\n', f) != EOF;
+        }
+    } else {
+        const int nc = 3;
+        const double a = L / N;
+        ok = std::fwrite(&N, sizeof(int), 1, f) == 1 && std::fwrite(&nc, sizeof(int), 1, f) == 1 &&
+             std::fwrite(&L, sizeof(double), 1, f) == 1 && std::fwrite(&a, sizeof(double), 1, f) == 1 &&
+             std::fwrite(&rapidity, sizeof(double), 1, f) == 1;
+        std::vector<double> row(18 * (size_t)N);
+        for (int p = 0; p < N && ok; p++) {
+            for (int q = 0; q < N; q++) std::memcpy(&row[18 * (size_t)q], V + 18 * ((size_t)q * N + p), 18 * sizeof(double));
+            ok = std::fwrite(row.data(), sizeof(double), row.size(), f) == row.size();
+        }
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        err = std::string("writing ") + path + " failed";
+        return IPG_ERR_IO;
+    }
+    return IPG_OK;
+}
+
+int read_wilson_lines(const char* path, int N, double L, double b, int nucleus, int format, double* V,
+                      std::string& err) {
+    if (format != 1 && format != 2) {
+        err = "unknown Wilson-line format " + std::to_string(format) + " (1 = text, 2 = binary)";
+        return IPG_ERR_ARG;
+    }
+    if (nucleus != 0 && nucleus != 1) {
+        err = "nucleus must be 0 (A) or 1 (B)";
+        return IPG_ERR_ARG;
+    }
+    const double shift = nucleus == 0 ? -b / 2. : b / 2.;
+    const size_t S = (size_t)N * N;
+    if (format == 1) {
+        std::ifstream in(path);
+        if (!in) {
+            err = std::string("Wilson-line file ") + path + " not found";
+            return IPG_ERR_IO;
+        }
+        double re[9] = {0}, im[9] = {0}, skip = 0.;
+        const double a = L / static_cast<double>(N);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) {
+                in >> skip >> skip;
+                for (int k = 0; k < 9; k++) in >> re[k] >> im[k];
+                // the row a line lands on: truncation toward zero, as the
+                // reference converts (a i -+ b/2) / a
+                const double x = a * i + shift;
+                const int ix = static_cast<int>(x / a);
+                if (ix < 0 || ix >= N) continue;
+                double* m = V + 18 * ((size_t)ix * N + j);
+                for (int k = 0; k < 9; k++) {
+                    m[2 * k] = re[k];
+                    m[2 * k + 1] = im[k];
+                }
+            }
+        return IPG_OK;
+    }
+    std::ifstream in(path, std::ios::binary);
+    if (!in.good()) {
+        err = std::string("Wilson-line file ") + path + " does not exist";
+        return IPG_ERR_IO;
+    }
+    int n = 0, nc = 0;
+    double Lf = 0., af = 0., rap = 0.;
+    in.read(reinterpret_cast<char*>(&n), sizeof(int));
+    in.read(reinterpret_cast<char*>(&nc), sizeof(int));
+    in.read(reinterpret_cast<char*>(&Lf), sizeof(double));
+    in.read(reinterpret_cast<char*>(&af), sizeof(double));
+    in.read(reinterpret_cast<char*>(&rap), sizeof(double));
+    if (!in) {
+        err = std::string(path) + ": truncated header";
+        return IPG_ERR_IO;
+    }
+    if (n != N) {
+        err = std::string(path) + ": wrong lattice size, data is " + std::to_string(n) + " but the run has " +
+              std::to_string(N);
+        return IPG_ERR_ARG;
+    }
+    if (std::abs(Lf - L) > 1e-5) {
+        err = std::string(path) + ": grid length " + std::to_string(Lf) + " but the run has " + std::to_string(L);
+        return IPG_ERR_ARG;
+    }
+    if (nc != 3) {
+        err = std::string(path) + ": Nc = " + std::to_string(nc) + ", only 3 is supported";
+        return IPG_ERR_ARG;
+    }
+    // values in file order; the spacing is the file's L over N
+    const double a = Lf / static_cast<double>(N);
+    double v[2];
+    for (long long t = 0; in.read(reinterpret_cast<char*>(v), sizeof(v)); t++) {
+        const long long pos = t / 9;
+        const int e = (int)(t - pos * 9);
+        const long long slow = pos / N, fast = pos - slow * N;
+        const int ix = (int)std::round((a * (double)fast + shift) / a);
+        const long long site = (long long)N * ix + slow;
+        if (site < 0 || site >= (long long)S) continue;
+        V[18 * (size_t)site + 2 * e] = v[0];
+        V[18 * (size_t)site + 2 * e + 1] = v[1];
+    }
     return IPG_OK;
 }
 

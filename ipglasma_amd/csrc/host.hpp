@@ -57,10 +57,23 @@ struct EventInfo {
 int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
                  std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err);
 
+// b from the first draw of Init::init (Init.cpp:2320-2347)
+double sample_b(MT64& rng, const ipg_params& p);
+
 int read_input(const char* path, ipg_params* p, std::string& err);
 void default_params(ipg_params* p);
 void derive(ipg_params* p);
 
 constexpr double HBARC = 0.1973269718;  // Phys_consts.h:6
+
+// Wilson-line files.  V is [N*N][18]: site s = i*N + j (i along x), entry
+// r*3 + c of the 3x3 matrix as (re, im).  format 1 = text, 2 = binary.
+// write_wilson_lines: Init.cpp:1857-1980 (one nucleus per file).
+int write_wilson_lines(const char* path, int N, double L, double rapidity, const double* V, int format,
+                       std::string& err);
+// read_wilson_lines: Init.cpp:1988-2301; nucleus 0 (A, rows shifted by -b/2)
+// or 1 (B, +b/2); sites the file does not reach keep what V holds.
+int read_wilson_lines(const char* path, int N, double L, double b, int nucleus, int format, double* V,
+                      std::string& err);
 
 }  // namespace ipg

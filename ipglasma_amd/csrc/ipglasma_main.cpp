@@ -36,10 +36,31 @@ int main(int argc, char** argv) {
         ipg_destroy(c);
         return 1;
     }
+    const int readV = p.readInitialWilsonLines, writeV = p.writeInitialWilsonLines;
     for (int iev = 0; iev < nev; iev++) {
+        // Wilson-line files (Init.cpp:1863-1877): event id iev, one process
+        const char* ext = (readV > 0 ? readV : writeV) == 1 ? ".txt" : "";
+        char fa[64], fb[64];
+        std::snprintf(fa, sizeof fa, "V-%llu%s", (unsigned long long)iev + 2ULL * seed, ext);
+        std::snprintf(fb, sizeof fb, "V-%llu%s", (unsigned long long)iev + 1ULL + 2ULL * seed, ext);
         int ok = 0, rc = IPG_OK;
         while (!ok) {
-            rc = ipg_init(c, &ok);
+            if (readV > 0) {  // Init::init with READFROMFILE (Init.cpp:2367-2369)
+                rc = ipg_read_wilson_lines(c, fa, fb, readV);
+                if (rc == IPG_OK) rc = ipg_init_fields(c);
+                ok = 1;
+            } else if (writeV > 0) {  // setV's output before the links
+                rc = ipg_init_nucleons(c);
+                if (rc == IPG_OK) rc = ipg_init_color_charge(c, &ok);
+                if (rc == IPG_OK && ok) rc = ipg_init_wilson_lines(c);
+                if (rc == IPG_OK && ok) rc = ipg_write_wilson_lines(c, fa, fb, writeV);
+                if (rc == IPG_OK && ok) {
+                    std::printf("wrote %s and %s\n", fa, fb);
+                    rc = ipg_init_fields(c);
+                }
+            } else {
+                rc = ipg_init(c, &ok);
+            }
             if (rc != IPG_OK) break;
             if (!ok) std::printf("No collision happened. Restarting with new random number...\n");
         }

@@ -367,3 +367,72 @@ def test_full_size_pbpb_event_invariants():
     for k in range(10):
         np.testing.assert_allclose(T2[k], T[k], rtol=0, atol=1e-12 * np.abs(T[k]).max(), err_msg=str(k))
     assert ev2.observables()["dEdeta"] == pytest.approx(o["dEdeta"], rel=1e-14)
+
+
+@pytest.mark.parametrize("fmt", [2, 1])
+def test_wilson_line_files_write_and_resume(tmp_path, fmt):
+    """setV's output written as the reference's Wilson-line files
+    (Init.cpp:1857-1980) and read back as Init::init does for
+    READFROMFILE > 0 (the b draw, then Init::readV: Init.cpp:2319-2369,
+    1988-2301) in a second event of the same seed (b = 0, so no row shift).  Binary files resume the event
+    exactly: the same V, links, root finder and dE/deta.  Text files carry
+    15 digits and are read back transposed, as in the reference."""
+    import ipglasma_amd as I
+    p = K.params("PbPb", N=32, Ny=4, bmin=0.0, bmax=0.0)
+    seed = 9
+    ev = I.Event(p, seed=seed)
+    ev.init_nucleons()
+    assert ev.init_color_charge()
+    ev.init_wilson_lines()
+    VA, VB = ev.field("VA"), ev.field("VB")
+    fa, fb = [tmp_path / n for n in I.wilson_line_file_names(seed, fmt=fmt)]
+    ev.write_wilson_lines(fa, fb, fmt)
+    ev.init_fields()
+    ev.run()
+    d1 = ev.observables()["dEdeta"]
+    ev2 = I.Event(p, seed=seed)
+    ev2.read_wilson_lines(fa, fb, fmt)  # draws b, no nucleons
+    o2 = ev2.observables()
+    assert o2["b"] == 0.0 and o2["success"] == 1
+    if fmt == 2:
+        assert np.array_equal(ev2.field("VA"), VA)
+        assert np.array_equal(ev2.field("VB"), VB)
+        ev2.init_fields()
+        ev2.run()
+        assert ev2.observables()["dEdeta"] == pytest.approx(d1, rel=1e-14)
+    else:
+        for got, want in ((ev2.field("VA"), VA), (ev2.field("VB"), VB)):
+            np.testing.assert_allclose(O.from_c(got), O.from_c(want).transpose(0, 2, 1), rtol=1e-14, atol=1e-15)
+
+
+def test_driver_writes_then_reads_wilson_lines(tmp_path):
+    """The drop-in driver with writeInitialWilsonLines 2 writes V-<2 seed> and
+    V-<2 seed + 1> (Init.cpp:1857-1980); run again with
+    readInitialWilsonLines 2 it starts the event from those files
+    (Init.cpp:2367-2369) and prints the same dE/deta (b = 0)."""
+    import os
+    import re
+    import subprocess
+
+    import ipglasma_amd as I
+    exe = os.path.join(os.path.dirname(I.LIB_PATH), "ipglasma")
+    keys = {"size": 32, "L": 30.0, "Ny": 4, "g": 2.0, "m": 0.2, "BG": 4.0, "UVdamp": 0.0, "QsmuRatio": 0.8,
+            "rmax": 1e7, "roots": 5020.0, "Rapidity": 0.0, "useFluctuatingx": 1, "xFromThisFactorTimesQs": 1.0,
+            "SigmaNN": 70.0, "maxtime": 0.4, "Projectile": "Pb", "Target": "Pb", "setWSDeformParams": 1,
+            "R_WS": 6.62, "a_WS": 0.546, "beta2": 0.0, "beta3": 0.0, "beta4": 0.0, "gamma": 0.0, "d_min": 0.9,
+            "bmin": 0.0, "bmax": 0.0, "samplebFromLinearDistribution": 1, "rotateReactionPlane": 1,
+            "gaussianWounding": 1, "computeGluonMultiplicity": 0, "writeOutputs": 0, "useTimeForSeed": 0,
+            "seed": 4, "NucleusQsTableFileName": I.QS_TABLE}
+
+    def run(extra):
+        text = "".join(f"{k} {v}\n" for k, v in {**keys, **extra}.items())
+        (tmp_path / "input").write_text(text)
+        r = subprocess.run([exe, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        return float(re.search(r"dE/deta=(\S+) GeV", r.stdout).group(1))
+
+    d_write = run({"writeInitialWilsonLines": 2})
+    assert (tmp_path / "V-8").stat().st_size == 32 + 32 * 32 * 9 * 16
+    assert (tmp_path / "V-9").exists()
+    d_read = run({"readInitialWilsonLines": 2})
+    assert d_read == pytest.approx(d_write, rel=1e-11)

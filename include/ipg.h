@@ -249,6 +249,17 @@ int ipg_get_landau(ipg_ctx *c, double *out);
  * themselves (Evolution::u + Evolution::eccentricity on given cells). */
 int ipg_landau(ipg_ctx *c, const double *T, int it);
 
+/* eccentricities<id>.dat (Evolution.cpp:1280-1810, cutoff 0, doAniso 0):
+ * appends one line for the Landau frame of step it (tau = it a dtau): tau,
+ * eps_n and Psi_n for n = 1..6, cutoff, sqrt<r^2>, maxX, maxY, b, Tpp, area,
+ * Rbar, <eps>, <Q_sA^2 Q_sB^2> hbarc, printed as an ostream prints them
+ * (%g).  The reference writes it when computeGluonMultiplicity is set. */
+int ipg_write_eccentricities(ipg_ctx *c, const char *path, int it);
+/* the maxX, maxY columns (Evolution.cpp:1492-1640) for eps, u^tau [N*N]
+ * (site ix*N + iy) and the centroid (avx, avy) of the first pass */
+int ipg_eccentricity_extents(const double *eps, const double *utau, int N, double L, double avx, double avy,
+                             double cutoff, double *maxX, double *maxY);
+
 /* field ids for ipg_get_field / ipg_set_field */
 #define IPG_F_UX 0
 #define IPG_F_UY 1

@@ -121,6 +121,9 @@ def lib():
             "ipg_read_wilson_line_file": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                                     dp]),
             "ipg_write_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
+            "ipg_write_eccentricities": (C.c_int, [vp, C.c_char_p, C.c_int]),
+            "ipg_eccentricity_extents": (C.c_int, [dp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                   dp, dp]),
             "ipg_read_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
             "ipg_evolve": (C.c_int, [vp]),
             "ipg_evolve_step": (C.c_int, [vp, C.c_int]),
@@ -237,6 +240,12 @@ class Event:
     # from the first draw, no nucleons; then init_fields
     def read_wilson_lines(self, path_a, path_b, fmt=2):
         _check(lib().ipg_read_wilson_lines(self._c, os.fsencode(path_a), os.fsencode(path_b), fmt))
+
+    # one line of eccentricities<id>.dat (Evolution.cpp:1797-1809) for the
+    # Landau frame of step it (default: itmax, where Evolution::run takes it)
+    def write_eccentricities(self, path, it=None):
+        it = self.p.itmax if it is None else it
+        _check(lib().ipg_write_eccentricities(self._c, os.fsencode(path), it))
 
     def init_fields(self):
         _check(lib().ipg_init_fields(self._c))
@@ -434,6 +443,17 @@ def read_wilson_line_file(path, N, L, b=0.0, nucleus=0, fmt=2, V=None):
     V = np.ascontiguousarray(V, dtype=np.float64)
     _check(lib().ipg_read_wilson_line_file(os.fsencode(path), N, L, b, nucleus, fmt, _dptr(V)))
     return V
+
+
+def eccentricity_extents(eps, utau, L, avx, avy, cutoff=0.0):
+    """maxX, maxY of Evolution::eccentricity (Evolution.cpp:1492-1640) for
+    eps, u^tau [N*N] and the centroid (avx, avy)."""
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    utau = np.ascontiguousarray(utau, dtype=np.float64)
+    N = int(round(np.sqrt(eps.size)))
+    mx, my = C.c_double(), C.c_double()
+    _check(lib().ipg_eccentricity_extents(_dptr(eps), _dptr(utau), N, L, avx, avy, cutoff, C.byref(mx), C.byref(my)))
+    return mx.value, my.value
 
 
 def autotune_report():

@@ -1610,6 +1610,29 @@ int ipg_landau(ipg_ctx* c, const double* T, int it) {
     g.desc[0].active = 1;
     return g.landau(it);
 }
+int ipg_eccentricity_extents(const double* eps, const double* utau, int N, double L, double avx, double avy,
+                             double cutoff, double* maxX, double* maxY) {
+    if (!eps || !utau || !maxX || !maxY || N < 5) return fail(IPG_ERR_ARG, "null argument or N < 5");
+    ipg::ecc_extents(eps, utau, N, L, avx, avy, cutoff, maxX, maxY);
+    return IPG_OK;
+}
+int ipg_write_eccentricities(ipg_ctx* c, const char* path, int it) {
+    if (!c || !path) return fail(IPG_ERR_ARG, "null argument");
+    Engine& g = c->eng;
+    const ipg_params& p = g.ev[0].p;
+    const ipg_observables& o = g.ev[0].obs;
+    const size_t S = (size_t)g.desc[0].S;
+    std::vector<double> land(15 * S);
+    int rc = ipg_get_landau(c, land.data());
+    if (rc) return rc;
+    double mx = 0., my = 0.;
+    ipg::ecc_extents(land.data(), land.data() + S, p.size, p.L, o.centroid[0], o.centroid[1], 0., &mx, &my);
+    const double a = p.L / p.size;
+    std::string err;
+    rc = ipg::append_eccentricity_line(path, it * a * p.dtau, o.eccentricity, o.Psi, 0., o.rms_r, mx, my, o.b, o.Tpp,
+                                       o.area, o.Rbar, o.avg_eps, o.avg_qs2A_qs2B, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
 int ipg_get_nucleons(ipg_ctx* c, double* out, int* A1, int* A2) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     Event& E = c->eng.ev[0];

@@ -665,4 +665,49 @@ int read_wilson_lines(const char* path, int N, double L, double b, int nucleus, 
     return IPG_OK;
 }
 
+
+// ---- eccentricities<id>.dat (Evolution.cpp:1280-1810) --------------------------
+
+void ecc_extents(const double* eps, const double* utau, int N, double L, double avx, double avy, double cutoff,
+                 double* maxX, double* maxY) {
+    const double a = L / N;
+    const int xshift = static_cast<int>(std::floor(avx / a + 0.00000000001));
+    const int yshift = static_cast<int>(std::floor(avy / a + 0.00000000001));
+    double mx = 0., my = 0.;
+    for (int ix = 2; ix < N - 2; ix++) {
+        const double x = -L / 2. + a * ix - avx;
+        for (int iy = 2; iy < N - 2; iy++) {
+            const double y = -L / 2. + a * iy - avy;
+            const size_t pos = (size_t)ix * N + iy;
+            const double w = eps[pos] < cutoff ? 0. : eps[pos] * utau[pos];
+            if (w > cutoff && iy == N / 2 + yshift) mx = x;
+            if (w > cutoff && ix == N / 2 + xshift) my = y;
+        }
+    }
+    *maxX = mx;
+    *maxY = my;
+}
+
+int append_eccentricity_line(const char* path, double tau, const double* ecc, const double* psi, double cutoff,
+                             double rms_r, double maxX, double maxY, double b, double Tpp, double area, double Rbar,
+                             double avg_eps, double avg_qs, std::string& err) {
+    FILE* f = std::fopen(path, "a");
+    if (!f) {
+        err = std::string("cannot open ") + path + " for appending";
+        return IPG_ERR_IO;
+    }
+    // an ostream's default format: %g, six significant digits
+    bool ok = std::fprintf(f, "%g", tau) > 0;
+    for (int n = 0; n < 6; n++) ok = ok && std::fprintf(f, " %g %g", ecc[n], psi[n]) > 0;
+    const double rest[10] = {cutoff, rms_r, maxX, maxY, b, Tpp, area, Rbar, avg_eps, avg_qs};
+    for (double v : rest) ok = ok && std::fprintf(f, " %g", v) > 0;
+    ok = ok && std::fputc('\n', f) != EOF;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        err = std::string("writing ") + path + " failed";
+        return IPG_ERR_IO;
+    }
+    return IPG_OK;
+}
+
 }  // namespace ipg

@@ -57,6 +57,20 @@ struct EventInfo {
 int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
                  std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err);
 
+// maxX, maxY of Evolution::eccentricity (Evolution.cpp:1492-1640): over
+// 2 <= ix, iy < N-2, the last x = -L/2 + a ix - avx on the row
+// iy = N/2 + yshift, and the last y on the column ix = N/2 + xshift, whose
+// weight eps u^tau exceeds cutoff (weight 0 where eps < cutoff);
+// shift = floor(<x>/a + 1e-11).  eps, utau: [N*N], site ix*N + iy.
+void ecc_extents(const double* eps, const double* utau, int N, double L, double avx, double avy, double cutoff,
+                 double* maxX, double* maxY);
+// one line appended to eccentricities<id>.dat (Evolution.cpp:1797-1809):
+// tau, (eps_n, Psi_n) for n = 1..6, cutoff, sqrt<r^2>, maxX, maxY, b, Tpp,
+// area, Rbar, <eps>, <Q_sA^2 Q_sB^2> hbarc, as an ostream prints them
+int append_eccentricity_line(const char* path, double tau, const double* ecc, const double* psi, double cutoff,
+                             double rms_r, double maxX, double maxY, double b, double Tpp, double area, double Rbar,
+                             double avg_eps, double avg_qs, std::string& err);
+
 // b from the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p);
 

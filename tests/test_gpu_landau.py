@@ -148,3 +148,30 @@ def test_batch_landau_matches_plain():
             assert o["eccentricity"][n] == pytest.approx(plain[e]["eccentricity"][n], rel=1e-12)
         assert o["area"] == plain[e]["area"]
     b.close()
+
+
+def test_eccentricity_file_line(tmp_path):
+    """eccentricities<id>.dat (Evolution.cpp:1797-1809): one appended line
+    per call, the 23 columns printed as an ostream prints them (%g), maxX
+    and maxY from the device's Landau frame by the reference's loop."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=32, Ny=4, bmin=3.0, bmax=3.0)
+    ev = I.Event(p, seed=13)
+    assert ev.init()
+    ev.run()
+    o = ev.observables()
+    f = tmp_path / "eccentricities0.dat"
+    ev.write_eccentricities(f)
+    ev.write_eccentricities(f)
+    lines = f.read_text().splitlines()
+    assert len(lines) == 2 and lines[0] == lines[1]
+    land = ev.landau()
+    S = p.size * p.size
+    mx, my = I.eccentricity_extents(land[0][:S], land[1][:S], p.L, o["centroid"][0], o["centroid"][1])
+    a = p.L / p.size
+    cols = [p.itmax * a * p.dtau]
+    for n in range(6):
+        cols += [o["eccentricity"][n], o["Psi"][n]]
+    cols += [0.0, o["rms_r"], mx, my, o["b"], o["Tpp"], o["area"], o["Rbar"], o["avg_eps"], o["avg_qs2A_qs2B"]]
+    assert lines[0] == " ".join("%g" % v for v in cols)
+    assert mx > 0 and my > 0

@@ -255,6 +255,18 @@ int ipg_landau(ipg_ctx *c, const double *T, int it);
  * Rbar, <eps>, <Q_sA^2 Q_sB^2> hbarc, printed as an ostream prints them
  * (%g).  The reference writes it when computeGluonMultiplicity is set. */
 int ipg_write_eccentricities(ipg_ctx *c, const char *path, int it);
+/* epsilon-u-Hydro-t<tau>-<id>.dat (MyEigen.cpp:326-800; writeOutputs odd,
+ * gfactor 1): the Landau frame of step it bilinearly interpolated onto a
+ * size_out x size_out x eta_size_out hydro grid of length L_out and rapidity
+ * step deta_out (the input keys sizeOutput, etaSizeOutput, LOutput,
+ * detaOutput); columns eta x y eps u^tau u^x u^y u^eta and the ten pi^{mu nu}
+ * (tau tau, tau x, tau y, tau eta, xx, xy, x eta, yy, y eta, eta eta), as an
+ * ostream prints them.  *Etot (may be NULL): the reference's printed Etot. */
+int ipg_write_hydro(ipg_ctx *c, const char *path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out, double *Etot);
+/* the same from a host Landau frame land [15][N*N] (ipg_get_landau's layout) */
+int ipg_write_hydro_file(const char *path, const double *land, int N, double L, int it, double dtau, int size_out,
+                         int eta_size_out, double L_out, double deta_out, double *Etot);
 /* the maxX, maxY columns (Evolution.cpp:1492-1640) for eps, u^tau [N*N]
  * (site ix*N + iy) and the centroid (avx, avy) of the first pass */
 int ipg_eccentricity_extents(const double *eps, const double *utau, int N, double L, double avx, double avy,

@@ -122,6 +122,9 @@ def lib():
                                                     dp]),
             "ipg_write_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
             "ipg_write_eccentricities": (C.c_int, [vp, C.c_char_p, C.c_int]),
+            "ipg_write_hydro": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "ipg_write_hydro_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                               C.c_int, C.c_double, C.c_double, dp]),
             "ipg_eccentricity_extents": (C.c_int, [dp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                                    dp, dp]),
             "ipg_read_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
@@ -246,6 +249,15 @@ class Event:
     def write_eccentricities(self, path, it=None):
         it = self.p.itmax if it is None else it
         _check(lib().ipg_write_eccentricities(self._c, os.fsencode(path), it))
+
+    # epsilon-u-Hydro file (MyEigen.cpp:326-800) of the Landau frame of step
+    # it (default itmax); returns the reference's printed Etot
+    def write_hydro(self, path, size_out, eta_size_out, L_out, deta_out, it=None):
+        it = self.p.itmax if it is None else it
+        e = C.c_double()
+        _check(lib().ipg_write_hydro(self._c, os.fsencode(path), it, size_out, eta_size_out, L_out, deta_out,
+                                     C.byref(e)))
+        return e.value
 
     def init_fields(self):
         _check(lib().ipg_init_fields(self._c))
@@ -454,6 +466,21 @@ def eccentricity_extents(eps, utau, L, avx, avy, cutoff=0.0):
     mx, my = C.c_double(), C.c_double()
     _check(lib().ipg_eccentricity_extents(_dptr(eps), _dptr(utau), N, L, avx, avy, cutoff, C.byref(mx), C.byref(my)))
     return mx.value, my.value
+
+
+def write_hydro_file(path, land, L, it, dtau, size_out, eta_size_out, L_out, deta_out):
+    """epsilon-u-Hydro file from a host Landau frame land [15, N*N]; returns Etot."""
+    land = np.ascontiguousarray(land, dtype=np.float64)
+    N = int(round(np.sqrt(land.shape[1])))
+    e = C.c_double()
+    _check(lib().ipg_write_hydro_file(os.fsencode(path), _dptr(land), N, L, it, dtau, size_out, eta_size_out, L_out,
+                                      deta_out, C.byref(e)))
+    return e.value
+
+
+def hydro_file_name(tau, event_id=0, final=False):
+    """The reference's name of the hydro file (MyEigen.cpp:379-386)."""
+    return f"epsilon-u-Hydro-TauHydro-{event_id}.dat" if final else f"epsilon-u-Hydro-t{tau:g}-{event_id}.dat"
 
 
 def autotune_report():

@@ -1633,6 +1633,25 @@ int ipg_write_eccentricities(ipg_ctx* c, const char* path, int it) {
                                        o.area, o.Rbar, o.avg_eps, o.avg_qs2A_qs2B, err);
     return rc ? fail(rc, err) : IPG_OK;
 }
+int ipg_write_hydro_file(const char* path, const double* land, int N, double L, int it, double dtau, int size_out,
+                         int eta_size_out, double L_out, double deta_out, double* Etot) {
+    if (!path || !land || N < 2 || size_out < 1 || eta_size_out < 1 || !(L_out > 0.))
+        return fail(IPG_ERR_ARG, "null argument or empty lattice / hydro grid");
+    std::string err;
+    const int rc =
+        ipg::write_hydro(path, land, N, L, it, dtau, size_out, eta_size_out, L_out, deta_out, Etot, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
+int ipg_write_hydro(ipg_ctx* c, const char* path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out, double* Etot) {
+    if (!c || !path) return fail(IPG_ERR_ARG, "null argument");
+    const ipg_params& p = c->eng.ev[0].p;
+    std::vector<double> land(15 * (size_t)c->eng.desc[0].S);
+    const int rc = ipg_get_landau(c, land.data());
+    if (rc) return rc;
+    return ipg_write_hydro_file(path, land.data(), p.size, p.L, it, p.dtau, size_out, eta_size_out, L_out, deta_out,
+                                Etot);
+}
 int ipg_get_nucleons(ipg_ctx* c, double* out, int* A1, int* A2) {
     if (!c) return fail(IPG_ERR_ARG, "null context");
     Event& E = c->eng.ev[0];

@@ -710,4 +710,93 @@ int append_eccentricity_line(const char* path, double tau, const double* ecc, co
     return IPG_OK;
 }
 
+
+// ---- epsilon-u-Hydro output (MyEigen.cpp:326-800) --------------------------------
+//
+// Each hydro cell (x, y) inside the lattice takes the four lattice sites
+// around it, xpos = floor((x + L/2)/a + 1e-10) and the next one (clamped at
+// N-1), and interpolates each quantity q as
+//   x1 = (1 - fx) q(xpos, ypos) + fx q(xpos+1, ypos)    (0 unless both sites
+//   x2 = (1 - fx) q(xpos, ypos+1) + fx q(xpos+1, ypos+1)  are in (0, N^2))
+//   q  = (1 - fy) x1 + fy x2,   fx = (x - x_low) / ha,  fy = (y - y_low) / ha
+// with ha the hydro spacing (the reference divides by ha, not by a), |eps|
+// for the energy density; u^tau is then rebuilt from u^x, u^y, u^eta.  The
+// columns: eta x y eps[GeV/fm^3] u^tau u^x u^y u^eta pi^{tt tx ty teta xx xy
+// xeta yy yeta etaeta}, a vacuum line "eta x y 0 1 0 ..." where eps <= 1e-10
+// or outside the lattice, an empty line after each eta slice.
+
+int write_hydro(const char* path, const double* land, int N, double L, int it, double dtau, int hx, int heta,
+                double hL, double deta, double* Etot, std::string& err) {
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        err = std::string("cannot open ") + path + " for writing";
+        return IPG_ERR_IO;
+    }
+    const size_t S = (size_t)N * N;
+    const double a = L / N;
+    const int hy = hx;
+    const double tau0 = it * dtau * a;
+    const double ha = hL / static_cast<double>(hx);
+    // land planes in the order of the columns after u^eta
+    const int pic[10] = {5, 9, 10, 11, 6, 12, 13, 7, 14, 8};
+    bool ok = std::fprintf(f, "# dummy %d etamax= %d xmax= %d ymax= %d deta= %g dx= %g dy= %g tau= %g\n", 1, heta, hx,
+                           hy, deta, ha, ha, tau0) > 0;
+    double etot = 0.;
+    for (int ieta = 0; ieta < heta && ok; ieta++) {
+        const double eta = -(heta - 1) / 2. * deta + deta * ieta;
+        for (int ix = 0; ix < hx && ok; ix++) {
+            for (int iy = 0; iy < hy && ok; iy++) {
+                const double x = -hL / 2. + ha * ix;
+                const double y = -hL / 2. + ha * iy;
+                double v[15];
+                bool inside = std::abs(x) < L / 2. && std::abs(y) < L / 2.;
+                if (inside) {
+                    const int xpos = static_cast<int>(std::floor((x + L / 2.) / a + 0.0000000001));
+                    const int ypos = static_cast<int>(std::floor((y + L / 2.) / a + 0.0000000001));
+                    const int xup = xpos < N - 1 ? xpos + 1 : xpos;
+                    const int yup = ypos < N - 1 ? ypos + 1 : ypos;
+                    const double xlow = -L / 2. + a * xpos, ylow = -L / 2. + a * ypos;
+                    const double fracx = (x - xlow) / ha;
+                    const double fracy = (y - ylow) / ha;
+                    const long long p1 = (long long)xpos * N + ypos, p2 = (long long)xup * N + ypos;
+                    const long long p3 = (long long)xpos * N + yup, p4 = (long long)xup * N + yup;
+                    const long long SS = (long long)S;
+                    const bool lo = p1 > 0 && p1 < SS && p2 > 0 && p2 < SS;
+                    const bool hi = p3 > 0 && p3 < SS && p4 > 0 && p4 < SS;
+                    for (int q = 0; q < 15; q++) {
+                        const double* F = land + (size_t)q * S;
+                        double f1 = 0., f2 = 0., f3 = 0., f4 = 0.;
+                        if (lo) f1 = F[p1], f2 = F[p2];
+                        if (hi) f3 = F[p3], f4 = F[p4];
+                        if (q == 0) f1 = std::abs(f1), f2 = std::abs(f2), f3 = std::abs(f3), f4 = std::abs(f4);
+                        const double x1 = lo ? (1 - fracx) * f1 + fracx * f2 : 0.;
+                        const double x2 = hi ? (1 - fracx) * f3 + fracx * f4 : 0.;
+                        v[q] = (1. - fracy) * x1 + fracy * x2;
+                    }
+                    v[1] = std::sqrt(1. + v[2] * v[2] + v[3] * v[3] + tau0 * tau0 * v[4] * v[4]);
+                    const double e = std::abs(HBARC * v[0] * 1.);
+                    etot += e * ha * ha * it * dtau * a;
+                    inside = e > 0.0000000001;
+                    if (inside) {
+                        ok = std::fprintf(f, "%g %g %g %g %g %g %g %g", eta, x, y, e, v[1], v[2], v[3], v[4]) > 0;
+                        for (int k = 0; k < 10; k++) ok = ok && std::fprintf(f, " %g", v[pic[k]] * 1.) > 0;
+                        ok = ok && std::fputc('\n', f) != EOF;
+                    }
+                }
+                if (!inside) {
+                    ok = std::fprintf(f, "%g %g %g 0 1 0 0 0 0 0 0 0 0 0 0 0 0 0\n", eta, x, y) > 0;
+                }
+            }
+        }
+        ok = ok && std::fputc('\n', f) != EOF;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        err = std::string("writing ") + path + " failed";
+        return IPG_ERR_IO;
+    }
+    if (Etot) *Etot = etot;
+    return IPG_OK;
+}
+
 }  // namespace ipg

@@ -71,6 +71,13 @@ int append_eccentricity_line(const char* path, double tau, const double* ecc, co
                              double rms_r, double maxX, double maxY, double b, double Tpp, double area, double Rbar,
                              double avg_eps, double avg_qs, std::string& err);
 
+// epsilon-u-Hydro file (MyEigen.cpp:326-414, 440-800; writeOutputs odd,
+// gfactor 1): the Landau frame land [15][N*N] (eps, u^tau, u^x, u^y, u^eta,
+// pi in T's order) interpolated onto hx x hx x heta hydro cells of size
+// hL/hx; *Etot as the reference sums it
+int write_hydro(const char* path, const double* land, int N, double L, int it, double dtau, int hx, int heta,
+                double hL, double deta, double* Etot, std::string& err);
+
 // b from the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p);
 

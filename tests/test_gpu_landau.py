@@ -175,3 +175,29 @@ def test_eccentricity_file_line(tmp_path):
     cols += [0.0, o["rms_r"], mx, my, o["b"], o["Tpp"], o["area"], o["Rbar"], o["avg_eps"], o["avg_qs2A_qs2B"]]
     assert lines[0] == " ".join("%g" % v for v in cols)
     assert mx > 0 and my > 0
+
+
+def test_hydro_file_of_an_event(tmp_path):
+    """ipg_write_hydro on an evolved event writes the file the host writer
+    makes from the device's Landau frame (MyEigen.cpp:326-800): header,
+    size_out^2 lines per eta slice plus an empty one, u^tau >= 1, eps >= 0
+    and the vacuum lines (eps = 0) at rest.  The weights are divided by the
+    hydro spacing, so a hydro grid finer than the lattice extrapolates, as in
+    the reference: eps is not bounded by the lattice's."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=32, Ny=4)
+    ev = I.Event(p, seed=17)
+    assert ev.init()
+    ev.run()
+    f1, f2 = tmp_path / "a.dat", tmp_path / "b.dat"
+    e1 = ev.write_hydro(f1, 40, 2, 0.9 * p.L, 0.5)
+    land = ev.landau()
+    e2 = I.write_hydro_file(f2, land, p.L, p.itmax, p.dtau, 40, 2, 0.9 * p.L, 0.5)
+    assert f1.read_text() == f2.read_text() and e1 == e2 and e1 > 0
+    rows = [r.split() for r in f1.read_text().splitlines()[1:] if r]
+    assert len(rows) == 2 * 40 * 40
+    vals = np.array(rows, dtype=float)
+    assert np.all(vals[:, 4] >= 1.0)
+    assert np.all(vals[:, 3] >= 0.0) and vals[:, 3].max() > 0
+    vac = vals[:, 3] == 0
+    assert np.all(vals[vac, 4] == 1.0) and np.all(vals[vac, 5:] == 0.0)

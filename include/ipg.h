@@ -264,6 +264,16 @@ int ipg_write_eccentricities(ipg_ctx *c, const char *path, int it);
  * ostream prints them.  *Etot (may be NULL): the reference's printed Etot. */
 int ipg_write_hydro(ipg_ctx *c, const char *path, int it, int size_out, int eta_size_out, double L_out,
                     double deta_out, double *Etot);
+/* Tmunu-t<tau>-<id>.dat (MyEigen.cpp:801-1046; writeOutputs / 4 == 1):
+ * T^{mu nu} at itmax interpolated as above onto the size_out^2 grid, y rows
+ * outer; columns ix iy T^tt T^xx T^yy tau^2 T^etaeta -T^tx -T^ty -tau T^teta
+ * -T^xy -tau T^yeta -tau T^xeta in GeV/fm^3 (a 1e-16 floor line where
+ * T^tt <= 1e-16 GeV/fm^3 or outside the lattice) */
+int ipg_write_tmunu(ipg_ctx *c, const char *path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out);
+/* the same from a host T [10][N*N] (ipg_get_tmunu's layout) */
+int ipg_write_tmunu_file(const char *path, const double *T, int N, double L, int it, double dtau, int size_out,
+                         int eta_size_out, double L_out, double deta_out);
 /* the same from a host Landau frame land [15][N*N] (ipg_get_landau's layout) */
 int ipg_write_hydro_file(const char *path, const double *land, int N, double L, int it, double dtau, int size_out,
                          int eta_size_out, double L_out, double deta_out, double *Etot);

@@ -123,6 +123,9 @@ def lib():
             "ipg_write_wilson_lines": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int]),
             "ipg_write_eccentricities": (C.c_int, [vp, C.c_char_p, C.c_int]),
             "ipg_write_hydro": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "ipg_write_tmunu": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "ipg_write_tmunu_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                               C.c_int, C.c_double, C.c_double]),
             "ipg_write_hydro_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                C.c_int, C.c_double, C.c_double, dp]),
             "ipg_eccentricity_extents": (C.c_int, [dp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -258,6 +261,11 @@ class Event:
         _check(lib().ipg_write_hydro(self._c, os.fsencode(path), it, size_out, eta_size_out, L_out, deta_out,
                                      C.byref(e)))
         return e.value
+
+    # Tmunu-t file (MyEigen.cpp:801-1046) of T^{mu nu} at itmax
+    def write_tmunu(self, path, size_out, eta_size_out, L_out, deta_out, it=None):
+        it = self.p.itmax if it is None else it
+        _check(lib().ipg_write_tmunu(self._c, os.fsencode(path), it, size_out, eta_size_out, L_out, deta_out))
 
     def init_fields(self):
         _check(lib().ipg_init_fields(self._c))
@@ -476,6 +484,14 @@ def write_hydro_file(path, land, L, it, dtau, size_out, eta_size_out, L_out, det
     _check(lib().ipg_write_hydro_file(os.fsencode(path), _dptr(land), N, L, it, dtau, size_out, eta_size_out, L_out,
                                       deta_out, C.byref(e)))
     return e.value
+
+
+def write_tmunu_file(path, T, L, it, dtau, size_out, eta_size_out, L_out, deta_out):
+    """Tmunu-t file from a host T [10, N*N]."""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    N = int(round(np.sqrt(T.shape[1])))
+    _check(lib().ipg_write_tmunu_file(os.fsencode(path), _dptr(T), N, L, it, dtau, size_out, eta_size_out, L_out,
+                                      deta_out))
 
 
 def hydro_file_name(tau, event_id=0, final=False):

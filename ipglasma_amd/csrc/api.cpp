@@ -1642,6 +1642,23 @@ int ipg_write_hydro_file(const char* path, const double* land, int N, double L, 
         ipg::write_hydro(path, land, N, L, it, dtau, size_out, eta_size_out, L_out, deta_out, Etot, err);
     return rc ? fail(rc, err) : IPG_OK;
 }
+int ipg_write_tmunu_file(const char* path, const double* T, int N, double L, int it, double dtau, int size_out,
+                         int eta_size_out, double L_out, double deta_out) {
+    if (!path || !T || N < 2 || size_out < 1 || !(L_out > 0.))
+        return fail(IPG_ERR_ARG, "null argument or empty lattice / hydro grid");
+    std::string err;
+    const int rc = ipg::write_tmunu(path, T, N, L, it, dtau, size_out, eta_size_out, L_out, deta_out, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
+int ipg_write_tmunu(ipg_ctx* c, const char* path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out) {
+    if (!c || !path) return fail(IPG_ERR_ARG, "null argument");
+    const ipg_params& p = c->eng.ev[0].p;
+    std::vector<double> T(10 * (size_t)c->eng.desc[0].S);
+    const int rc = ipg_get_tmunu(c, T.data());
+    if (rc) return rc;
+    return ipg_write_tmunu_file(path, T.data(), p.size, p.L, it, p.dtau, size_out, eta_size_out, L_out, deta_out);
+}
 int ipg_write_hydro(ipg_ctx* c, const char* path, int it, int size_out, int eta_size_out, double L_out,
                     double deta_out, double* Etot) {
     if (!c || !path) return fail(IPG_ERR_ARG, "null argument");

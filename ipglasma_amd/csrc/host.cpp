@@ -799,4 +799,76 @@ int write_hydro(const char* path, const double* land, int N, double L, int it, d
     return IPG_OK;
 }
 
+
+// ---- Tmunu-t output (MyEigen.cpp:801-1046) -----------------------------------------
+//
+// The same interpolation as the hydro file (without |.|) for the ten
+// components, y rows outer and x inner, an empty line after each row.
+// Columns: ix iy, then in GeV/fm^3 T^tt, T^xx, T^yy, tau^2 T^etaeta, -T^tx,
+// -T^ty, -tau T^teta, -T^xy, -tau T^yeta, -tau T^xeta; a floor line
+// "ix iy 1e-16 5e-17 5e-17 0 ..." where T^tt hbarc <= 1e-16 or outside the
+// lattice (small_eps, Phys_consts.h:5).
+
+int write_tmunu(const char* path, const double* T, int N, double L, int it, double dtau, int hx, int heta, double hL,
+                double deta, std::string& err) {
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        err = std::string("cannot open ") + path + " for writing";
+        return IPG_ERR_IO;
+    }
+    const double small_eps = 1e-16;
+    const long long S = (long long)N * N;
+    const double a = L / N;
+    const int hy = hx;
+    const double tau0 = it * dtau * a;
+    const double ha = hL / static_cast<double>(hx);
+    bool ok = std::fprintf(f, "# dummy %d etamax= %d xmax= %d ymax= %d deta= %g dx= %g dy= %g\n", 1, heta, hx, hy,
+                           deta, ha, ha) > 0;
+    for (int iy = 0; iy < hy && ok; iy++) {
+        for (int ix = 0; ix < hx && ok; ix++) {
+            const double x = -hL / 2. + ha * ix;
+            const double y = -hL / 2. + ha * iy;
+            bool written = false;
+            if (std::abs(x) < L / 2. && std::abs(y) < L / 2.) {
+                const int xpos = static_cast<int>(std::floor((x + L / 2.) / a + 0.0000000001));
+                const int ypos = static_cast<int>(std::floor((y + L / 2.) / a + 0.0000000001));
+                const int xup = xpos < N - 1 ? xpos + 1 : xpos;
+                const int yup = ypos < N - 1 ? ypos + 1 : ypos;
+                const double xlow = -L / 2. + a * xpos, ylow = -L / 2. + a * ypos;
+                const long long p1 = (long long)xpos * N + ypos, p2 = (long long)xup * N + ypos;
+                const long long p3 = (long long)xpos * N + yup, p4 = (long long)xup * N + yup;
+                const double fracx = (x - xlow) / ha;
+                const double fracy = (y - ylow) / ha;
+                const bool lo = p1 > 0 && p1 < S && p2 > 0 && p2 < S;
+                const bool hi = p3 > 0 && p3 < S && p4 > 0 && p4 < S;
+                double r[10];
+                for (int q = 0; q < 10; q++) {
+                    const double* F = T + (size_t)q * S;
+                    const double x1 = lo ? (1 - fracx) * F[p1] + fracx * F[p2] : 0.;
+                    const double x2 = hi ? (1 - fracx) * F[p3] + fracx * F[p4] : 0.;
+                    r[q] = (1. - fracy) * x1 + fracy * x2;
+                }
+                // T's order: tt 0, xx 1, yy 2, ee 3, tx 4, ty 5, te 6, xy 7, xe 8, ye 9
+                if (r[0] * 1. * HBARC > small_eps) {
+                    ok = std::fprintf(f, "%d %d %g %g %g %g %g %g %g %g %g %g\n", ix, iy, r[0] * 1. * HBARC,
+                                      r[1] * 1. * HBARC, r[2] * 1. * HBARC, tau0 * tau0 * r[3] * 1. * HBARC,
+                                      -r[4] * 1. * HBARC, -r[5] * 1. * HBARC, -tau0 * r[6] * 1. * HBARC,
+                                      -r[7] * 1. * HBARC, -tau0 * r[9] * 1. * HBARC, -tau0 * r[8] * 1. * HBARC) > 0;
+                    written = true;
+                }
+            }
+            if (!written && ok)
+                ok = std::fprintf(f, "%d %d %g %g %g 0 0 0 0 0 0 0\n", ix, iy, small_eps, small_eps / 2.,
+                                  small_eps / 2.) > 0;
+        }
+        ok = ok && std::fputc('\n', f) != EOF;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        err = std::string("writing ") + path + " failed";
+        return IPG_ERR_IO;
+    }
+    return IPG_OK;
+}
+
 }  // namespace ipg

@@ -78,6 +78,12 @@ int append_eccentricity_line(const char* path, double tau, const double* ecc, co
 int write_hydro(const char* path, const double* land, int N, double L, int it, double dtau, int hx, int heta,
                 double hL, double deta, double* Etot, std::string& err);
 
+// Tmunu-t file (MyEigen.cpp:801-1046; writeOutputs / 4 == 1, gfactor 1):
+// T [10][N*N] in T's order and physical units, interpolated as for the
+// hydro file onto the hx x hx grid (y rows outer, x inner)
+int write_tmunu(const char* path, const double* T, int N, double L, int it, double dtau, int hx, int heta, double hL,
+                double deta, std::string& err);
+
 // b from the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p);
 

@@ -201,3 +201,11 @@ def test_hydro_file_of_an_event(tmp_path):
     assert np.all(vals[:, 3] >= 0.0) and vals[:, 3].max() > 0
     vac = vals[:, 3] == 0
     assert np.all(vals[vac, 4] == 1.0) and np.all(vals[vac, 5:] == 0.0)
+    # Tmunu-t (MyEigen.cpp:801-1046): the device path against the host path
+    # on the event's T^{mu nu}
+    t1, t2 = tmp_path / "t1.dat", tmp_path / "t2.dat"
+    ev.write_tmunu(t1, 40, 1, 0.9 * p.L, 0.5)
+    I.write_tmunu_file(t2, ev.tmunu(), p.L, p.itmax, p.dtau, 40, 1, 0.9 * p.L, 0.5)
+    assert t1.read_text() == t2.read_text()
+    trows = np.array([r.split() for r in t1.read_text().splitlines()[1:] if r], dtype=float)
+    assert trows.shape == (40 * 40, 12) and trows[:, 2].max() > 1e-16

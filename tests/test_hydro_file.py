@@ -1,9 +1,10 @@
-"""The epsilon-u-Hydro output of MyEigen::flowVelocity4D (MyEigen.cpp:326-800,
-writeOutputs odd, gfactor 1) through the library's host entry point, against
-a plain-Python restatement of the reference's loops on a synthetic Landau
-frame: the hydro grid, the bilinear weights divided by the hydro spacing,
-the site-0 and clamped-neighbour rules, |eps|, the rebuilt u^tau, the vacuum
-lines, the header, the printed Etot.  No GPU."""
+"""The epsilon-u-Hydro and Tmunu-t outputs of MyEigen::flowVelocity4D
+(MyEigen.cpp:326-800 and 801-1046, gfactor 1) through the library's host
+entry points, against plain-Python restatements of the reference's loops on
+synthetic fields: the hydro grid, the bilinear weights divided by the hydro
+spacing, the site-0 and clamped-neighbour rules, |eps|, the rebuilt u^tau,
+the vacuum and floor lines, the column signs and tau factors, the headers,
+the printed Etot.  No GPU."""
 import math
 
 import numpy as np
@@ -99,3 +100,56 @@ def test_hydro_file_name_and_errors(tmp_path):
         I.write_hydro_file(tmp_path / "h", land, 4.0, 1, 0.1, 0, 1, 4.0, 1.0)
     with pytest.raises(I.IPGError, match="cannot open"):
         I.write_hydro_file(tmp_path / "no" / "h", land, 4.0, 1, 0.1, 4, 1, 4.0, 1.0)
+
+
+def _reference_tmunu_text(T, N, L, it, dtau, hx, heta, hL, deta):
+    S = N * N
+    a = L / N
+    tau0 = it * dtau * a
+    ha = hL / float(hx)
+    small = 1e-16
+    out = ["# dummy %d etamax= %d xmax= %d ymax= %d deta= %g dx= %g dy= %g" % (1, heta, hx, hx, deta, ha, ha)]
+    for iy in range(hx):
+        for ix in range(hx):
+            x = -hL / 2.0 + ha * ix
+            y = -hL / 2.0 + ha * iy
+            line = None
+            if abs(x) < L / 2.0 and abs(y) < L / 2.0:
+                xpos = int(math.floor((x + L / 2.0) / a + 0.0000000001))
+                ypos = int(math.floor((y + L / 2.0) / a + 0.0000000001))
+                xup = xpos + 1 if xpos < N - 1 else xpos
+                yup = ypos + 1 if ypos < N - 1 else ypos
+                fx = (x - (-L / 2.0 + a * xpos)) / ha
+                fy = (y - (-L / 2.0 + a * ypos)) / ha
+                p1, p2, p3, p4 = xpos * N + ypos, xup * N + ypos, xpos * N + yup, xup * N + yup
+                r = []
+                for q in range(10):
+                    F = T[q]
+                    x1 = (1 - fx) * F[p1] + fx * F[p2] if 0 < p1 < S and 0 < p2 < S else 0.0
+                    x2 = (1 - fx) * F[p3] + fx * F[p4] if 0 < p3 < S and 0 < p4 < S else 0.0
+                    r.append((1.0 - fy) * x1 + fy * x2)
+                if r[0] * 1.0 * HBARC > small:
+                    cols = [r[0] * 1.0 * HBARC, r[1] * 1.0 * HBARC, r[2] * 1.0 * HBARC,
+                            tau0 * tau0 * r[3] * 1.0 * HBARC, -r[4] * 1.0 * HBARC, -r[5] * 1.0 * HBARC,
+                            -tau0 * r[6] * 1.0 * HBARC, -r[7] * 1.0 * HBARC, -tau0 * r[9] * 1.0 * HBARC,
+                            -tau0 * r[8] * 1.0 * HBARC]
+                    line = f"{ix} {iy} " + " ".join("%g" % c for c in cols)
+            if line is None:
+                line = f"{ix} {iy} " + " ".join("%g" % c for c in [small, small / 2.0, small / 2.0] + [0.0] * 7)
+            out.append(line)
+        out.append("")
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.parametrize("N,L,hx,hL", [(8, 4.0, 12, 3.6), (8, 4.0, 5, 6.0), (12, 6.0, 12, 6.0)])
+def test_tmunu_file_matches_reference_loops(tmp_path, N, L, hx, hL):
+    rng = np.random.default_rng(N * hx)
+    a = L / N
+    x = -L / 2 + a * np.arange(N)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    T = rng.normal(size=(10, N * N))
+    T[0] = (20 * np.exp(-(X ** 2 + Y ** 2) / (0.1 * L * L))).reshape(-1)
+    T[0][T[0] < 1e-3] = 1e-17  # below the file's floor
+    f = tmp_path / "t.dat"
+    I.write_tmunu_file(f, T, L, 5, 0.1, hx, 1, hL, 1.0)
+    assert f.read_text() == _reference_tmunu_text(T, N, L, 5, 0.1, hx, 1, hL, 1.0)

@@ -274,6 +274,14 @@ int ipg_write_tmunu(ipg_ctx *c, const char *path, int it, int size_out, int eta_
 /* the same from a host T [10][N*N] (ipg_get_tmunu's layout) */
 int ipg_write_tmunu_file(const char *path, const double *T, int N, double L, int it, double dtau, int size_out,
                          int eta_size_out, double L_out, double deta_out);
+/* Jazma-Hydro-t<tau>-<id>.dat (MyEigen.cpp:1047-1293; (writeOutputs % 4) / 2
+ * == 1): g2mu2A g2mu2B interpolated onto the hydro grid and scaled to
+ * integrate to Etot (ipg_write_hydro's; the reference passes 0 when the hydro
+ * file is not written), in the hydro file's columns with u^tau = 1 */
+int ipg_write_jazma(ipg_ctx *c, const char *path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out, double Etot);
+int ipg_write_jazma_file(const char *path, const double *g2mu2A, const double *g2mu2B, int N, double L, int it,
+                         double dtau, int size_out, int eta_size_out, double L_out, double deta_out, double Etot);
 /* the same from a host Landau frame land [15][N*N] (ipg_get_landau's layout) */
 int ipg_write_hydro_file(const char *path, const double *land, int N, double L, int it, double dtau, int size_out,
                          int eta_size_out, double L_out, double deta_out, double *Etot);

@@ -124,6 +124,10 @@ def lib():
             "ipg_write_eccentricities": (C.c_int, [vp, C.c_char_p, C.c_int]),
             "ipg_write_hydro": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
             "ipg_write_tmunu": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+            "ipg_write_jazma": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.c_double]),
+            "ipg_write_jazma_file": (C.c_int, [C.c_char_p, dp, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                               C.c_int, C.c_double, C.c_double, C.c_double]),
             "ipg_write_tmunu_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                C.c_int, C.c_double, C.c_double]),
             "ipg_write_hydro_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
@@ -266,6 +270,11 @@ class Event:
     def write_tmunu(self, path, size_out, eta_size_out, L_out, deta_out, it=None):
         it = self.p.itmax if it is None else it
         _check(lib().ipg_write_tmunu(self._c, os.fsencode(path), it, size_out, eta_size_out, L_out, deta_out))
+
+    # Jazma-Hydro file (MyEigen.cpp:1047-1293), scaled to Etot
+    def write_jazma(self, path, size_out, eta_size_out, L_out, deta_out, Etot, it=None):
+        it = self.p.itmax if it is None else it
+        _check(lib().ipg_write_jazma(self._c, os.fsencode(path), it, size_out, eta_size_out, L_out, deta_out, Etot))
 
     def init_fields(self):
         _check(lib().ipg_init_fields(self._c))
@@ -492,6 +501,15 @@ def write_tmunu_file(path, T, L, it, dtau, size_out, eta_size_out, L_out, deta_o
     N = int(round(np.sqrt(T.shape[1])))
     _check(lib().ipg_write_tmunu_file(os.fsencode(path), _dptr(T), N, L, it, dtau, size_out, eta_size_out, L_out,
                                       deta_out))
+
+
+def write_jazma_file(path, g2mu2A, g2mu2B, L, it, dtau, size_out, eta_size_out, L_out, deta_out, Etot):
+    """Jazma-Hydro file from host g2mu2A, g2mu2B [N*N]."""
+    A = np.ascontiguousarray(g2mu2A, dtype=np.float64)
+    B = np.ascontiguousarray(g2mu2B, dtype=np.float64)
+    N = int(round(np.sqrt(A.size)))
+    _check(lib().ipg_write_jazma_file(os.fsencode(path), _dptr(A), _dptr(B), N, L, it, dtau, size_out, eta_size_out,
+                                      L_out, deta_out, Etot))
 
 
 def hydro_file_name(tau, event_id=0, final=False):

@@ -1659,6 +1659,27 @@ int ipg_write_tmunu(ipg_ctx* c, const char* path, int it, int size_out, int eta_
     if (rc) return rc;
     return ipg_write_tmunu_file(path, T.data(), p.size, p.L, it, p.dtau, size_out, eta_size_out, L_out, deta_out);
 }
+int ipg_write_jazma_file(const char* path, const double* g2mu2A, const double* g2mu2B, int N, double L, int it,
+                         double dtau, int size_out, int eta_size_out, double L_out, double deta_out, double Etot) {
+    if (!path || !g2mu2A || !g2mu2B || N < 2 || size_out < 1 || eta_size_out < 1 || !(L_out > 0.))
+        return fail(IPG_ERR_ARG, "null argument or empty lattice / hydro grid");
+    std::string err;
+    const int rc = ipg::write_jazma(path, g2mu2A, g2mu2B, N, L, it, dtau, size_out, eta_size_out, L_out, deta_out,
+                                    Etot, err);
+    return rc ? fail(rc, err) : IPG_OK;
+}
+int ipg_write_jazma(ipg_ctx* c, const char* path, int it, int size_out, int eta_size_out, double L_out,
+                    double deta_out, double Etot) {
+    if (!c || !path) return fail(IPG_ERR_ARG, "null argument");
+    const ipg_params& p = c->eng.ev[0].p;
+    const size_t S = (size_t)c->eng.desc[0].S;
+    std::vector<double> g(2 * S);
+    int rc = ipg_get_field(c, IPG_F_G2MU2A, g.data());
+    if (!rc) rc = ipg_get_field(c, IPG_F_G2MU2B, g.data() + S);
+    if (rc) return rc;
+    return ipg_write_jazma_file(path, g.data(), g.data() + S, p.size, p.L, it, p.dtau, size_out, eta_size_out, L_out,
+                                deta_out, Etot);
+}
 int ipg_write_hydro(ipg_ctx* c, const char* path, int it, int size_out, int eta_size_out, double L_out,
                     double deta_out, double* Etot) {
     if (!c || !path) return fail(IPG_ERR_ARG, "null argument");

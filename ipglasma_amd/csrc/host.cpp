@@ -871,4 +871,82 @@ int write_tmunu(const char* path, const double* T, int N, double L, int it, doub
     return IPG_OK;
 }
 
+
+// ---- Jazma-Hydro output (MyEigen.cpp:1047-1293) ---------------------------------
+//
+// Two passes over the hydro grid with the hydro file's interpolation of
+// |g2mu2A| and |g2mu2B|: the sum J of g2mu2A g2mu2B ha^2 tau over all eta
+// slices, then one line per cell, "eta x y (g2mu2A g2mu2B / J) Etot 1 0 ...",
+// the vacuum line outside the lattice, no empty lines.
+
+namespace {
+struct HydroCell {
+    bool inside = false, lo = false, hi = false;
+    double fx = 0., fy = 0.;
+    long long p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+};
+HydroCell hydro_cell(double x, double y, int N, double L, double ha) {
+    HydroCell c;
+    c.inside = std::abs(x) < L / 2. && std::abs(y) < L / 2.;
+    if (!c.inside) return c;
+    const double a = L / N;
+    const long long S = (long long)N * N;
+    const int xpos = static_cast<int>(std::floor((x + L / 2.) / a + 0.0000000001));
+    const int ypos = static_cast<int>(std::floor((y + L / 2.) / a + 0.0000000001));
+    const int xup = xpos < N - 1 ? xpos + 1 : xpos;
+    const int yup = ypos < N - 1 ? ypos + 1 : ypos;
+    c.fx = (x - (-L / 2. + a * xpos)) / ha;
+    c.fy = (y - (-L / 2. + a * ypos)) / ha;
+    c.p1 = (long long)xpos * N + ypos;
+    c.p2 = (long long)xup * N + ypos;
+    c.p3 = (long long)xpos * N + yup;
+    c.p4 = (long long)xup * N + yup;
+    c.lo = c.p1 > 0 && c.p1 < S && c.p2 > 0 && c.p2 < S;
+    c.hi = c.p3 > 0 && c.p3 < S && c.p4 > 0 && c.p4 < S;
+    return c;
+}
+double interp_abs(const HydroCell& c, const double* F) {
+    const double x1 = c.lo ? (1 - c.fx) * std::abs(F[c.p1]) + c.fx * std::abs(F[c.p2]) : 0.;
+    const double x2 = c.hi ? (1 - c.fx) * std::abs(F[c.p3]) + c.fx * std::abs(F[c.p4]) : 0.;
+    return (1. - c.fy) * x1 + c.fy * x2;
+}
+}  // namespace
+
+int write_jazma(const char* path, const double* g2A, const double* g2B, int N, double L, int it, double dtau, int hx,
+                int heta, double hL, double deta, double Etot, std::string& err) {
+    const double a = L / N;
+    const int hy = hx;
+    const double ha = hL / static_cast<double>(hx);
+    double J = 0.;
+    for (int ieta = 0; ieta < heta; ieta++)
+        for (int ix = 0; ix < hx; ix++)
+            for (int iy = 0; iy < hy; iy++) {
+                const HydroCell c = hydro_cell(-hL / 2. + ha * ix, -hL / 2. + ha * iy, N, L, ha);
+                if (c.inside) J += interp_abs(c, g2A) * interp_abs(c, g2B) * ha * ha * it * dtau * a;
+            }
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        err = std::string("cannot open ") + path + " for writing";
+        return IPG_ERR_IO;
+    }
+    bool ok = std::fprintf(f, "# dummy %d etamax= %d xmax= %d ymax= %d deta= %g dx= %g dy= %g\n", 1, heta, hx, hy,
+                           deta, ha, ha) > 0;
+    for (int ieta = 0; ieta < heta && ok; ieta++) {
+        const double eta = -(heta - 1) / 2. * deta + deta * ieta;
+        for (int ix = 0; ix < hx && ok; ix++)
+            for (int iy = 0; iy < hy && ok; iy++) {
+                const double x = -hL / 2. + ha * ix, y = -hL / 2. + ha * iy;
+                const HydroCell c = hydro_cell(x, y, N, L, ha);
+                const double e = c.inside ? interp_abs(c, g2A) * interp_abs(c, g2B) / J * Etot : 0.;
+                ok = std::fprintf(f, "%g %g %g %g 1 0 0 0 0 0 0 0 0 0 0 0 0 0\n", eta, x, y, e) > 0;
+            }
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        err = std::string("writing ") + path + " failed";
+        return IPG_ERR_IO;
+    }
+    return IPG_OK;
+}
+
 }  // namespace ipg

@@ -84,6 +84,13 @@ int write_hydro(const char* path, const double* land, int N, double L, int it, d
 int write_tmunu(const char* path, const double* T, int N, double L, int it, double dtau, int hx, int heta, double hL,
                 double deta, std::string& err);
 
+// Jazma-Hydro file (MyEigen.cpp:1047-1293; (writeOutputs % 4) / 2 == 1):
+// g2mu2A g2mu2B [N*N] interpolated onto the hydro grid, normalised so that
+// it integrates to Etot (the hydro file's sum, 0 when that file was not
+// written, as in the reference)
+int write_jazma(const char* path, const double* g2A, const double* g2B, int N, double L, int it, double dtau, int hx,
+                int heta, double hL, double deta, double Etot, std::string& err);
+
 // b from the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p);
 

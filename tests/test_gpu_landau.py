@@ -209,3 +209,25 @@ def test_hydro_file_of_an_event(tmp_path):
     assert t1.read_text() == t2.read_text()
     trows = np.array([r.split() for r in t1.read_text().splitlines()[1:] if r], dtype=float)
     assert trows.shape == (40 * 40, 12) and trows[:, 2].max() > 1e-16
+
+
+def test_jazma_file_of_an_event(tmp_path):
+    """ipg_write_jazma (MyEigen.cpp:1047-1293) on an evolved event: the same
+    text as the host writer on the event's g2mu2 fields, which the evolution
+    leaves as the colour charge set them; the eps column integrates to Etot."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=32, Ny=4)
+    ev = I.Event(p, seed=19)
+    assert ev.init()
+    gA, gB = ev.field("g2mu2A"), ev.field("g2mu2B")
+    ev.run()
+    assert np.array_equal(ev.field("g2mu2A"), gA) and np.array_equal(ev.field("g2mu2B"), gB)
+    etot = ev.write_hydro(tmp_path / "h.dat", 40, 1, 0.9 * p.L, 0.5)
+    f1, f2 = tmp_path / "j1.dat", tmp_path / "j2.dat"
+    ev.write_jazma(f1, 40, 1, 0.9 * p.L, 0.5, etot)
+    I.write_jazma_file(f2, gA, gB, p.L, p.itmax, p.dtau, 40, 1, 0.9 * p.L, 0.5, etot)
+    assert f1.read_text() == f2.read_text()
+    vals = np.array([r.split() for r in f1.read_text().splitlines()[1:]], dtype=float)
+    ha = 0.9 * p.L / 40
+    tau = p.itmax * p.dtau * p.L / p.size
+    assert vals[:, 3].sum() * ha * ha * tau == pytest.approx(etot, rel=1e-4)

@@ -153,3 +153,54 @@ def test_tmunu_file_matches_reference_loops(tmp_path, N, L, hx, hL):
     f = tmp_path / "t.dat"
     I.write_tmunu_file(f, T, L, 5, 0.1, hx, 1, hL, 1.0)
     assert f.read_text() == _reference_tmunu_text(T, N, L, 5, 0.1, hx, 1, hL, 1.0)
+
+
+def _reference_jazma_text(A, B, N, L, it, dtau, hx, heta, hL, deta, Etot):
+    S = N * N
+    a = L / N
+    ha = hL / float(hx)
+
+    def cell(x, y):
+        if not (abs(x) < L / 2.0 and abs(y) < L / 2.0):
+            return None
+        xpos = int(math.floor((x + L / 2.0) / a + 0.0000000001))
+        ypos = int(math.floor((y + L / 2.0) / a + 0.0000000001))
+        xup = xpos + 1 if xpos < N - 1 else xpos
+        yup = ypos + 1 if ypos < N - 1 else ypos
+        fx = (x - (-L / 2.0 + a * xpos)) / ha
+        fy = (y - (-L / 2.0 + a * ypos)) / ha
+        p1, p2, p3, p4 = xpos * N + ypos, xup * N + ypos, xpos * N + yup, xup * N + yup
+
+        def f(F):
+            x1 = (1 - fx) * abs(F[p1]) + fx * abs(F[p2]) if 0 < p1 < S and 0 < p2 < S else 0.0
+            x2 = (1 - fx) * abs(F[p3]) + fx * abs(F[p4]) if 0 < p3 < S and 0 < p4 < S else 0.0
+            return (1.0 - fy) * x1 + fy * x2
+        return f(A), f(B)
+
+    J = 0.0
+    for ieta in range(heta):
+        for ix in range(hx):
+            for iy in range(hx):
+                c = cell(-hL / 2.0 + ha * ix, -hL / 2.0 + ha * iy)
+                if c is not None:
+                    J += c[0] * c[1] * ha * ha * it * dtau * a
+    out = ["# dummy %d etamax= %d xmax= %d ymax= %d deta= %g dx= %g dy= %g" % (1, heta, hx, hx, deta, ha, ha)]
+    for ieta in range(heta):
+        eta = -(heta - 1) / 2.0 * deta + deta * ieta
+        for ix in range(hx):
+            for iy in range(hx):
+                x, y = -hL / 2.0 + ha * ix, -hL / 2.0 + ha * iy
+                c = cell(x, y)
+                e = 0.0 if c is None else c[0] * c[1] / J * Etot
+                out.append(" ".join("%g" % v for v in [eta, x, y, e, 1.0] + [0.0] * 13))
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.parametrize("N,L,hx,heta,hL", [(8, 4.0, 12, 2, 3.6), (8, 4.0, 5, 1, 6.0)])
+def test_jazma_file_matches_reference_loops(tmp_path, N, L, hx, heta, hL):
+    rng = np.random.default_rng(3 * N + hx)
+    A = np.abs(rng.normal(size=N * N))
+    B = -np.abs(rng.normal(size=N * N))  # |g2mu2| is what the file takes
+    f = tmp_path / "j.dat"
+    I.write_jazma_file(f, A, B, L, 6, 0.1, hx, heta, hL, 0.5, 123.4)
+    assert f.read_text() == _reference_jazma_text(A, B, N, L, 6, 0.1, hx, heta, hL, 0.5, 123.4)

@@ -59,6 +59,11 @@ typedef struct ipg_params {
     /* project options (not in the reference input; default 0) */
     int rho_rng; /* 0: device Philox counter map, 1: the reference's
                     MT19937-64 Gauss stream drawn on the host in its order */
+    /* the hydro output grid (main.cpp:477-487): cells per side, rapidity
+     * cells, side length [fm], rapidity step; used by the drop-in driver's
+     * writeOutputs files */
+    int sizeOutput, etaSizeOutput;
+    double LOutput, detaOutput;
 } ipg_params;
 
 typedef struct ipg_observables {

@@ -16,6 +16,7 @@
 // exit(1)) and that the event state lives on the GPU inside Lattice.
 #pragma once
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -215,6 +216,22 @@ class Evolution {
         ipg_observables o;
         check(ipg_get_observables(lat->ctx(), &o));
         param->setObservables(o);
+        const ipg_params& p = param->raw();
+        if (p.writeOutputs <= 0) return;
+        // MyEigen::flowVelocity4D's files at itmax (MyEigen.cpp:326-1293)
+        char tau[32];
+        std::snprintf(tau, sizeof tau, "%g", p.itmax * p.dtau * (p.L / p.size));
+        const std::string id = std::to_string(param->getEventId());
+        double etot = 0.;
+        if (p.writeOutputs % 2 == 1)
+            check(ipg_write_hydro(lat->ctx(), ("epsilon-u-Hydro-TauHydro-" + id + ".dat").c_str(), p.itmax,
+                                  p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput, &etot));
+        if (p.writeOutputs / 4 == 1)
+            check(ipg_write_tmunu(lat->ctx(), ("Tmunu-t" + std::string(tau) + "-" + id + ".dat").c_str(), p.itmax,
+                                  p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput));
+        if ((p.writeOutputs % 4) / 2 == 1)
+            check(ipg_write_jazma(lat->ctx(), ("Jazma-Hydro-t" + std::string(tau) + "-" + id + ".dat").c_str(),
+                                  p.itmax, p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput, etot));
     }
     // In place of run(): the loss at tau_max (IPG_LOSS_DEDETA, IPG_LOSS_ECC2,
     // or IPG_LOSS_TMUNU with ten weights) and its gradient with respect to

@@ -51,7 +51,8 @@ class Parameters(C.Structure):
                 [(n, C.c_int) for n in _INT_FIELDS_2] + [(n, C.c_double) for n in _DBL_FIELDS_2] +
                 [("seed", C.c_ulonglong), ("Projectile", C.c_char * 16), ("Target", C.c_char * 16),
                  ("NucleusQsTableFileName", C.c_char * 512), ("dtau", C.c_double), ("itmax", C.c_int),
-                 ("rho_rng", C.c_int)])
+                 ("rho_rng", C.c_int), ("sizeOutput", C.c_int), ("etaSizeOutput", C.c_int),
+                 ("LOutput", C.c_double), ("detaOutput", C.c_double)])
 
     def set(self, **kw):
         for k, v in kw.items():

@@ -382,6 +382,10 @@ void default_params(ipg_params* p) {
     p->BG = 4.;
     p->QsmuRatio = 0.8;
     p->rmax = 1e7;
+    p->LOutput = 30.;
+    p->sizeOutput = 512;
+    p->etaSizeOutput = 1;
+    p->detaOutput = 0.;
     p->roots = 7000;
     p->Rapidity = 0.;
     p->xFromThisFactorTimesQs = 1;
@@ -522,6 +526,10 @@ int read_input(const char* path, ipg_params* p, std::string& err) {
     }
     I("SubNucleonParamType", p->SubNucleonParamType);
     I("rho_rng", p->rho_rng);
+    I("sizeOutput", p->sizeOutput);
+    I("etaSizeOutput", p->etaSizeOutput);
+    D("LOutput", p->LOutput);
+    D("detaOutput", p->detaOutput);
     derive(p);
     return IPG_OK;
 }

@@ -70,6 +70,32 @@ int main(int argc, char** argv) {
             ipg_destroy(c);
             return 1;
         }
+        if (p.writeOutputs > 0) {
+            // MyEigen::flowVelocity4D's files at itmax (finalFlag set,
+            // MyEigen.cpp:326-1293; Evolution.cpp:406-409)
+            const int it = p.itmax;
+            const double tau = it * p.dtau * (p.L / p.size);
+            char name[160];
+            double etot = 0.;
+            if (rc == IPG_OK && p.writeOutputs % 2 == 1) {
+                std::snprintf(name, sizeof name, "epsilon-u-Hydro-TauHydro-%d.dat", iev);
+                rc = ipg_write_hydro(c, name, it, p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput, &etot);
+                if (rc == IPG_OK) std::printf("Etot = %g GeV \n", etot);
+            }
+            if (rc == IPG_OK && p.writeOutputs / 4 == 1) {
+                std::snprintf(name, sizeof name, "Tmunu-t%g-%d.dat", tau, iev);
+                rc = ipg_write_tmunu(c, name, it, p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput);
+            }
+            if (rc == IPG_OK && (p.writeOutputs % 4) / 2 == 1) {
+                std::snprintf(name, sizeof name, "Jazma-Hydro-t%g-%d.dat", tau, iev);
+                rc = ipg_write_jazma(c, name, it, p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput, etot);
+            }
+            if (rc != IPG_OK) {
+                std::fprintf(stderr, "event %d: %s\n", iev, ipg_last_error());
+                ipg_destroy(c);
+                return 1;
+            }
+        }
         ipg_observables o;
         ipg_get_observables(c, &o);
         std::printf("event %d: b=%.6g fm Npart=%d Ncoll=%d <Qs>=%.6g GeV tau=%.6g fm dE/deta=%.12g GeV "

@@ -436,3 +436,32 @@ def test_driver_writes_then_reads_wilson_lines(tmp_path):
     assert (tmp_path / "V-9").exists()
     d_read = run({"readInitialWilsonLines": 2})
     assert d_read == pytest.approx(d_write, rel=1e-11)
+
+
+def test_driver_writes_output_files(tmp_path):
+    """writeOutputs 7 in the drop-in driver: the epsilon-u-Hydro-TauHydro,
+    Tmunu-t<tau> and Jazma-Hydro-t<tau> files of MyEigen::flowVelocity4D at
+    itmax (MyEigen.cpp:326-1293), on the sizeOutput x sizeOutput grid, and the
+    printed Etot, which the Jazma file integrates to."""
+    import os
+    import re
+    import subprocess
+
+    import numpy as np
+    import ipglasma_amd as I
+    exe = os.path.join(os.path.dirname(I.LIB_PATH), "ipglasma")
+    keys = {"size": 32, "L": 30.0, "Ny": 4, "g": 2.0, "m": 0.2, "BG": 4.0, "QsmuRatio": 0.8, "roots": 5020.0,
+            "SigmaNN": 70.0, "maxtime": 0.4, "Projectile": "Pb", "Target": "Pb", "setWSDeformParams": 1,
+            "R_WS": 6.62, "a_WS": 0.546, "bmin": 0.0, "bmax": 0.0, "useTimeForSeed": 0, "seed": 5,
+            "NucleusQsTableFileName": I.QS_TABLE, "writeOutputs": 7, "sizeOutput": 40, "LOutput": 27.0,
+            "etaSizeOutput": 1, "detaOutput": 0.1}
+    (tmp_path / "input").write_text("".join(f"{k} {v}\n" for k, v in keys.items()))
+    r = subprocess.run([exe, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    etot = float(re.search(r"Etot = (\S+) GeV", r.stdout).group(1))
+    names = sorted(x.name for x in tmp_path.iterdir() if x.suffix == ".dat")
+    assert any(n.startswith("Jazma-Hydro-t") for n in names) and any(n.startswith("Tmunu-t") for n in names)
+    assert "epsilon-u-Hydro-TauHydro-0.dat" in names
+    hyd = np.array([ln.split() for ln in (tmp_path / "epsilon-u-Hydro-TauHydro-0.dat").read_text().splitlines()[1:]
+                    if ln], dtype=float)
+    assert hyd.shape == (40 * 40, 18) and hyd[:, 3].max() > 0 and etot > 0

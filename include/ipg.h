@@ -114,6 +114,11 @@ int ipg_read_input(const char *path, ipg_params *p);
 void ipg_derive(ipg_params *p);
 const char *ipg_last_error(void);
 int ipg_version(void);
+/* Per-event device memory is kept, per device and size, when a context or
+ * batch is destroyed (up to a quarter of the device's memory) and handed,
+ * cleared, to the next one that needs that size.  This frees the kept
+ * blocks.  No counterpart in the reference. */
+int ipg_release_cached_memory(void);
 /* sizeof(ipg_params) (which=0) or sizeof(ipg_observables) (which=1), so that
  * bindings can check their struct layout */
 int ipg_struct_size(int which);

@@ -108,6 +108,7 @@ def lib():
             "ipg_derive": (None, [C.POINTER(Parameters)]),
             "ipg_last_error": (C.c_char_p, []),
             "ipg_version": (C.c_int, []),
+            "ipg_release_cached_memory": (C.c_int, []),
             "ipg_struct_size": (C.c_int, [C.c_int]),
             "ipg_create": (vp, [C.POINTER(Parameters)]),
             "ipg_destroy": (None, [vp]),

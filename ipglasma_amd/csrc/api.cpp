@@ -156,6 +156,54 @@ struct TimedLaunch {
 };
 
 void hipFreeVoid(double* p) { (void)hipFree(p); }
+
+// Per-event device blocks outlive their context: a released block is kept,
+// per device and byte size, for the next context that needs the same size,
+// up to a quarter of the device's memory.  Allocating and freeing a few GB
+// costs milliseconds per context, more than the kernels of a small event.  A
+// reused block is cleared first, so a context sees what a fresh allocation
+// gives it.  ipg_release_cached_memory hands the kept blocks back.
+struct BlockCache {
+    std::mutex m;
+    std::multimap<std::pair<int, size_t>, void*> blocks;
+    size_t bytes = 0;
+};
+BlockCache& block_cache() {
+    static BlockCache c;
+    return c;
+}
+hipError_t block_alloc(void** p, size_t n) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        auto it = c.blocks.find({dev, n});
+        if (it != c.blocks.end()) {
+            *p = it->second;
+            c.blocks.erase(it);
+            c.bytes -= n;
+            return hipMemset(*p, 0, n);
+        }
+    }
+    return hipMalloc(p, n);
+}
+void block_release(void* p, size_t n) {
+    if (!p) return;
+    int dev = 0;
+    size_t fr = 0, tot = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        if (c.bytes + n <= tot / 4) {
+            c.blocks.insert({{dev, n}, p});
+            c.bytes += n;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
 // [18][S] planes -> [S][18]
 __global__ void k_soa_to_aos_ext(const double* __restrict__ f, long long S, double* __restrict__ out) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,6 +228,7 @@ struct Event {
     ipg::EventInfo info;
     ipg_observables obs;
     double* mem = nullptr;   // one device allocation per event
+    size_t mem_bytes = 0;
     double* d_nuc = nullptr;
     size_t nuc_cap = 0;      // doubles allocated at d_nuc
     double* d_z = nullptr;   // injected or MT normals
@@ -277,12 +326,15 @@ struct Engine {
     }
 
     ~Engine() {
+        // the per-event blocks go back to the cache, not through hipFree
+        // (which would wait): finish this context's work first
+        if (st) (void)hipStreamSynchronize(st);
         for (int k = 0; k < 3; k++) {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
         }
         for (Event& e : ev) {
-            if (e.mem) (void)hipFree(e.mem);
+            if (e.mem) block_release(e.mem, e.mem_bytes);
             if (e.d_nuc) (void)hipFree(e.d_nuc);
             if (e.d_z) (void)hipFree(e.d_z);
         }
@@ -377,7 +429,8 @@ struct Engine {
             // partials, int planes (2 + 1)
             const size_t nd = (size_t)S * (72 + 45 + 5 + nfft + 10 + 16) + (size_t)32 * d.nblocks +
                               (size_t)S * 2 + ((size_t)S + 1) / 2;
-            HIPCHK(hipMalloc(&E.mem, nd * sizeof(double)));
+            HIPCHK(block_alloc((void**)&E.mem, nd * sizeof(double)));
+            E.mem_bytes = nd * sizeof(double);
             double* q = E.mem;
             auto take = [&](size_t n) {
                 double* r = q;
@@ -1380,6 +1433,20 @@ double* field_ptr(Engine& g, int which, int* kind) {
 extern "C" {
 
 const char* ipg_last_error(void) { return g_err.c_str(); }
+int ipg_release_cached_memory(void) {
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.m);
+    int dev = 0;
+    const bool have = hipGetDevice(&dev) == hipSuccess;
+    for (auto& kv : c.blocks) {
+        if (have) (void)hipSetDevice(kv.first.first);
+        (void)hipFree(kv.second);
+    }
+    if (have) (void)hipSetDevice(dev);
+    c.blocks.clear();
+    c.bytes = 0;
+    return IPG_OK;
+}
 int ipg_version(void) { return 100; }
 int ipg_struct_size(int which) { return which == 0 ? (int)sizeof(ipg_params) : (int)sizeof(ipg_observables); }
 void ipg_default_params(ipg_params* p) { ipg::default_params(p); }

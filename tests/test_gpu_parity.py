@@ -465,3 +465,32 @@ def test_driver_writes_output_files(tmp_path):
     hyd = np.array([ln.split() for ln in (tmp_path / "epsilon-u-Hydro-TauHydro-0.dat").read_text().splitlines()[1:]
                     if ln], dtype=float)
     assert hyd.shape == (40 * 40, 18) and hyd[:, 3].max() > 0 and etot > 0
+
+
+def test_reused_event_memory_gives_identical_results():
+    """A destroyed context's device block is kept and handed, cleared, to the
+    next context of the same size: an event run on a block that another event
+    used before reproduces a fresh run exactly (T^{mu nu}, dE/deta), also
+    in a batch; ipg_release_cached_memory returns the blocks."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=32, Ny=4)
+
+    def one(seed):
+        ev = I.Event(p, seed=seed)
+        assert ev.init()
+        ev.run()
+        out = ev.tmunu(), ev.observables()["dEdeta"]
+        ev.close()
+        return out
+
+    assert I.lib().ipg_release_cached_memory() == 0
+    T1, d1 = one(31)           # fresh block
+    one(32)                    # reuses it, leaves other values behind
+    T2, d2 = one(31)           # reuses it again
+    assert np.array_equal(T1, T2) and d1 == d2
+    b = I.Batch([p] * 3, [31, 33, 31])
+    b.run()
+    o = [b.observables(e) for e in range(3)]
+    b.close()
+    assert o[0]["dEdeta"] == d1 and o[2]["dEdeta"] == d1
+    assert I.lib().ipg_release_cached_memory() == 0

@@ -349,7 +349,7 @@ struct Engine {
         if (d_work) (void)hipFree(d_work);
         if (d_out) (void)hipFree(d_out);
         if (d_obs) (void)hipFree(d_obs);
-        if (adj_ws) (void)hipFree(adj_ws);
+        if (adj_ws) block_release(adj_ws, sizeof(double) * adj_ws_n);
         if (d_rdesc) (void)hipFree(d_rdesc);
         for (hipEvent_t t : tev) (void)hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
@@ -1085,12 +1085,16 @@ struct Engine {
     // grows the workspace to at least n doubles (contents not kept)
     int adj_reserve(size_t n) {
         if (n <= adj_ws_n) return IPG_OK;
-        if (adj_ws) HIPCHK(hipFree(adj_ws));
+        if (adj_ws) {
+            HIPCHK(hipStreamSynchronize(st));
+            block_release(adj_ws, sizeof(double) * adj_ws_n);
+        }
         adj_ws = nullptr;
         adj_ws_n = 0;
         rho_grad = nullptr;
         rho_grad_n = 0;
-        HIPCHK(hipMalloc(&adj_ws, sizeof(double) * n));
+        // through the block cache, like the per-event blocks
+        HIPCHK(block_alloc((void**)&adj_ws, sizeof(double) * n));
         adj_ws_n = n;
         return IPG_OK;
     }

@@ -168,9 +168,11 @@ struct BlockCache {
     std::multimap<std::pair<int, size_t>, void*> blocks;
     size_t bytes = 0;
 };
+// never destroyed: a context released during process teardown, after this
+// library's static destructors, still finds a live cache
 BlockCache& block_cache() {
-    static BlockCache c;
-    return c;
+    static BlockCache* c = new BlockCache;
+    return *c;
 }
 hipError_t block_alloc(void** p, size_t n) {
     int dev = 0;

@@ -143,33 +143,40 @@ struct Lat {
 };
 
 // ---- state copies between the live buffers and a slot -----------------------------
+// A state slot is 72 planes of S doubles: U_x, U_y (18 each), E1, E2, phi,
+// pi (9 each).  One thread per double; S is a multiple of the block size
+// (N >= 16), so each block stays inside one plane.
+__device__ __forceinline__ double* state_plane(const EvDesc& ev, int parity, int q, int* k) {
+    if (q < 18) return *k = q, ev.U[parity][0];
+    if (q < 36) return *k = q - 18, ev.U[parity][1];
+    if (q < 45) return *k = q - 36, ev.E1;
+    if (q < 54) return *k = q - 45, ev.E2;
+    if (q < 63) return *k = q - 54, ev.phi[parity];
+    return *k = q - 63, ev.pi;
+}
 __global__ __launch_bounds__(IPG_BLOCK) void k_state_save(const EvDesc* __restrict__ evs, int parity,
                                                          double* __restrict__ slot) {
     const EvDesc& ev = evs[0];
     const long long S = ev.S;
     const long long t = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
-    if (t >= S) return;
-    const double* src[6] = {ev.U[parity][0], ev.U[parity][1], ev.E1, ev.E2, ev.phi[parity], ev.pi};
-    const int np[6] = {18, 18, 9, 9, 9, 9};
-    int o = 0;
-    for (int f = 0; f < 6; f++) {
-        for (int k = 0; k < np[f]; k++) slot[(size_t)(o + k) * S + t] = src[f][(size_t)k * S + t];
-        o += np[f];
-    }
+    if (t >= 72 * S) return;
+    const int q = (int)(t / S);
+    const long long s = t - (long long)q * S;
+    int k = 0;
+    const double* src = state_plane(ev, parity, q, &k);
+    slot[t] = src[(size_t)k * S + s];
 }
 __global__ __launch_bounds__(IPG_BLOCK) void k_state_load(const EvDesc* __restrict__ evs, int parity,
                                                          const double* __restrict__ slot) {
     const EvDesc& ev = evs[0];
     const long long S = ev.S;
     const long long t = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
-    if (t >= S) return;
-    double* dst[6] = {ev.U[parity][0], ev.U[parity][1], ev.E1, ev.E2, ev.phi[parity], ev.pi};
-    const int np[6] = {18, 18, 9, 9, 9, 9};
-    int o = 0;
-    for (int f = 0; f < 6; f++) {
-        for (int k = 0; k < np[f]; k++) dst[f][(size_t)k * S + t] = slot[(size_t)(o + k) * S + t];
-        o += np[f];
-    }
+    if (t >= 72 * S) return;
+    const int q = (int)(t / S);
+    const long long s = t - (long long)q * S;
+    int k = 0;
+    double* dst = state_plane(ev, parity, q, &k);
+    dst[(size_t)k * S + s] = slot[t];
 }
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_zero(double* __restrict__ p, long long n) {
@@ -817,10 +824,10 @@ void ipg_adj_links_newton(const EvDesc* d, long long S, const double* s0, const 
 static unsigned nb_of(long long n) { return (unsigned)((n + IPG_BLOCK - 1) / IPG_BLOCK); }
 
 void ipg_adj_state_save(const EvDesc* d, long long S, int parity, double* slot, hipStream_t st) {
-    hipLaunchKernelGGL(k_state_save, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, parity, slot);
+    hipLaunchKernelGGL(k_state_save, dim3(nb_of(72 * S)), dim3(IPG_BLOCK), 0, st, d, parity, slot);
 }
 void ipg_adj_state_load(const EvDesc* d, long long S, int parity, const double* slot, hipStream_t st) {
-    hipLaunchKernelGGL(k_state_load, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, parity, slot);
+    hipLaunchKernelGGL(k_state_load, dim3(nb_of(72 * S)), dim3(IPG_BLOCK), 0, st, d, parity, slot);
 }
 void ipg_adj_zero(double* p, long long n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(nb_of(n)), dim3(IPG_BLOCK), 0, st, p, n);

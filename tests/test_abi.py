@@ -59,7 +59,9 @@ def test_version():
 
 def test_cxx_call_surface_compiles_and_links(tmp_path):
     """A driver written against the reference's class names (main.cpp:167-360)
-    compiles against include/ipglasma.hpp and links to the library."""
+    compiles against include/ipglasma.hpp and links to the library, with the
+    Wilson-line files (READFROMFILE, Init::readV), the per-event id and MPI
+    size that name them, and the writeOutputs files of Evolution::run."""
     src = tmp_path / "drv.cpp"
     src.write_text('''
 #include "ipglasma.hpp"
@@ -67,15 +69,22 @@ using namespace ipglasma;
 int main(int argc, char** argv) {
     Parameters param; Setup setup;
     if (argc > 1) setup.read(argv[1], &param);
+    param.setMPISize(1);
     Random random; random.init_genrand64(param.getSeed());
     int nn[2] = {param.getSize(), param.getSize()};
     Init init(nn); Group group(param.getNc()); Glauber glauber;
     Evolution evolution(nn);
-    Lattice lat(&param, param.getNc(), param.getSize());
-    while (param.getSuccess() == 0) {
-        init.init(&lat, &group, &param, &random, &glauber, 0);
-        if (param.getSuccess() == 0) continue;
-        evolution.run(&lat, &group, &param);
+    for (int iev = 0; iev < 2; iev++) {
+        param.setEventId(iev);
+        param.setSuccess(0);
+        Lattice lat(&param, param.getNc(), param.getSize());
+        const int READFROMFILE = param.getReadInitialWilsonLines();
+        while (param.getSuccess() == 0) {
+            init.init(&lat, &group, &param, &random, &glauber, READFROMFILE);
+            if (param.getSuccess() == 0) continue;
+            evolution.run(&lat, &group, &param);
+        }
+        if (Init::vname(&param, 1, 2) != "V-" + std::to_string(iev + 1 + 2 * param.getSeed())) return 2;
     }
     return 0;
 }

@@ -494,3 +494,60 @@ def test_reused_event_memory_gives_identical_results():
     b.close()
     assert o[0]["dEdeta"] == d1 and o[2]["dEdeta"] == d1
     assert I.lib().ipg_release_cached_memory() == 0
+
+
+def test_cxx_surface_writes_then_reads_wilson_lines(tmp_path):
+    """The C++ call surface (include/ipglasma.hpp) run as the reference's
+    main runs it: with writeInitialWilsonLines 2 Init::init writes
+    V-<eventId + 2 seed> and V-<eventId + 2 seed + 1>; with
+    readInitialWilsonLines 2 (READFROMFILE) the same event starts from them
+    and Evolution::run gives the same dE/deta (b = 0)."""
+    import os
+    import re
+    import subprocess
+
+    import ipglasma_amd as I
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "drv.cpp"
+    src.write_text('''
+#include <cstdio>
+#include "ipglasma.hpp"
+using namespace ipglasma;
+int main(int argc, char** argv) {
+    Parameters param; Setup setup;
+    setup.read(argv[1], &param);
+    param.setMPISize(1);
+    param.setEventId(3);
+    Random random; random.init_genrand64(param.getSeed());
+    int nn[2] = {param.getSize(), param.getSize()};
+    Init init(nn); Group group(param.getNc()); Glauber glauber;
+    Evolution evolution(nn);
+    Lattice lat(&param, param.getNc(), param.getSize());
+    while (param.getSuccess() == 0) {
+        init.init(&lat, &group, &param, &random, &glauber, param.getReadInitialWilsonLines());
+        if (param.getSuccess() == 0) continue;
+        evolution.run(&lat, &group, &param);
+    }
+    std::printf("dEdeta %.17g\\n", param.observables().dEdeta);
+    return 0;
+}
+''')
+    libdir = os.path.dirname(I.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(root, "include"), str(src), "-o", str(tmp_path / "drv"),
+                    "-L", libdir, "-lipglasma_amd", "-Wl,-rpath," + libdir], check=True)
+    keys = {"size": 32, "L": 30.0, "Ny": 4, "g": 2.0, "m": 0.2, "BG": 4.0, "QsmuRatio": 0.8, "roots": 5020.0,
+            "SigmaNN": 70.0, "maxtime": 0.4, "Projectile": "Pb", "Target": "Pb", "setWSDeformParams": 1,
+            "R_WS": 6.62, "a_WS": 0.546, "bmin": 0.0, "bmax": 0.0, "useTimeForSeed": 0, "seed": 6,
+            "NucleusQsTableFileName": I.QS_TABLE}
+
+    def run(extra):
+        (tmp_path / "input").write_text("".join(f"{k} {v}\n" for k, v in {**keys, **extra}.items()))
+        r = subprocess.run([str(tmp_path / "drv"), "input"], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        return float(re.search(r"dEdeta (\S+)", r.stdout).group(1))
+
+    d_write = run({"writeInitialWilsonLines": 2})
+    assert (tmp_path / "V-15").exists() and (tmp_path / "V-16").exists()
+    d_read = run({"readInitialWilsonLines": 2})
+    assert d_read == pytest.approx(d_write, rel=1e-14)

@@ -217,6 +217,11 @@ class Evolution {
         check(ipg_get_observables(lat->ctx(), &o));
         param->setObservables(o);
         const ipg_params& p = param->raw();
+        // Evolution.cpp:644-651 at itmax: the eccentricity line (no gluon
+        // spectra, no it = 1 line)
+        if (p.computeGluonMultiplicity == 1)
+            check(ipg_write_eccentricities(
+                lat->ctx(), ("eccentricities" + std::to_string(param->getEventId()) + ".dat").c_str(), p.itmax));
         if (p.writeOutputs <= 0) return;
         // MyEigen::flowVelocity4D's files at itmax (MyEigen.cpp:326-1293)
         char tau[32];

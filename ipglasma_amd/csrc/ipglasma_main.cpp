@@ -70,6 +70,22 @@ int main(int argc, char** argv) {
             ipg_destroy(c);
             return 1;
         }
+        if (p.computeGluonMultiplicity == 1) {
+            // Evolution.cpp:644-651 at itmax: the eccentricity line (the
+            // gauge-fixed spectra, and the it = 1 line before any Landau
+            // frame exists, are not produced)
+            if (iev == 0)
+                std::printf("computeGluonMultiplicity: writing eccentricities<id>.dat at itmax; gluon spectra "
+                            "(GaugeFix, multiplicity) are not computed\n");
+            char ename[64];
+            std::snprintf(ename, sizeof ename, "eccentricities%d.dat", iev);
+            rc = ipg_write_eccentricities(c, ename, p.itmax);
+            if (rc != IPG_OK) {
+                std::fprintf(stderr, "event %d: %s\n", iev, ipg_last_error());
+                ipg_destroy(c);
+                return 1;
+            }
+        }
         if (p.writeOutputs > 0) {
             // MyEigen::flowVelocity4D's files at itmax (finalFlag set,
             // MyEigen.cpp:326-1293; Evolution.cpp:406-409)

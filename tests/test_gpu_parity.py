@@ -462,6 +462,9 @@ def test_driver_writes_output_files(tmp_path):
     names = sorted(x.name for x in tmp_path.iterdir() if x.suffix == ".dat")
     assert any(n.startswith("Jazma-Hydro-t") for n in names) and any(n.startswith("Tmunu-t") for n in names)
     assert "epsilon-u-Hydro-TauHydro-0.dat" in names
+    # computeGluonMultiplicity (default 1): the itmax line of eccentricities0.dat
+    ecc = (tmp_path / "eccentricities0.dat").read_text().splitlines()
+    assert len(ecc) == 1 and len(ecc[0].split()) == 23
     hyd = np.array([ln.split() for ln in (tmp_path / "epsilon-u-Hydro-TauHydro-0.dat").read_text().splitlines()[1:]
                     if ln], dtype=float)
     assert hyd.shape == (40 * 40, 18) and hyd[:, 3].max() > 0 and etot > 0

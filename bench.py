@@ -18,8 +18,9 @@ Printed JSON fields beyond metric, value, unit, steps, config and the timing key
   small_events        64 p+Pb events at N=128, batched against plain: where one
                       event leaves most of the GPU idle and the plain path is
                       bound by launch latency
-  cpu_baseline        the C oracle (OpenMP) on a bounded sample of the same
-                      event, extrapolated to one event
+  cpu_baseline        the C oracle (OpenMP, the host's threads) on whole
+                      events per phase, median of 3, at Pb+Pb N=512 (the
+                      value) and Au+Au N=256, with the CPU model
 """
 import argparse
 import json
@@ -35,6 +36,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_SITE_STEP = 1728
+UNIQUE_BYTES_PER_SITE_STEP = 1152
 
 
 def pbpb_params(I, N=512, L=30.0, Ny=50, seed=1):
@@ -187,10 +189,21 @@ def roofline(I, p):
     S = p.size * p.size
     avg_ms = t["step_ms_total"] / max(1, t["nsteps"])
     achieved = BYTES_PER_SITE_STEP * S / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(p.size),
-            "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5), "launches": t["nsteps"],
-            "bytes_per_launch": BYTES_PER_SITE_STEP * S}
+    traffic = measured_traffic(p.size)
+    # the same launch against the bytes the Hermitian storage of E, phi, pi
+    # actually moves (U: 2 x 144 B, E1, E2, phi, pi: 4 x 72 B, read + write)
+    unique = UNIQUE_BYTES_PER_SITE_STEP * S / (avg_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "kernel": "k_step", "avg_launch_ms": round(avg_ms, 5), "launches": t["nsteps"],
+           "bytes_per_launch": BYTES_PER_SITE_STEP * S,
+           "unique_bytes_per_launch": UNIQUE_BYTES_PER_SITE_STEP * S,
+           "achieved_unique": round(unique, 1), "frac_unique": round(unique / HBM_PEAK_GBS, 4)}
+    if traffic:
+        counted = traffic / (avg_ms * 1e-3) / 1e9
+        out["counter_GBps"] = round(counted, 1)
+        out["frac_counter"] = round(counted / HBM_PEAK_GBS, 4)
+    return out
 
 
 def copy_probe(mib):
@@ -212,65 +225,118 @@ def copy_probe(mib):
     print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p):
-    """Oracle timing on a bounded sample of the event, extrapolated."""
+def cpu_model():
+    """CPU model and logical CPU count of this host (lscpu)"""
+    import subprocess
+    model, ncpu = "unknown", os.cpu_count() or 1
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                model = v.strip()
+    except Exception:
+        pass
+    return model, ncpu
+
+
+def oracle_event_phases(p, seed, tab, serial=()):
+    """One full event of the C oracle, timed per phase: Glauber + g^2 mu^2 +
+    statistics, the colour-charge normals (the reference's serial MT19937-64
+    Gauss stream), setV (FFT Poisson solves and the Wilson-line products),
+    the tau = 0 links + root finder + pi, the leapfrog, T^{mu nu} + dE/deta +
+    Gauss law.  Phases named in `serial` run on one thread."""
     import ctypes as C
-    import numpy as np
     import _cases as K
     import _oracle as O
-    import ipglasma_amd as I
+    L = O.lib()
+    L.orc_set_threads.argtypes = [C.c_int]
+    nthr = L.orc_get_threads()
     N, S = p.size, p.size ** 2
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    tab = K.load_table(I.QS_TABLE)
+
+    def threads(phase):
+        L.orc_set_threads(1 if phase in serial else nthr)
+
+    t = {}
+    threads("glauber")
     t0 = time.perf_counter()
-    st, nA, nB, info = K.oracle_event(p, 3)
+    st, nA, nB, info = K.oracle_event(p, seed)
     TpA, TpB, gA, gB = K.oracle_color_charge(p, nA, nB, tab)
-    t_cc = time.perf_counter() - t0
-    ny_sample = 1
+    threads("stats")
+    out = np.zeros(8)
+    qc = K.qs_cfg(p)
+    L.orc_stats(C.byref(qc), p.SigmaNN, info.Npart, p.minimumQs2ST, nA, len(nA), nB, len(nB), gA, gB, TpA, TpB, out)
+    t["glauber_g2mu2_stats"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    z = np.empty(2 * p.Ny * S * 8)
+    L.orc_mt_fill(C.byref(st), 4, z.size, z)
+    t["rho_normals"] = time.perf_counter() - t0
+    threads("setv")
     V = np.zeros((2, S, 18))
     m_lat = p.m * (p.L / N) / 0.1973269718
     t0 = time.perf_counter()
-    O.lib().orc_setv(N, ny_sample, p.g, m_lat, 0.0, gA * (ny_sample / p.Ny), gB * (ny_sample / p.Ny),
-                     None, None, 99, V.reshape(-1))
-    t_setv = (time.perf_counter() - t0) * p.Ny / ny_sample
-    # root finder on a sample of links (single thread), scaled to 2S links
-    # over the OpenMP thread count the full oracle uses
-    VA, VB = O.from_c(V[0]), O.from_c(V[1])
-    sample = list(range(N * (N // 2), N * (N // 2) + 4 * N, 3))
-    out18 = np.zeros(18)
+    L.orc_setv(N, p.Ny, p.g, m_lat, 0.0, gA, gB, z.ctypes.data, None, 0, V.reshape(-1))
+    t["setv"] = time.perf_counter() - t0
+    del z
+    threads("newton")
+    ref = {k: np.zeros((S, 18)) for k in O.FIELDS}
+    stats = np.zeros(5)
     t0 = time.perf_counter()
-    for s in sample:
-        i, j = divmod(s, N)
-        px = min(i + 1, N - 1) * N + j
-        U1 = O.to_c(VA[s] @ VA[px].conj().T)
-        U2 = O.to_c(VB[s] @ VB[px].conj().T)
-        O.lib().orc_newton_link(U1, U2, 7, 2 * s, out18, None)
-    t_newton = (time.perf_counter() - t0) * (2 * S) / len(sample) / max(1, threads)
-    # leapfrog: 2 full steps on random-free fields
-    st2 = {k: np.zeros((S, 18)) for k in O.FIELDS}
-    one = np.zeros(18)
-    one[0] = one[8] = one[16] = 1.0
-    st2["Ux"][:] = one
-    st2["Uy"][:] = one
-    t0 = time.perf_counter()
-    nst = 2
-    for it in range(1, nst + 1):
-        O.lib().orc_evolve_pi(N, p.dtau, it * p.dtau, st2["Ux"], st2["Uy"], st2["phi"], st2["pi"])
-        O.lib().orc_evolve_e(N, p.g, p.dtau, it * p.dtau, st2["Ux"], st2["Uy"], st2["phi"], st2["E1"], st2["E2"])
-        O.lib().orc_evolve_phi(N, p.dtau, it * p.dtau, st2["pi"], st2["phi"])
-        O.lib().orc_evolve_u(N, p.g, p.dtau, it * p.dtau, st2["E1"], st2["E2"], st2["Ux"], st2["Uy"])
-    t_step = (time.perf_counter() - t0) / nst
+    L.orc_init_fields(N, p.g, V[0], V[1], 17, *[ref[k] for k in O.FIELDS], stats)
+    t["links_newton_pi"] = time.perf_counter() - t0
+    del V
+    # T^{mu nu} alone on the tau = 0 state, then the whole Evolution::run
+    # (which evaluates it once at itmax): evolution = run - T^{mu nu}
+    threads("tmunu")
     T = np.zeros((10, S))
     t0 = time.perf_counter()
-    O.lib().orc_tmunu(N, p.g, p.L / N, p.dtau, p.itmax, *[st2[k] for k in O.FIELDS], T.reshape(-1))
+    L.orc_tmunu(N, p.g, p.L / N, p.dtau, p.itmax, *[ref[k] for k in O.FIELDS], T.reshape(-1))
     t_tmunu = time.perf_counter() - t0
-    total = t_cc + t_setv + t_newton + p.itmax * t_step + t_tmunu
+    threads("evolve")
+    obs = np.zeros(3)
+    t0 = time.perf_counter()
+    L.orc_run(N, p.g, p.L, p.dtau, p.itmax, *[ref[k] for k in O.FIELDS], T.reshape(-1), obs)
+    t_run = time.perf_counter() - t0
+    t["leapfrog"] = max(0.0, t_run - t_tmunu)
+    t["tmunu_dedeta_gauss"] = t_tmunu
+    L.orc_set_threads(nthr)
+    t["total"] = sum(v for k, v in t.items())
+    return t, obs[0]
+
+
+def cpu_baseline(p, reps=3):
+    """The C oracle (OpenMP, all host threads) on whole events: Au+Au b=5 fm
+    at N=256 and Pb+Pb b=0 at N=512 (the bench's own workload), Ny=50,
+    tau_max=0.4 fm/c, `reps` events each, median per phase; and one Au+Au
+    event with the phases the reference runs on one thread (the rho draws,
+    the FFTs of setV, T^{mu nu} and the statistics loop) on one thread."""
+    import statistics
+    import _cases as K
+    import ipglasma_amd as I
+    tab = K.load_table(I.QS_TABLE)
+    model, ncpu = cpu_model()
+    import _oracle as O
+    threads = O.lib().orc_get_threads()
+    res = {}
+    for name, q in (("auau_n256", K.params("AuAu", N=256, Ny=50)), ("pbpb_n512", p)):
+        runs = []
+        for r in range(reps):
+            ph, dedeta = oracle_event_phases(q, 100 + r, tab)
+            runs.append(ph)
+            print(f"[bench] cpu oracle {name} event {r}: {ph['total']:.2f} s", file=sys.stderr, flush=True)
+        res[name] = {k: round(statistics.median(x[k] for x in runs), 3) for k in runs[0]}
+    ser, _ = oracle_event_phases(K.params("AuAu", N=256, Ny=50), 100, tab, serial=("setv", "tmunu", "stats"))
+    print(f"[bench] cpu oracle auau_n256 reference-serial phases: {ser['total']:.2f} s", file=sys.stderr, flush=True)
+    res["auau_n256_reference_serial"] = {k: round(v, 3) for k, v in ser.items()}
+    total = res["pbpb_n512"]["total"]
     return {"value": round(1.0 / total, 6), "unit": "events/s", "cores": threads, "kind": "port",
-            "sample": (f"C oracle, Pb+Pb N={N}: colour charge once, setV 1 of {p.Ny} slices x{p.Ny}, "
-                       f"root finder on {len(sample)} central links scaled to {2 * S} over {threads} "
-                       f"threads, 2 leapfrog steps x{p.itmax}, T^munu once; "
-                       f"seconds per event {total:.2f} (cc {t_cc:.2f}, setV {t_setv:.2f}, "
-                       f"newton {t_newton:.2f}, evolve {p.itmax * t_step:.2f}, tmunu {t_tmunu:.2f})")}
+            "cpu_model": model, "host_cpus": ncpu,
+            "sample": (f"C oracle (OpenMP, {threads} threads), whole events, median of {reps}: Pb+Pb b=0 N={p.size} "
+                       f"Ny={p.Ny} {total:.2f} s/event (the value); Au+Au b=5 fm N=256 "
+                       f"{res['auau_n256']['total']:.2f} s/event; seconds per phase in phases_s"),
+            "phases_s": res,
+            "serial_in_reference": ["rho_normals", "setv (FFTW)", "tmunu_dedeta_gauss (Tmunu)",
+                                    "glauber_g2mu2_stats (statistics loop)"]}
 
 
 def main():

@@ -59,6 +59,11 @@ typedef struct ipg_params {
     /* project options (not in the reference input; default 0) */
     int rho_rng; /* 0: device Philox counter map, 1: the reference's
                     MT19937-64 Gauss stream drawn on the host in its order */
+    int newton_polish; /* extra Newton iterations of the tau = 0 root finder
+                          after the reference's stopping rule holds (0, the
+                          default, stops where Init.cpp:3384 stops); a
+                          test-only knob that takes every link to round-off
+                          so that difference quotients see no stopping noise */
     /* the hydro output grid (main.cpp:477-487): cells per side, rapidity
      * cells, side length [fm], rapidity step; used by the drop-in driver's
      * writeOutputs files */
@@ -229,11 +234,15 @@ int ipg_adjoint_rho(ipg_ctx *c, double *dedeta, double *grad);
 #define IPG_WRT_TMUNU 2
 int ipg_adjoint_loss(ipg_ctx *c, int loss, const double *weights, int wrt, double *value, double *grad);
 
-/* Steps between the reverse mode's checkpoints: 0 (the default) takes
- * ceil(sqrt(itmax-1)), storing about 2 sqrt(T) states and recomputing each
- * segment once; 1 stores every state (the full tape, nothing recomputed).
+/* Steps between the reverse mode's checkpoints.  0 (the default) chooses by
+ * memory: every state is stored (K = 1, the full tape, nothing recomputed)
+ * when that fits in the free device memory with 4 GiB to spare, else
+ * K = ceil(sqrt(itmax-1)) (about 2 sqrt(T) states, each segment recomputed
+ * once), else the K in between that fits.  K > 0 is used as given.
  * Gradients do not depend on it.  No counterpart in the reference. */
 int ipg_set_checkpoint_interval(ipg_ctx *c, int K);
+/* the checkpoint interval the last reverse pass of the context used */
+int ipg_get_checkpoint_interval(ipg_ctx *c, int *K);
 
 /* Host copy of the last ipg_adjoint_rho gradient, grad[2][Ny][S][8].
  * IPG_ERR_STATE when there is none.  No counterpart in the reference. */
@@ -347,6 +356,26 @@ int ipg_batch_load_qs_table(ipg_batch *b, const char *path);
  * HIP graph. */
 int ipg_batch_run(ipg_batch *b, int use_graph, int max_retries);
 int ipg_batch_get_observables(ipg_batch *b, int ev, ipg_observables *o);
+
+/* Reverse mode over a batch (no counterpart in the reference): Init::init
+ * for every event in one launch sequence (failures re-drawn up to
+ * max_retries, as ipg_batch_run does), then for each event the loss at
+ * tau_max (IPG_LOSS_DEDETA, IPG_LOSS_ECC2, or IPG_LOSS_TMUNU with ten
+ * weights) and its gradient with respect to rho, through the evolution,
+ * pi, the root finder, the links and setV as ipg_adjoint_rho.  The events
+ * go one after the other through one reverse-mode workspace, whose
+ * checkpoint interval is chosen for the device memory left after the
+ * per-event gradients (see ipg_set_checkpoint_interval; the batch's own
+ * interval is set with ipg_batch_set_checkpoint_interval).  Each event's
+ * gradient [2][Ny][S][8] stays in device memory (ipg_batch_get_rho_grad
+ * copies it out).  values[nev] receive the losses (NaN for events that
+ * never passed the success test). */
+int ipg_batch_adjoint(ipg_batch *b, int loss, const double *weights, int max_retries, double *values);
+int ipg_batch_get_rho_grad(ipg_batch *b, int ev, double *grad);
+int ipg_batch_set_checkpoint_interval(ipg_batch *b, int K);
+/* the largest device memory in use (all of the device, bytes) seen during
+ * the last ipg_batch_adjoint, and the checkpoint interval it used */
+int ipg_batch_adjoint_info(ipg_batch *b, unsigned long long *peak_bytes, int *K);
 
 #ifdef __cplusplus
 }

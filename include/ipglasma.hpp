@@ -17,6 +17,10 @@
 #pragma once
 
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -33,53 +37,343 @@ inline void check(int rc) {
     if (rc != IPG_OK) throw Error(ipg_last_error());
 }
 
-// Parameters: the parsed input file (Parameters.h getters for the hot path)
+// Parameters: the parsed input file and the per-event results, with the
+// reference's accessors (Parameters.h:215-493): every setX/getX pair that
+// readInput (main.cpp:448-578) and the event loop (main.cpp:129-360) call.
+// Keys the hot path reads live in the ipg_params the library takes (raw());
+// the others are kept for callers and ignored by the library.  b, Npart, the
+// <Q_s> averages, alpha_s, T_pp, the area, eps_2 and Psi_2 are filled by
+// Init::init and Evolution::run (setObservables).  setSize, setL and
+// setMaxtime re-derive dtau and itmax (main.cpp:531-534).
 class Parameters {
    public:
     Parameters() { ipg_default_params(&p_); }
     ipg_params& raw() { return p_; }
     const ipg_params& raw() const { return p_; }
-    int getSize() const { return p_.size; }
-    int getNc() const { return p_.Nc; }
-    int getNy() const { return p_.Ny; }
-    double getL() const { return p_.L; }
-    double getg() const { return p_.g; }
-    double getdtau() const { return p_.dtau; }
-    double getMaxtime() const { return p_.maxtime; }
-    unsigned long long getSeed() const { return p_.seed; }
-    int getUseTimeForSeed() const { return p_.useTimeForSeed; }
-    std::string getNucleusQsTableFileName() const { return p_.NucleusQsTableFileName; }
-    int getReadInitialWilsonLines() const { return p_.readInitialWilsonLines; }
-    int getWriteInitialWilsonLines() const { return p_.writeInitialWilsonLines; }
-    // set by the driver per event (main.cpp:71,148); they name the
-    // Wilson-line files
-    int getEventId() const { return event_id_; }
-    void setEventId(int id) { event_id_ = id; }
-    int getMPISize() const { return mpi_size_; }
-    void setMPISize(int n) { mpi_size_ = n; }
-    // filled by Init::init (Init.cpp:1614-1644)
-    int getSuccess() const { return success_; }
-    void setSuccess(int s) { success_ = s; }
-    double getb() const { return obs_.b; }
-    int getNpart() const { return obs_.Npart; }
-    double getAverageQs() const { return obs_.averageQs; }
-    double getAverageQsAvg() const { return obs_.averageQsAvg; }
-    double getAverageQsmin() const { return obs_.averageQsmin; }
-    double getTpp() const { return obs_.Tpp; }
-    double getalphas() const { return obs_.alphas; }
     const ipg_observables& observables() const { return obs_; }
     void setObservables(const ipg_observables& o) { obs_ = o; }
+    int getItmax() const { return p_.itmax; }
+    // the posterior sub-nucleon parameter sets (Parameters.cpp) belong to the
+    // constituent-quark proton, which the library does not support
+    void loadPosteriorParameterSets(const int itype) {
+        if (itype > 0) throw Error("SubNucleonParamType > 0 (sub-nucleon posterior sets) is not supported");
+    }
+    void setParamsWithPosteriorParameterSet(const int itype, int iset) {
+        (void)iset;
+        loadPosteriorParameterSets(itype);
+    }
+
+    void setSubNucleonParamType(int x) { p_.SubNucleonParamType = x; }
+    int getSubNucleonParamType() const { return p_.SubNucleonParamType; }
+    void setSubNucleonParamSet(int x) { subNucleonParamSet_ = x; }
+    int getSubNucleonParamSet() const { return subNucleonParamSet_; }
+    void setSeed(unsigned long long x) { p_.seed = x; }
+    unsigned long long getSeed() const { return p_.seed; }
+    void setA(int x) { a_ = x; }
+    int getA() const { return a_; }
+    void setNc(int x) { p_.Nc = x; }
+    int getNc() const { return p_.Nc; }
+    void setNy(int x) { p_.Ny = x; }
+    int getNy() const { return p_.Ny; }
+    void setSize(int x) { p_.size = x; ipg_derive(&p_); }
+    int getSize() const { return p_.size; }
+    void setProtonAnisotropy(double x) { p_.protonAnisotropy = x; }
+    double getProtonAnisotropy() const { return p_.protonAnisotropy; }
+    void setSizeOutput(int x) { p_.sizeOutput = x; }
+    int getSizeOutput() const { return p_.sizeOutput; }
+    void setEtaSizeOutput(int x) { p_.etaSizeOutput = x; }
+    int getEtaSizeOutput() const { return p_.etaSizeOutput; }
+    void setDetaOutput(double x) { p_.detaOutput = x; }
+    double getDetaOutput() const { return p_.detaOutput; }
+    void setAverageOverNuclei(int x) { p_.averageOverThisManyNuclei = x; }
+    int getAverageOverNuclei() const { return p_.averageOverThisManyNuclei; }
+    void setR(double x) { r_ = x; }
+    double getR() const { return r_; }
+    void setDs(double x) { ds_ = x; }
+    double getDs() const { return ds_; }
+    void setg2mu(double x) { p_.g2mu = x; }
+    double getg2mu() const { return p_.g2mu; }
+    void setQs(double x) { qs_ = x; }
+    double getQs() const { return qs_; }
+    void setSteps(int x) { steps_ = x; }
+    int getSteps() const { return steps_; }
+    void setMeasureSteps(int x) { measureSteps_ = x; }
+    int getMeasureSteps() const { return measureSteps_; }
+    void setMode(int x) { mode_ = x; }
+    int getMode() const { return mode_; }
+    void setRunningCoupling(int x) { p_.runningCoupling = x; }
+    int getRunningCoupling() const { return p_.runningCoupling; }
+    void setMu0(double x) { mu0_ = x; }
+    double getMu0() const { return mu0_; }
+    void setg(double x) { p_.g = x; }
+    double getg() const { return p_.g; }
+    void setLambdaQCD(double x) { lambdaQCD_ = x; }
+    double getLambdaQCD() const { return lambdaQCD_; }
+    void setkappa4Factor(double x) { kappa4Factor_ = x; }
+    double getkappa4Factor() const { return kappa4Factor_; }
+    void setSigmaNN(double x) { p_.SigmaNN = x; }
+    double getSigmaNN() const { return p_.SigmaNN; }
+    void setb(double x) { obs_.b = x; }
+    double getb() const { return obs_.b; }
+    void setbmin(double x) { p_.bmin = x; }
+    double getbmin() const { return p_.bmin; }
+    void setbmax(double x) { p_.bmax = x; }
+    double getbmax() const { return p_.bmax; }
+    void setTarget(std::string x) { copy_name(p_.Target, sizeof(p_.Target), x); }
+    std::string getTarget() const { return std::string(p_.Target); }
+    void setProjectile(std::string x) { copy_name(p_.Projectile, sizeof(p_.Projectile), x); }
+    std::string getProjectile() const { return std::string(p_.Projectile); }
+    void setL(double x) { p_.L = x; ipg_derive(&p_); }
+    double getL() const { return p_.L; }
+    void setLOutput(double x) { p_.LOutput = x; }
+    double getLOutput() const { return p_.LOutput; }
+    void setm(double x) { p_.m = x; }
+    double getm() const { return p_.m; }
+    void setJacobianm(double x) { p_.Jacobianm = x; }
+    double getJacobianm() const { return p_.Jacobianm; }
+    void setQsmuRatio(double x) { p_.QsmuRatio = x; }
+    double getQsmuRatio() const { return p_.QsmuRatio; }
+    void setQsmuRatioB(double x) { qsmuRatioB_ = x; }
+    double getQsmuRatioB() const { return qsmuRatioB_; }
+    void setRapidity(double x) { p_.Rapidity = x; }
+    double getRapidity() const { return p_.Rapidity; }
+    void setMaxtime(double x) { p_.maxtime = x; ipg_derive(&p_); }
+    double getMaxtime() const { return p_.maxtime; }
+    void setdtau(double x) { p_.dtau = x; }
+    double getdtau() const { return p_.dtau; }
+    void setNpart(int x) { obs_.Npart = x; }
+    int getNpart() const { return obs_.Npart; }
+    void setAverageQs(double x) { obs_.averageQs = x; }
+    double getAverageQs() const { return obs_.averageQs; }
+    void setAverageQsAvg(double x) { obs_.averageQsAvg = x; }
+    double getAverageQsAvg() const { return obs_.averageQsAvg; }
+    void setAverageQsmin(double x) { obs_.averageQsmin = x; }
+    double getAverageQsmin() const { return obs_.averageQsmin; }
+    void setalphas(double x) { obs_.alphas = x; }
+    double getalphas() const { return obs_.alphas; }
+    void setxExponent(double x) { xExponent_ = x; }
+    double getxExponent() const { return xExponent_; }
+    void setRandomSeed(unsigned long long x) { randomSeed_ = x; }
+    unsigned long long getRandomSeed() const { return randomSeed_; }
+    void setUseTimeForSeed(int x) { p_.useTimeForSeed = x; }
+    int getUseTimeForSeed() const { return p_.useTimeForSeed; }
+    void setUseSeedList(int x) { p_.useSeedList = x; }
+    int getUseSeedList() const { return p_.useSeedList; }
+    void setNucleusQsTableFileName(std::string x) { copy_name(p_.NucleusQsTableFileName, sizeof(p_.NucleusQsTableFileName), x); }
+    std::string getNucleusQsTableFileName() const { return std::string(p_.NucleusQsTableFileName); }
+    void setA1FromFile(int x) { a1FromFile_ = x; }
+    int getA1FromFile() const { return a1FromFile_; }
+    void setA2FromFile(int x) { a2FromFile_ = x; }
+    int getA2FromFile() const { return a2FromFile_; }
+    void setBG(double x) { p_.BG = x; }
+    double getBG() const { return p_.BG; }
+    void setBGq(double x) { bGq_ = x; }
+    double getBGq() const { return bGq_; }
+    void setBGqVar(double x) { bGqVar_ = x; }
+    double getBGqVar() const { return bGqVar_; }
+    void setDqmin(double x) { dqmin_ = x; }
+    double getDqmin() const { return dqmin_; }
+    void setMuZero(double x) { muZero_ = x; }
+    double getMuZero() const { return muZero_; }
+    void setc(double x) { c_ = x; }
+    double getc() const { return c_; }
+    void setRoots(double x) { p_.roots = x; }
+    double getRoots() const { return p_.roots; }
+    void setUseFluctuatingx(int x) { p_.useFluctuatingx = x; }
+    int getUseFluctuatingx() const { return p_.useFluctuatingx; }
+    void setRunWithThisFactorTimesQs(double x) { runWithThisFactorTimesQs_ = x; }
+    double getRunWithThisFactorTimesQs() const { return runWithThisFactorTimesQs_; }
+    void setxFromThisFactorTimesQs(double x) { p_.xFromThisFactorTimesQs = x; }
+    double getxFromThisFactorTimesQs() const { return p_.xFromThisFactorTimesQs; }
+    void setTpp(double x) { obs_.Tpp = x; }
+    double getTpp() const { return obs_.Tpp; }
+    void setNu(double x) { nu_ = x; }
+    double getNu() const { return nu_; }
+    void setUseFixedNpart(int x) { p_.useFixedNpart = x; }
+    int getUseFixedNpart() const { return p_.useFixedNpart; }
+    void setArea(double x) { obs_.area = x; }
+    double getArea() const { return obs_.area; }
+    void setEccentricity2(double x) { obs_.eccentricity[1] = x; }
+    double getEccentricity2() const { return obs_.eccentricity[1]; }
+    void setRnp(double x) { rnp_ = x; }
+    double getRnp() const { return rnp_; }
+    void setPsi(double x) { obs_.Psi[1] = x; }
+    double getPsi() const { return obs_.Psi[1]; }
+    void setSmearingWidth(double x) { p_.smearingWidth = x; }
+    double getSmearingWidth() const { return p_.smearingWidth; }
+    void setMPIRank(int x) { mPIRank_ = x; }
+    int getMPIRank() const { return mPIRank_; }
+    void setEventId(int x) { eventId_ = x; }
+    int getEventId() const { return eventId_; }
+    void setMPISize(int x) { mPISize_ = x; }
+    int getMPISize() const { return mPISize_; }
+    void setSuccess(int x) { success_ = x; }
+    int getSuccess() const { return success_; }
+    void setRmax(double x) { p_.rmax = x; }
+    double getRmax() const { return p_.rmax; }
+    void setUVdamp(double x) { p_.UVdamp = x; }
+    double getUVdamp() const { return p_.UVdamp; }
+    void setSetWSDeformParams(int x) { p_.setWSDeformParams = x != 0; }
+    bool getSetWSDeformParams() const { return p_.setWSDeformParams != 0; }
+    void setR_WS(double x) { p_.R_WS = x; }
+    double getR_WS() const { return p_.R_WS; }
+    void setA_WS(double x) { p_.a_WS = x; }
+    double getA_WS() const { return p_.a_WS; }
+    void setBeta2(double x) { p_.beta2 = x; }
+    double getBeta2() const { return p_.beta2; }
+    void setBeta3(double x) { p_.beta3 = x; }
+    double getBeta3() const { return p_.beta3; }
+    void setBeta4(double x) { p_.beta4 = x; }
+    double getBeta4() const { return p_.beta4; }
+    void setGamma(double x) { p_.gamma = x; }
+    double getGamma() const { return p_.gamma; }
+    void setDmin(double x) { p_.d_min = x; }
+    double getDmin() const { return p_.d_min; }
+    void setForceDmin(int x) { p_.force_dmin_flag = x != 0; }
+    bool getForceDmin() const { return p_.force_dmin_flag != 0; }
+    void setWSdR_np(double x) { p_.dR_np = x; }
+    double getWSdR_np() const { return p_.dR_np; }
+    void setWSda_np(double x) { p_.da_np = x; }
+    double getWSda_np() const { return p_.da_np; }
+    void setRotateReactionPlane(int x) { p_.rotateReactionPlane = x != 0; }
+    bool getRotateReactionPlane() const { return p_.rotateReactionPlane != 0; }
+    void setInitMethod(int x) { initMethod_ = x; }
+    int getInitMethod() const { return initMethod_; }
+    void setUseNucleus(int x) { p_.useNucleus = x; }
+    int getUseNucleus() const { return p_.useNucleus; }
+    void setUseGaussian(int x) { useGaussian_ = x; }
+    int getUseGaussian() const { return useGaussian_; }
+    void setlightNucleusOption(int x) { lightNucleusOption_ = x; }
+    int getlightNucleusOption() const { return lightNucleusOption_; }
+    void setRunWithQs(int x) { runWithQs_ = x; }
+    int getRunWithQs() const { return runWithQs_; }
+    void setRunWithkt(int x) { runWithkt_ = x; }
+    int getRunWithkt() const { return runWithkt_; }
+    void setRunWithLocalQs(int x) { runWithLocalQs_ = x; }
+    int getRunWithLocalQs() const { return runWithLocalQs_; }
+    void setLinearb(int x) { p_.samplebFromLinearDistribution = x; }
+    int getLinearb() const { return p_.samplebFromLinearDistribution; }
+    void setWriteOutputs(int x) { p_.writeOutputs = x; }
+    int getWriteOutputs() const { return p_.writeOutputs; }
+    void setWriteOutputsToHDF5(int x) { writeOutputsToHDF5_ = x; }
+    int getWriteOutputsToHDF5() const { return writeOutputsToHDF5_; }
+    void setWriteEvolution(int x) { writeEvolution_ = x; }
+    int getWriteEvolution() const { return writeEvolution_; }
+    void setWriteInitialWilsonLines(int x) { p_.writeInitialWilsonLines = x; }
+    int getWriteInitialWilsonLines() const { return p_.writeInitialWilsonLines; }
+    void setReadInitialWilsonLines(int x) { p_.readInitialWilsonLines = x; }
+    int getReadInitialWilsonLines() const { return p_.readInitialWilsonLines; }
+    void setNucleonPositionsFromFile(int x) { p_.nucleonPositionsFromFile = x; }
+    int getNucleonPositionsFromFile() const { return p_.nucleonPositionsFromFile; }
+    void setInverseQsForMaxTime(int x) { p_.inverseQsForMaxTime = x; }
+    int getInverseQsForMaxTime() const { return p_.inverseQsForMaxTime; }
+    void setUseFatTails(int x) { useFatTails_ = x; }
+    int getUseFatTails() const { return useFatTails_; }
+    void setSmearQs(int x) { p_.smearQs = x; }
+    int getSmearQs() const { return p_.smearQs; }
+    void setReadMultFromFile(int x) { readMultFromFile_ = x; }
+    int getReadMultFromFile() const { return readMultFromFile_; }
+    void setGaussianWounding(int x) { p_.gaussianWounding = x; }
+    int getGaussianWounding() const { return p_.gaussianWounding; }
+    void setUsePseudoRapidity(int x) { p_.usePseudoRapidity = x; }
+    int getUsePseudoRapidity() const { return p_.usePseudoRapidity; }
+    void setUseConstituentQuarkProton(int x) { p_.useConstituentQuarkProton = x; }
+    int getUseConstituentQuarkProton() const { return p_.useConstituentQuarkProton; }
+    void setNqBase(double x) { nqBase_ = x; }
+    double getNqBase() const { return nqBase_; }
+    void setNqFluc(double x) { nqFluc_ = x; }
+    double getNqFluc() const { return nqFluc_; }
+    void setUseSmoothNucleus(int x) { p_.useSmoothNucleus = x; }
+    int getUseSmoothNucleus() const { return p_.useSmoothNucleus; }
+    void setShiftConstituentQuarkProtonOrigin(int x) { shiftConstituentQuarkProtonOrigin_ = x; }
+    int getShiftConstituentQuarkProtonOrigin() const { return shiftConstituentQuarkProtonOrigin_; }
+    void setMinimumQs2ST(int x) { p_.minimumQs2ST = x; }
+    int getMinimumQs2ST() const { return static_cast<int>(p_.minimumQs2ST); }
+    void setComputeGluonMultiplicity(int x) { p_.computeGluonMultiplicity = x != 0; }
+    bool getComputeGluonMultiplicity() const { return p_.computeGluonMultiplicity != 0; }
 
    private:
+    static void copy_name(char* dst, size_t n, const std::string& v) {
+        if (v.size() + 1 > n) throw Error("name too long: " + v);
+        std::memcpy(dst, v.c_str(), v.size() + 1);
+    }
     ipg_params p_;
-    int event_id_ = 0, mpi_size_ = 1;
-    int success_ = 0;
     ipg_observables obs_{};
+    int subNucleonParamSet_ = 0;
+    int a_ = 0;
+    double r_ = 0.;
+    double ds_ = 0.;
+    double qs_ = 0.;
+    int steps_ = 0;
+    int measureSteps_ = 0;
+    int mode_ = 0;
+    double mu0_ = 0.;
+    double lambdaQCD_ = 0.;
+    double kappa4Factor_ = 0.;
+    double qsmuRatioB_ = 0.;
+    double xExponent_ = 0.;
+    unsigned long long randomSeed_ = 0;
+    int a1FromFile_ = 0;
+    int a2FromFile_ = 0;
+    double bGq_ = 0.;
+    double bGqVar_ = 0.;
+    double dqmin_ = 0.;
+    double muZero_ = 0.;
+    double c_ = 0.;
+    double runWithThisFactorTimesQs_ = 0.;
+    double nu_ = 0.;
+    double rnp_ = 0.;
+    int mPIRank_ = 0;
+    int eventId_ = 0;
+    int mPISize_ = 1;
+    int success_ = 0;
+    int initMethod_ = 0;
+    int useGaussian_ = 0;
+    int lightNucleusOption_ = 0;
+    int runWithQs_ = 0;
+    int runWithkt_ = 0;
+    int runWithLocalQs_ = 0;
+    int writeOutputsToHDF5_ = 0;
+    int writeEvolution_ = 0;
+    int useFatTails_ = 0;
+    int readMultFromFile_ = 0;
+    double nqBase_ = 0.;
+    double nqFluc_ = 0.;
+    int shiftConstituentQuarkProtonOrigin_ = 0;
 };
 
-// Setup: the reference's key/value input reader (Setup.cpp:24-104)
+// Setup: the reference's key/value input reader (Setup.cpp:24-104).  A file
+// is whitespace-separated "key value" pairs up to the word EndOfFile.
+// StringFind returns the value after the first occurrence of a key; DFind
+// converts with atof, IFind truncates (static_cast<int>), ULLIFind rounds
+// (+0.5).  A missing file or key throws ipglasma::Error where the reference
+// prints and calls exit(1).
 class Setup {
    public:
+    Setup() {}
+    std::string StringFind(std::string file_name, std::string st) {
+        std::ifstream in(file_name);
+        if (!in) throw Error("The input file named " + file_name + " is absent.");
+        std::string key, value;
+        while (in >> key && key != "EndOfFile") {
+            if (!(in >> value)) break;
+            if (key == st) return value;
+        }
+        throw Error(st + " not found in " + file_name);
+    }
+    double DFind(std::string file_name, std::string st) {
+        return std::atof(StringFind(file_name, st).c_str());
+    }
+    int IFind(std::string file_name, std::string st) { return static_cast<int>(DFind(file_name, st)); }
+    unsigned long long int ULLIFind(std::string file_name, std::string st) {
+        return static_cast<unsigned long long int>(DFind(file_name, st) + 0.5);
+    }
+    int IsFile(std::string file_name) {
+        std::ifstream in(file_name);
+        return in.good() ? 1 : 0;
+    }
+    // the whole file at once (ipg_read_input: keys absent from the file keep
+    // their current values instead of being an error)
     void read(const std::string& file_name, Parameters* param) {
         check(ipg_read_input(file_name.c_str(), &param->raw()));
     }
@@ -93,13 +387,23 @@ class Random {
     void init_genrand64(unsigned long long seed) {
         seed_ = seed;
         seeded_ = true;
+        host_.seed(seed);
     }
+    // the GSL generator of the reference feeds only samplers outside the
+    // library's path (fat tails, Poisson smearing)
+    void gslRandomInit(unsigned long long seed) { (void)seed; }
+    // draws from a host copy of the stream; the library's Init::init keeps
+    // its own stream from the same seed, so draws taken here do not shift it
+    // (in the reference they would)
+    unsigned long long genrand64_int64() { return host_(); }
+    long long genrand64_int63() { return (long long)(host_() >> 1); }
     unsigned long long seed() const { return seed_; }
     bool seeded() const { return seeded_; }
 
    private:
     unsigned long long seed_ = 5489ULL;
     bool seeded_ = false;
+    std::mt19937_64 host_{5489ULL};  // the MT19937-64 of Random.cpp:67-120
 };
 
 // Group and Glauber keep their places in the call sequence; SU(3) and the

@@ -19,7 +19,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libipglasma_amd.so")
-QS_TABLE = os.path.join(os.path.dirname(_HERE), "tests", "golden", "qs2Adj_vs_Tp_vs_Y_200.in")
+# the Q_s^2(T_p, y) table the reference ships (data, Init.cpp:441-480)
+QS_TABLE = os.path.join(_HERE, "data", "qs2Adj_vs_Tp_vs_Y_200.in")
 
 FIELDS = {"Ux": 0, "Uy": 1, "E1": 2, "E2": 3, "phi": 4, "pi": 5, "VA": 6, "VB": 7,
           "g2mu2A": 8, "g2mu2B": 9, "TpA": 10, "TpB": 11}
@@ -51,7 +52,7 @@ class Parameters(C.Structure):
                 [(n, C.c_int) for n in _INT_FIELDS_2] + [(n, C.c_double) for n in _DBL_FIELDS_2] +
                 [("seed", C.c_ulonglong), ("Projectile", C.c_char * 16), ("Target", C.c_char * 16),
                  ("NucleusQsTableFileName", C.c_char * 512), ("dtau", C.c_double), ("itmax", C.c_int),
-                 ("rho_rng", C.c_int), ("sizeOutput", C.c_int), ("etaSizeOutput", C.c_int),
+                 ("rho_rng", C.c_int), ("newton_polish", C.c_int), ("sizeOutput", C.c_int), ("etaSizeOutput", C.c_int),
                  ("LOutput", C.c_double), ("detaOutput", C.c_double)])
 
     def set(self, **kw):
@@ -149,6 +150,11 @@ def lib():
             "ipg_adjoint_rho": (C.c_int, [vp, dp, dp]),
             "ipg_get_rho_grad": (C.c_int, [vp, dp]),
             "ipg_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
+            "ipg_get_checkpoint_interval": (C.c_int, [vp, ip]),
+            "ipg_batch_adjoint": (C.c_int, [vp, C.c_int, dp, C.c_int, dp]),
+            "ipg_batch_get_rho_grad": (C.c_int, [vp, C.c_int, dp]),
+            "ipg_batch_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
+            "ipg_batch_adjoint_info": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip]),
             "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
@@ -319,9 +325,16 @@ class Event:
     LOSSES = {"dEdeta": 0, "eps2": 1, "tmunu": 2}
 
     def set_checkpoint_interval(self, K):
-        """steps between the reverse mode's checkpoints (0: ceil(sqrt(T));
-        1: every state stored, nothing recomputed)"""
+        """steps between the reverse mode's checkpoints (0: chosen by the
+        free device memory, the full tape when it fits; 1: every state
+        stored, nothing recomputed; K: every K-th state)"""
         _check(lib().ipg_set_checkpoint_interval(self._c, int(K)))
+
+    def checkpoint_interval(self):
+        """the interval the last reverse pass used"""
+        k = C.c_int(0)
+        _check(lib().ipg_get_checkpoint_interval(self._c, C.byref(k)))
+        return k.value
 
     def adjoint(self, loss="dEdeta", weights=None, wrt="fields", fetch=True):
         """Reverse mode with a choice of loss at tau_max: "dEdeta", "eps2"
@@ -437,6 +450,32 @@ class Batch:
         o = Observables()
         _check(lib().ipg_batch_get_observables(self._b, e, C.byref(o)))
         return o.as_dict()
+
+    def adjoint(self, loss="dEdeta", weights=None, max_retries=20):
+        """Init::init for all events, then each event's loss at tau_max and
+        its gradient with respect to rho, kept on the device per event
+        (rho_grad(e) copies one out).  Returns the losses [nev]."""
+        v = np.zeros(self.n)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        _check(lib().ipg_batch_adjoint(self._b, Event.LOSSES[loss], None if w is None else _dptr(w), max_retries,
+                                       _dptr(v)))
+        return v
+
+    def rho_grad(self, e, Ny, S):
+        g = np.empty((2, Ny, S, 8))
+        _check(lib().ipg_batch_get_rho_grad(self._b, e, _dptr(g)))
+        return g
+
+    def set_checkpoint_interval(self, K):
+        _check(lib().ipg_batch_set_checkpoint_interval(self._b, int(K)))
+
+    def adjoint_info(self):
+        """(largest device memory in use during the last adjoint, bytes;
+        the checkpoint interval it used)"""
+        peak = C.c_ulonglong(0)
+        k = C.c_int(0)
+        _check(lib().ipg_batch_adjoint_info(self._b, C.byref(peak), C.byref(k)))
+        return peak.value, k.value
 
     def close(self):
         if self._b:

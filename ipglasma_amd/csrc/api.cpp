@@ -63,8 +63,7 @@ int pad8(long long n) { return (int)(((n + 7) / 8) * 8); }
 // colour-charge slices per setV chunk: the Wilson lines are read and written
 // once per chunk; the FFT buffer holds 256 bytes per site and slice
 constexpr int kSetvChunk = 8;
-// doubles of end-of-evolution observables per event (kernels_landau.hip)
-constexpr int kObs = 32;
+constexpr int kObs = kObsDoubles;
 int log2i(int n) {
     int l = 0;
     while ((1 << l) < n) l++;
@@ -318,14 +317,9 @@ struct Engine {
     size_t rho_grad_n = 0;
     EvDesc* d_rdesc = nullptr;   // descriptors of the recomputed steps
     size_t rdesc_n = 0;
-    int adj_K = 0;               // checkpoint interval (0: ceil(sqrt(T)))
-
-    // steps between checkpoints of the reverse mode for T leapfrog steps;
-    // 1 stores every state (the full tape: nothing is recomputed)
-    int ckpt_interval(int T) const {
-        if (adj_K > 0) return std::max(1, std::min(adj_K, std::max(T, 1)));
-        return std::max(1, (int)std::ceil(std::sqrt((double)std::max(T, 1))));
-    }
+    int adj_K = 0;               // checkpoint interval (0: chosen by memory, adj_interval)
+    int last_K = 0;              // the interval the last reverse pass used
+    EvDesc* d_adesc = nullptr;   // one-event descriptor of the event being reversed
 
     ~Engine() {
         // the per-event blocks go back to the cache, not through hipFree
@@ -353,6 +347,8 @@ struct Engine {
         if (d_obs) (void)hipFree(d_obs);
         if (adj_ws) block_release(adj_ws, sizeof(double) * adj_ws_n);
         if (d_rdesc) (void)hipFree(d_rdesc);
+        if (d_adesc) (void)hipFree(d_adesc);
+        if (bgrad) (void)hipFree(bgrad);
         for (hipEvent_t t : tev) (void)hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
             (void)hipEventDestroy(t.a);
@@ -419,6 +415,8 @@ struct Engine {
             d.xi = p.protonAnisotropy;
             d.rcut = std::sqrt(0.1 * p.SigmaNN / M_PI);
             d.useFluctuatingx = p.useFluctuatingx;
+            if (p.newton_polish < 0 || p.newton_polish > 50) return fail(IPG_ERR_ARG, "newton_polish must be in [0, 50]");
+            d.newton_polish = p.newton_polish;
             // device memory: 9 matrix fields (162 planes), 5 scalars, 2 int
             // planes, FFT buffer (18), T (10), partials, root-finder counters
             // setV chunk: kc slices of 2 nuclei x 4 packed complex planes; the
@@ -1029,22 +1027,7 @@ struct Engine {
         HIPCHK(hipMemcpy(h.data(), d_obs, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
         for (size_t e = 0; e < ev.size(); e++) {
             if (!desc[e].active) continue;
-            const double* o = &h[(size_t)kObs * e];
-            ipg_observables& q = ev[e].obs;
-            for (int n = 0; n < 6; n++) {
-                q.eccentricity[n] = o[n];
-                q.Psi[n] = o[6 + n];
-            }
-            q.area = o[12];
-            q.Rbar = o[13];
-            q.avg_eps = o[14];
-            q.avg_qs2A_qs2B = o[15];
-            q.rms_r = o[16];
-            q.centroid[0] = o[17];
-            q.centroid[1] = o[18];
-            for (int k = 0; k < 3; k++) q.u_rms[k] = std::sqrt(o[19 + k] / o[22]);
-            q.landau_failures = (int)o[23];
-            q.landau_no_timelike = (int)o[24];
+            ipg_obs_from_device(&h[(size_t)kObs * e], &ev[e].obs);
         }
         return IPG_OK;
     }
@@ -1062,26 +1045,52 @@ struct Engine {
     // dE/deta at itmax and its gradient with respect to the state left by
     // the initial conditions (U_x, U_y, E1, E2, phi, pi at tau = 0), by
     // reverse mode through the tau = 0 kick and itmax-1 leapfrog steps with
-    // checkpoints every K = ceil(sqrt(itmax-1)) steps.  Consumes the state.
+    // checkpoints (adj_interval).  Consumes the state.
     int adjoint_dedeta(double* dedeta, double* grad) { return adjoint_fields(IPG_LOSS_DEDETA, nullptr, dedeta, grad); }
 
     // the loss's gradient with respect to the tau = 0 fields
     int adjoint_fields(int loss, const double* w10, double* value, double* grad) {
         const double *s_init = nullptr, *cs0 = nullptr;
-        return adjoint_evolution(loss, w10, value, grad, &s_init, &cs0);
+        return adjoint_evolution(0, loss, w10, value, grad, &s_init, &cs0);
     }
     // the loss's gradient with respect to the ten T^{mu nu} planes at itmax
     int adjoint_tmunu(int loss, const double* w10, double* value, double* grad) {
         const double *s_init = nullptr, *cs0 = nullptr;
-        return adjoint_evolution(loss, w10, value, grad, &s_init, &cs0, true);
+        return adjoint_evolution(0, loss, w10, value, grad, &s_init, &cs0, true);
     }
 
-    // doubles of the evolution part of the workspace
-    size_t adj_evolution_doubles(const EvDesc& d) const {
+    // doubles of the evolution part of the workspace for checkpoint interval K
+    static size_t adj_evolution_doubles(const EvDesc& d, int K) {
         const int T = d.itmax - 1;
-        const int K = ckpt_interval(T);
         const size_t nslot = 1 + (size_t)(T / K + 1) + (size_t)K + 1 + 1;
         return nslot * 72 * (size_t)d.S + (2 * IPG_PEND_PLANES + 108 + 10) * (size_t)d.S + 16;
+    }
+    // doubles the reverse mode to rho needs beyond the evolution part: V_A,
+    // V_B and the link and D cotangents (2 x 18 planes each), and the
+    // gradient unless it goes to a buffer of the caller's
+    static size_t adj_rho_extra_doubles(const EvDesc& d, bool own_grad) {
+        return 6 * 18 * (size_t)d.S + (own_grad ? 2 * (size_t)d.Ny * d.S * 8 : 0);
+    }
+
+    // Steps between checkpoints for event d.  An explicit interval
+    // (ipg_set_checkpoint_interval) is used as given.  Otherwise every state
+    // is stored (K = 1: nothing is recomputed) when that workspace fits in
+    // the device memory that is free (plus the workspace already held) with
+    // 4 GiB to spare, else K = ceil(sqrt(T)) (about 2 sqrt(T) states, each
+    // segment recomputed once), else the K in between that fits.
+    int adj_interval(const EvDesc& d, size_t extra_doubles) const {
+        const int T = std::max(d.itmax - 1, 1);
+        if (adj_K > 0) return std::max(1, std::min(adj_K, T));
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (int)std::ceil(std::sqrt((double)T));
+        const size_t reserve = (size_t)4 << 30;
+        const size_t avail = fr + sizeof(double) * adj_ws_n;
+        const int Ksq = std::max(1, (int)std::ceil(std::sqrt((double)T)));
+        for (int K = 1; K <= Ksq; K++) {
+            const size_t need = sizeof(double) * (adj_evolution_doubles(d, K) + extra_doubles);
+            if (need + reserve <= avail) return K;
+        }
+        return Ksq;
     }
 
     // grows the workspace to at least n doubles (contents not kept)
@@ -1098,6 +1107,27 @@ struct Engine {
         // through the block cache, like the per-event blocks
         HIPCHK(block_alloc((void**)&adj_ws, sizeof(double) * n));
         adj_ws_n = n;
+        note_memory();
+        return IPG_OK;
+    }
+
+    // device memory in use (all of the device), kept as a peak
+    size_t peak_used = 0;
+    void note_memory() {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) peak_used = std::max(peak_used, tot - fr);
+    }
+
+    // the descriptor of event e as a one-event launch (its blocks from 0),
+    // in d_adesc
+    int adj_descriptor(int e, EvDesc& d) {
+        d = desc[e];
+        d.block0 = 0;
+        d.lblock0 = 0;
+        d.active = 1;
+        if (!d_adesc) HIPCHK(hipMalloc(&d_adesc, sizeof(EvDesc)));
+        HIPCHK(hipMemcpyAsync(d_adesc, &d, sizeof(EvDesc), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
         return IPG_OK;
     }
 
@@ -1108,47 +1138,58 @@ struct Engine {
     // ipg_init: the Wilson lines of that call are still in the V buffers.
     int adjoint_rho(double* dedeta, double* grad) { return adjoint_rho_loss(IPG_LOSS_DEDETA, nullptr, dedeta, grad); }
 
-    int adjoint_rho_loss(int loss, const double* w10, double* dedeta, double* grad) {
-        if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
-        EvDesc& d = desc[0];
+    // the reverse mode to rho for event e of the engine; gr_ext (device,
+    // 2 Ny S 8 doubles) receives the gradient in place of the workspace
+    int adjoint_rho_loss(int loss, const double* w10, double* dedeta, double* grad, int e = 0,
+                         double* gr_ext = nullptr) {
+        if (e < 0 || e >= (int)ev.size()) return fail(IPG_ERR_ARG, "event index out of range");
+        if (ev.size() != 1 && !gr_ext) return fail(IPG_ERR_ARG, "a batch keeps its gradients per event");
+        EvDesc d;
+        int rc = adj_descriptor(e, d);
+        if (rc) return rc;
         const long long S = d.S;
         const size_t F = 18 * (size_t)S;
-        const size_t nev_part = adj_evolution_doubles(d);
+        const size_t extra = adj_rho_extra_doubles(d, gr_ext == nullptr);
+        const int K = adj_interval(d, extra);
+        const size_t nev_part = adj_evolution_doubles(d, K);
         const size_t ng = 2 * (size_t)d.Ny * S * 8;
-        int rc = adj_reserve(nev_part + 6 * F + ng);
+        rc = adj_reserve(nev_part + extra);
         if (rc) return rc;
         rho_grad = nullptr;
         rho_grad_n = 0;
         // [evolution part (offset 0) | V_A, V_B | link cotangents | D cotangents | gradient]
-        if (nev_part + 6 * F + ng > adj_ws_n) return fail(IPG_ERR_STATE, "adjoint workspace too small");
+        if (nev_part + extra > adj_ws_n) return fail(IPG_ERR_STATE, "adjoint workspace too small");
         double* keep = adj_ws + nev_part;
         double* vab = keep;           // V_A, V_B of the initial conditions
         double* ulb = vab + 2 * F;    // cotangents of the tau = 0 links (x, y)
         double* dbb = ulb + 2 * F;    // cotangents of D = L1 - L2 (x, y)
-        double* gr = dbb + 2 * F;     // d/d rho
+        double* gr = gr_ext ? gr_ext : dbb + 2 * F;  // d/d rho
         HIPCHK(hipMemcpyAsync(vab, d.VA, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(vab + F, d.VB, sizeof(double) * F, hipMemcpyDeviceToDevice, st));
         const double *s_init = nullptr, *cs0 = nullptr;
-        rc = adjoint_evolution(loss, w10, dedeta, nullptr, &s_init, &cs0);
+        rc = adjoint_evolution(e, loss, w10, dedeta, nullptr, &s_init, &cs0, false, K);
         if (rc) return rc;
         // tau = 0 links: total cotangent = from the evolution + from pi
         HIPCHK(hipMemcpyAsync(ulb, cs0, sizeof(double) * 2 * F, hipMemcpyDeviceToDevice, st));
         ipg_adj_zero(dbb, (long long)(2 * F), st);
-        ipg_adj_pi_init(d_desc, S, s_init, vab, cs0, ulb, dbb, st);
+        ipg_adj_pi_init(d_adesc, S, s_init, vab, cs0, ulb, dbb, st);
         // Wilson-line cotangents (reuse the evolution's cotangent buffer)
         double* vb = const_cast<double*>(cs0);
         ipg_adj_zero(vb, (long long)(2 * F), st);
-        ipg_adj_links_newton(d_desc, S, s_init, vab, ulb, dbb, vb, st);
+        ipg_adj_links_newton(d_adesc, S, s_init, vab, ulb, dbb, vb, st);
         // setV in reverse, chunk by chunk (the chunk's A_k are recomputed)
         const Plan& pl = plans[d.N];
         FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
         const int kc = d.kc;
         const int nch = (d.Ny + kc - 1) / kc;
-        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_desc, dp, c * kc, kc, vab, vb, gr, st);
+        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_adesc, dp, c * kc, kc, vab, vb, gr, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        rho_grad = gr;
-        rho_grad_n = ng;
+        note_memory();
+        if (!gr_ext) {
+            rho_grad = gr;
+            rho_grad_n = ng;
+        }
         if (grad) HIPCHK(hipMemcpy(grad, gr, sizeof(double) * ng, hipMemcpyDeviceToHost));
         return IPG_OK;
     }
@@ -1162,26 +1203,30 @@ struct Engine {
     // the evolution part, at the start of the workspace (grown if needed;
     // adjoint_rho reserves its own buffers behind it first); *s_init is the
     // tau = 0 state slot and *cs0 its cotangent set (U_x, U_y, E1, E2, phi,
-    // pi, 18 planes each)
-    int adjoint_evolution(int loss, const double* w10, double* dedeta, double* grad, const double** s_init_out,
-                          const double** cs0_out, bool tb_only = false) {
-        if (ev.size() != 1) return fail(IPG_ERR_ARG, "the adjoint runs on single-event contexts");
+    // pi, 18 planes each).  Kfix > 0: the checkpoint interval the caller
+    // sized the workspace for.
+    int adjoint_evolution(int e, int loss, const double* w10, double* dedeta, double* grad,
+                          const double** s_init_out, const double** cs0_out, bool tb_only = false, int Kfix = 0) {
+        if (ev.size() != 1 && e == 0 && Kfix == 0)
+            return fail(IPG_ERR_ARG, "the field and T^{mu nu} gradients run on single-event contexts");
         if (loss < IPG_LOSS_DEDETA || loss > IPG_LOSS_TMUNU) return fail(IPG_ERR_ARG, "unknown loss");
         if (loss == IPG_LOSS_TMUNU && !w10) return fail(IPG_ERR_ARG, "the T^{mu nu} loss needs 10 weights");
-        EvDesc& d = desc[0];
+        EvDesc d;
+        int rc = adj_descriptor(e, d);
+        if (rc) return rc;
+        const int nb = d.nblocks;
         const int itm = d.itmax, T = itm - 1;
         if (itm < 1) return fail(IPG_ERR_ARG, "itmax must be at least 1");
         const long long S = d.S;
-        const int K = ckpt_interval(T);
+        const int K = Kfix > 0 ? Kfix : adj_interval(d, 0);
+        last_K = K;
         const int nck = T / K + 1;
         const size_t slot = 72 * (size_t)S, pend = IPG_PEND_PLANES * (size_t)S, cset = 108 * (size_t)S;
         rho_grad = nullptr;
         rho_grad_n = 0;
-        const size_t need = adj_evolution_doubles(d);
-        {
-            int rc = adj_reserve(need);
-            if (rc) return rc;
-        }
+        const size_t need = adj_evolution_doubles(d, K);
+        rc = adj_reserve(need);
+        if (rc) return rc;
         if ((size_t)(nck + K + 3) * slot + 2 * pend + cset + 10 * (size_t)S + 16 > need)
             return fail(IPG_ERR_STATE, "adjoint workspace layout");
         double* mem = adj_ws;
@@ -1194,21 +1239,19 @@ struct Engine {
         double* c0 = cB + pend;      // compact cotangents of the tau = 0 fields
         double* tb = c0 + cset;      // cotangents of the ten T^{mu nu} planes at itmax
         double* wdev = tb + 10 * (size_t)S;
-        int rc = upload_desc();
-        if (rc) return rc;
         const int sv = step_decided();
         // forward with checkpoints
-        ipg_adj_state_save(d_desc, S, 0, s_init, st);
-        ipg_launch_step0(d_desc, 1, nblk, st);
-        ipg_adj_state_save(d_desc, S, 0, s_ck, st);
+        ipg_adj_state_save(d_adesc, S, 0, s_init, st);
+        ipg_launch_step0(d_adesc, 1, nb, st);
+        ipg_adj_state_save(d_adesc, S, 0, s_ck, st);
         for (int it = 1; it <= T; it++) {
-            ipg_launch_step(d_desc, 1, nblk, it, st, sv);
-            if (it % K == 0) ipg_adj_state_save(d_desc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
+            ipg_launch_step(d_adesc, 1, nb, it, st, sv);
+            if (it % K == 0) ipg_adj_state_save(d_adesc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
         }
-        ipg_adj_state_save(d_desc, S, T & 1, s_T, st);
+        ipg_adj_state_save(d_adesc, S, T & 1, s_T, st);
         // the loss: T^{mu nu} at itmax from the state after step itmax-1
-        ipg_launch_tmunu(d_desc, 1, nblk, itm, st);
-        ipg_launch_finish(d_desc, 1, itm, d_out, st);
+        ipg_launch_tmunu(d_adesc, 1, nb, itm, st);
+        ipg_launch_finish(d_adesc, 1, itm, d_out, st);
         HIPCHK(hipGetLastError());
         double tt = 0.;
         HIPCHK(hipMemcpy(&tt, d_out, sizeof(double), hipMemcpyDeviceToHost));
@@ -1217,9 +1260,9 @@ struct Engine {
         double L = dEdeta;
         if (loss == IPG_LOSS_ECC2) {
             // Landau frame and eccentricities at itmax (k_landau ... k_ecc_finish)
-            ipg_launch_landau(d_desc, 1, nblk, itm, 0, st);
+            ipg_launch_landau(d_adesc, 1, nb, itm, 0, st);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpy(&L, d_obs + 1, sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&L, d.obs + 1, sizeof(double), hipMemcpyDeviceToHost));
         } else if (loss == IPG_LOSS_TMUNU) {
             std::vector<double> hT(10 * (size_t)S);
             HIPCHK(hipStreamSynchronize(st));
@@ -1234,7 +1277,7 @@ struct Engine {
         // The recomputed steps go from one stored state to the next (step it
         // reads the slot of state it-1 and writes the slot of state it), each
         // with a descriptor of its own: no copies through the live fields.
-        if (T > 0) {
+        if (T > 0 && K > 1) {
             std::vector<EvDesc> rd((size_t)T);
             for (int k = 0; k < nck; k++) {
                 const int s0 = k * K, s1 = std::min(s0 + K, T);
@@ -1270,7 +1313,7 @@ struct Engine {
         if (tb_only) {
             // dL/dT^{mu nu} at itmax only
             if (loss == IPG_LOSS_ECC2) {
-                ipg_adj_ecc2(d_desc, S, itm, tb, st);
+                ipg_adj_ecc2(d_adesc, S, itm, tb, st);
             } else {
                 double w[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
                 if (loss == IPG_LOSS_TMUNU)
@@ -1291,15 +1334,15 @@ struct Engine {
         // reverse: the loss's cotangents of the state after step itmax-1
         ipg_adj_zero(cA, (long long)pend, st);
         if (loss == IPG_LOSS_DEDETA) {
-            ipg_adj_loss(d_desc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
+            ipg_adj_loss(d_adesc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
         } else {
             if (loss == IPG_LOSS_ECC2) {
-                ipg_adj_ecc2(d_desc, S, itm, tb, st);
+                ipg_adj_ecc2(d_adesc, S, itm, tb, st);
             } else {
                 HIPCHK(hipMemcpy(wdev, w10, sizeof(double) * 10, hipMemcpyHostToDevice));
                 ipg_adj_fill_tb(S, wdev, tb, st);
             }
-            ipg_adj_tmunu(d_desc, S, s_T, itm, tb, cA, st);
+            ipg_adj_tmunu(d_adesc, S, s_T, itm, tb, cA, st);
         }
         double *cin = cA, *cout = cB;
         for (int k = nck - 1; k >= 0; k--) {
@@ -1307,16 +1350,16 @@ struct Engine {
             if (s1 <= s0) continue;
             const double* x0 = s_ck + (size_t)k * slot;
             if (K > 1)
-                for (int it = s0 + 1; it <= s1; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nblk, it, st, sv);
+                for (int it = s0 + 1; it <= s1; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nb, it, st, sv);
             for (int it = s1; it > s0; it--) {
                 const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
                 // K = 1: the state after the step is the next stored state
                 const double* xa = K > 1 ? s_seg + (size_t)(it - s0) * slot : s_ck + (size_t)it * slot;
-                ipg_adj_step(d_desc, S, xb, xa, it, cin, cout, st);
+                ipg_adj_step(d_adesc, S, xb, xa, it, cin, cout, st);
                 std::swap(cin, cout);
             }
         }
-        ipg_adj_kick(d_desc, S, s_init, cin, c0, st);
+        ipg_adj_kick(d_adesc, S, s_init, cin, c0, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         if (grad) {
@@ -1332,10 +1375,92 @@ struct Engine {
             }
         }
         if (dedeta) *dedeta = L;
-        ev[0].obs.dEdeta = dEdeta;
-        ev[0].obs.tau_fm = tau_fm;
+        ev[e].obs.dEdeta = dEdeta;
+        ev[e].obs.tau_fm = tau_fm;
         *s_init_out = s_init;
         *cs0_out = c0;
+        return IPG_OK;
+    }
+
+    // ---- the batched reverse mode -------------------------------------------------
+    double* bgrad = nullptr;   // per-event rho gradients, [nev][2][Ny][S][8]
+    size_t bgrad_n = 0;
+    std::vector<size_t> bgrad_off;
+    std::vector<int> bgrad_ok;
+
+    // Init::init for every event (failures re-drawn up to max_retries), as
+    // the batched forward does
+    int batch_init(int max_retries) {
+        const int nev = (int)ev.size();
+        std::vector<int> pending(nev, 1);
+        for (int attempt = 0; attempt <= max_retries; attempt++) {
+            std::vector<int> which;
+            for (int e = 0; e < nev; e++) {
+                desc[e].active = pending[e];
+                if (pending[e]) which.push_back(e);
+            }
+            if (which.empty()) break;
+            int rc = nucleons_many(which);
+            if (rc) return rc;
+            rc = color_charge();
+            if (rc) return rc;
+            for (int e = 0; e < nev; e++)
+                if (pending[e] && ev[e].success) pending[e] = 0;
+        }
+        for (int e = 0; e < nev; e++) desc[e].active = ev[e].success;
+        int rc = upload_desc();
+        if (rc) return rc;
+        rc = wilson_lines();
+        if (rc) return rc;
+        return fields();
+    }
+
+    // initial conditions of all events in one launch sequence, then each
+    // event's loss and its gradient with respect to rho, one event after the
+    // other through one reverse-mode workspace (its checkpoint interval sized
+    // to the device memory left after the per-event gradients)
+    int batch_adjoint(int loss, const double* w10, int max_retries, double* values) {
+        const int nev = (int)ev.size();
+        peak_used = 0;
+        note_memory();
+        int rc = batch_init(max_retries);
+        if (rc) return rc;
+        size_t total = 0;
+        bgrad_off.assign(nev, 0);
+        bgrad_ok.assign(nev, 0);
+        for (int e = 0; e < nev; e++) {
+            bgrad_off[e] = total;
+            total += 2 * (size_t)desc[e].Ny * desc[e].S * 8;
+        }
+        if (total > bgrad_n) {
+            if (bgrad) HIPCHK(hipFree(bgrad));
+            bgrad = nullptr;
+            bgrad_n = 0;
+            HIPCHK(hipMalloc(&bgrad, sizeof(double) * total));
+            bgrad_n = total;
+        }
+        note_memory();
+        for (int e = 0; e < nev; e++) {
+            if (!ev[e].success) {
+                if (values) values[e] = std::nan("");
+                continue;
+            }
+            double v = 0.;
+            rc = adjoint_rho_loss(loss, w10, &v, nullptr, e, bgrad + bgrad_off[e]);
+            if (rc) return rc;
+            if (values) values[e] = v;
+            bgrad_ok[e] = 1;
+        }
+        note_memory();
+        return IPG_OK;
+    }
+
+    int batch_get_rho_grad(int e, double* grad) {
+        if (e < 0 || e >= (int)ev.size()) return fail(IPG_ERR_ARG, "event index out of range");
+        if (bgrad_ok.size() != ev.size() || !bgrad_ok[e])
+            return fail(IPG_ERR_STATE, "no gradient for this event: call ipg_batch_adjoint first");
+        const size_t n = 2 * (size_t)desc[e].Ny * desc[e].S * 8;
+        HIPCHK(hipMemcpy(grad, bgrad + bgrad_off[e], sizeof(double) * n, hipMemcpyDeviceToHost));
         return IPG_OK;
     }
 
@@ -1435,6 +1560,9 @@ double* field_ptr(Engine& g, int which, int* kind) {
 }
 
 }  // namespace
+
+// the message ipg_last_error returns, for the entry points of other files
+void ipg_record_error(const std::string& msg) { g_err = msg; }
 
 extern "C" {
 
@@ -1657,6 +1785,11 @@ int ipg_set_checkpoint_interval(ipg_ctx* c, int K) {
     c->eng.adj_K = K;
     return IPG_OK;
 }
+int ipg_get_checkpoint_interval(ipg_ctx* c, int* K) {
+    if (!c || !K) return fail(IPG_ERR_ARG, "null argument");
+    *K = c->eng.last_K;
+    return IPG_OK;
+}
 int ipg_get_rho_grad(ipg_ctx* c, double* grad) {
     if (!c || !grad) return fail(IPG_ERR_ARG, "null argument");
     return c->eng.get_rho_grad(grad);
@@ -1865,30 +1998,30 @@ int ipg_batch_load_qs_table(ipg_batch* b, const char* path) {
 int ipg_batch_run(ipg_batch* b, int use_graph, int max_retries) {
     if (!b) return fail(IPG_ERR_ARG, "null batch");
     Engine& g = b->eng;
-    const int nev = (int)g.ev.size();
-    std::vector<int> pending(nev, 1);
-    for (int attempt = 0; attempt <= max_retries; attempt++) {
-        std::vector<int> which;
-        for (int e = 0; e < nev; e++) {
-            g.desc[e].active = pending[e];
-            if (pending[e]) which.push_back(e);
-        }
-        if (which.empty()) break;
-        int rc = g.nucleons_many(which);
-        if (rc) return rc;
-        rc = g.color_charge();
-        if (rc) return rc;
-        for (int e = 0; e < nev; e++)
-            if (pending[e] && g.ev[e].success) pending[e] = 0;
-    }
-    for (int e = 0; e < nev; e++) g.desc[e].active = g.ev[e].success;
-    int rc = g.upload_desc();
-    if (rc) return rc;
-    rc = g.wilson_lines();
-    if (rc) return rc;
-    rc = g.fields();
+    int rc = g.batch_init(max_retries);
     if (rc) return rc;
     return g.evolve(use_graph != 0);
+}
+int ipg_batch_adjoint(ipg_batch* b, int loss, const double* weights, int max_retries, double* values) {
+    if (!b) return fail(IPG_ERR_ARG, "null batch");
+    if (loss < IPG_LOSS_DEDETA || loss > IPG_LOSS_TMUNU) return fail(IPG_ERR_ARG, "unknown loss");
+    if (loss == IPG_LOSS_TMUNU && !weights) return fail(IPG_ERR_ARG, "the T^{mu nu} loss needs 10 weights");
+    return b->eng.batch_adjoint(loss, weights, max_retries, values);
+}
+int ipg_batch_get_rho_grad(ipg_batch* b, int e, double* grad) {
+    if (!b || !grad) return fail(IPG_ERR_ARG, "null argument");
+    return b->eng.batch_get_rho_grad(e, grad);
+}
+int ipg_batch_set_checkpoint_interval(ipg_batch* b, int K) {
+    if (!b || K < 0) return fail(IPG_ERR_ARG, "bad argument");
+    b->eng.adj_K = K;
+    return IPG_OK;
+}
+int ipg_batch_adjoint_info(ipg_batch* b, unsigned long long* peak_bytes, int* K) {
+    if (!b) return fail(IPG_ERR_ARG, "null batch");
+    if (peak_bytes) *peak_bytes = b->eng.peak_used;
+    if (K) *K = b->eng.last_K;
+    return IPG_OK;
 }
 int ipg_batch_get_observables(ipg_batch* b, int e, ipg_observables* o) {
     if (!b || !o || e < 0 || e >= (int)b->eng.ev.size()) return fail(IPG_ERR_ARG, "bad argument");

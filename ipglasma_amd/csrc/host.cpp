@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <numeric>
 
 namespace ipg {
@@ -76,8 +77,9 @@ struct Row {
     double R, a, b2, b3, b4;
     int user_b2;  // beta2 taken from the input (U, Xe)
 };
-// Glauber.cpp:19-245 (Woods-Saxon nuclei; deuteron and He3 need samplers
-// that are not on this path)
+// Glauber.cpp:19-245: Woods-Saxon nuclei, the proton and the deuteron
+// (Hulthen a = 0.228 fm^-1 in the a column, R_WS = 1 its starting
+// normalisation); He3 needs a sampler that is not on this path
 const Row kNuclei[] = {
     {"Au", 197, 79, 6.37, 0.535, -0.13, 0., -0.03, 0}, {"Pb", 208, 82, 6.62, 0.546, 0., 0., 0., 0},
     {"p", 1, 1, 1., 1., 0., 0., 0., 0},                {"d", 2, 1, 1.0, 0.228, 0., 0., 0., 0},
@@ -118,6 +120,7 @@ bool find_nucleus(const std::string& name, const ipg_params& p, NucleusData* out
         }
         d.force_dmin = p.force_dmin_flag;
         d.d_min = p.d_min;
+        d.w = r.A == 2 ? 1.18 : 0.;  // Glauber.cpp:68-77 (not overridden)
         *out = d;
         return true;
     }
@@ -279,6 +282,150 @@ void rotate_3d(MT64& rng, std::vector<Nucleon>& n) {
 
 }  // namespace
 
+// ---- deuteron --------------------------------------------------------------------------
+namespace {
+
+// Hulthen thickness integrand in xi = e^{-z a} (Glauber.cpp:874-903): the
+// transverse distance s is in units of 1/a, the factor 2 folds z < 0
+struct HulthenTA {
+    double a, w, rho, sigma, s;
+    double f(double xi) const {
+        if (xi == 0.0) xi = 1e-10;
+        if (xi == 1.0) xi = 1.0 - 1e-10;
+        const double sa = s * a;
+        const double z = -std::log(xi);
+        const double r = std::sqrt(sa * sa + z * z);
+        double v = 2.0 * a * rho * sigma;
+        const double g = (1.0 / r) * (std::exp(-r) - std::exp(-(w / a) * r));
+        v *= g * g;
+        return v;
+    }
+};
+
+// adaptive 7-point closed Newton-Cotes with interval halving
+// (Glauber.cpp:905-1100): each level compares the halves against the
+// parent estimate, tolerance divided by sqrt 2 per level, at most 10000
+// integrand evaluations in all
+double nc7(const HulthenTA& F, double tol, double lo, double dx, const double* fo, double pre, double area,
+           int* count) {
+    static const double wt[7] = {41.0 / 140.0, 54.0 / 35.0, 27.0 / 140.0, 68.0 / 35.0,
+                                 27.0 / 140,   54.0 / 35.0, 41.0 / 140.0};
+    dx /= 2.0;
+    double fl[7], fr[7];
+    fl[1] = F.f(lo + dx);
+    fl[3] = F.f(lo + 3.0 * dx);
+    fl[5] = F.f(lo + 5.0 * dx);
+    fl[0] = fo[0];
+    fl[2] = fo[1];
+    fl[4] = fo[2];
+    fl[6] = fo[3];
+    *count += 3;
+    double left = 0.0;
+    for (int i = 0; i < 7; i++) left += wt[i] * fl[i];
+    left *= dx;
+    fr[1] = F.f(lo + 7.0 * dx);
+    fr[3] = F.f(lo + 9.0 * dx);
+    fr[5] = F.f(lo + 11.0 * dx);
+    fr[0] = fo[3];
+    fr[2] = fo[4];
+    fr[4] = fo[5];
+    fr[6] = fo[6];
+    *count += 3;
+    double right = 0.0;
+    for (int i = 0; i < 7; i++) right += wt[i] * fr[i];
+    right *= dx;
+    double ans = left + right;
+    area += -std::fabs(pre) + std::fabs(left) + std::fabs(right);
+    if (std::fabs(ans - pre) > tol * std::fabs(area) && *count < 10000) {
+        tol /= 1.414213562;
+        left = nc7(F, tol, lo, dx, fl, left, area, count);
+        right = nc7(F, tol, lo + dx * 6., dx, fr, right, area, count);
+        ans = left + right;
+    }
+    return ans;
+}
+
+double nc7_integral(const HulthenTA& F) {
+    const double lo = 0.0, dx = (1.0 - 0.0) / 6.0;
+    double g[7];
+    for (int i = 0; i < 7; i++) g[i] = F.f(lo + i * dx);
+    int count = 7;
+    return nc7(F, 1.0e-6, lo, dx, g, 0.0, 0.0, &count);
+}
+
+struct DeuteronTable {
+    std::vector<double> vx, vy;
+};
+
+// 101 nodes on [0, 2 x 12 fm] (Glauber.cpp:400-430, initGlauber's 100
+// intervals and SCutOff 12)
+const DeuteronTable& deuteron_table(double a, double w, double R0, double sigma) {
+    static std::mutex mu;
+    static std::map<std::vector<double>, DeuteronTable> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const std::vector<double> key = {a, w, R0, sigma};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    // Glauber::CalcRho with AnumHulthen: int d^3r (e^{-ar} - e^{-wr})^2/r^2 =
+    // 2 pi (a - w)^2 / (a w (a + w)); rho0 starts at R_WS (Glauber.cpp:246)
+    const double f0 = 2.0 * (a - w) * (a - w) * M_PI / a / w / (a + w);
+    const double rho_start = R0;
+    const double f = (f0 * rho_start) / rho_start;
+    HulthenTA F{a, w, 2.0 / f, sigma, 0.};
+    DeuteronTable t;
+    const int n = 100;
+    const double up = 2.0 * 12.0, dx = (up - 0.0) / n;
+    t.vx.resize(n + 1);
+    t.vy.resize(n + 1);
+    for (int i = 0; i <= n; i++) {
+        t.vx[i] = dx * i;
+        F.s = t.vx[i];
+        t.vy[i] = nc7_integral(F);
+    }
+    return cache.emplace(key, std::move(t)).first->second;
+}
+
+}  // namespace
+
+double deuteron_thickness(double s, double a, double w, double R0, double sigma) {
+    const DeuteronTable& t = deuteron_table(a, w, R0, sigma);
+    const int n = (int)t.vx.size() - 1;
+    if (s > t.vx[n]) return 0.0;
+    // cubic through four evenly spaced nodes, x between the second and third
+    // (Glauber.cpp:300-358)
+    int k = (int)(n * (s - t.vx[0]) / (t.vx[n] - t.vx[0])) - 1;
+    if (k <= 0)
+        k = 0;
+    else if (k >= n - 3)
+        k = n - 3;
+    const double h = (t.vx[n] - t.vx[0]) / n;
+    const double f0 = t.vy[k], f1 = t.vy[k + 1], f2 = t.vy[k + 2], f3 = t.vy[k + 3];
+    const double c3 = (-f0 + 3.0 * f1 - 3.0 * f2 + f3) / (6.0 * h * h * h);
+    const double c2 = (2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3) / (2.0 * h * h);
+    const double c1 = (-11.0 * f0 + 18.0 * f1 - 9.0 * f2 + 2.0 * f3) / (6.0 * h);
+    const double u = s - t.vx[k];
+    double y = c3 * std::pow(u, 3.);
+    y += c2 * std::pow(u, 2.);
+    y += c1 * u;
+    y += f0;
+    return y < 0.0 ? 0.0 : y;
+}
+
+void sample_deuteron_separation(MT64& rng, double a, double w, double R0, double sigma, double* x, double* y) {
+    // envelope A r 11 e^{-r^2/40}, scaled with sigma_NN against 42.13 mb;
+    // r drawn from it by inverting its integral up to 15 fm
+    const double A = 1.2 * sigma / 4.21325504715;
+    const double area15 = A * 220. * (1. - std::exp(-0.025 * 15. * 15.));
+    double phi, r, u;
+    do {
+        phi = 2. * M_PI * rng.real1();
+        r = 6.32456 * std::sqrt(-std::log((-0.00454545 * (-220. * A + area15 * rng.real1())) / A));
+        u = rng.real1();
+    } while (u > r * deuteron_thickness(r, a, w, R0, sigma) / (A * r * 11. * std::exp(-r * r / 40.)));
+    *x = r * std::cos(phi);
+    *y = r * std::sin(phi);
+}
+
 // the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p) {
     const double xb = rng.real1();
@@ -296,19 +443,33 @@ int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const 
         err = "constituent quarks, smearQs and nucleus averaging are not supported";
         return IPG_ERR_ARG;
     }
-    if (proj.A == 2 || targ.A == 2 || proj.A == 3 || targ.A == 3) {
-        err = "deuteron and He3 samplers are not supported";
+    if (proj.A == 3 || targ.A == 3) {
+        err = "the He3 sampler is not supported";
         return IPG_ERR_ARG;
     }
     const double b = sample_b(rng, p);
     A.clear();
     B.clear();
+    const double sigma_fm2 = 0.1 * p.SigmaNN;
+    // the deuteron (Init.cpp:90-113, 129-152): p-n separation, nucleons at
+    // +-1/2 of it; the second nucleon keeps proton = 1 in the projectile and
+    // gets proton = 0 in the target, as the reference writes them
+    auto deuteron = [&](const NucleusData& d, std::vector<Nucleon>& out, int second_proton) {
+        double x, y;
+        sample_deuteron_separation(rng, d.a, d.w, d.R, sigma_fm2, &x, &y);
+        out.push_back(Nucleon{x / 2., y / 2., 0., 0., 0, 1});
+        out.push_back(Nucleon{-(x / 2.), -(y / 2.), -0., 0., 0, second_proton});
+    };
     if (proj.A == 1)
         A.push_back(Nucleon{0., 0., 0., 0., 0, 1});
+    else if (proj.A == 2)
+        deuteron(proj, A, 1);
     else
         generate_nucleus(rng, proj, A);
     if (targ.A == 1)
         B.push_back(Nucleon{0., 0., 0., 0., 0, 1});
+    else if (targ.A == 2)
+        deuteron(targ, B, 0);
     else
         generate_nucleus(rng, targ, B);
     rotate_3d(rng, A);
@@ -526,6 +687,7 @@ int read_input(const char* path, ipg_params* p, std::string& err) {
     }
     I("SubNucleonParamType", p->SubNucleonParamType);
     I("rho_rng", p->rho_rng);
+    I("newton_polish", p->newton_polish);
     I("sizeOutput", p->sizeOutput);
     I("etaSizeOutput", p->etaSizeOutput);
     D("LOutput", p->LOutput);

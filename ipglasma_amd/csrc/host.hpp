@@ -38,6 +38,7 @@ struct Nucleon {
 struct NucleusData {
     int A = 0, Z = 0;
     double R = 0, a = 0, beta2 = 0, beta3 = 0, beta4 = 0, gamma = 0, dR = 0, da = 0;
+    double w = 0;  // the second Hulthen range of the deuteron (w_WS)
     int force_dmin = 0;
     double d_min = 0;
 };
@@ -56,6 +57,18 @@ struct EventInfo {
 // (1117-1216).  Returns IPG_OK or IPG_ERR_ARG for unsupported options.
 int sample_event(MT64& rng, const ipg_params& p, const NucleusData& proj, const NucleusData& targ,
                  std::vector<Nucleon>& A, std::vector<Nucleon>& B, EventInfo& info, std::string& err);
+
+// The deuteron's nucleon thickness (Glauber.cpp:513-576, 847-1100): the
+// Hulthen density rho(r) = rho0 (e^{-a r} - e^{-w r})^2 / r^2 integrated
+// along the beam axis as the reference integrates it (integrand in
+// xi = e^{-z a}, adaptive 7-point Newton-Cotes to 1e-6), tabulated at 101
+// points on [0, 24] fm and read back by 4-point interpolation; units
+// sigma_NN [fm^2] / fm^2 as in the reference.  R0 is the density's starting
+// normalisation (R_WS), sigma the nucleon cross section in fm^2.
+double deuteron_thickness(double s, double a, double w, double R0, double sigma);
+// Glauber::SampleTARejection (Glauber.cpp:1188-1243): proton-neutron
+// separation (x, y) by rejection against r e^{-r^2/40} envelopes
+void sample_deuteron_separation(MT64& rng, double a, double w, double R0, double sigma, double* x, double* y);
 
 // maxX, maxY of Evolution::eccentricity (Evolution.cpp:1492-1640): over
 // 2 <= ix, iy < N-2, the last x = -L/2 + a ix - avx on the row

@@ -67,6 +67,7 @@ struct EvDesc {
     const double* znorm;  // optional host-provided normals [2][Ny][S][8]
     uint64_t rho_seed;
     uint64_t newton_seed;
+    int newton_polish;  // Newton iterations after the stopping rule holds
     // outputs
     double* T;        // 10 planes of S doubles
     double* partial;  // per-block reduction partials (32 slots of nblocks)

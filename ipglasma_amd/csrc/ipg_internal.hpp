@@ -2,7 +2,35 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <string>
+
+#include "../../include/ipg.h"
 #include "ipg_device.hpp"
+
+// sets the message ipg_last_error returns on the calling thread
+void ipg_record_error(const std::string& msg);
+
+// doubles of end-of-evolution observables per event (EvDesc::obs)
+constexpr int kObsDoubles = 32;
+// EvDesc::obs (the k_ecc_mid / k_ecc_finish layout) -> the Landau-frame and
+// eccentricity fields of ipg_observables
+inline void ipg_obs_from_device(const double* o, ipg_observables* q) {
+    for (int n = 0; n < 6; n++) {
+        q->eccentricity[n] = o[n];
+        q->Psi[n] = o[6 + n];
+    }
+    q->area = o[12];
+    q->Rbar = o[13];
+    q->avg_eps = o[14];
+    q->avg_qs2A_qs2B = o[15];
+    q->rms_r = o[16];
+    q->centroid[0] = o[17];
+    q->centroid[1] = o[18];
+    for (int k = 0; k < 3; k++) q->u_rms[k] = std::sqrt(o[19 + k] / o[22]);
+    q->landau_failures = (int)o[23];
+    q->landau_no_timelike = (int)o[24];
+}
 
 // site-parallel helpers: nblk = total blocks over all events of the launch
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);

@@ -759,7 +759,7 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
 #pragma unroll
     for (int a = 0; a < 8; a++) alpha[a] = 0.;
     double Fzero = 0., Fmin = 0.;
-    int iter = 0, nRestart = 0, total = 0, analytic = 0, best_one = 1;
+    int iter = 0, nRestart = 0, total = 0, analytic = 0, best_one = 1, npol = 0;
     uint32_t ndraw = 0;
     if (c == 0) {
         lds_st(mU0, m_one());
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
                         best_one = 1;
                         Fzero = 10.;
                         Fmin = 1e6;
-                        iter = nRestart = total = analytic = 0;
+                        iter = nRestart = total = analytic = npol = 0;
                         ndraw = 0;
                         state = 1;
                     }
@@ -837,20 +837,20 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
                 Fzero += fabs(F[a]);
             }
         }
-        // the serial loop adds the step to alpha[b] and subtracts it again in
-        // place, so column b sees (alpha[k] + d_k) - d_k for k < b
-        double dk[8], ar[8], p[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            dk[k] = fmax(0.001, fmin(10., 0.01 * fabs(alpha[k])));
-            ar[k] = (alpha[k] + dk[k]) - dk[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++) p[k] = k < c ? ar[k] : (k == c ? alpha[k] + dk[k] : alpha[k]);
-        const double d = pick8(dk, c);
         double col[8];
         bool good = true;
-        {
+        if (npol == 0) {
+            // the serial loop adds the step to alpha[b] and subtracts it again
+            // in place, so column b sees (alpha[k] + d_k) - d_k for k < b
+            double dk[8], ar[8], p[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                dk[k] = fmax(0.001, fmin(10., 0.01 * fabs(alpha[k])));
+                ar[k] = (alpha[k] + dk[k]) - dk[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) p[k] = k < c ? ar[k] : (k == c ? alpha[k] + dk[k] : alpha[k]);
+            const double d = pick8(dk, c);
             const M3 E = exp_iQt(p) * lds_m3(mU0);
             M3 Mp = lds_m3(mW) * (dag(E) - dag(lds_m3(mUs)));
             cd t[8];
@@ -862,11 +862,19 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
                 chk += fabs(col[a]);
                 if (chk < 1e-15) good = false;
             }
-        }
 #pragma unroll
-        for (int k = 0; k < 8; k++) alpha[k] = ar[k];
+            for (int k = 0; k < 8; k++) alpha[k] = ar[k];
+        } else {
+            // newton_polish: the exact Jacobian about the iterate,
+            // U <- exp(i beta^a t^a) U, J_ab = -2 Re tr(t^a W U^+ t^b)
+            const M3 X = dag(gen_mul(c, mul_bd(lds_m3(mUs), lds_m3(mW))));
+            cd t[8];
+            tr_gen(X, t);
+#pragma unroll
+            for (int a = 0; a < 8; a++) col[a] = -2. * t[a].r;
+        }
         const unsigned long long bad = __ballot(!good);
-        if ((bad >> (grp * NG)) & 0xFFull) {
+        if (npol == 0 && ((bad >> (grp * NG)) & 0xFFull)) {
             analytic = 1;
             const M3 Mb = gen_mul(c, dag(lds_m3(mUs)));
             const M3 W = lds_m3(mW);
@@ -937,14 +945,19 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
 #pragma unroll
         for (int a = 0; a < 8; a++)
             if (fabs(y[a]) > 1e5) okstep = false;
-        if (okstep) {
-#pragma unroll
-            for (int a = 0; a < 8; a++) alpha[a] = alpha[a] + y[a];
+        M3 Us;
+        if (npol > 0) {
+            Us = exp_iQt(y) * lds_m3(mUs);
         } else {
-            restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
+            if (okstep) {
+#pragma unroll
+                for (int a = 0; a < 8; a++) alpha[a] = alpha[a] + y[a];
+            } else {
+                restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
+            }
+            Us = exp_iQt(alpha) * lds_m3(mU0);
         }
-        M3 Us = exp_iQt(alpha) * lds_m3(mU0);
-        if (Fzero < Fmin) {
+        if (npol == 0 && Fzero < Fmin) {
             // best = Us, kept as its exponent (Us is exp(i alpha t) U0)
             Fmin = Fzero;
             if (c == 0)
@@ -952,14 +965,20 @@ __global__ __launch_bounds__(NWB, OCC) void k_newton(const EvDesc* __restrict__ 
                 for (int a = 0; a < 8; a++) best_alpha[a] = alpha[a];
             best_one = 0;
         }
-        if (iter == maxIt) {
+        if (npol == 0 && iter == maxIt) {
             restart_alpha(evs[e].newton_seed, link, ndraw++, alpha);
             Us = exp_iQt(alpha) * lds_m3(mU0);
             nRestart++;
             iter = 0;
         }
         if (c == 0) lds_st(mUs, Us);
-        if (state == 1 && !(Fzero > 1e-6 && iter < maxIt && nRestart < maxRe)) {
+        bool stop = state == 1 && !(Fzero > 1e-6 && iter < maxIt && nRestart < maxRe);
+        // optional exact Newton steps of a converged link (newton_polish)
+        if (stop && nRestart != maxRe && (Fzero <= 1e-6 || npol > 0) && npol < evs[e].newton_polish) {
+            npol++;
+            stop = false;
+        }
+        if (stop) {
             const EvDesc& ev = evs[e];
             const int conv = nRestart != maxRe;
             if (c == 0) {

@@ -109,8 +109,10 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_landau(const EvDesc* __restrict__
             w = o.eps * o.u[0];
             acc[8] = 1.;
             acc[9] = o.eps * kHbarc;
-            acc[10] = ev.g2A[s] * ev.QsmuRatio * ev.QsmuRatio * ev.g2B[s] * ev.QsmuRatioB * ev.QsmuRatioB / a / a /
-                      a / a;
+            // <Q_sA^2 Q_sB^2> needs the colour-charge densities (0 without them)
+            if (ev.g2A && ev.g2B)
+                acc[10] = ev.g2A[s] * ev.QsmuRatio * ev.QsmuRatio * ev.g2B[s] * ev.QsmuRatioB * ev.QsmuRatioB / a /
+                          a / a / a;
         }
         acc[6] = x * w;
         acc[7] = y * w;

@@ -47,6 +47,44 @@
 extern "C" {
 #endif
 
+/* ---- mutation tests ------------------------------------------------------------
+ * A test plants a deliberate mistake by scaling one coefficient of the
+ * restatement (factor != 1) and checks that its pin then fails; factors are
+ * 1 otherwise (multiplication by 1 is exact, so results are unchanged).
+ * Not thread-safe with oracle calls in flight. */
+enum {
+    ORC_MUT_EVOLVE_U = 0,   /* i g^2 dtau/(tau + dtau/2) of evolveU            */
+    ORC_MUT_EVOLVE_PHI = 1, /* (tau + dtau/2) dtau of evolvePhi                */
+    ORC_MUT_EVOLVE_PI = 2,  /* dtau/tau of evolvePi                            */
+    ORC_MUT_E_PLAQ = 3,     /* i tau dtau/(2 g^2) of evolveE's plaquettes      */
+    ORC_MUT_E_COMM = 4,     /* i dtau/tau of evolveE's [phi~, phi]             */
+    ORC_MUT_T_ELEC = 5,     /* transverse electric part of T^{mu nu}           */
+    ORC_MUT_T_LONG = 6,     /* longitudinal pi part                            */
+    ORC_MUT_T_MAG = 7,      /* plaquette part                                  */
+    ORC_MUT_T_PHIGRAD = 8,  /* transverse phi-gradient part                    */
+    ORC_MUT_TX_F = 9,       /* T^{tau x}: E^y x plaquette clover term          */
+    ORC_MUT_TX_K = 10,      /* T^{tau x}: pi x D_x phi term                    */
+    ORC_MUT_TY_F = 11,
+    ORC_MUT_TY_K = 12,
+    ORC_MUT_TE = 13,        /* T^{tau eta}                                     */
+    ORC_MUT_XY_E = 14,      /* T^{xy}: E^x E^y term                            */
+    ORC_MUT_XY_B = 15,      /* T^{xy}: D_x phi D_y phi terms                   */
+    ORC_MUT_XE_E = 16,      /* T^{x eta}: E^x pi term                          */
+    ORC_MUT_XE_B = 17,      /* T^{x eta}: plaquette x D_y phi term             */
+    ORC_MUT_YE_E = 18,
+    ORC_MUT_YE_B = 19,
+    ORC_MUT_PI_INIT = 20,   /* -2i/g of pi = -(2i/g) A^eta at tau = 0          */
+    ORC_MUT_GAUSS_COMM = 21 /* i [phi, pi] of the Gauss-law check              */
+};
+void orc_set_mutation(int k, double factor);
+void orc_clear_mutations(void);
+/* extra Newton iterations of orc_newton_link after its stopping rule holds
+ * (the library's newton_polish option; 0, the default, is the reference's) */
+void orc_set_newton_polish(int n);
+/* OpenMP thread count of the oracle's parallel loops */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* ---- random numbers (Random.cpp:67-204) ---------------------------------- */
 typedef struct {
     uint64_t mt[312];
@@ -118,6 +156,10 @@ typedef struct {
 /* b draw, sampleTA (A then B, rotations), reaction plane shift, wounding.
  * nucleus arrays must hold A1 / A2 entries.  Returns 0, or <0 on an
  * unsupported configuration (deuteron, file-based nuclei). */
+/* the deuteron's nucleon thickness at transverse distance s [fm] for the
+ * Hulthen ranges a, w [fm^-1], starting normalisation R0 and cross section
+ * sigma [fm^2] (Glauber.cpp:513-576, 847-1100) */
+double orc_deuteron_thickness(double s, double a, double w, double R0, double sigma_fm2);
 int orc_sample_event(orc_mt *r, const orc_glauber_cfg *cfg, orc_nucleon *nucA,
                      orc_nucleon *nucB, orc_event_info *info);
 

@@ -32,7 +32,7 @@ static inline int at(int i, int j, int N) { return cl(i, N) * N + cl(j, N); }
 void orc_evolve_u(int N, double g, double dtau, double tau, const double *E1,
                   const double *E2, double *Ux, double *Uy) {
     const int S = N * N;
-    const cplx c = CMPLX(0., g * g * dtau / (tau + dtau / 2.));
+    const cplx c = CMPLX(0., g * g * dtau / (tau + dtau / 2.) * orc_mut[ORC_MUT_EVOLVE_U]);
 #pragma omp parallel for
     for (int s = 0; s < S; s++) {
         M3 one = m_one();
@@ -49,7 +49,7 @@ void orc_evolve_u(int N, double g, double dtau, double tau, const double *E1,
 
 void orc_evolve_phi(int N, double dtau, double tau, const double *pi, double *phi) {
     const int S = N * N;
-    const double c = (tau + dtau / 2.) * dtau;
+    const double c = (tau + dtau / 2.) * dtau * orc_mut[ORC_MUT_EVOLVE_PHI];
 #pragma omp parallel for
     for (int s = 0; s < S; s++) ST(phi, s, m_add(LD(phi, s), m_scale(c, LD(pi, s))));
 }
@@ -67,15 +67,15 @@ void orc_evolve_pi(int N, double dtau, double tau, const double *Ux, const doubl
             M3 pmX = m_mul(m_mul_ad(uxm, LD(phi, at(i - 1, j, N))), uxm);
             M3 pmY = m_mul(m_mul_ad(uym, LD(phi, at(i, j - 1, N))), uym);
             M3 br = m_sub(m_add(m_add(m_add(pX, pmX), pY), pmY), m_scale(4., LD(phi, s)));
-            ST(pi, s, m_add(LD(pi, s), m_scale(dtau / (tau), br)));
+            ST(pi, s, m_add(LD(pi, s), m_scale(dtau / (tau) * orc_mut[ORC_MUT_EVOLVE_PI], br)));
         }
     }
 }
 
 void orc_evolve_e(int N, double g, double dtau, double tau, const double *Ux,
                   const double *Uy, const double *phi, double *E1, double *E2) {
-    const cplx c1 = CMPLX(0., 1.) * tau * dtau / (2. * g * g);
-    const cplx c2 = CMPLX(0., 1.) * dtau / tau;
+    const cplx c1 = CMPLX(0., 1.) * tau * dtau / (2. * g * g) * orc_mut[ORC_MUT_E_PLAQ];
+    const cplx c2 = CMPLX(0., 1.) * dtau / tau * orc_mut[ORC_MUT_E_COMM];
 #pragma omp parallel for
     for (int i = 0; i < N; i++) {
         for (int j = 0; j < N; j++) {
@@ -117,7 +117,7 @@ double orc_gauss_law(int N, const double *Ux, const double *Uy, const double *E1
             G = m_sub(G, LD(E1, s));
             G = m_add(G, m_mul(m_mul(m_dag(uym), LD(E2, my)), uym));
             G = m_sub(G, LD(E2, s));
-            G = m_sub(G, m_cscale(CMPLX(0., 1.), m_sub(m_mul(ph, p), m_mul(p, ph))));
+            G = m_sub(G, m_cscale(CMPLX(0., orc_mut[ORC_MUT_GAUSS_COMM]), m_sub(m_mul(ph, p), m_mul(p, ph))));
             double sq = m_square(G);
             if (sq > largest) largest = sq;
         }
@@ -163,8 +163,9 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
             cplx t11 = m_tr(m_mul(e1, e1)), t1p = m_tr(m_mul(e1p, e1p));
             cplx t22 = m_tr(m_mul(e2, e2)), t2p = m_tr(m_mul(e2p, e2p));
             double lng = (creal(m_tr(m_mul(p, p))) + creal(m_tr(m_mul(pXm, pXm))) +
-                          creal(m_tr(m_mul(pYm, pYm))) + creal(m_tr(m_mul(pXYm, pXYm)))) / 4.;
-            double gt = g * g / tau / tau;
+                          creal(m_tr(m_mul(pYm, pYm))) + creal(m_tr(m_mul(pXYm, pXYm)))) / 4. *
+                         orc_mut[ORC_MUT_T_LONG];
+            double gt = g * g / tau / tau * orc_mut[ORC_MUT_T_ELEC];
             double tt = gt * creal(t11 + t1p + t22 + t2p) / 2. + lng;
             double xx = gt * creal(-1. * t11 - t1p + t22 + t2p) / 2. + lng;
             double yy = gt * creal(t11 + t1p - t22 - t2p) / 2. + lng;
@@ -188,12 +189,12 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
             double C = creal(m_tr(m_mul(d, d)));
             d = m_sub(phX, ptXY2);
             double D = creal(m_tr(m_mul(d, d)));
-            double mag = 2. / pow(g, 2.) * (Nc - creal(m_tr(plaq)));
-            double h = 0.5 / tau / tau;
+            double mag = 2. / pow(g, 2.) * (Nc - creal(m_tr(plaq))) * orc_mut[ORC_MUT_T_MAG];
+            double h = 0.5 / tau / tau * orc_mut[ORC_MUT_T_PHIGRAD];
             tt = (tt + mag) + h * (A + B + C + D);
             xx = (xx + mag) + h * (A + B - C - D);
             yy = (yy + mag) + h * (-A - B + C + D);
-            ee = ee + 1. / tau / tau * (-2. / pow(g, 2.) * (Nc - creal(m_tr(plaq))) + h * (A + B + C + D));
+            ee = ee + 1. / tau / tau * (-mag + h * (A + B + C + D));
             TT[s] = tt * 1 / pow(a, 4.);
             Txx[s] = xx * 1 / pow(a, 4.);
             Tyy[s] = yy * 1 / pow(a, 4.);
@@ -229,7 +230,8 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
                 m_mul(pYm, m_sub(m_mul3(UxpY, phXY, UDxpY), m_mul3(UDxmXpY, phmXpY, UxmXpY)))),
                 m_mul(pXm, m_sub(m_mul3(UxpX, ph2X, UDxpX), m_mul3(Udx, ph, ux)))),
                 m_mul(pXYm, m_sub(m_mul3(UxpXpY, ph2XY, UDxpXpY), m_mul3(UDxpY, phY, UxpY))));
-            double ttx = +2. / tau / 8. * cimag(m_tr(fx)) - 2. / 8. / tau * creal(m_tr(kx));
+            double ttx = +2. / tau / 8. * cimag(m_tr(fx)) * orc_mut[ORC_MUT_TX_F] -
+                         2. / 8. / tau * creal(m_tr(kx)) * orc_mut[ORC_MUT_TX_K];
 
             M3 Q1 = m_mul4(ux, UypX, UDxpY, Udy), Q2 = m_mul4(uy, UxpY, UDypX, Udx);
             M3 Q3 = m_mul4(UDymY, UxmY, UypXmY, Udx), Q4 = m_mul4(ux, UDypXmY, UDxmY, UymY);
@@ -241,14 +243,15 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
                 m_mul(pXm, m_sub(m_mul3(UypX, phXY, UDypX), m_mul3(UDypXmY, phpXmY, UypXmY)))),
                 m_mul(pYm, m_sub(m_mul3(UypY, ph2Y, UDypY), m_mul3(Udy, ph, uy)))),
                 m_mul(pXYm, m_sub(m_mul3(UypXpY, phX2Y, UDypXpY), m_mul3(UDypX, phX, UypX))));
-            double tty = +2. / tau / 8. * cimag(m_tr(fy)) - 2. / 8. / tau * creal(m_tr(ky));
+            double tty = +2. / tau / 8. * cimag(m_tr(fy)) * orc_mut[ORC_MUT_TY_F] -
+                         2. / 8. / tau * creal(m_tr(ky)) * orc_mut[ORC_MUT_TY_K];
 
             M3 gX = m_sub(m_mul3(ux, phX, Udx), ph);       /* Ux phiX Ux^+ - phi */
             M3 gXY1 = m_sub(m_mul3(UxpY, phXY, UDxpY), phY);
             M3 gY = m_sub(m_mul3(uy, phY, Udy), ph);
             M3 gXY2 = m_sub(m_mul3(UypX, phXY, UDypX), phX);
             M3 fe = m_add(m_add(m_add(m_mul(e1, gX), m_mul(e1p, gXY1)), m_mul(e2, gY)), m_mul(e2p, gXY2));
-            double tte = g / tau / tau / tau * creal(m_tr(fe));
+            double tte = g / tau / tau / tau * creal(m_tr(fe)) * orc_mut[ORC_MUT_TE];
 
             M3 el = m_mul(m_scale(-1. / 4. * g * g, m_add(e1, m_mul3(uy, e1p, Udy))),
                           m_add(e2, m_mul3(ux, e2p, Udx)));
@@ -256,7 +259,8 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
             M3 b2 = m_mul(m_mul(m_mul(uy, gXY1), Udy), gY);
             M3 b3 = m_mul(m_mul(m_mul(gX, ux), gXY2), Udx);
             M3 b4 = m_mul(m_mul(m_mul(m_mul(m_mul(uy, gXY1), Udy), ux), gXY2), Udx);
-            M3 fxy = m_add(el, m_scale(1. / 4., m_add(m_add(m_add(b1, b2), b3), b4)));
+            M3 fxy = m_add(m_scale(orc_mut[ORC_MUT_XY_E], el),
+                           m_scale(1. / 4. * orc_mut[ORC_MUT_XY_B], m_add(m_add(m_add(b1, b2), b3), b4)));
             double txy = 2. / tau / tau * creal(m_tr(fxy));
 
             M3 R1 = m_mul4(ux, UypX, UDxpY, Udy), R2 = m_mul4(uy, UxpY, UDypX, Udx);
@@ -269,7 +273,7 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
             double xe2 = 1. / 8. / g *
                          cimag(m_tr(m_add(m_mul(tlquad(R1, R2, R3, R4), gY),
                                           m_mul(tlquad(R5, R6, R7, R8), gXY2))));
-            double txe = -2. / tau / tau * (xe1 + xe2);
+            double txe = -2. / tau / tau * (xe1 * orc_mut[ORC_MUT_XE_E] + xe2 * orc_mut[ORC_MUT_XE_B]);
 
             M3 S1 = m_mul4(uy, UxpY, UDypX, Udx), S2 = m_mul4(ux, UypX, UDxpY, Udy);
             M3 S3 = m_mul4(ux, UDypXmY, UDxmY, UymY), S4 = m_mul4(UDymY, UxmY, UypXmY, Udx);
@@ -281,7 +285,7 @@ void orc_tmunu(int N, double g, double a, double dtau, int it, const double *Ux,
             double ye2 = 1. / 8. / g *
                          cimag(m_tr(m_add(m_mul(tlquad(S1, S2, S3, S4), gX),
                                           m_mul(tlquad(S5, S6, S7, S8), gXY1))));
-            double tye = -2. / tau / tau * (ye1 + ye2);
+            double tye = -2. / tau / tau * (ye1 * orc_mut[ORC_MUT_YE_E] + ye2 * orc_mut[ORC_MUT_YE_B]);
 
             Ttx[s] = ttx * 1 / pow(a, 4.);
             Tty[s] = tty * 1 / pow(a, 4.);

@@ -3,8 +3,10 @@
  *
  * Follows, in draw order:
  *   Init.cpp:2320-2345   b = sqrt((bmax^2-bmin^2) xb + bmin^2) (or linear in xb)
- *   Init.cpp:65-439      sampleTA: p at the origin, Woods-Saxon nuclei, then
- *                        rotate_nucleus_3D on A and on B
+ *   Init.cpp:65-439      sampleTA: p at the origin, the deuteron (90-152),
+ *                        Woods-Saxon nuclei, then rotate_nucleus_3D on A and B
+ *   Glauber.cpp:1188-1243 SampleTARejection (deuteron p-n separation) with the
+ *                        Hulthen thickness of Glauber.cpp:513-576, 847-1100
  *   Init.cpp:2721-3037   generate_nucleus_configuration{,_with_woods_saxon,
  *                        _with_deformed_woods_saxon{,2,_force_dmin}}
  *   Init.cpp:2818-2831   sample_r_from_woods_saxon
@@ -26,6 +28,130 @@
 #include "ipg_oracle.h"
 
 static const double HBARC = 0.1973269718; /* Phys_consts.h:6 */
+
+/* ---- deuteron ------------------------------------------------------------ */
+/* the Hulthen density (1/r^2)(e^{-a r} - e^{-w r})^2, normalised to 2
+ * nucleons, integrated over z in xi = e^{-a z} as the reference writes the
+ * integrand (Glauber.cpp:874-903); s in fm */
+typedef struct {
+    double a, w, rho, sigma, s;
+} hulthen_t;
+
+static double hulthen_f(const hulthen_t *h, double xi) {
+    if (xi == 0.0) xi = 1.0e-10;
+    if (xi == 1.0) xi = 1.0 - 1.0e-10;
+    double sa = h->s * h->a;
+    double z = -log(xi);
+    double r = sqrt(sa * sa + z * z);
+    double v = 2.0 * h->a * h->rho * h->sigma;
+    double g = (1.0 / r) * (exp(-r) - exp(-(h->w / h->a) * r));
+    v *= g * g;
+    return v;
+}
+
+/* Glauber::qnc7 (Glauber.cpp:946-1100) */
+static double qnc7(const hulthen_t *h, double tol, double down, double dx, const double *f_of, double pre_sum,
+                   double area, int *count) {
+    static const double w[7] = {41.0 / 140.0, 54.0 / 35.0, 27.0 / 140.0, 68.0 / 35.0,
+                                27.0 / 140,   54.0 / 35.0, 41.0 / 140.0};
+    double fl[7], fr[7], left_sum = 0.0, right_sum = 0.0, ans;
+    dx /= 2.0;
+    fl[1] = hulthen_f(h, down + dx);
+    fl[3] = hulthen_f(h, down + 3.0 * dx);
+    fl[5] = hulthen_f(h, down + 5.0 * dx);
+    fl[0] = f_of[0];
+    fl[2] = f_of[1];
+    fl[4] = f_of[2];
+    fl[6] = f_of[3];
+    *count += 3;
+    for (int i = 0; i < 7; i++) left_sum += w[i] * fl[i];
+    left_sum *= dx;
+    fr[1] = hulthen_f(h, down + 7.0 * dx);
+    fr[3] = hulthen_f(h, down + 9.0 * dx);
+    fr[5] = hulthen_f(h, down + 11.0 * dx);
+    fr[0] = f_of[3];
+    fr[2] = f_of[4];
+    fr[4] = f_of[5];
+    fr[6] = f_of[6];
+    *count += 3;
+    for (int i = 0; i < 7; i++) right_sum += w[i] * fr[i];
+    right_sum *= dx;
+    ans = left_sum + right_sum;
+    area += -fabs(pre_sum) + fabs(left_sum) + fabs(right_sum);
+    if (fabs(ans - pre_sum) > tol * fabs(area) && (*count < 10000)) {
+        tol /= 1.414213562;
+        left_sum = qnc7(h, tol, down, dx, fl, left_sum, area, count);
+        right_sum = qnc7(h, tol, down + dx * 6., dx, fr, right_sum, area, count);
+        ans = left_sum + right_sum;
+    }
+    return ans;
+}
+
+/* InterNuPInSP / InterNuTInST: the table of NuInS over [0, 2 SCutOff] with
+ * InterMax = 100 intervals (initGlauber, Glauber.cpp:1135-1180), cubic
+ * through four nodes (FourPtInterpolate, LinearFindXorg) */
+enum { DT_IMAX = 100 };
+/* the last table built (the sampler calls this per trial); not thread-safe,
+ * like the sampler's generator */
+static struct {
+    int valid;
+    double a, w, R0, sigma;
+    double vx[DT_IMAX + 1], vy[DT_IMAX + 1];
+} dtab;
+
+double orc_deuteron_thickness(double s, double a, double wr, double R0, double sigma_fm2) {
+    enum { IMAX = DT_IMAX };
+    double up = 2.0 * 12.0, dxt = (up - 0.0) / IMAX;
+    if (!(dtab.valid && dtab.a == a && dtab.w == wr && dtab.R0 == R0 && dtab.sigma == sigma_fm2)) {
+        double f0n = 2.0 * (a - wr) * (a - wr) * M_PI / a / wr / (a + wr);
+        double f = (f0n * R0) / R0; /* AnumHulthen() / rho_WS with rho_WS = R_WS */
+        hulthen_t h = {a, wr, 2.0 / f, sigma_fm2, 0.};
+        for (int i = 0; i <= IMAX; i++) {
+            dtab.vx[i] = dxt * i;
+            h.s = dtab.vx[i];
+            double g1[7], dx = (1.0 - 0.0) / 6.0;
+            for (int k = 0; k < 7; k++) g1[k] = hulthen_f(&h, 0.0 + k * dx);
+            int count = 7;
+            dtab.vy[i] = qnc7(&h, 1.0e-6, 0.0, dx, g1, 0.0, 0.0, &count);
+        }
+        dtab.a = a;
+        dtab.w = wr;
+        dtab.R0 = R0;
+        dtab.sigma = sigma_fm2;
+        dtab.valid = 1;
+    }
+    const double *vx = dtab.vx, *vy = dtab.vy;
+    if (s > up) return 0.0;
+    double nx = IMAX * (s - vx[0]) / (vx[IMAX] - vx[0]);
+    int x_org = (int)nx - 1;
+    if (x_org <= 0) x_org = 0;
+    else if (x_org >= IMAX - 3) x_org = IMAX - 3;
+    double hh = (vx[IMAX] - vx[0]) / IMAX;
+    double f0 = vy[x_org], f1 = vy[x_org + 1], f2 = vy[x_org + 2], f3 = vy[x_org + 3];
+    double ca = (-f0 + 3.0 * f1 - 3.0 * f2 + f3) / (6.0 * hh * hh * hh);
+    double cb = (2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3) / (2.0 * hh * hh);
+    double cc = (-11.0 * f0 + 18.0 * f1 - 9.0 * f2 + 2.0 * f3) / (6.0 * hh);
+    double y = ca * pow(s - vx[x_org], 3.);
+    y += cb * pow(s - vx[x_org], 2.);
+    y += cc * (s - vx[x_org]);
+    y += f0;
+    return y < 0.0 ? 0.0 : y;
+}
+
+/* Glauber::SampleTARejection (Glauber.cpp:1188-1243) */
+static void sample_deuteron(orc_mt *rng, double a, double R0, double sigma_fm2, double *x, double *y) {
+    const double wr = 1.18; /* Glauber.cpp:72, w_WS of "d" */
+    double A = 1.2 * sigma_fm2 / 4.21325504715;
+    double phi, r, tmp;
+    do {
+        phi = 2. * M_PI * orc_mt_real1(rng);
+        r = 6.32456 * sqrt(-log((-0.00454545 * (-220. * A + A * 220. * (1. - exp(-0.025 * 15. * 15.)) *
+                                                                 orc_mt_real1(rng))) / A));
+        tmp = orc_mt_real1(rng);
+    } while (tmp > r * orc_deuteron_thickness(r, a, wr, R0, sigma_fm2) / (A * r * 11. * exp(-r * r / 40.)));
+    *x = r * cos(phi);
+    *y = r * sin(phi);
+}
 
 /* stable sort of idx by key (ascending), insertion sort */
 static void stable_sort_idx(int n, const double *key, int *idx) {
@@ -224,18 +350,28 @@ static void rotate3d(orc_mt *rng, int A, orc_nucleon *n) {
 
 int orc_sample_event(orc_mt *rng, const orc_glauber_cfg *c, orc_nucleon *nA,
                      orc_nucleon *nB, orc_event_info *info) {
-    if (c->A1 == 2 || c->A2 == 2) return -1; /* deuteron: unsupported here */
+    if (c->A1 == 3 || c->A2 == 3) return -1; /* He3: unsupported here */
     double xb = orc_mt_real1(rng);
     double b = c->linear_b ? sqrt((c->bmax * c->bmax - c->bmin * c->bmin) * xb + c->bmin * c->bmin)
                            : (c->bmax - c->bmin) * xb + c->bmin;
     if (c->A1 == 1) {
         nA[0] = (orc_nucleon){0, 0, 0, 0, 0, 1};
+    } else if (c->A1 == 2) {
+        double dx, dy;
+        sample_deuteron(rng, c->a1, c->R1, 0.1 * c->sigma_nn_mb, &dx, &dy);
+        nA[0] = (orc_nucleon){dx / 2., dy / 2., 0., 0, 0, 1};
+        nA[1] = (orc_nucleon){-(dx / 2.), -(dy / 2.), -0., 0, 0, 1};
     } else {
         gen_nucleus(rng, c->A1, c->Z1, c->a1, c->R1, c->beta2_1, c->beta3_1, c->beta4_1,
                     c->gamma_1, c->force_dmin, c->d_min, c->dR1, c->da1, nA);
     }
     if (c->A2 == 1) {
         nB[0] = (orc_nucleon){0, 0, 0, 0, 0, 1};
+    } else if (c->A2 == 2) {
+        double dx, dy;
+        sample_deuteron(rng, c->a2, c->R2, 0.1 * c->sigma_nn_mb, &dx, &dy);
+        nB[0] = (orc_nucleon){dx / 2., dy / 2., 0., 0, 0, 1};
+        nB[1] = (orc_nucleon){-(dx / 2.), -(dy / 2.), -0., 0, 0, 0};
     } else {
         gen_nucleus(rng, c->A2, c->Z2, c->a2, c->R2, c->beta2_2, c->beta3_2, c->beta4_2,
                     c->gamma_2, c->force_dmin, c->d_min, c->dR2, c->da2, nB);

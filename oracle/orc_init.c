@@ -136,6 +136,15 @@ static void solve8(double *J /* row-major 8x8, destroyed */, const double *F, do
     }
 }
 
+/* extra Newton iterations after the stopping rule holds (the library's
+ * newton_polish option); 0 is the reference's rule.  They are exact Newton
+ * steps about the current iterate: U <- exp(i beta^a t^a) U with
+ * J_ab = -dF_a/dbeta_b = -2 Re tr(t^a W U^+ t^b) at beta = 0, which
+ * converges quadratically (the reference's forward-difference Jacobian in
+ * alpha, with steps of at least 1e-3, does not converge past ~1e-9). */
+static int newton_polish = 0;
+void orc_set_newton_polish(int n) { newton_polish = n < 0 ? 0 : n; }
+
 int orc_newton_link(const double *pU1, const double *pU2, uint64_t seed, uint32_t link,
                     double *pUsol, orc_newton_stat *st) {
     const int maxIterations = 2000, maxRetrys = 200;
@@ -157,7 +166,32 @@ int orc_newton_link(const double *pU1, const double *pU2, uint64_t seed, uint32_
     M3 Ud = m_dag(Usol);
     int iter = 0, nRestart = 0, total = 0, analytic = 0;
     uint32_t ndraw = 0; /* restart draws taken so far on this link */
-    while (Fzero > 1e-6 && iter < maxIterations && nRestart < maxRetrys) {
+    int npol = 0;
+    for (;;) {
+        if (!(Fzero > 1e-6 && iter < maxIterations && nRestart < maxRetrys)) {
+            if (!(nRestart != maxRetrys && (Fzero <= 1e-6 || npol > 0) && npol < newton_polish)) break;
+            npol++;
+        }
+        if (npol > 0) {
+            /* exact Newton step about the current iterate */
+            iter++;
+            total++;
+            M3 Mt = m_sub(m_mul(W, Ud), m_mul(Usol, Wd));
+            Fzero = 0.;
+            for (int a = 0; a < 8; a++) {
+                Fa[a] = cimag((-1.) * (trc[a] + m_trprod(t[a], Mt)));
+                Fzero += fabs(Fa[a]);
+            }
+            M3 WUd = m_mul(W, Ud);
+            for (int b = 0; b < 8; b++) {
+                M3 X = m_mul(WUd, t[b]);
+                for (int a = 0; a < 8; a++) Jab[8 * a + b] = -2. * creal(m_trprod(t[a], X));
+            }
+            solve8(Jab, Fa, dal);
+            Usol = m_mul(orc_exp_iQt(dal), Usol);
+            Ud = m_dag(Usol);
+            continue;
+        }
         iter++;
         total++;
         Fzero = 0.;
@@ -311,7 +345,7 @@ int orc_init_fields(int N, double g, double *VA, double *VB, uint64_t seed, doub
         Udg = m_dag(U);
         acc = m_sub(m_add(m_add(m_sub(acc, m_mul(Udg, D)), D), m_mul(Dd, U)), Dd);
         M3 Aeta = m_scale(1. / 8., acc);
-        M3 p = m_cscale(CMPLX(0., -2. / g), Aeta);
+        M3 p = m_cscale(CMPLX(0., -2. / g * orc_mut[ORC_MUT_PI_INIT]), Aeta);
         m_store(&pi[18 * (size_t)s], p);
         m_store(&E1[18 * (size_t)s], m_zero());
         m_store(&E2[18 * (size_t)s], m_zero());

@@ -14,6 +14,11 @@
 
 typedef double complex cplx;
 
+/* coefficient factors for mutation tests (orc_set_mutation); all 1.0 unless
+ * a test plants a deliberate mistake, and x * 1.0 == x, so they change
+ * nothing otherwise */
+extern double orc_mut[32];
+
 typedef struct {
     cplx e[9];
 } M3;

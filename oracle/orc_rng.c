@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include "ipg_oracle.h"
+#include "orc_internal.h"
 
 #define MT_N 312
 #define MT_M 156
@@ -176,4 +177,35 @@ void orc_newton_restart_normals(uint64_t seed, uint32_t link, uint32_t restart,
         orc_philox4x32(ctr, key, w);
         box_muller(w, &alpha[2 * p], &alpha[2 * p + 1]);
     }
+}
+
+/* ---- mutation factors ------------------------------------------------------ */
+#define ORC_ONES4 1.0, 1.0, 1.0, 1.0
+double orc_mut[32] = {ORC_ONES4, ORC_ONES4, ORC_ONES4, ORC_ONES4, ORC_ONES4, ORC_ONES4, ORC_ONES4, ORC_ONES4};
+
+void orc_set_mutation(int k, double factor) {
+    if (k >= 0 && k < 32) orc_mut[k] = factor;
+}
+void orc_clear_mutations(void) {
+    for (int k = 0; k < 32; k++) orc_mut[k] = 1.0;
+}
+
+/* OpenMP threads of the parallel oracle loops (the CPU baseline times the
+ * phases the reference runs serially with one thread) */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

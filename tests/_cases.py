@@ -14,6 +14,7 @@ NUCLEI = {
     "Pb": (208, 82, 6.62, 0.546, 0.0, 0.0, 0.0),
     "O": (16, 8, 2.608, 0.513, -0.01, 0.0, -0.122),
     "p": (1, 1, 1.0, 1.0, 0.0, 0.0, 0.0),
+    "d": (2, 1, 1.0, 0.228, 0.0, 0.0, 0.0),  # Hulthen: a in the a_WS slot, R_WS 1
 }
 
 # (projectile, target, R_WS, a_WS, sigma_NN, roots, L, bmin, bmax, linear_b)
@@ -23,6 +24,7 @@ SYSTEMS = {
     "OO": ("O", "O", 2.61, 0.513, 70.0, 5020.0, 15.0, 0.0, 3.0, 1),
     "pPb": ("p", "Pb", 6.62, 0.546, 70.0, 5020.0, 10.0, 0.0, 5.0, 0),
     "PbPbdef": ("Pb", "Pb", 6.62, 0.546, 70.0, 5020.0, 30.0, 0.0, 4.0, 1),
+    "dAu": ("d", "Au", 6.38, 0.535, 42.0, 200.0, 20.0, 0.0, 3.0, 1),
 }
 
 
@@ -113,3 +115,15 @@ def oracle_color_charge(p, nA, nB, table):
     T, Q = table
     O.lib().orc_g2mu2(C.byref(qc), T, Q, TpA, TpB, gA, gB)
     return TpA, TpB, gA, gB
+
+
+def c3_params():
+    """The 64-event heterogeneous batch: 16 p+Pb (N=256, L=10 fm, b uniform
+    in [0,5] fm), 16 O+O (N=256, L=15 fm, R=2.61 fm, a=0.513 fm), 24 Pb+Pb
+    (N=512, L=30 fm, b uniform in [0,12] fm), 8 Pb+Pb (N=768, L=30 fm, b=0);
+    Ny=50, tau_max=0.4 fm/c."""
+    def mk(N, L, pr, tg, R, a, bmax):
+        return params("PbPb", N=N, Ny=50, L=L, Projectile=pr, Target=tg, R_WS=R, a_WS=a, bmin=0.0, bmax=bmax,
+                      samplebFromLinearDistribution=0)
+    return ([mk(256, 10.0, "p", "Pb", 6.62, 0.546, 5.0)] * 16 + [mk(256, 15.0, "O", "O", 2.61, 0.513, 0.0)] * 16 +
+            [mk(512, 30.0, "Pb", "Pb", 6.62, 0.546, 12.0)] * 24 + [mk(768, 30.0, "Pb", "Pb", 6.62, 0.546, 0.0)] * 8)

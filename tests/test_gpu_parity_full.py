@@ -1,0 +1,246 @@
+"""GPU parity at the workload sizes: the HIP path through the C ABI against
+the C oracle on the same seeded inputs, with the full lattice and Ny = 50.
+
+  Au+Au b = 5 fm, N = 256, L = 30 fm, Ny = 50, tau_max = 0.4 fm/c (35 steps),
+      colour-charge normals from the reference's MT19937-64 Gauss stream in
+      its draw order (Init.cpp:1697-1705, continuing after the nucleon draws):
+      nucleons and N_coll/N_part bit-exact, Wilson lines, root finder,
+      dE/deta within 1e-9, pointwise T^{mu nu} within 1e-6 of each
+      component's maximum, the Gauss-law maximum;
+  Pb+Pb b = 0, N = 512, L = 30 fm, Ny = 50 (69 steps), device normals: the
+      same chain from the seed, and the evolution from the device's tau = 0
+      state;
+  deformed Au (the Glauber.cpp defaults beta2 = -0.13, beta4 = -0.03):
+      nucleons bit-exact and one event end to end;
+  the 64-event heterogeneous batch (16 p+Pb N=256 L=10 fm, 16 O+O N=256
+      L=15 fm, 24 Pb+Pb N=512 b < 12 fm, 8 Pb+Pb N=768): batched equal to
+      one launch sequence per event, and one p+Pb and one O+O event of it
+      against the oracle.
+The oracle is OpenMP C (oracle/); each case costs it seconds.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HBARC = 0.1973269718
+_TABLE = None
+
+
+def table():
+    global _TABLE
+    if _TABLE is None:
+        import ipglasma_amd as I
+        _TABLE = K.load_table(I.QS_TABLE)
+    return _TABLE
+
+
+def say(msg):
+    print(f"[parity-full] {msg}", file=sys.stderr, flush=True)
+
+
+def tmunu_check(Tg, To, tol=1e-6):
+    worst = 0.0
+    for c in range(10):
+        scale = np.abs(To[c]).max()
+        if scale == 0:
+            assert np.abs(Tg[c]).max() == 0
+            continue
+        err = np.abs(Tg[c] - To[c]).max() / scale
+        worst = max(worst, err)
+        assert err < tol, (c, err)
+    return worst
+
+
+def oracle_setv(p, gA, gB, z=None, rho_seed=0):
+    N = p.size
+    S = N * N
+    V = np.zeros((2, S, 18))
+    m_lat = p.m * (p.L / N) / HBARC
+    O.lib().orc_setv(N, p.Ny, p.g, m_lat, 0.0, gA, gB, None if z is None else z.ctypes.data, None, rho_seed,
+                     V.reshape(-1))
+    return V
+
+
+def oracle_fields(p, VA, VB, newton_seed):
+    N = p.size
+    S = N * N
+    ref = {k: np.zeros((S, 18)) for k in O.FIELDS}
+    stats = np.zeros(5)
+    O.lib().orc_init_fields(N, p.g, VA.copy(), VB.copy(), newton_seed, *[ref[k] for k in O.FIELDS], stats)
+    return ref, stats
+
+
+def oracle_run(p, state):
+    N = p.size
+    T = np.zeros((10, N * N))
+    obs = np.zeros(3)
+    st = {k: v.copy() for k, v in state.items()}
+    O.lib().orc_run(N, p.g, p.L, p.dtau, p.itmax, *[st[k] for k in O.FIELDS], T.reshape(-1), obs)
+    return T, obs, st
+
+
+def run_chain(p, seed, mt_normals):
+    """device event through Init::init stages and Evolution::run, with the
+    oracle chain on the same seed"""
+    import ipglasma_amd as I
+    N = p.size
+    S = N * N
+    ev = I.Event(p, seed=seed)
+    ev.init_nucleons()
+    A, B = ev.nucleons()
+    o = ev.observables()
+    st, nA, nB, info = K.oracle_event(p, seed)
+    np.testing.assert_array_equal(A, K.nucleon_array(nA))
+    np.testing.assert_array_equal(B, K.nucleon_array(nB))
+    assert (o["Ncoll"], o["Npart"]) == (info.Ncoll, info.Npart)
+    assert o["b"] == info.b and o["phiRP"] == info.phiRP
+    say(f"N={N}: nucleons bit-exact, Npart {info.Npart}, Ncoll {info.Ncoll}")
+    assert ev.init_color_charge()
+    TpA, TpB, gA, gB = K.oracle_color_charge(p, nA, nB, table())
+    np.testing.assert_allclose(ev.field("TpA"), TpA, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(ev.field("TpB"), TpB, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(ev.field("g2mu2A"), gA, rtol=1e-9, atol=0)
+    np.testing.assert_allclose(ev.field("g2mu2B"), gB, rtol=1e-9, atol=0)
+    ev.init_wilson_lines()
+    if mt_normals:
+        # the MT Gauss stream continues after the nucleon draws
+        z = np.zeros(2 * p.Ny * S * 8)
+        O.lib().orc_mt_fill(C.byref(st), 4, z.size, z)
+        V = oracle_setv(p, gA, gB, z=z)
+        del z
+    else:
+        V = oracle_setv(p, gA, gB, rho_seed=o["rho_seed"])
+    VA, VB = ev.field("VA"), ev.field("VB")
+    np.testing.assert_allclose(VA, V[0], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(VB, V[1], rtol=0, atol=1e-10)
+    say(f"N={N}: Wilson lines agree, max |dV| {max(np.abs(VA - V[0]).max(), np.abs(VB - V[1]).max()):.2e}")
+    ev.init_fields()
+    o = ev.observables()
+    assert o["newton_unconverged"] == 0
+    dev_state = {k: ev.field(k).copy() for k in O.FIELDS}
+    # the oracle's own chain from its own Wilson lines
+    ref, stats = oracle_fields(p, V[0], V[1], o["newton_seed"])
+    del V
+    assert stats[3] == 0
+    for k in ("Ux", "Uy", "pi"):
+        np.testing.assert_allclose(dev_state[k], ref[k], rtol=0, atol=1e-7, err_msg=k)
+    say(f"N={N}: tau=0 fields agree to the solver tolerance "
+        f"(max |dU| {np.abs(dev_state['Ux'] - ref['Ux']).max():.2e})")
+    ev.run()
+    o = ev.observables()
+    Tg = ev.tmunu()
+    T, obs, _ = oracle_run(p, ref)
+    rel = abs(o["dEdeta"] - obs[0]) / abs(obs[0])
+    assert rel <= 1e-9, (o["dEdeta"], obs[0], rel)
+    worst = tmunu_check(Tg, T)
+    assert o["gauss_max"] == pytest.approx(obs[1], rel=1e-6, abs=1e-14)
+    say(f"N={N}: end to end dE/deta {o['dEdeta']:.10g} GeV rel {rel:.1e}, T^munu worst {worst:.1e}, "
+        f"Gauss max {o['gauss_max']:.3e} vs {obs[1]:.3e}")
+    return ev, dev_state, Tg, o
+
+
+def test_c1_auau_n256_reference_rho_stream_end_to_end():
+    p = K.params("AuAu", N=256, Ny=50, seed=5, rho_rng=1)
+    assert p.itmax == 35
+    ev, dev_state, Tg, o = run_chain(p, 5, mt_normals=True)
+    # the evolution alone from the device's tau = 0 state
+    T, obs, fin = oracle_run(p, dev_state)
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-12 * abs(obs[0])
+    tmunu_check(Tg, T, tol=1e-9)
+    for k in O.FIELDS:
+        np.testing.assert_allclose(ev.field(k), fin[k], rtol=0, atol=1e-10, err_msg=k)
+    ev.close()
+
+
+def test_c2_pbpb_n512_end_to_end_and_from_device_state():
+    p = K.params("PbPb", N=512, Ny=50, seed=7)
+    assert p.itmax == 69
+    ev, dev_state, Tg, o = run_chain(p, 7, mt_normals=False)
+    T, obs, _ = oracle_run(p, dev_state)
+    assert abs(o["dEdeta"] - obs[0]) <= 1e-12 * abs(obs[0])
+    tmunu_check(Tg, T, tol=1e-9)
+    ev.close()
+
+
+def test_deformed_au_nucleons_and_event():
+    """setWSDeformParams 0: Au keeps R = 6.37, a = 0.535, beta2 = -0.13,
+    beta4 = -0.03 (Glauber.cpp:19-287) and is sampled by the deformed
+    Woods-Saxon sampler (Init.cpp:2833-2904)."""
+    for seed in (3, 4, 5):
+        p = K.params("AuAu", N=32, Ny=3, seed=seed, ws_override=False)
+        import ipglasma_amd as I
+        ev = I.Event(p, seed=seed)
+        ev.init_nucleons()
+        A, B = ev.nucleons()
+        st, nA, nB, info = K.oracle_event(p, seed)
+        np.testing.assert_array_equal(A, K.nucleon_array(nA))
+        np.testing.assert_array_equal(B, K.nucleon_array(nB))
+        assert ev.observables()["Ncoll"] == info.Ncoll
+        ev.close()
+    p = K.params("AuAu", N=64, Ny=4, seed=6, ws_override=False)
+    ev, _, _, _ = run_chain(p, 6, mt_normals=False)
+    ev.close()
+
+
+def test_c3_heterogeneous_batch_matches_plain_and_oracle():
+    import ipglasma_amd as I
+    ps = K.c3_params()
+    assert len(ps) == 64
+    seeds = list(range(5000, 5064))
+    b = I.Batch(ps, seeds)
+    b.run(use_graph=True)
+    batched = [b.observables(e) for e in range(64)]
+    b.close()
+    say("64-event batch done")
+    picks = {0: None, 16: None}  # one p+Pb (N=256, L=10 fm) and one O+O (N=256, L=15 fm)
+    for e, (p, s) in enumerate(zip(ps, seeds)):
+        ev = I.Event(p, seed=s)
+        tries = 0
+        while not ev.init():
+            tries += 1
+            assert tries < 50
+        if e in picks:
+            picks[e] = {k: ev.field(k).copy() for k in O.FIELDS}
+        ev.run()
+        o = ev.observables()
+        assert o["Ncoll"] == batched[e]["Ncoll"] and o["Npart"] == batched[e]["Npart"], e
+        assert o["dEdeta"] == pytest.approx(batched[e]["dEdeta"], rel=1e-12), e
+        assert o["eccentricity"][1] == pytest.approx(batched[e]["eccentricity"][1], rel=1e-9, abs=1e-12), e
+        if e in picks:
+            # 103 (p+Pb) and 69 (O+O) steps from the same tau = 0 state
+            T, obs, _ = oracle_run(p, picks[e])
+            assert abs(o["dEdeta"] - obs[0]) <= 1e-10 * abs(obs[0]), e
+            tmunu_check(ev.tmunu(), T, tol=1e-9)
+        ev.close()
+    say("plain path agrees event by event")
+
+
+@pytest.mark.parametrize("ws_override", [False, True])
+def test_deuteron_gold_nucleons_and_event(ws_override):
+    """d+Au: the deuteron's proton-neutron separation by rejection sampling
+    against its Hulthen thickness (Glauber.cpp:1188-1243, Init.cpp:90-152),
+    nucleons bit-exact over several events; with setWSDeformParams 1 the
+    input's R_WS and a_WS replace the deuteron's too (Glauber.cpp:251-262)"""
+    import ipglasma_amd as I
+    for seed in (2, 3, 4, 5):
+        p = K.params("dAu", N=32, Ny=3, seed=seed, ws_override=ws_override)
+        ev = I.Event(p, seed=seed)
+        ev.init_nucleons()
+        A, B = ev.nucleons()
+        st, nA, nB, info = K.oracle_event(p, seed)
+        assert A.shape[0] == 2
+        np.testing.assert_array_equal(A, K.nucleon_array(nA))
+        np.testing.assert_array_equal(B, K.nucleon_array(nB))
+        o = ev.observables()
+        assert (o["Ncoll"], o["Npart"], o["b"]) == (info.Ncoll, info.Npart, info.b)
+        ev.close()
+    p = K.params("dAu", N=64, Ny=3, seed=8, ws_override=ws_override)
+    ev, _, _, _ = run_chain(p, 8, mt_normals=False)
+    ev.close()

@@ -18,6 +18,12 @@ Printed JSON fields beyond metric, value, unit, steps, config and the timing key
   small_events        64 p+Pb events at N=128, batched against plain: where one
                       event leaves most of the GPU idle and the plain path is
                       bound by launch latency
+  adjoint             reverse mode to rho against the forward event, Au+Au
+                      N=256 and Pb+Pb N=512 (adjoint_pbpb_n512)
+  large_batched_adjoint  8 Pb+Pb N=1024 Ny=100 events, one batched backward
+                      per loss, events/s forward+backward and peak memory
+  gluon_spectra       the evolution with the Coulomb gauge fix and spectra
+                      after steps 1 and itmax, against the plain evolution
   cpu_baseline        the C oracle (OpenMP, the host's threads) on whole
                       events per phase, median of 3, at Pb+Pb N=512 (the
                       value) and Au+Au N=256, with the CPU model
@@ -88,12 +94,14 @@ def batch_vs_plain(I, ps, seeds, label):
             "batched_vs_plain_max_rel_dEdeta": rel}
 
 
-def adjoint_timing(I, seed=77):
-    """Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c: one forward event
-    (initial conditions + evolution) against the reverse mode to rho
-    (initial conditions + ipg_adjoint_rho, whose forward pass is included)."""
-    p = pbpb_params(I, N=256, L=30.0)
-    p.set(Projectile="Au", Target="Au", R_WS=6.38, a_WS=0.535, SigmaNN=42.0, roots=200.0, bmin=5.0, bmax=5.0)
+def adjoint_timing(I, seed=77, system="AuAu", N=256):
+    """Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c (or Pb+Pb b=0 at
+    N=512): one forward event (initial conditions + evolution) against the
+    reverse mode to rho (initial conditions + ipg_adjoint_rho, whose forward
+    pass is included)."""
+    p = pbpb_params(I, N=N, L=30.0)
+    if system == "AuAu":
+        p.set(Projectile="Au", Target="Au", R_WS=6.38, a_WS=0.535, SigmaNN=42.0, roots=200.0, bmin=5.0, bmax=5.0)
     def timed(adj):
         ev = I.Event(p, seed=seed)
         t0 = time.perf_counter()
@@ -122,13 +130,94 @@ def adjoint_timing(I, seed=77):
     i_f, e_f, _, Lf = timed(False)
     i_a, e_a, f_a, La = timed(True)
     i_e, e_e, _, Le = timed("eps2")
-    return {"workload": "Au+Au b=5 fm, N=256, L=30 fm, Ny=50, tau_max=0.4 fm/c, d(dE/deta)/d rho",
+    wl = ("Au+Au b=5 fm" if system == "AuAu" else "Pb+Pb b=0") + f", N={N}, L=30 fm, Ny=50, tau_max=0.4 fm/c"
+    return {"workload": wl + ", d(dE/deta)/d rho and d eps_2/d rho", "itmax": p.itmax,
             "forward_ms": round(1e3 * (i_f + e_f), 2), "adjoint_ms": round(1e3 * (i_a + e_a), 2),
             "adjoint_over_forward": round((i_a + e_a) / (i_f + e_f), 2),
             "gradient_to_host_ms": round(1e3 * f_a, 2),
             "dEdeta_rel_diff": abs(La - Lf) / abs(Lf),
             "eps2_adjoint_ms": round(1e3 * (i_e + e_e), 2),
             "eps2_adjoint_over_forward": round((i_e + e_e) / (i_f + e_f), 2), "eps2": Le}
+
+
+def large_batched_adjoint(I, reps=1):
+    """8 Pb+Pb events at N=1024, L=40 fm, Ny=100, b uniform in [0,10] fm,
+    tau_max=0.6 fm/c: the batched forward pass (initial conditions +
+    evolution), then for dE/deta and for eps_2 the initial conditions of all
+    events and one batched backward to rho (ipg_batch_adjoint, the gradients
+    of all 8 events kept in HBM).  The checkpoint interval is the largest
+    the device memory allows below sqrt(T)."""
+    p = pbpb_params(I, N=1024, L=40.0, Ny=100)
+    p.set(maxtime=0.6, bmin=0.0, bmax=10.0, samplebFromLinearDistribution=0)
+    seeds = list(range(9000, 9008))
+    I.release_cached_memory()
+    out = {"workload": "8 Pb+Pb b in [0,10] fm, N=1024, L=40 fm, Ny=100, tau_max=0.6 fm/c "
+                       f"(itmax={p.itmax}); gradient 2 x 100 x 8 x 1024^2 doubles per event"}
+    for r in range(reps + 1):
+        dt_f, ob = run_batched(I, [p] * 8, seeds)
+        res = {}
+        for loss in ("dEdeta", "eps2"):
+            b = I.Batch([p] * 8, seeds)
+            t0 = time.perf_counter()
+            vals = b.adjoint(loss)
+            dt = time.perf_counter() - t0
+            peak, K = b.adjoint_info()
+            ok = True
+            if r == reps:  # one event's gradient to the host (13.4 GB), outside the timing
+                g = b.rho_grad(0, p.Ny, p.size ** 2)
+                ok = bool(np.all(np.isfinite(g)))
+                del g
+            b.close()
+            res[loss] = (dt, vals, peak, K, ok)
+            print(f"[bench] large batched adjoint {loss}: {dt:.2f} s for 8 events, K={K}, peak "
+                  f"{peak / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        print(f"[bench] large batched forward: {dt_f:.2f} s for 8 events", file=sys.stderr, flush=True)
+    I.release_cached_memory()
+    dE = res["dEdeta"]
+    e2 = res["eps2"]
+    rel = max(abs(v - o["dEdeta"]) / abs(o["dEdeta"]) for v, o in zip(dE[1], ob))
+    out.update({"forward_s": round(dt_f, 3), "forward_events_per_s": round(8 / dt_f, 3),
+                "fwd_bwd_dEdeta_s": round(dE[0], 3), "fwd_bwd_dEdeta_events_per_s": round(8 / dE[0], 3),
+                "fwd_bwd_eps2_s": round(e2[0], 3), "fwd_bwd_eps2_events_per_s": round(8 / e2[0], 3),
+                "adjoint_over_forward_dEdeta": round(dE[0] / dt_f, 2),
+                "adjoint_over_forward_eps2": round(e2[0] / dt_f, 2),
+                "peak_device_memory_GiB": round(max(dE[2], e2[2]) / 2**30, 1),
+                "checkpoint_interval": dE[3], "gradient_finite": dE[4] and e2[4],
+                "adjoint_vs_forward_max_rel_dEdeta": rel,
+                "eps2": [round(float(v), 6) for v in e2[1]]})
+    return out
+
+
+def gluon_spectra_timing(I, seed=3):
+    """Evolution::run with computeGluonMultiplicity on one Pb+Pb b=0 event at
+    N=512 (Ny=50, tau_max=0.4 fm/c): the gauge fix and the spectra after
+    steps 1 and itmax (GaugeFix::FFTChi up to 4000 steps, three passes of
+    Evolution::multiplicity), timed against the plain evolution"""
+    p = pbpb_params(I, N=512)
+    ev = I.Event(p, seed=seed)
+    while not ev.init():
+        pass
+    t0 = time.perf_counter()
+    m = ev.run_with_multiplicity()
+    dt_m = time.perf_counter() - t0
+    d_m = ev.observables()["dEdeta"]
+    ev.close()
+    ev = I.Event(p, seed=seed)
+    while not ev.init():
+        pass
+    t0 = time.perf_counter()
+    ev.run()
+    dt_r = time.perf_counter() - t0
+    d_r = ev.observables()["dEdeta"]
+    ev.close()
+    it1, itm = 1, p.itmax
+    return {"workload": f"Pb+Pb b=0, N=512, L=30 fm, Ny=50, tau_max=0.4 fm/c (itmax={p.itmax})",
+            "evolution_with_spectra_ms": round(1e3 * dt_m, 2), "evolution_ms": round(1e3 * dt_r, 2),
+            "gauge_steps": [m[it1]["gauge_iterations"], m[itm]["gauge_iterations"]],
+            "gauge_residual": [m[it1]["gauge_residual"], m[itm]["gauge_residual"]],
+            "dNdy": [m[it1]["dNdy"], m[itm]["dNdy"]], "dEdy_GeV": [m[it1]["dEdy"], m[itm]["dEdy"]],
+            "mean_pt_GeV": [m[it1]["mean_pt"], m[itm]["mean_pt"]],
+            "dEdeta_rel_diff_with_gauge_fix": abs(d_m - d_r) / abs(d_r)}
 
 
 def run_plain(I, p, seeds):
@@ -354,6 +443,12 @@ def main():
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the reverse-mode timing (for rocprofv3)")
+    ap.add_argument("--no-large-adjoint", action="store_true",
+                    help="skip the batched reverse mode of 8 Pb+Pb events at N=1024, Ny=100")
+    ap.add_argument("--large-adjoint-only", action="store_true",
+                    help="run only the large batched reverse mode (for rocprofv3)")
+    ap.add_argument("--spectra-only", action="store_true",
+                    help="run only the gauge fix + gluon spectra timing (for rocprofv3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (the HBM counter calibration)")
     a = ap.parse_args()
@@ -371,6 +466,12 @@ def main():
         return
     if a.adjoint_only:
         print(json.dumps(adjoint_timing(I)))
+        return
+    if a.large_adjoint_only:
+        print(json.dumps(large_batched_adjoint(I)))
+        return
+    if a.spectra_only:
+        print(json.dumps(gluon_spectra_timing(I)))
         return
     seeds = list(range(1000, 1000 + a.batch))
     # untimed runs until the autotuner has decided every launch variant of
@@ -410,10 +511,17 @@ def main():
         sp = small_params(I)
         small = {"workload": "64 p+Pb N=128 L=10 fm b<5 fm; Ny=50, tau_max=0.4 fm/c"}
         small.update(batch_vs_plain(I, sp, list(range(7000, 7000 + len(sp))), "small events"))
-    adj = None
+    adj = adj512 = large = None
     if not a.no_adjoint:
         adj = adjoint_timing(I)
         print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+        adj512 = adjoint_timing(I, seed=78, system="PbPb", N=512)
+        print(f"[bench] adjoint N=512: {adj512}", file=sys.stderr, flush=True)
+    if not a.no_adjoint and not a.no_large_adjoint:
+        large = large_batched_adjoint(I)
+        print(f"[bench] large batched adjoint: {large}", file=sys.stderr, flush=True)
+    spectra = gluon_spectra_timing(I)
+    print(f"[bench] gluon spectra: {spectra}", file=sys.stderr, flush=True)
     rf = roofline(I, p)
     line = {
         "metric": "events/s: batched vs per-event launch, N=512 FP64, dE/deta rel err<=1e-9",
@@ -431,6 +539,9 @@ def main():
         "heterogeneous_batch": hetero,
         "small_events": small,
         "adjoint": adj,
+        "adjoint_pbpb_n512": adj512,
+        "large_batched_adjoint": large,
+        "gluon_spectra": spectra,
         "autotune": I.autotune_report(),
     }
     if not a.no_cpu:

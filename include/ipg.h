@@ -69,6 +69,9 @@ typedef struct ipg_params {
      * writeOutputs files */
     int sizeOutput, etaSizeOutput;
     double LOutput, detaOutput;
+    /* the running-coupling keys (main.cpp:476-477, 540-541) that the last
+     * column of the NpartdNdy file evaluates */
+    double muZero, c, runWithThisFactorTimesQs;
 } ipg_params;
 
 typedef struct ipg_observables {
@@ -124,8 +127,9 @@ int ipg_version(void);
  * cleared, to the next one that needs that size.  This frees the kept
  * blocks.  No counterpart in the reference. */
 int ipg_release_cached_memory(void);
-/* sizeof(ipg_params) (which=0) or sizeof(ipg_observables) (which=1), so that
- * bindings can check their struct layout */
+/* sizeof(ipg_params) (which=0), sizeof(ipg_observables) (1) or
+ * sizeof(ipg_multiplicity) (2), so that bindings can check their struct
+ * layout */
 int ipg_struct_size(int which);
 
 /* ---- one event: Init(nn) + Evolution(nn) + Lattice (main.cpp:167-187) --- */
@@ -308,6 +312,38 @@ int ipg_write_hydro_file(const char *path, const double *land, int N, double L, 
  * (site ix*N + iy) and the centroid (avx, avy) of the first pass */
 int ipg_eccentricity_extents(const double *eps, const double *utau, int N, double L, double avx, double avy,
                              double cutoff, double *maxX, double *maxY);
+
+/* ---- Coulomb gauge and gluon spectra (Evolution::multiplicity) -------- */
+typedef struct ipg_multiplicity {
+    double dNdy, dEdy;           /* sums over the modes: "dN/dy 1", "dE/dy 1" [GeV] */
+    double dNdy_kt, dEdy_kt;     /* from the k_T histogram: "dN/dy 2", "dE/dy 2"    */
+    double dNdy_cut3, dEdy_cut3; /* histogram above k_T = 3 GeV                     */
+    double dNdy_cut6, dEdy_cut6; /* above 6 GeV                                     */
+    double mean_pt;              /* gluon <p_T> = dEdy / dNdy [GeV]                 */
+    double kt_bin;               /* bin width 2.83/100 in lattice units             */
+    double n_kt[100], E_kt[100]; /* per-bin spectra n[ik], E[ik] after averaging     */
+    int counts[100];             /* modes per bin                                   */
+    int gauge_iterations;        /* gauge-fixing steps applied                      */
+    double gauge_residual;       /* <tr(chi^+ chi)/Nc> when it stopped             */
+    double tau_fm;
+} ipg_multiplicity;
+/* GaugeFix::FFTChi (GaugeFix.cpp:17-165): Fourier-accelerated steepest
+ * descent to transverse Coulomb gauge on the current fields (periodic
+ * neighbours), at most max_iterations steps, stopping when the residual is
+ * below 1e-9 or rises again below 1e-6 (checked against its value every
+ * tenth step); the links, E, phi and pi are transformed in place. */
+int ipg_gauge_fix(ipg_ctx *c, int max_iterations, int *iterations, double *residual);
+/* Evolution::multiplicity (Evolution.cpp:2284-3188) at step it (1 or itmax
+ * in the reference): gauge fix (4000 steps at most), then the gluon number
+ * and energy per unit rapidity from the transverse E and from pi,
+ * n(k) = 2/(omega N^2) [g^2/((it - 1/2) dtau) Re tr E^i(k)E^i(-k) + (it - 1/2)
+ * dtau Re tr pi(k)pi(-k)], and their 100-bin k_T spectra (runningCoupling 0,
+ * usePseudoRapidity 0). */
+int ipg_gluon_multiplicity(ipg_ctx *c, int it, ipg_multiplicity *out);
+/* the NpartdNdy-t<tau>-<id>.dat line (Evolution.cpp:3153-3177): Npart
+ * dN/dy Tpp b dE/dy seed N/A N/A N/A and the k_T > 3 and > 6 GeV cuts, then
+ * the last column's coupling expression, as an ostream prints them */
+int ipg_write_multiplicity(ipg_ctx *c, const char *path, const ipg_multiplicity *m);
 
 /* field ids for ipg_get_field / ipg_set_field */
 #define IPG_F_UX 0

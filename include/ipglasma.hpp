@@ -173,16 +173,16 @@ class Parameters {
     double getBGqVar() const { return bGqVar_; }
     void setDqmin(double x) { dqmin_ = x; }
     double getDqmin() const { return dqmin_; }
-    void setMuZero(double x) { muZero_ = x; }
-    double getMuZero() const { return muZero_; }
-    void setc(double x) { c_ = x; }
-    double getc() const { return c_; }
+    void setMuZero(double x) { p_.muZero = x; }
+    double getMuZero() const { return p_.muZero; }
+    void setc(double x) { p_.c = x; }
+    double getc() const { return p_.c; }
     void setRoots(double x) { p_.roots = x; }
     double getRoots() const { return p_.roots; }
     void setUseFluctuatingx(int x) { p_.useFluctuatingx = x; }
     int getUseFluctuatingx() const { return p_.useFluctuatingx; }
-    void setRunWithThisFactorTimesQs(double x) { runWithThisFactorTimesQs_ = x; }
-    double getRunWithThisFactorTimesQs() const { return runWithThisFactorTimesQs_; }
+    void setRunWithThisFactorTimesQs(double x) { p_.runWithThisFactorTimesQs = x; }
+    double getRunWithThisFactorTimesQs() const { return p_.runWithThisFactorTimesQs; }
     void setxFromThisFactorTimesQs(double x) { p_.xFromThisFactorTimesQs = x; }
     double getxFromThisFactorTimesQs() const { return p_.xFromThisFactorTimesQs; }
     void setTpp(double x) { obs_.Tpp = x; }
@@ -318,9 +318,6 @@ class Parameters {
     double bGq_ = 0.;
     double bGqVar_ = 0.;
     double dqmin_ = 0.;
-    double muZero_ = 0.;
-    double c_ = 0.;
-    double runWithThisFactorTimesQs_ = 0.;
     double nu_ = 0.;
     double rnp_ = 0.;
     int mPIRank_ = 0;
@@ -516,13 +513,36 @@ class Evolution {
     explicit Evolution(const int nn[]) { (void)nn; }
     void run(Lattice* lat, Group* group, Parameters* param) {
         (void)group;
-        check(ipg_evolve(lat->ctx()));
+        const ipg_params& p = param->raw();
+        if (p.computeGluonMultiplicity == 1) {
+            // Evolution.cpp:366-657 step by step: after the updates of steps 1
+            // and itmax the gauge fix and the spectra (Evolution.cpp:644-651);
+            // NpartdNdy-t<tau>-<id>.dat at itmax; a step whose dN/dy is zero
+            // ends the evolution as the reference's success = 0 does
+            for (int it = 0; it <= p.itmax; it++) {
+                check(ipg_evolve_step(lat->ctx(), it));
+                if (it >= 1 && (it == 1 || it == p.itmax)) {
+                    check(ipg_gluon_multiplicity(lat->ctx(), it, &mult_));
+                    if (mult_.dNdy == 0.) {
+                        param->setSuccess(0);
+                        return;
+                    }
+                    if (it == p.itmax) {
+                        char name[96];
+                        std::snprintf(name, sizeof name, "NpartdNdy-t%g-%d.dat", it * p.dtau * (p.L / p.size),
+                                      param->getEventId());
+                        check(ipg_write_multiplicity(lat->ctx(), name, &mult_));
+                    }
+                }
+            }
+        } else {
+            check(ipg_evolve(lat->ctx()));
+        }
         ipg_observables o;
         check(ipg_get_observables(lat->ctx(), &o));
         param->setObservables(o);
-        const ipg_params& p = param->raw();
-        // Evolution.cpp:644-651 at itmax: the eccentricity line (no gluon
-        // spectra, no it = 1 line)
+        // the eccentricity line at itmax (the it = 1 line reads a Landau
+        // frame that does not exist before itmax and is not written)
         if (p.computeGluonMultiplicity == 1)
             check(ipg_write_eccentricities(
                 lat->ctx(), ("eccentricities" + std::to_string(param->getEventId()) + ".dat").c_str(), p.itmax));
@@ -542,6 +562,8 @@ class Evolution {
             check(ipg_write_jazma(lat->ctx(), ("Jazma-Hydro-t" + std::string(tau) + "-" + id + ".dat").c_str(),
                                   p.itmax, p.sizeOutput, p.etaSizeOutput, p.LOutput, p.detaOutput, etot));
     }
+    // the spectra of the last multiplicity step (dN/dy, dE/dy, k_T bins)
+    const ipg_multiplicity& lastMultiplicity() const { return mult_; }
     // In place of run(): the loss at tau_max (IPG_LOSS_DEDETA, IPG_LOSS_ECC2,
     // or IPG_LOSS_TMUNU with ten weights) and its gradient with respect to
     // rho ([2][Ny][S][8]), the tau = 0 fields ([6][S][18]) or T^{mu nu} at
@@ -553,6 +575,9 @@ class Evolution {
         check(ipg_adjoint_loss(lat->ctx(), loss, weights, wrt, &v, grad.data()));
         return v;
     }
+
+   private:
+    ipg_multiplicity mult_{};
 };
 
 }  // namespace ipglasma

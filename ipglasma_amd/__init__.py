@@ -53,7 +53,8 @@ class Parameters(C.Structure):
                 [("seed", C.c_ulonglong), ("Projectile", C.c_char * 16), ("Target", C.c_char * 16),
                  ("NucleusQsTableFileName", C.c_char * 512), ("dtau", C.c_double), ("itmax", C.c_int),
                  ("rho_rng", C.c_int), ("newton_polish", C.c_int), ("sizeOutput", C.c_int), ("etaSizeOutput", C.c_int),
-                 ("LOutput", C.c_double), ("detaOutput", C.c_double)])
+                 ("LOutput", C.c_double), ("detaOutput", C.c_double),
+                 ("muZero", C.c_double), ("c", C.c_double), ("runWithThisFactorTimesQs", C.c_double)])
 
     def set(self, **kw):
         for k, v in kw.items():
@@ -89,6 +90,20 @@ class Observables(C.Structure):
         for k, _ in self._fields_:
             v = getattr(self, k)
             d[k] = list(v) if isinstance(v, C.Array) else v
+        return d
+
+
+class Multiplicity(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("dNdy", "dEdy", "dNdy_kt", "dEdy_kt", "dNdy_cut3", "dEdy_cut3",
+                                           "dNdy_cut6", "dEdy_cut6", "mean_pt", "kt_bin")] +
+                [("n_kt", C.c_double * 100), ("E_kt", C.c_double * 100), ("counts", C.c_int * 100),
+                 ("gauge_iterations", C.c_int), ("gauge_residual", C.c_double), ("tau_fm", C.c_double)])
+
+    def as_dict(self):
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = np.array(v) if isinstance(v, C.Array) else v
         return d
 
 
@@ -157,6 +172,9 @@ def lib():
             "ipg_batch_adjoint_info": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip]),
             "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
+            "ipg_gauge_fix": (C.c_int, [vp, C.c_int, ip, dp]),
+            "ipg_gluon_multiplicity": (C.c_int, [vp, C.c_int, C.POINTER(Multiplicity)]),
+            "ipg_write_multiplicity": (C.c_int, [vp, C.c_char_p, C.POINTER(Multiplicity)]),
             "ipg_get_nucleons": (C.c_int, [vp, dp, ip, ip]),
             "ipg_set_nucleons": (C.c_int, [vp, dp, C.c_int, C.c_int]),
             "ipg_get_newton_iterations": (C.c_int, [vp, ip]),
@@ -174,8 +192,10 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        if L.ipg_struct_size(0) != C.sizeof(Parameters) or L.ipg_struct_size(1) != C.sizeof(Observables):
-            raise IPGError("ipg_params / ipg_observables layout mismatch between ipg.h and the binding")
+        if (L.ipg_struct_size(0) != C.sizeof(Parameters) or L.ipg_struct_size(1) != C.sizeof(Observables) or
+                L.ipg_struct_size(2) != C.sizeof(Multiplicity)):
+            raise IPGError("ipg_params / ipg_observables / ipg_multiplicity layout mismatch between ipg.h and the "
+                           "binding")
         _lib = L
     return _lib
 
@@ -298,6 +318,34 @@ class Event:
         o = Observables()
         _check(lib().ipg_get_observables(self._c, C.byref(o)))
         return o.as_dict()
+
+    # GaugeFix::FFTChi on the current fields
+    def gauge_fix(self, max_iterations=4000):
+        it, res = C.c_int(), C.c_double()
+        _check(lib().ipg_gauge_fix(self._c, max_iterations, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    # Evolution::multiplicity at step it (gauge fix included)
+    def multiplicity(self, it):
+        m = Multiplicity()
+        _check(lib().ipg_gluon_multiplicity(self._c, it, C.byref(m)))
+        self._last_mult = m
+        return m.as_dict()
+
+    def write_multiplicity(self, path, m=None):
+        m = self._last_mult if m is None else m
+        _check(lib().ipg_write_multiplicity(self._c, os.fsencode(path), C.byref(m)))
+
+    def run_with_multiplicity(self):
+        """Evolution::run with computeGluonMultiplicity: steps 0..itmax with
+        the gauge fix and spectra after steps 1 and itmax; returns both"""
+        itmax = self.p.itmax
+        out = {}
+        for it in range(itmax + 1):
+            self.step(it)
+            if it >= 1 and (it == 1 or it == itmax):
+                out[it] = self.multiplicity(it)
+        return out
 
     def field(self, name):
         which = FIELDS[name]

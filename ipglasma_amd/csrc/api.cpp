@@ -1464,6 +1464,122 @@ struct Engine {
         return IPG_OK;
     }
 
+    // ---- Coulomb gauge and gluon spectra -------------------------------------
+    // GaugeFix::FFTChi on event 0's current fields.  The rotations, the
+    // transform buffer and the residual's row partials use the event's FFT
+    // region and partial slots.
+    int gauge_fix(int max_iter, int* iters, double* residual) {
+        if (ev.size() != 1) return fail(IPG_ERR_ARG, "the gauge fixing runs on single-event contexts");
+        EvDesc& d = desc[0];
+        const int N = d.N;
+        const long long S = d.S;
+        const int par = ev[0].parity;
+        const Plan& pl = plans[N];
+        FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
+        double2* buf = d.fftbuf;                  // 4 complex planes of chi
+        double* G = (double*)d.fftbuf + 8 * S;    // the rotations, 18 planes
+        double* part = d.partial;                 // N row partials
+        int rc = upload_desc();
+        if (rc) return rc;
+        std::vector<double> hp((size_t)N);
+        double prev = 10000., res = 0.;
+        int k = 0;
+        for (k = 0; k < max_iter; k++) {
+            ipg_launch_gf_iteration(d_desc, par, N, dp, buf, G, part, 0, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            res = 0.;
+            for (int i = 0; i < N; i++) res += hp[(size_t)i];
+            res /= (double)N * (double)N;
+            if (k % 10 == 0) prev = res;
+            if (res < 1e-9) break;
+            if (res > prev && res < 1e-6) break;
+            ipg_launch_gf_iteration(d_desc, par, N, dp, buf, G, part, 1, st);
+            ipg_launch_gf_transform(d_desc, par, S, G, st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        if (iters) *iters = k;
+        if (residual) *residual = res;
+        return IPG_OK;
+    }
+
+    // Evolution::multiplicity at step it: gauge fix, then the spectra of
+    // E1, E2 (coefficient g^2 / ((it - 1/2) dtau)) and pi ((it - 1/2) dtau)
+    int multiplicity(int it, ipg_multiplicity* m) {
+        int iters = 0;
+        double res = 0.;
+        int rc = gauge_fix(4000, &iters, &res);
+        if (rc) return rc;
+        EvDesc& d = desc[0];
+        const int N = d.N;
+        const long long S = d.S;
+        const Plan& pl = plans[N];
+        FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
+        const double a = d.a, g = d.g, dtau = d.dtau;
+        const double ahbarc = ipg::HBARC / a;
+        double2* buf = d.fftbuf;                      // 5 complex planes
+        double* part = (double*)d.fftbuf + 10 * S;    // per-block sums
+        const int nb = (int)((S + IPG_BLOCK - 1) / IPG_BLOCK);
+        const int slots = ipg_mult_slots();
+        constexpr int B = 100;
+        std::vector<double> hp((size_t)nb * slots);
+        double dN = 0., dE = 0., n[B] = {0.}, E[B] = {0.};
+        int cnt[B] = {0};
+        const int par = ev[0].parity;
+        const double* fields[3] = {d.E1, d.E2, d.pi};
+        const double pref[3] = {g * g / ((it - 0.5) * dtau), g * g / ((it - 0.5) * dtau), (it - 0.5) * dtau};
+        (void)par;
+        for (int f = 0; f < 3; f++) {
+            ipg_launch_mult(d_desc, N, dp, fields[f], buf, pref[f], ahbarc, part, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int b = 0; b < nb; b++) {
+                const double* o = &hp[(size_t)b * slots];
+                dN += o[0];
+                dE += o[1];
+                for (int k = 0; k < B; k++) {
+                    n[k] += o[2 + k];
+                    E[k] += o[2 + B + k];
+                    if (f == 0) cnt[k] += (int)o[2 + 2 * B + k];
+                }
+            }
+        }
+        std::memset(m, 0, sizeof(*m));
+        const double dkt = 2.83 / (double)B;
+        for (int k = 0; k < B; k++) {
+            if (cnt[k] > 0) {
+                n[k] = n[k] / (double)cnt[k];
+                E[k] = E[k] / (double)cnt[k];
+                const double wn = n[k] * (k + 0.5) * dkt * dkt * 2. * M_PI;
+                const double we = E[k] * (k + 0.5) * dkt * dkt * 2. * M_PI;
+                m->dNdy_kt += wn;
+                m->dEdy_kt += we;
+                if (k * dkt / a * ipg::HBARC > 3.) {
+                    m->dNdy_cut3 += wn;
+                    m->dEdy_cut3 += we;
+                }
+                if (k * dkt / a * ipg::HBARC > 6.) {
+                    m->dNdy_cut6 += wn;
+                    m->dEdy_cut6 += we;
+                }
+            }
+            m->n_kt[k] = n[k];
+            m->E_kt[k] = E[k];
+            m->counts[k] = cnt[k];
+        }
+        m->dNdy = dN;
+        m->dEdy = dE;
+        m->mean_pt = dE / dN;
+        m->kt_bin = dkt;
+        m->gauge_iterations = iters;
+        m->gauge_residual = res;
+        m->tau_fm = it * dtau * a;
+        return IPG_OK;
+    }
+
     int single_step(int e, int it) {
         // one schedule piece for a single-event engine
         int rc = upload_desc();
@@ -1582,7 +1698,10 @@ int ipg_release_cached_memory(void) {
     return IPG_OK;
 }
 int ipg_version(void) { return 100; }
-int ipg_struct_size(int which) { return which == 0 ? (int)sizeof(ipg_params) : (int)sizeof(ipg_observables); }
+int ipg_struct_size(int which) {
+    return which == 0 ? (int)sizeof(ipg_params)
+                      : (which == 1 ? (int)sizeof(ipg_observables) : (int)sizeof(ipg_multiplicity));
+}
 void ipg_default_params(ipg_params* p) { ipg::default_params(p); }
 void ipg_derive(ipg_params* p) { ipg::derive(p); }
 int ipg_read_input(const char* path, ipg_params* p) {
@@ -1895,6 +2014,24 @@ int ipg_write_hydro(ipg_ctx* c, const char* path, int it, int size_out, int eta_
     if (rc) return rc;
     return ipg_write_hydro_file(path, land.data(), p.size, p.L, it, p.dtau, size_out, eta_size_out, L_out, deta_out,
                                 Etot);
+}
+int ipg_gauge_fix(ipg_ctx* c, int max_iterations, int* iterations, double* residual) {
+    if (!c || max_iterations < 0) return fail(IPG_ERR_ARG, "bad argument");
+    return c->eng.gauge_fix(max_iterations, iterations, residual);
+}
+int ipg_gluon_multiplicity(ipg_ctx* c, int it, ipg_multiplicity* out) {
+    if (!c || !out || it < 1) return fail(IPG_ERR_ARG, "bad argument (it >= 1)");
+    return c->eng.multiplicity(it, out);
+}
+int ipg_write_multiplicity(ipg_ctx* c, const char* path, const ipg_multiplicity* m) {
+    if (!c || !path || !m) return fail(IPG_ERR_ARG, "null argument");
+    const Event& E = c->eng.ev[0];
+    const ipg_params& p = E.p;
+    std::string err;
+    int rc = ipg::write_npart_dndy(path, E.obs.Npart, m->dNdy, E.obs.Tpp, E.obs.b, m->dEdy, E.seed, m->dNdy_cut3,
+                                   m->dEdy_cut3, m->dNdy_cut6, m->dEdy_cut6, p.g, p.muZero, p.c,
+                                   p.runWithThisFactorTimesQs, E.obs.averageQs, err);
+    return rc ? fail(rc, err) : IPG_OK;
 }
 int ipg_get_nucleons(ipg_ctx* c, double* out, int* A1, int* A2) {
     if (!c) return fail(IPG_ERR_ARG, "null context");

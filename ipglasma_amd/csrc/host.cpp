@@ -426,6 +426,23 @@ void sample_deuteron_separation(MT64& rng, double a, double w, double R0, double
     *y = r * std::sin(phi);
 }
 
+int write_npart_dndy(const char* path, int Npart, double dNdy, double Tpp, double b, double dEdy,
+                     unsigned long long seed, double n3, double e3, double n6, double e6, double g, double muZero,
+                     double c, double fQs, double avgQs, std::string& err) {
+    std::ofstream out(path, std::ios::out);
+    if (!out) {
+        err = std::string("cannot write ") + path;
+        return IPG_ERR_IO;
+    }
+    const double last =
+        g * g /
+        (4. * M_PI * 4. * M_PI /
+         (9. * std::log(std::pow(std::pow(muZero / 0.2, 2. / c) + std::pow(fQs * avgQs / 0.2, 2. / c), c))));
+    out << Npart << " " << dNdy << " " << Tpp << " " << b << " " << dEdy << " " << seed << " N/A N/A N/A " << n3
+        << " " << e3 << " " << n6 << " " << e6 << " " << last << std::endl;
+    return IPG_OK;
+}
+
 // the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p) {
     const double xb = rng.real1();
@@ -570,6 +587,9 @@ void default_params(ipg_params* p) {
     p->smearingWidth = 0.6;
     p->Jacobianm = 0.35;
     p->g2mu = 0.1;
+    p->muZero = 0.3;
+    p->c = 0.2;
+    p->runWithThisFactorTimesQs = 0.5;
     p->seed = 13;
     std::strcpy(p->Projectile, "p");
     std::strcpy(p->Target, "p");
@@ -626,6 +646,9 @@ int read_input(const char* path, ipg_params* p, std::string& err) {
     I("useSeedList", p->useSeedList);
     I("Ny", p->Ny);
     D("roots", p->roots);
+    D("muZero", p->muZero);
+    D("c", p->c);
+    D("runWithThisFactorTimesQs", p->runWithThisFactorTimesQs);
     D("g", p->g);
     D("m", p->m);
     D("Jacobianm", p->Jacobianm);

@@ -104,6 +104,13 @@ int write_tmunu(const char* path, const double* T, int N, double L, int it, doub
 int write_jazma(const char* path, const double* g2A, const double* g2B, int N, double L, int it, double dtau, int hx,
                 int heta, double hL, double deta, double Etot, std::string& err);
 
+// the NpartdNdy-t<tau>-<id>.dat line (Evolution.cpp:3153-3177), as an
+// ostream prints it: Npart dN/dy Tpp b dE/dy seed N/A N/A N/A cut3 cut3
+// cut6 cut6 and g^2 / (4 pi 4 pi / (9 log((mu0/0.2)^(2/c) + (f <Q_s>/0.2)^(2/c))^c))
+int write_npart_dndy(const char* path, int Npart, double dNdy, double Tpp, double b, double dEdy,
+                     unsigned long long seed, double n3, double e3, double n6, double e6, double g, double muZero,
+                     double c, double fQs, double avgQs, std::string& err);
+
 // b from the first draw of Init::init (Init.cpp:2320-2347)
 double sample_b(MT64& rng, const ipg_params& p);
 

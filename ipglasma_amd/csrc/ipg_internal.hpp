@@ -106,3 +106,19 @@ void ipg_adj_links_newton(const EvDesc* d_evs, long long S, const double* s0, co
 void ipg_adj_setv_chunk(const EvDesc* d_evs, const FftPlanDev& plan, int k0, int kc, double* vv, double* vbar,
                         double* grad, hipStream_t st);
 
+
+// Coulomb gauge fixing and gluon spectra (one event, evs[0]; par: the buffer
+// holding the current U and phi).  Stage 0 computes chi = Im tr(div A t^a)
+// and its row transform into buf (4 complex planes) with the residual's row
+// partials in part[N]; stage 1 filters and transforms back into the site
+// rotations G (18 planes); the transform applies G to the fields.
+void ipg_launch_gf_iteration(const EvDesc* d, int par, int N, const FftPlanDev& pl, double2* buf, double* G,
+                             double* part, int stage, hipStream_t st);
+void ipg_launch_gf_transform(const EvDesc* d, int par, long long S, const double* G, hipStream_t st);
+// the 2D transform of the Hermitian field F (9 planes) into buf (5 complex
+// planes) and the per-block sums of n(k) = 2/(omega N^2) pref Re tr F(k)F(-k),
+// omega n(k) ahbarc and the k_T histogram into part (ipg_mult_slots() doubles
+// per block of 256 momenta)
+void ipg_launch_mult(const EvDesc* d, int N, const FftPlanDev& pl, const double* F, double2* buf, double pref,
+                     double ahbarc, double* part, hipStream_t st);
+int ipg_mult_slots();

@@ -64,19 +64,41 @@ int main(int argc, char** argv) {
             if (rc != IPG_OK) break;
             if (!ok) std::printf("No collision happened. Restarting with new random number...\n");
         }
-        if (rc == IPG_OK) rc = ipg_evolve(c);
+        if (rc == IPG_OK && p.computeGluonMultiplicity == 1) {
+            // Evolution::run step by step with the gauge fix and the spectra
+            // after steps 1 and itmax (Evolution.cpp:644-651, 2284-3188)
+            ipg_multiplicity m;
+            for (int it = 0; rc == IPG_OK && it <= p.itmax; it++) {
+                rc = ipg_evolve_step(c, it);
+                if (rc != IPG_OK || it < 1 || (it != 1 && it != p.itmax)) continue;
+                rc = ipg_gluon_multiplicity(c, it, &m);
+                if (rc != IPG_OK) break;
+                std::printf("tau = %g fm: %d gauge-fixing steps (residual %g)\n", m.tau_fm, m.gauge_iterations,
+                            m.gauge_residual);
+                std::printf("dN/dy 1 = %g, dE/dy 1 = %g\n", m.dNdy, m.dEdy);
+                std::printf("dN/dy 2 = %g, dE/dy 2 = %g\n", m.dNdy_kt, m.dEdy_kt);
+                std::printf("gluon <p_T> = %g\n", m.mean_pt);
+                if (m.dNdy == 0.) {
+                    std::printf("No collision happened. Stopping the evolution.\n");
+                    break;
+                }
+                if (it == p.itmax) {
+                    char nname[96];
+                    std::snprintf(nname, sizeof nname, "NpartdNdy-t%g-%d.dat", m.tau_fm, iev);
+                    rc = ipg_write_multiplicity(c, nname, &m);
+                }
+            }
+        } else if (rc == IPG_OK) {
+            rc = ipg_evolve(c);
+        }
         if (rc != IPG_OK) {
             std::fprintf(stderr, "event %d: %s\n", iev, ipg_last_error());
             ipg_destroy(c);
             return 1;
         }
         if (p.computeGluonMultiplicity == 1) {
-            // Evolution.cpp:644-651 at itmax: the eccentricity line (the
-            // gauge-fixed spectra, and the it = 1 line before any Landau
-            // frame exists, are not produced)
-            if (iev == 0)
-                std::printf("computeGluonMultiplicity: writing eccentricities<id>.dat at itmax; gluon spectra "
-                            "(GaugeFix, multiplicity) are not computed\n");
+            // the eccentricity line at itmax (the it = 1 line reads a Landau
+            // frame that does not exist before itmax and is not written)
             char ename[64];
             std::snprintf(ename, sizeof ename, "eccentricities%d.dat", iev);
             rc = ipg_write_eccentricities(c, ename, p.itmax);

@@ -36,6 +36,14 @@
  *                    (eps, u^mu, pi^{mu nu}); u.u = 1; sign u^tau > 0
  *   orc_eccentricity elliptic and rotated Gaussian closed forms; moments of
  *                    point sources; translation invariance
+ *   orc_expm_pade    scipy.linalg.expm
+ *   orc_reunitarize  QR factorisation of the first two rows; det = 1
+ *   orc_fftn_shifted numpy fftshift(fft2(ifftshift(.)))
+ *   orc_gauge_fix    linearised step divides div A by 4 (the -1.5/k^2
+ *                    filter's closed form); gauge-invariant traces kept;
+ *                    Coulomb residual below 1e-9
+ *   orc_multiplicity single-mode E and pi closed forms; excluded momentum
+ *                    lines; Parseval over the included modes
  * Functions with no check of their own are marked "parity unpinned" below.
  */
 #ifndef IPG_ORACLE_H
@@ -270,6 +278,31 @@ void orc_landau(int N, double L, double dtau, int it, const double *T,
 void orc_eccentricity(int N, double L, double QsmuRatio, double QsmuRatioB,
                       double cutoff, const double *eps, const double *utau,
                       const double *g2A, const double *g2B, double *res);
+
+/* ---- Coulomb gauge and gluon spectra (GaugeFix.cpp, Evolution.cpp) ----- */
+/* Matrix::expm, Pade order 6 (Matrix.cpp:431-560); nonzero on NaN/inf */
+int orc_expm_pade(const double *A, double *out);
+/* Matrix::reu (Matrix.h:67-90) */
+void orc_reunitarize(const double *M, double *out);
+/* FFT::fftn (FFT.cpp:260-372) on a [S][18] field: quadrant-swapped DFT,
+ * isign +1 forward (exp(-i...)), -1 inverse divided by N^2 */
+void orc_fftn_shifted(int N, double *f, int isign);
+/* GaugeFix::FFTChi + gaugeTransform (GaugeFix.cpp:17-210) in place; any of
+ * E1, E2, phi, pi may be NULL; history (max_iter doubles, may be NULL)
+ * receives the residual of every step */
+int orc_gauge_fix(int N, double *Ux, double *Uy, double *E1, double *E2, double *phi, double *pi,
+                  int max_iter, int *iterations, double *residual, double *history);
+/* Evolution::multiplicity (Evolution.cpp:2284-3188) on gauge-fixed fields:
+ * out[0..7] = dN/dy 1, dE/dy 1, dN/dy 2, dE/dy 2, dN/dy and dE/dy above
+ * k_T = 3 GeV, above 6 GeV; out[8..107] n[ik], out[108..207] E[ik],
+ * out[208..307] counts */
+void orc_multiplicity(int N, double L, double g, double dtau, int it, const double *E1, const double *E2,
+                      const double *pi, double *out);
+/* orc_run with the gauge fix and spectra after steps 1 and itmax;
+ * gf = iterations, residual at step 1 then at itmax */
+int orc_run_multiplicity(int N, double g, double L, double dtau, int itmax, double *Ux, double *Uy, double *E1,
+                         double *E2, double *phi, double *pi, double *T, double obs[3], double *m1, double *mmax,
+                         double gf[4]);
 
 /* lattice-unit derived step count and dtau (main.cpp:531-534,
  * Evolution.cpp:396) */

@@ -111,6 +111,16 @@ def lib():
         L.orc_landau.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                                  C.POINTER(C.c_int)]
         L.orc_eccentricity.argtypes = [C.c_int] + [C.c_double] * 4 + [_dp] * 5
+        L.orc_expm_pade.argtypes = [_dp, _dp]
+        L.orc_expm_pade.restype = C.c_int
+        L.orc_reunitarize.argtypes = [_dp, _dp]
+        L.orc_fftn_shifted.argtypes = [C.c_int, _dp, C.c_int]
+        L.orc_gauge_fix.argtypes = [C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.POINTER(C.c_int),
+                                                                 C.POINTER(C.c_double), C.c_void_p]
+        L.orc_multiplicity.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       _dp, _dp, _dp, _dp]
+        L.orc_run_multiplicity.argtypes = ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
+                                           + [_dp] * 11)
         _lib = L
     return _lib
 
@@ -219,3 +229,57 @@ def new_state(N):
 
 
 FIELDS = ("Ux", "Uy", "E1", "E2", "phi", "pi")
+
+
+# ---- Coulomb gauge and gluon spectra ------------------------------------------
+
+def expm_pade(A):
+    out = np.zeros(18)
+    rc = lib().orc_expm_pade(to_c(A), out)
+    assert rc == 0
+    return from_c(out)
+
+
+def reunitarize(M):
+    out = np.zeros(18)
+    lib().orc_reunitarize(to_c(M), out)
+    return from_c(out)
+
+
+def fftn_shifted(f, sign):
+    """[S, 18] field -> its quadrant-swapped transform"""
+    N = int(round(np.sqrt(f.shape[0])))
+    w = np.ascontiguousarray(f, dtype=np.float64).copy()
+    lib().orc_fftn_shifted(N, w, sign)
+    return w
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def gauge_fix(N, st, max_iter=4000, history=False):
+    """in place on the state dict; returns (iterations, residual[, history])"""
+    it = C.c_int()
+    res = C.c_double()
+    h = np.zeros(max_iter) if history else None
+    for k in FIELDS:
+        assert st[k].flags.c_contiguous and st[k].dtype == np.float64
+    lib().orc_gauge_fix(N, *[_ptr(st[k]) for k in FIELDS], max_iter, C.byref(it), C.byref(res), _ptr(h))
+    if history:
+        return it.value, res.value, h[:it.value + 1 if it.value < max_iter else max_iter]
+    return it.value, res.value
+
+
+def multiplicity(N, L, g, dtau, it, E1, E2, pi):
+    out = np.zeros(308)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    lib().orc_multiplicity(N, L, g, dtau, it, c(E1), c(E2), c(pi), out)
+    return out
+
+
+def run_multiplicity(N, g, L, dtau, itmax, st):
+    T = np.zeros(10 * N * N)
+    obs, m1, mmax, gf = np.zeros(3), np.zeros(308), np.zeros(308), np.zeros(4)
+    lib().orc_run_multiplicity(N, g, L, dtau, itmax, *[st[k] for k in FIELDS], T, obs, m1, mmax, gf)
+    return T.reshape(10, N * N), obs, m1, mmax, gf

@@ -267,3 +267,31 @@ def test_adjoint_eps2_rho_matches_finite_differences():
         print(f"eps2 rho gradient trial {trial}: adjoint {pred:.10g} central difference {fd:.10g} "
               f"rel {abs(pred - fd) / abs(fd):.2e}")
         assert abs(pred - fd) <= 1e-3 * abs(fd), (trial, pred, fd)
+
+
+def test_batched_adjoint_matches_single_event_contexts():
+    """ipg_batch_adjoint: initial conditions of three events of two lattice
+    sizes in one launch sequence, then each event's reverse mode through one
+    shared workspace; the losses and rho gradients equal those of one
+    context per event (to the round-off of the reverse kernels' atomic
+    sums), for dE/deta and for eps_2"""
+    import ipglasma_amd as I
+    ps = [K.params("AuAu", N=32, Ny=3, L=8.0), K.params("pPb", N=16, Ny=3), K.params("AuAu", N=32, Ny=3, L=8.0)]
+    seeds = [21, 5, 22]
+    for loss in ("dEdeta", "eps2"):
+        b = I.Batch(ps, seeds)
+        vals = b.adjoint(loss=loss)
+        peak, kint = b.adjoint_info()
+        assert peak > 0 and kint >= 1
+        for e, (p, s) in enumerate(zip(ps, seeds)):
+            ev = I.Event(p, seed=s)
+            tries = 0
+            while not ev.init():
+                tries += 1
+                assert tries < 20
+            L0, g = ev.adjoint(loss=loss, wrt="rho")
+            ev.close()
+            assert vals[e] == pytest.approx(L0, rel=1e-13), (loss, e)
+            gb = b.rho_grad(e, p.Ny, p.size ** 2)
+            assert np.abs(gb - g).max() <= 1e-12 * np.abs(g).max(), (loss, e)
+        b.close()

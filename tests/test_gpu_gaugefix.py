@@ -1,0 +1,190 @@
+"""GPU parity of the Coulomb gauge fixing and the gluon spectra
+(GaugeFix.cpp:17-210, Evolution.cpp:2284-3188) against the oracle's
+restatement in the reference's own form (shifted transforms of full 3x3
+matrices, site-by-site gauge transform), on the same fields.
+
+Tolerances: gauge-fixing step counts equal; the residual 1e-6 relative;
+dN/dy, dE/dy, the k_T spectra and their cuts 1e-8 relative; bin counts
+exact.  The fixed fields are compared through tr U_x(x), tr U_y(x) (1e-11),
+which a local gauge rotation changes and a global one does not, and
+directly to 1e-7: a constant rotation leaves the Coulomb condition and the
+residual unchanged, so nothing in the iteration damps it and round-off
+differences of the two computations drift along it (1e-8 after a few
+hundred steps).
+
+The fix uses periodic neighbours (GaugeFix.cpp:53-64) and the evolution
+clamped ones, so at the lattice edges the gauge-transformed fields do not
+evolve as a gauge transformation of the untransformed ones, and dE/deta at
+itmax moves by ~1e-9 .. 1e-7 relative when the spectra are taken after
+step 1, in the reference as here (the oracle, which follows the reference,
+moves the same way).  At the full 512^2 Pb+Pb size the checks are
+size-independent: gauge-invariant traces kept site by site, the Coulomb
+residual, and that small change of dE/deta."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HBARC = 0.1973269718
+
+
+def say(msg):
+    print(f"[gaugefix] {msg}", file=sys.stderr, flush=True)
+
+
+def at_fields(p, seed):
+    import ipglasma_amd as I
+    ev = I.Event(p, seed=seed)
+    ev.init_nucleons()
+    assert ev.init_color_charge()
+    ev.init_wilson_lines()
+    ev.init_fields()
+    return ev
+
+
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("PbPb", 48, 7), ("pPb", 64, 2)])
+def test_gauge_fix_after_first_step(system, N, seed):
+    p = K.params(system, N=N, Ny=3)
+    ev = at_fields(p, seed)
+    ev.step(0)
+    ev.step(1)
+    st = {k: ev.field(k).copy() for k in O.FIELDS}
+    it_g, res_g = ev.gauge_fix(4000)
+    it_o, res_o = O.gauge_fix(N, st)
+    say(f"{system} N={N}: steps gpu {it_g} oracle {it_o}; residual gpu {res_g:.6e} oracle {res_o:.6e}")
+    assert it_g == it_o
+    assert res_g == pytest.approx(res_o, rel=1e-6)
+    for k in ("Ux", "Uy"):
+        tg = np.trace(O.from_c(ev.field(k)), axis1=-2, axis2=-1)
+        to = np.trace(O.from_c(st[k]), axis1=-2, axis2=-1)
+        np.testing.assert_allclose(tg, to, rtol=0, atol=1e-11, err_msg=k)
+    for k in O.FIELDS:
+        np.testing.assert_allclose(ev.field(k), st[k], rtol=0, atol=1e-7, err_msg=k)
+    ev.close()
+
+
+def _compare(m, ref, tag):
+    keys = ("dNdy", "dEdy", "dNdy_kt", "dEdy_kt", "dNdy_cut3", "dEdy_cut3", "dNdy_cut6", "dEdy_cut6")
+    for i, k in enumerate(keys):
+        assert m[k] == pytest.approx(ref[i], rel=1e-8, abs=1e-12 * abs(ref[0])), (tag, k, m[k], ref[i])
+    np.testing.assert_allclose(m["n_kt"], ref[8:108], rtol=1e-8, atol=1e-12 * np.abs(ref[8:108]).max())
+    np.testing.assert_allclose(m["E_kt"], ref[108:208], rtol=1e-8, atol=1e-12 * np.abs(ref[108:208]).max())
+    assert np.array_equal(m["counts"], ref[208:308].astype(int))
+
+
+@pytest.mark.parametrize("system,N,seed", [("AuAu", 32, 5), ("PbPb", 48, 7)])
+def test_run_with_multiplicity_matches_oracle(system, N, seed, tmp_path):
+    p = K.params(system, N=N, Ny=3)
+    ev = at_fields(p, seed)
+    st = {k: ev.field(k).copy() for k in O.FIELDS}
+    res = ev.run_with_multiplicity()
+    o = ev.observables()
+    T, obs, m1, mmax, gf = O.run_multiplicity(N, p.g, p.L, p.dtau, p.itmax, st)
+    itmax = p.itmax
+    say(f"{system} N={N} itmax={itmax}: dN/dy(1) gpu {res[1]['dNdy']:.10g} oracle {m1[0]:.10g}; "
+        f"dN/dy(itmax) gpu {res[itmax]['dNdy']:.10g} oracle {mmax[0]:.10g}; gauge steps "
+        f"{res[1]['gauge_iterations']}/{int(gf[0])}, {res[itmax]['gauge_iterations']}/{int(gf[2])}")
+    _compare(res[1], m1, "it=1")
+    _compare(res[itmax], mmax, "itmax")
+    assert res[1]["gauge_iterations"] == int(gf[0]) and res[itmax]["gauge_iterations"] == int(gf[2])
+    assert res[1]["dNdy"] > 0 and res[itmax]["mean_pt"] == pytest.approx(res[itmax]["dEdy"] / res[itmax]["dNdy"])
+    assert o["dEdeta"] == pytest.approx(obs[0], rel=1e-9)
+    # against the evolution without the fix: the edges differ (periodic fix,
+    # clamped evolution), a small change
+    plain = at_fields(p, seed)
+    plain.run()
+    d0 = plain.observables()["dEdeta"]
+    say(f"{system} N={N}: dE/deta with the spectra {o['dEdeta']:.12g}, without {d0:.12g}")
+    assert o["dEdeta"] == pytest.approx(d0, rel=1e-6)
+    plain.close()
+    # NpartdNdy-t<tau>-<id>.dat
+    path = str(tmp_path / "NpartdNdy.dat")
+    ev.write_multiplicity(path)
+    f = open(path).read().split()
+    assert len(f) == 14 and f[6:9] == ["N/A"] * 3
+    m = res[itmax]
+    assert int(f[0]) == o["Npart"]
+    for col, v in ((1, m["dNdy"]), (2, o["Tpp"]), (3, o["b"]), (4, m["dEdy"]), (9, m["dNdy_cut3"]),
+                   (10, m["dEdy_cut3"]), (11, m["dNdy_cut6"]), (12, m["dEdy_cut6"])):
+        assert float(f[col]) == pytest.approx(v, rel=1e-5, abs=1e-300), (col, f[col], v)
+    ev.close()
+
+
+def test_single_mode_closed_form():
+    """unit links (already in Coulomb gauge), E1 and pi single cosine waves:
+    dN/dy = sum over the waves of c A^2 N^2 / (2 omega)"""
+    import ipglasma_amd as I
+    N = 32
+    p = K.params("AuAu", N=N, L=4.0)
+    ev = I.Event(p, seed=1)
+    S = N * N
+    one = O.to_c(np.broadcast_to(np.eye(3), (S, 3, 3)))
+    t3 = np.diag([0.5, -0.5, 0.0])
+    i, j = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+
+    def wave(q, A):
+        c = A * np.cos(2 * np.pi * (q[0] * i + q[1] * j) / N).reshape(-1)
+        return O.to_c(c[:, None, None] * t3)
+
+    z = np.zeros((S, 18))
+    fields = {"Ux": one, "Uy": one, "E1": wave((3, 7), 0.4), "E2": z, "phi": z, "pi": wave((10, 2), 0.3)}
+    for k, v in fields.items():
+        ev.set_field(k, v)
+    it = 2
+    m = ev.multiplicity(it)
+    assert m["gauge_iterations"] == 0 and m["gauge_residual"] < 1e-20
+    w = lambda q: np.sqrt(4 * np.sin(np.pi * q[0] / N) ** 2 + 4 * np.sin(np.pi * q[1] / N) ** 2)
+    cE, cP = p.g ** 2 / ((it - 0.5) * p.dtau), (it - 0.5) * p.dtau
+    dN = cE * 0.16 * N * N / (2 * w((3, 7))) + cP * 0.09 * N * N / (2 * w((10, 2)))
+    a = p.L / N
+    dE = (cE * 0.16 * N * N / 2 + cP * 0.09 * N * N / 2) * HBARC / a
+    assert m["dNdy"] == pytest.approx(dN, rel=1e-12)
+    assert m["dEdy"] == pytest.approx(dE, rel=1e-12)
+    ref = O.multiplicity(N, p.L, p.g, p.dtau, it, fields["E1"], z, fields["pi"])
+    _compare(m, ref, "waves")
+    ev.close()
+
+
+def test_full_size_pbpb_gauge_fix_invariants():
+    """Pb+Pb at 512^2, L = 30 fm: the fix after step 1 reaches the Coulomb
+    residual, keeps tr(E^2), tr(pi^2) and the plaquette traces site by site,
+    and changes dE/deta at itmax only at the edge level"""
+    p = K.params("PbPb", N=512, Ny=10, maxtime=0.2)
+    ev = at_fields(p, 7)
+    ev.step(0)
+    ev.step(1)
+    N = p.size
+
+    def invariants():
+        U = {k: O.from_c(ev.field(k)).reshape(N, N, 3, 3) for k in ("Ux", "Uy")}
+        P = U["Ux"] @ np.roll(U["Uy"], -1, 0) @ np.roll(U["Ux"], -1, 1).conj().swapaxes(-1, -2) @ \
+            U["Uy"].conj().swapaxes(-1, -2)
+        out = {"plaq": np.trace(P, axis1=-2, axis2=-1)}
+        for k in ("E1", "E2", "pi", "phi"):
+            M = O.from_c(ev.field(k))
+            out[k] = np.einsum("sij,sji->s", M, M).real
+        return out
+
+    before = invariants()
+    its, res = ev.gauge_fix(4000)
+    after = invariants()
+    say(f"PbPb 512^2: {its} gauge-fixing steps, residual {res:.3e}")
+    assert res < 1e-6
+    for k in before:
+        np.testing.assert_allclose(after[k], before[k], rtol=0, atol=1e-10 * max(1.0, np.abs(before[k]).max()),
+                                   err_msg=k)
+    for it in range(2, p.itmax + 1):
+        ev.step(it)
+    plain = at_fields(p, 7)
+    plain.run()
+    d0, d1 = plain.observables()["dEdeta"], ev.observables()["dEdeta"]
+    say(f"PbPb 512^2: dE/deta with the fix {d1:.12g}, without {d0:.12g}")
+    assert d1 == pytest.approx(d0, rel=1e-8)
+    plain.close()
+    ev.close()

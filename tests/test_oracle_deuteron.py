@@ -6,9 +6,14 @@ Init::sampleTA, Init.cpp:90-152):
   the density (e^{-a r} - e^{-w r})^2 / r^2 integrated in xi = e^{-a z});
 - the proton-neutron separation against its target density r T(r) on
   [0, 15] fm by a Kolmogorov-Smirnov test (rotations keep the 3D distance);
-- the two nucleons sit at +-1/2 of it (centre at the origin).
-The integrand itself has no check beyond being the reference's expression
-(no closed form or library routine computes the same quantity)."""
+- the two nucleons sit at +-1/2 of it (centre at the origin);
+- at s = 0 the integral has a closed form: in the reference's variable
+  xi = e^{-z} (z in units of 1/a, no 1/xi Jacobian, Glauber.cpp:874-903) it
+  is int_0^inf e^{-z} (e^{-z} - e^{-k z})^2 / z^2 dz with k = w / a, a sum
+  c_i e^{-l_i z} / z^2 with sum c_i = sum c_i l_i = 0, whose integral is
+  sum c_i l_i ln l_i = 3 ln 3 - 2 (2+k) ln(2+k) + (1+2k) ln(1+2k); this pins
+  the integrand's form (the exponents, the 1/r^2, the missing Jacobian) and
+  the normalisation rho = A / f of CalcRho independently of the quadrature."""
 import ctypes as C
 
 import numpy as np
@@ -63,6 +68,15 @@ def test_thickness_table_matches_quadrature(a, R0):
         assert got == pytest.approx(want, rel=1e-5, abs=1e-9 * tq[0]), s
     # beyond the table (2 x 12 fm) the thickness is 0
     assert L.orc_deuteron_thickness(24.5, a, W_HUL, R0, sigma) == 0.0
+
+
+@pytest.mark.parametrize("a,R0,sigma", [(A_HUL, 1.0, 4.2), (0.535, 6.38, 7.0)])
+def test_thickness_at_zero_closed_form(a, R0, sigma):
+    k = W_HUL / a
+    I = 3 * np.log(3) - 2 * (2 + k) * np.log(2 + k) + (1 + 2 * k) * np.log(1 + 2 * k)
+    f0 = 2.0 * (a - W_HUL) ** 2 * np.pi / a / W_HUL / (a + W_HUL)
+    want = 2.0 * a * (2.0 / f0) * sigma * I
+    assert lib().orc_deuteron_thickness(0.0, a, W_HUL, R0, sigma) == pytest.approx(want, rel=1e-7)
 
 
 def test_separation_distribution_and_placement():

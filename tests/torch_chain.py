@@ -97,71 +97,77 @@ def links(V):
 
 # ---- the tau = 0 root ------------------------------------------------------------
 
-def residual(alpha, U1, U2):
+def residual(beta, U0, U1, U2):
     """F_a = -Im[tr t^a (W - W^+) + tr t^a (W U^+ - U W^+)] for
-    U = exp(i alpha t) U1 U2, W = U1 + U2"""
+    U = exp(i beta t) U0, W = U1 + U2 (U0 a fixed point of the coset
+    SU(3) U1 U2, so that U stays on it)"""
     W = U1 + U2
-    U = expi(alpha) @ U1 @ U2
+    U = expi(beta) @ U0
     M = W - dag(W) + W @ dag(U) - U @ dag(W)
     return -torch.einsum("aij,...ji->...a", GEN, M).imag
 
 
-def _jacobian(alpha, U1, U2):
-    """dF/dalpha [n, 8, 8] by forward-mode autograd over the 8 directions"""
+def _jacobian(beta, U0, U1, U2):
+    """dF/dbeta [n, 8, 8] by forward-mode autograd over the 8 directions"""
     cols = []
     for b in range(8):
-        e = torch.zeros_like(alpha)
+        e = torch.zeros_like(beta)
         e[:, b] = 1.0
-        _, jv = torch.autograd.functional.jvp(lambda a: residual(a, U1, U2), (alpha,), (e,))
+        _, jv = torch.autograd.functional.jvp(lambda x: residual(x, U0, U1, U2), (beta,), (e,))
         cols.append(jv)
     return torch.stack(cols, -1)
 
 
-def solve_root(U1, U2, iters=60, tol=1e-14):
-    """Newton on alpha (no graph), from alpha = 0"""
+def solve_root(U0, U1, U2, iters=60, tol=1e-14):
+    """Newton on beta (no graph), from beta = 0 (U = U0)"""
     with torch.no_grad():
-        a = torch.zeros(U1.shape[0], 8, dtype=DT)
+        b = torch.zeros(U1.shape[0], 8, dtype=DT)
         for _ in range(iters):
-            F = residual(a, U1, U2)
+            F = residual(b, U0, U1, U2)
             if F.abs().sum(-1).max() < tol:
                 break
-            J = _jacobian(a, U1, U2)
-            a = a - torch.linalg.solve(J, F)
-        return a
+            J = _jacobian(b, U0, U1, U2)
+            b = b - torch.linalg.solve(J, F)
+        return b
 
 
-class RootAlpha(torch.autograd.Function):
-    """alpha(U1, U2) with F(alpha; U1, U2) = 0; backward by the implicit
-    function theorem: lambda = J^-T g, dL/dU = -(lambda . dF/dU)"""
-
-    @staticmethod
-    def forward(ctx, U1, U2):
-        a = solve_root(U1, U2)
-        ctx.save_for_backward(a, U1, U2)
-        return a
+class RootBeta(torch.autograd.Function):
+    """beta(U1, U2) with F(beta; U1, U2) = 0 for U = exp(i beta t) U0, U0
+    held fixed; backward by the implicit function theorem:
+    lambda = J^-T g, dL/dU_n = -(lambda . dF/dU_n)"""
 
     @staticmethod
-    def backward(ctx, ga):
-        a, U1, U2 = ctx.saved_tensors
+    def forward(ctx, U0, U1, U2):
+        b = solve_root(U0, U1, U2)
+        ctx.save_for_backward(b, U0, U1, U2)
+        return b
+
+    @staticmethod
+    def backward(ctx, gb):
+        b, U0, U1, U2 = ctx.saved_tensors
         with torch.enable_grad():
-            J = _jacobian(a, U1, U2)
-            lam = torch.linalg.solve(J.transpose(-1, -2), ga)
+            J = _jacobian(b, U0, U1, U2)
+            lam = torch.linalg.solve(J.transpose(-1, -2), gb)
             u1 = U1.detach().requires_grad_(True)
             u2 = U2.detach().requires_grad_(True)
-            F = residual(a.detach(), u1, u2)
+            F = residual(b.detach(), U0, u1, u2)
             g1, g2 = torch.autograd.grad(F, (u1, u2), grad_outputs=-lam)
-        return g1, g2
+        return None, g1, g2
 
 
-def tau0_fields(VA, VB, g):
-    """links, root, pi = -(2i/g) A^eta (Init.cpp:2437-2690)"""
+def tau0_fields(VA, VB, g, start=None):
+    """links, root, pi = -(2i/g) A^eta (Init.cpp:2437-2690).  The root is
+    sought near `start` (Ux, Uy [N, N, 3, 3], e.g. another solver's roots,
+    which picks the same root where several exist), else near U1 U2."""
     N = VA.shape[0]
     U1x, U1y = links(VA)
     U2x, U2y = links(VB)
     out = []
-    for A1, A2 in ((U1x, U2x), (U1y, U2y)):
-        a = RootAlpha.apply(A1.reshape(-1, 3, 3), A2.reshape(-1, 3, 3))
-        out.append((expi(a) @ A1.reshape(-1, 3, 3) @ A2.reshape(-1, 3, 3)).reshape(N, N, 3, 3))
+    for k, (A1, A2) in enumerate(((U1x, U2x), (U1y, U2y))):
+        a1, a2 = A1.reshape(-1, 3, 3), A2.reshape(-1, 3, 3)
+        U0 = (a1 @ a2).detach() if start is None else torch.as_tensor(start[k], dtype=CT).reshape(-1, 3, 3)
+        b = RootBeta.apply(U0, a1, a2)
+        out.append((expi(b) @ U0).reshape(N, N, 3, 3))
     Ux, Uy = out
     acc = torch.zeros(N, N, 3, 3, dtype=CT)
     for ax, (A1, A2, U) in enumerate(((U1x, U2x, Ux), (U1y, U2y, Uy))):
@@ -246,12 +252,13 @@ def ttautau(st, g, tau):
     return el + lng + mag + 0.5 / tau / tau * (A + B + Cc + D)
 
 
-def dedeta(z, g2mu2, g, m_lat, Ny, L, dtau, itmax):
-    """dE/deta [GeV] from the normals z [2, Ny, N, N, 8] (requires_grad)"""
+def dedeta(z, g2mu2, g, m_lat, Ny, L, dtau, itmax, start=None):
+    """dE/deta [GeV] from the normals z [2, Ny, N, N, 8] (requires_grad);
+    `start`: starting points of the root finder (tau0_fields)"""
     N = z.shape[2]
     a = L / N
     V = wilson_lines(z, g2mu2, g, m_lat, Ny)
-    Ux, Uy, pi = tau0_fields(V[0], V[1], g)
+    Ux, Uy, pi = tau0_fields(V[0], V[1], g, start)
     zero = torch.zeros_like(pi)
     # tau = 0 kick: phi <- phi + (dtau/2) dtau pi; U unchanged (E = 0)
     st = (Ux, Uy, zero, zero, (dtau / 2) * dtau * pi, pi)

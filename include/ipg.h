@@ -272,6 +272,23 @@ int ipg_get_landau(ipg_ctx *c, double *out);
  * themselves (Evolution::u + Evolution::eccentricity on given cells). */
 int ipg_landau(ipg_ctx *c, const double *T, int it);
 
+/* ipg_landau without a context and without device allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  T [10][S] (T^{mu nu} in ipg_get_tmunu's order and units),
+ * g2mu2A and g2mu2B [S] (or both NULL: <Q_sA^2 Q_sB^2> is then 0) and land
+ * [15][S] (ipg_get_landau's layout) are DEVICE pointers; p gives size, L,
+ * maxtime (for dtau) and QsmuRatio; it >= 1 is the step (tau = it dtau a).
+ * Writes the Landau frame to land and the Landau-frame and eccentricity
+ * fields of *o (eccentricity, Psi, area, Rbar, avg_eps, avg_qs2A_qs2B, rms_r,
+ * centroid, u_rms, landau_failures, landau_no_timelike, tau_fm); leaves its
+ * other fields as they are.  The work space (per-block partial sums, per-site
+ * flags and running eps, 13.6 MB) is one static device array per GPU, so
+ * size <= 1024 (IPG_ERR_ARG beyond that), and calls that overlap on one GPU
+ * race.  Runs on the default stream and returns when the results are there. */
+int ipg_landau_noalloc(const ipg_params *p, const double *T, const double *g2mu2A, const double *g2mu2B, int it,
+                       double *land, ipg_observables *o);
+
 /* eccentricities<id>.dat (Evolution.cpp:1280-1810, cutoff 0, doAniso 0):
  * appends one line for the Landau frame of step it (tau = it a dtau): tau,
  * eps_n and Psi_n for n = 1..6, cutoff, sqrt<r^2>, maxX, maxY, b, Tpp, area,

@@ -172,6 +172,8 @@ def lib():
             "ipg_batch_adjoint_info": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip]),
             "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
+            "ipg_landau_noalloc": (C.c_int, [C.POINTER(Parameters), vp, vp, vp, C.c_int, vp,
+                                             C.POINTER(Observables)]),
             "ipg_gauge_fix": (C.c_int, [vp, C.c_int, ip, dp]),
             "ipg_gluon_multiplicity": (C.c_int, [vp, C.c_int, C.POINTER(Multiplicity)]),
             "ipg_write_multiplicity": (C.c_int, [vp, C.c_char_p, C.POINTER(Multiplicity)]),

@@ -116,6 +116,8 @@ def adjoint_timing(I, seed=77, system="AuAu", N=256):
             ev.run()
             L = ev.observables()["dEdeta"]
         t2 = time.perf_counter()
+        if adj:
+            phases[adj] = {k: round(v, 2) for k, v in ev.adjoint_phases().items() if k != "batch_init"}
         t3 = t2
         if adj:
             g = ev.rho_grad()
@@ -124,6 +126,7 @@ def adjoint_timing(I, seed=77, system="AuAu", N=256):
                 raise RuntimeError("non-finite rho gradient")
         ev.close()
         return t1 - t0, t2 - t1, t3 - t2, L
+    phases = {}
     timed(False)
     timed(True)
     timed("eps2")
@@ -137,7 +140,9 @@ def adjoint_timing(I, seed=77, system="AuAu", N=256):
             "gradient_to_host_ms": round(1e3 * f_a, 2),
             "dEdeta_rel_diff": abs(La - Lf) / abs(Lf),
             "eps2_adjoint_ms": round(1e3 * (i_e + e_e), 2),
-            "eps2_adjoint_over_forward": round((i_e + e_e) / (i_f + e_f), 2), "eps2": Le}
+            "eps2_adjoint_over_forward": round((i_e + e_e) / (i_f + e_f), 2), "eps2": Le,
+            "initial_conditions_ms": round(1e3 * i_f, 2),
+            "phases_ms_dEdeta": phases.get(True), "phases_ms_eps2": phases.get("eps2")}
 
 
 def large_batched_adjoint(I, reps=1):
@@ -162,15 +167,16 @@ def large_batched_adjoint(I, reps=1):
             vals = b.adjoint(loss)
             dt = time.perf_counter() - t0
             peak, K = b.adjoint_info()
+            phases = {k: round(v, 1) for k, v in b.adjoint_phases().items()}
             ok = True
             if r == reps:  # one event's gradient to the host (13.4 GB), outside the timing
                 g = b.rho_grad(0, p.Ny, p.size ** 2)
                 ok = bool(np.all(np.isfinite(g)))
                 del g
             b.close()
-            res[loss] = (dt, vals, peak, K, ok)
+            res[loss] = (dt, vals, peak, K, ok, phases)
             print(f"[bench] large batched adjoint {loss}: {dt:.2f} s for 8 events, K={K}, peak "
-                  f"{peak / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+                  f"{peak / 2**30:.1f} GiB, phases (ms) {phases}", file=sys.stderr, flush=True)
         print(f"[bench] large batched forward: {dt_f:.2f} s for 8 events", file=sys.stderr, flush=True)
     I.release_cached_memory()
     dE = res["dEdeta"]
@@ -183,6 +189,7 @@ def large_batched_adjoint(I, reps=1):
                 "adjoint_over_forward_eps2": round(e2[0] / dt_f, 2),
                 "peak_device_memory_GiB": round(max(dE[2], e2[2]) / 2**30, 1),
                 "checkpoint_interval": dE[3], "gradient_finite": dE[4] and e2[4],
+                "phases_ms_dEdeta": dE[5], "phases_ms_eps2": e2[5],
                 "adjoint_vs_forward_max_rel_dEdeta": rel,
                 "eps2": [round(float(v), 6) for v in e2[1]]})
     return out
@@ -439,6 +446,7 @@ def main():
     ap.add_argument("--no-hetero", action="store_true",
                     help="skip the heterogeneous 64-event batch (N=256/512/768) and the 64 small events (N=128)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reverse-mode timing")
+    ap.add_argument("--no-spectra", action="store_true", help="skip the gauge fix + gluon spectra timing")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -520,8 +528,10 @@ def main():
     if not a.no_adjoint and not a.no_large_adjoint:
         large = large_batched_adjoint(I)
         print(f"[bench] large batched adjoint: {large}", file=sys.stderr, flush=True)
-    spectra = gluon_spectra_timing(I)
-    print(f"[bench] gluon spectra: {spectra}", file=sys.stderr, flush=True)
+    spectra = None
+    if not a.no_spectra:
+        spectra = gluon_spectra_timing(I)
+        print(f"[bench] gluon spectra: {spectra}", file=sys.stderr, flush=True)
     rf = roofline(I, p)
     line = {
         "metric": "events/s: batched vs per-event launch, N=512 FP64, dE/deta rel err<=1e-9",

@@ -429,6 +429,13 @@ int ipg_batch_set_checkpoint_interval(ipg_batch *b, int K);
 /* the largest device memory in use (all of the device, bytes) seen during
  * the last ipg_batch_adjoint, and the checkpoint interval it used */
 int ipg_batch_adjoint_info(ipg_batch *b, unsigned long long *peak_bytes, int *K);
+/* wall time of the last reverse-mode call's phases, ms (summed over the
+ * events of a batch): [0] workspace allocation, [1] forward pass with its
+ * checkpoints and the loss, [2] reverse leapfrog, [3] reverse initial
+ * conditions (pi, root finder, links, Wilson lines), [4] the batch's initial
+ * conditions (0 for a context).  No counterpart in the reference. */
+int ipg_adjoint_phases(ipg_ctx *c, double *ms);
+int ipg_batch_adjoint_phases(ipg_batch *b, double *ms);
 
 #ifdef __cplusplus
 }

@@ -170,6 +170,8 @@ def lib():
             "ipg_batch_get_rho_grad": (C.c_int, [vp, C.c_int, dp]),
             "ipg_batch_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
             "ipg_batch_adjoint_info": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip]),
+            "ipg_adjoint_phases": (C.c_int, [vp, dp]),
+            "ipg_batch_adjoint_phases": (C.c_int, [vp, dp]),
             "ipg_adjoint_loss": (C.c_int, [vp, C.c_int, dp, C.c_int, dp, dp]),
             "ipg_landau": (C.c_int, [vp, dp, C.c_int]),
             "ipg_landau_noalloc": (C.c_int, [C.POINTER(Parameters), vp, vp, vp, C.c_int, vp,
@@ -200,6 +202,11 @@ def lib():
                            "binding")
         _lib = L
     return _lib
+
+
+def release_cached_memory():
+    """hand the device blocks kept for later contexts back to the device"""
+    _check(lib().ipg_release_cached_memory())
 
 
 def _check(rc):
@@ -471,6 +478,12 @@ class Event:
         _check(lib().ipg_get_rho_normals(self._c, nucleus, k, _dptr(out)))
         return out
 
+    def adjoint_phases(self):
+        """wall time of the last reverse-mode call's phases, ms"""
+        out = np.zeros(5)
+        _check(lib().ipg_adjoint_phases(self._c, _dptr(out)))
+        return dict(zip(ADJOINT_PHASES, out.tolist()))
+
     def enable_timing(self, on=True):
         _check(lib().ipg_enable_timing(self._c, int(on)))
 
@@ -478,6 +491,9 @@ class Event:
         out = np.zeros(3)
         _check(lib().ipg_get_timing(self._c, _dptr(out)))
         return {"step_ms_total": out[0], "tmunu_ms": out[1], "nsteps": int(out[2])}
+
+
+ADJOINT_PHASES = ("workspace", "forward", "reverse_evolution", "reverse_initial_conditions", "batch_init")
 
 
 class Batch:
@@ -526,6 +542,12 @@ class Batch:
         k = C.c_int(0)
         _check(lib().ipg_batch_adjoint_info(self._b, C.byref(peak), C.byref(k)))
         return peak.value, k.value
+
+    def adjoint_phases(self):
+        """wall time of the last adjoint's phases summed over the events, ms"""
+        out = np.zeros(5)
+        _check(lib().ipg_batch_adjoint_phases(self._b, _dptr(out)))
+        return dict(zip(ADJOINT_PHASES, out.tolist()))
 
     def close(self):
         if self._b:

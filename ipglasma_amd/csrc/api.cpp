@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -158,10 +159,12 @@ void hipFreeVoid(double* p) { (void)hipFree(p); }
 
 // Per-event device blocks outlive their context: a released block is kept,
 // per device and byte size, for the next context that needs the same size,
-// up to a quarter of the device's memory.  Allocating and freeing a few GB
-// costs milliseconds per context, more than the kernels of a small event.  A
-// reused block is cleared first, so a context sees what a fresh allocation
-// gives it.  ipg_release_cached_memory hands the kept blocks back.
+// up to four fifths of the device's memory.  Allocating and freeing a few GB
+// costs milliseconds per context, more than the kernels of a small event,
+// and a hipMalloc of ~100 GB after a free of as much took seconds.  A reused
+// block is cleared first, so a context sees what a fresh allocation gives it.
+// An allocation that fails hands the kept blocks of its device back and tries
+// once more; ipg_release_cached_memory hands them all back.
 struct BlockCache {
     std::mutex m;
     std::multimap<std::pair<int, size_t>, void*> blocks;
@@ -188,7 +191,37 @@ hipError_t block_alloc(void** p, size_t n) {
             return hipMemset(*p, 0, n);
         }
     }
-    return hipMalloc(p, n);
+    e = hipMalloc(p, n);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        {
+            BlockCache& c = block_cache();
+            std::lock_guard<std::mutex> lk(c.m);
+            for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+                if (it->first.first == dev) {
+                    (void)hipFree(it->second);
+                    c.bytes -= it->first.second;
+                    it = c.blocks.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        e = hipMalloc(p, n);
+    }
+    return e;
+}
+// bytes the cache keeps on the current device (memory a failed allocation
+// would hand back)
+size_t block_cached_bytes() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.m);
+    size_t b = 0;
+    for (auto& kv : c.blocks)
+        if (kv.first.first == dev) b += kv.first.second;
+    return b;
 }
 void block_release(void* p, size_t n) {
     if (!p) return;
@@ -197,7 +230,7 @@ void block_release(void* p, size_t n) {
     if (hipGetDevice(&dev) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess) {
         BlockCache& c = block_cache();
         std::lock_guard<std::mutex> lk(c.m);
-        if (c.bytes + n <= tot / 4) {
+        if (c.bytes + n <= tot / 5 * 4) {
             c.blocks.insert({{dev, n}, p});
             c.bytes += n;
             return;
@@ -319,6 +352,14 @@ struct Engine {
     size_t rdesc_n = 0;
     int adj_K = 0;               // checkpoint interval (0: chosen by memory, adj_interval)
     int last_K = 0;              // the interval the last reverse pass used
+    // wall time of the reverse mode's phases, summed over the events of the
+    // last ipg_batch_adjoint (or the last single-event call), ms: workspace,
+    // forward with the tape and the loss, reverse evolution, reverse initial
+    // conditions (pi, root finder, links, setV), batch initial conditions
+    double adj_phase_ms[5] = {0., 0., 0., 0., 0.};
+    static double ms_since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    }
     EvDesc* d_adesc = nullptr;   // one-event descriptor of the event being reversed
 
     ~Engine() {
@@ -348,7 +389,7 @@ struct Engine {
         if (adj_ws) block_release(adj_ws, sizeof(double) * adj_ws_n);
         if (d_rdesc) (void)hipFree(d_rdesc);
         if (d_adesc) (void)hipFree(d_adesc);
-        if (bgrad) (void)hipFree(bgrad);
+        if (bgrad) block_release(bgrad, sizeof(double) * bgrad_n);
         for (hipEvent_t t : tev) (void)hipEventDestroy(t);
         for (TimedLaunch& t : timed) {
             (void)hipEventDestroy(t.a);
@@ -732,15 +773,27 @@ struct Engine {
             int minNy = 1 << 30;
             for (int e : es) minNy = std::min(minNy, desc[e].Ny);
             auto chunk = [&](const EvDesc* dd, int n, int k0, int kcc, int ny) {
-                // keyed by the launch's own event count and chunk length
-                const std::string ck = "setv_rows_inv_block/N=" + std::to_string(kv.first) +
-                                       "/events=" + std::to_string(n) + "/kc=" + std::to_string(kcc);
+                // keyed by the launch's own event count and chunk length; the
+                // 512-thread forms of the forward row and column passes are
+                // candidates from N = 512 on
+                const std::string tail = "/N=" + std::to_string(kv.first) + "/events=" + std::to_string(n) +
+                                         "/kc=" + std::to_string(kcc);
+                SetvVariant v;
+                SetvEvents t;
                 bool meas = false;
-                const int v = g_tuner.pick(ck, 2, &meas);
-                hipEvent_t a, b;
                 // only full chunks are compared (the last one may be short)
-                timed_begin(ck, v, meas && k0 + kcc <= ny, &a, &b);
-                ipg_launch_setv_chunk(dd, n, dp, k0, kcc, v, a, b, st);
+                const bool full = k0 + kcc <= ny;
+                v.inv = g_tuner.pick("setv_rows_inv_block" + tail, 2, &meas);
+                timed_begin("setv_rows_inv_block" + tail, v.inv, meas && full, &t.inv[0], &t.inv[1]);
+                if (kv.first >= 256) {
+                    // 0: 256-thread blocks, 1: 512-thread blocks, 2: one
+                    // wavefront per transformed row, synchronised alone
+                    v.fwd = g_tuner.pick("setv_rows_fwd_block" + tail, 3, &meas);
+                    timed_begin("setv_rows_fwd_block" + tail, v.fwd, meas && full, &t.fwd[0], &t.fwd[1]);
+                    v.cols = g_tuner.pick("setv_cols_block" + tail, 3, &meas);
+                    timed_begin("setv_cols_block" + tail, v.cols, meas && full, &t.cols[0], &t.cols[1]);
+                }
+                ipg_launch_setv_chunk(dd, n, dp, k0, kcc, v, t, st);
             };
             // schedule (a tuner choice when several events share the size):
             // 0: each chunk launch covers all events; 1: one event at a
@@ -965,7 +1018,7 @@ struct Engine {
                     // full steps of every active event are timed for the tuner
                     const std::string sk = step_key();
                     bool sm = false;
-                    const int v = timing ? step_decided() : g_tuner.pick(sk, 2, &sm);
+                    const int v = timing ? step_decided() : g_tuner.pick(sk, 3, &sm);
                     hipEvent_t a = nullptr, b = nullptr;
                     if (timed_begin(sk, v, sm && !fin && !timing, &a, &b)) (void)hipEventRecord(a, st);
                     ipg_launch_step(d_desc, nev, nblk, it, st, v);
@@ -1050,11 +1103,13 @@ struct Engine {
 
     // the loss's gradient with respect to the tau = 0 fields
     int adjoint_fields(int loss, const double* w10, double* value, double* grad) {
+        for (double& x : adj_phase_ms) x = 0.;
         const double *s_init = nullptr, *cs0 = nullptr;
         return adjoint_evolution(0, loss, w10, value, grad, &s_init, &cs0);
     }
     // the loss's gradient with respect to the ten T^{mu nu} planes at itmax
     int adjoint_tmunu(int loss, const double* w10, double* value, double* grad) {
+        for (double& x : adj_phase_ms) x = 0.;
         const double *s_init = nullptr, *cs0 = nullptr;
         return adjoint_evolution(0, loss, w10, value, grad, &s_init, &cs0, true);
     }
@@ -1075,16 +1130,18 @@ struct Engine {
     // Steps between checkpoints for event d.  An explicit interval
     // (ipg_set_checkpoint_interval) is used as given.  Otherwise every state
     // is stored (K = 1: nothing is recomputed) when that workspace fits in
-    // the device memory that is free (plus the workspace already held) with
-    // 4 GiB to spare, else K = ceil(sqrt(T)) (about 2 sqrt(T) states, each
-    // segment recomputed once), else the K in between that fits.
+    // the device memory that is free (plus the workspace already held and
+    // the blocks the cache keeps, which an allocation hands back when it
+    // needs them) with 4 GiB to spare, else K = ceil(sqrt(T)) (about
+    // 2 sqrt(T) states, each segment recomputed once), else the K in between
+    // that fits.
     int adj_interval(const EvDesc& d, size_t extra_doubles) const {
         const int T = std::max(d.itmax - 1, 1);
         if (adj_K > 0) return std::max(1, std::min(adj_K, T));
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (int)std::ceil(std::sqrt((double)T));
         const size_t reserve = (size_t)4 << 30;
-        const size_t avail = fr + sizeof(double) * adj_ws_n;
+        const size_t avail = fr + sizeof(double) * adj_ws_n + block_cached_bytes();
         const int Ksq = std::max(1, (int)std::ceil(std::sqrt((double)T)));
         for (int K = 1; K <= Ksq; K++) {
             const size_t need = sizeof(double) * (adj_evolution_doubles(d, K) + extra_doubles);
@@ -1144,6 +1201,8 @@ struct Engine {
                          double* gr_ext = nullptr) {
         if (e < 0 || e >= (int)ev.size()) return fail(IPG_ERR_ARG, "event index out of range");
         if (ev.size() != 1 && !gr_ext) return fail(IPG_ERR_ARG, "a batch keeps its gradients per event");
+        if (!gr_ext)
+            for (double& x : adj_phase_ms) x = 0.;
         EvDesc d;
         int rc = adj_descriptor(e, d);
         if (rc) return rc;
@@ -1153,8 +1212,10 @@ struct Engine {
         const int K = adj_interval(d, extra);
         const size_t nev_part = adj_evolution_doubles(d, K);
         const size_t ng = 2 * (size_t)d.Ny * S * 8;
+        auto t0 = std::chrono::steady_clock::now();
         rc = adj_reserve(nev_part + extra);
         if (rc) return rc;
+        adj_phase_ms[0] += ms_since(t0);
         rho_grad = nullptr;
         rho_grad_n = 0;
         // [evolution part (offset 0) | V_A, V_B | link cotangents | D cotangents | gradient]
@@ -1169,6 +1230,7 @@ struct Engine {
         const double *s_init = nullptr, *cs0 = nullptr;
         rc = adjoint_evolution(e, loss, w10, dedeta, nullptr, &s_init, &cs0, false, K);
         if (rc) return rc;
+        auto t2 = std::chrono::steady_clock::now();
         // tau = 0 links: total cotangent = from the evolution + from pi
         HIPCHK(hipMemcpyAsync(ulb, cs0, sizeof(double) * 2 * F, hipMemcpyDeviceToDevice, st));
         ipg_adj_zero(dbb, (long long)(2 * F), st);
@@ -1182,9 +1244,18 @@ struct Engine {
         FftPlanDev dp{pl.N, pl.logN, pl.C, pl.tw, pl.s2};
         const int kc = d.kc;
         const int nch = (d.Ny + kc - 1) / kc;
-        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_adesc, dp, c * kc, kc, vab, vb, gr, st);
+        // the launch shapes the tuner settled for this size's one-event chunks
+        SetvVariant sv;
+        {
+            const std::string tail =
+                "/N=" + std::to_string(d.N) + "/events=1/kc=" + std::to_string(kc);
+            sv.fwd = std::max(0, g_tuner.chosen("setv_rows_fwd_block" + tail));
+            sv.cols = std::max(0, g_tuner.chosen("setv_cols_block" + tail));
+        }
+        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_adesc, dp, c * kc, kc, vab, vb, gr, sv, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        adj_phase_ms[3] += ms_since(t2);
         note_memory();
         if (!gr_ext) {
             rho_grad = gr;
@@ -1240,17 +1311,75 @@ struct Engine {
         double* tb = c0 + cset;      // cotangents of the ten T^{mu nu} planes at itmax
         double* wdev = tb + 10 * (size_t)S;
         const int sv = step_decided();
+        auto tf = std::chrono::steady_clock::now();
+        // a descriptor whose step `it` reads the stored state `in` and writes
+        // the state after the step to `out` (no copies through the live
+        // fields); with out null, one that reads `in` at step it
+        auto slot_desc = [&](const double* in, double* out, int it) {
+            EvDesc q = d;
+            const int pp = (it - 1) & 1;
+            double* i0 = const_cast<double*>(in);
+            q.U[pp][0] = i0;
+            q.U[pp][1] = i0 + 18 * (size_t)S;
+            q.E1 = i0 + 36 * (size_t)S;
+            q.E2 = i0 + 45 * (size_t)S;
+            q.phi[pp] = i0 + 54 * (size_t)S;
+            q.pi = i0 + 63 * (size_t)S;
+            if (out) {
+                q.U[pp ^ 1][0] = out;
+                q.U[pp ^ 1][1] = out + 18 * (size_t)S;
+                q.E1o = out + 36 * (size_t)S;
+                q.E2o = out + 45 * (size_t)S;
+                q.phi[pp ^ 1] = out + 54 * (size_t)S;
+                q.pio = out + 63 * (size_t)S;
+            } else {
+                q.E1o = q.E1;
+                q.E2o = q.E2;
+                q.pio = q.pi;
+            }
+            return q;
+        };
+        auto upload_rdesc = [&](const std::vector<EvDesc>& rd) -> int {
+            if (rdesc_n < rd.size()) {
+                if (d_rdesc) HIPCHK(hipFree(d_rdesc));
+                d_rdesc = nullptr;
+                rdesc_n = 0;
+                HIPCHK(hipMalloc(&d_rdesc, sizeof(EvDesc) * rd.size()));
+                rdesc_n = rd.size();
+            }
+            HIPCHK(hipMemcpy(d_rdesc, rd.data(), sizeof(EvDesc) * rd.size(), hipMemcpyHostToDevice));
+            return IPG_OK;
+        };
+        // K = 1 (every state kept): the forward steps write the stored states
+        // themselves, step it from slot it-1 into slot it, and T^{mu nu}
+        // reads the last slot (descriptor T)
+        const bool tape = K == 1;
+        if (tape) {
+            std::vector<EvDesc> rd((size_t)T + 1);
+            for (int it = 1; it <= T; it++)
+                rd[(size_t)it - 1] = slot_desc(s_ck + (size_t)(it - 1) * slot, s_ck + (size_t)it * slot, it);
+            rd[(size_t)T] = slot_desc(s_ck + (size_t)T * slot, nullptr, itm);
+            rc = upload_rdesc(rd);
+            if (rc) return rc;
+        }
         // forward with checkpoints
         ipg_adj_state_save(d_adesc, S, 0, s_init, st);
         ipg_launch_step0(d_adesc, 1, nb, st);
         ipg_adj_state_save(d_adesc, S, 0, s_ck, st);
-        for (int it = 1; it <= T; it++) {
-            ipg_launch_step(d_adesc, 1, nb, it, st, sv);
-            if (it % K == 0) ipg_adj_state_save(d_adesc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
+        if (tape) {
+            for (int it = 1; it <= T; it++) ipg_launch_step(d_rdesc + (it - 1), 1, nb, it, st, sv);
+        } else {
+            for (int it = 1; it <= T; it++) {
+                ipg_launch_step(d_adesc, 1, nb, it, st, sv);
+                if (it % K == 0) ipg_adj_state_save(d_adesc, S, it & 1, s_ck + (size_t)(it / K) * slot, st);
+            }
+            ipg_adj_state_save(d_adesc, S, T & 1, s_T, st);
         }
-        ipg_adj_state_save(d_adesc, S, T & 1, s_T, st);
+        // the state after step itmax-1
+        const double* sT = tape ? s_ck + (size_t)T * slot : s_T;
+        const EvDesc* dT = tape ? d_rdesc + T : d_adesc;
         // the loss: T^{mu nu} at itmax from the state after step itmax-1
-        ipg_launch_tmunu(d_adesc, 1, nb, itm, st);
+        ipg_launch_tmunu(dT, 1, nb, itm, st);
         ipg_launch_finish(d_adesc, 1, itm, d_out, st);
         HIPCHK(hipGetLastError());
         double tt = 0.;
@@ -1282,33 +1411,13 @@ struct Engine {
             for (int k = 0; k < nck; k++) {
                 const int s0 = k * K, s1 = std::min(s0 + K, T);
                 for (int it = s0 + 1; it <= s1; it++) {
-                    double* in = it - 1 == s0 ? s_ck + (size_t)k * slot : s_seg + (size_t)(it - 1 - s0) * slot;
-                    double* out = s_seg + (size_t)(it - s0) * slot;
-                    EvDesc q = d;
-                    const int pp = (it - 1) & 1;
-                    q.U[pp][0] = in;
-                    q.U[pp][1] = in + 18 * (size_t)S;
-                    q.E1 = in + 36 * (size_t)S;
-                    q.E2 = in + 45 * (size_t)S;
-                    q.phi[pp] = in + 54 * (size_t)S;
-                    q.pi = in + 63 * (size_t)S;
-                    q.U[pp ^ 1][0] = out;
-                    q.U[pp ^ 1][1] = out + 18 * (size_t)S;
-                    q.E1o = out + 36 * (size_t)S;
-                    q.E2o = out + 45 * (size_t)S;
-                    q.phi[pp ^ 1] = out + 54 * (size_t)S;
-                    q.pio = out + 63 * (size_t)S;
-                    rd[(size_t)it - 1] = q;
+                    const double* in =
+                        it - 1 == s0 ? s_ck + (size_t)k * slot : s_seg + (size_t)(it - 1 - s0) * slot;
+                    rd[(size_t)it - 1] = slot_desc(in, s_seg + (size_t)(it - s0) * slot, it);
                 }
             }
-            if (rdesc_n < (size_t)T) {
-                if (d_rdesc) HIPCHK(hipFree(d_rdesc));
-                d_rdesc = nullptr;
-                rdesc_n = 0;
-                HIPCHK(hipMalloc(&d_rdesc, sizeof(EvDesc) * (size_t)T));
-                rdesc_n = (size_t)T;
-            }
-            HIPCHK(hipMemcpy(d_rdesc, rd.data(), sizeof(EvDesc) * (size_t)T, hipMemcpyHostToDevice));
+            rc = upload_rdesc(rd);
+            if (rc) return rc;
         }
         if (tb_only) {
             // dL/dT^{mu nu} at itmax only
@@ -1331,10 +1440,13 @@ struct Engine {
             *cs0_out = nullptr;
             return IPG_OK;
         }
+        HIPCHK(hipStreamSynchronize(st));
+        adj_phase_ms[1] += ms_since(tf);
+        auto tr = std::chrono::steady_clock::now();
         // reverse: the loss's cotangents of the state after step itmax-1
         ipg_adj_zero(cA, (long long)pend, st);
         if (loss == IPG_LOSS_DEDETA) {
-            ipg_adj_loss(d_adesc, S, s_T, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
+            ipg_adj_loss(d_adesc, S, sT, itm, tau_fm * ipg::HBARC / (d.a * d.a), cA, st);
         } else {
             if (loss == IPG_LOSS_ECC2) {
                 ipg_adj_ecc2(d_adesc, S, itm, tb, st);
@@ -1342,7 +1454,7 @@ struct Engine {
                 HIPCHK(hipMemcpy(wdev, w10, sizeof(double) * 10, hipMemcpyHostToDevice));
                 ipg_adj_fill_tb(S, wdev, tb, st);
             }
-            ipg_adj_tmunu(d_adesc, S, s_T, itm, tb, cA, st);
+            ipg_adj_tmunu(d_adesc, S, sT, itm, tb, cA, st);
         }
         double *cin = cA, *cout = cB;
         for (int k = nck - 1; k >= 0; k--) {
@@ -1362,6 +1474,7 @@ struct Engine {
         ipg_adj_kick(d_adesc, S, s_init, cin, c0, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        adj_phase_ms[2] += ms_since(tr);
         if (grad) {
             // [6][S][18]: U_x, U_y, E1, E2, phi, pi
             double* tmp = (double*)d.fftbuf;
@@ -1422,9 +1535,13 @@ struct Engine {
     int batch_adjoint(int loss, const double* w10, int max_retries, double* values) {
         const int nev = (int)ev.size();
         peak_used = 0;
+        for (double& x : adj_phase_ms) x = 0.;
         note_memory();
+        auto ti = std::chrono::steady_clock::now();
         int rc = batch_init(max_retries);
         if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        adj_phase_ms[4] = ms_since(ti);
         size_t total = 0;
         bgrad_off.assign(nev, 0);
         bgrad_ok.assign(nev, 0);
@@ -1433,10 +1550,14 @@ struct Engine {
             total += 2 * (size_t)desc[e].Ny * desc[e].S * 8;
         }
         if (total > bgrad_n) {
-            if (bgrad) HIPCHK(hipFree(bgrad));
+            if (bgrad) {
+                HIPCHK(hipStreamSynchronize(st));
+                block_release(bgrad, sizeof(double) * bgrad_n);
+            }
             bgrad = nullptr;
             bgrad_n = 0;
-            HIPCHK(hipMalloc(&bgrad, sizeof(double) * total));
+            // through the block cache, like the per-event blocks
+            HIPCHK(block_alloc((void**)&bgrad, sizeof(double) * total));
             bgrad_n = total;
         }
         note_memory();
@@ -1466,8 +1587,11 @@ struct Engine {
 
     // ---- Coulomb gauge and gluon spectra -------------------------------------
     // GaugeFix::FFTChi on event 0's current fields.  The rotations, the
-    // transform buffer and the residual's row partials use the event's FFT
-    // region and partial slots.
+    // transform buffer, the residual's row partials and the stopping state
+    // use the event's FFT region and partial slots.  The stopping rule runs
+    // on the device (k_gf_decide), so the host launches batches of steps (8,
+    // growing to 64) and reads the 4-double state between them; the steps a
+    // batch holds after the stop return at once.
     int gauge_fix(int max_iter, int* iters, double* residual) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the gauge fixing runs on single-event contexts");
         EvDesc& d = desc[0];
@@ -1479,29 +1603,24 @@ struct Engine {
         double2* buf = d.fftbuf;                  // 4 complex planes of chi
         double* G = (double*)d.fftbuf + 8 * S;    // the rotations, 18 planes
         double* part = d.partial;                 // N row partials
+        double* gs = d.partial + ((N + 31) / 32) * 32;  // the stopping state
         int rc = upload_desc();
         if (rc) return rc;
-        std::vector<double> hp((size_t)N);
-        double prev = 10000., res = 0.;
-        int k = 0;
-        for (k = 0; k < max_iter; k++) {
-            ipg_launch_gf_iteration(d_desc, par, N, dp, buf, G, part, 0, st);
+        ipg_launch_gf_reset(gs, st);
+        double h[4] = {0., 0., 0., 0.};
+        int launched = 0, batch = 8;
+        while (launched < max_iter) {
+            const int nb = std::min(batch, max_iter - launched);
+            for (int b = 0; b < nb; b++) ipg_launch_gf_step(d_desc, par, N, dp, buf, G, part, gs, st);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+            launched += nb;
+            HIPCHK(hipMemcpyAsync(h, gs, sizeof(h), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            res = 0.;
-            for (int i = 0; i < N; i++) res += hp[(size_t)i];
-            res /= (double)N * (double)N;
-            if (k % 10 == 0) prev = res;
-            if (res < 1e-9) break;
-            if (res > prev && res < 1e-6) break;
-            ipg_launch_gf_iteration(d_desc, par, N, dp, buf, G, part, 1, st);
-            ipg_launch_gf_transform(d_desc, par, S, G, st);
+            if (h[0] != 0.) break;
+            batch = std::min(2 * batch, 64);
         }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-        if (iters) *iters = k;
-        if (residual) *residual = res;
+        if (iters) *iters = h[0] != 0. ? (int)h[1] : max_iter;
+        if (residual) *residual = h[3];
         return IPG_OK;
     }
 
@@ -2152,6 +2271,16 @@ int ipg_batch_get_rho_grad(ipg_batch* b, int e, double* grad) {
 int ipg_batch_set_checkpoint_interval(ipg_batch* b, int K) {
     if (!b || K < 0) return fail(IPG_ERR_ARG, "bad argument");
     b->eng.adj_K = K;
+    return IPG_OK;
+}
+int ipg_adjoint_phases(ipg_ctx* c, double* ms) {
+    if (!c || !ms) return fail(IPG_ERR_ARG, "null argument");
+    for (int k = 0; k < 5; k++) ms[k] = c->eng.adj_phase_ms[k];
+    return IPG_OK;
+}
+int ipg_batch_adjoint_phases(ipg_batch* b, double* ms) {
+    if (!b || !ms) return fail(IPG_ERR_ARG, "null argument");
+    for (int k = 0; k < 5; k++) ms[k] = b->eng.adj_phase_ms[k];
     return IPG_OK;
 }
 int ipg_batch_adjoint_info(ipg_batch* b, unsigned long long* peak_bytes, int* K) {

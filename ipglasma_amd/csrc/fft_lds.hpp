@@ -193,8 +193,11 @@ IPGF_HD constexpr int lds_slots(int N, int NROW) { return NROW * row_stride(N); 
 IPGF_HD constexpr int at(int N, int row, int i) { return row * row_stride(N) + pad(i); }
 
 // ---- block-cooperative transform of NROW rows of length N in LDS ---------------
-// (device only; BLK threads; element i of row `row` at x[at(N, row, i)])
-template <int N, int NROW, int BLK, bool INV, int S>
+// (device only; BLK threads; element i of row `row` at x[at(N, row, i)]).
+// WAVE: the rows belong to one wavefront (BLK = 64, lane = threadIdx.x % 64)
+// and the stages synchronise that wavefront only, so the waves of a block
+// run their own rows without waiting for each other.
+template <int N, int NROW, int BLK, bool INV, int S, bool WAVE = false>
 __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ tw) {
     if constexpr (S < nstages(N)) {
         constexpr int R = radix(N, S);
@@ -203,20 +206,25 @@ __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ t
         constexpr int TOT = NROW * NB;
         constexpr int K = (TOT + BLK - 1) / BLK;
         constexpr int RS = row_stride(N);
+        static_assert(!WAVE || BLK == 64, "a wavefront transform runs on 64 lanes");
+        const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
         double2 v[K][R];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int q = threadIdx.x + k * BLK;
+            const int q = tid + k * BLK;
             if (q < TOT) {
                 const int row = q / NB, j = q - row * NB;
 #pragma unroll
                 for (int r = 0; r < R; r++) v[k][r] = x[row * RS + pad(j + r * NB)];
             }
         }
-        __syncthreads();
+        if constexpr (WAVE)
+            __syncwarp();
+        else
+            __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int q = threadIdx.x + k * BLK;
+            const int q = tid + k * BLK;
             if (q < TOT) {
                 const int row = q / NB, j = q - row * NB;
                 butterfly<N, R, NS, INV>(v[k], j, tw);
@@ -225,14 +233,23 @@ __device__ __forceinline__ void stages(double2* x, const double2* __restrict__ t
                 for (int r = 0; r < R; r++) x[row * RS + pad(o + r * NS)] = v[k][r];
             }
         }
-        __syncthreads();
-        stages<N, NROW, BLK, INV, S + 1>(x, tw);
+        if constexpr (WAVE)
+            __syncwarp();
+        else
+            __syncthreads();
+        stages<N, NROW, BLK, INV, S + 1, WAVE>(x, tw);
     }
 }
 
 template <int N, int NROW, int BLK, bool INV>
 __device__ __forceinline__ void lds_fft(double2* x, const double2* __restrict__ tw) {
     stages<N, NROW, BLK, INV, 0>(x, tw);
+}
+// NROW rows starting at x (row stride row_stride(N)) transformed by the
+// calling wavefront alone
+template <int N, int NROW, bool INV>
+__device__ __forceinline__ void wave_fft(double2* x, const double2* __restrict__ tw) {
+    stages<N, NROW, 64, INV, 0, true>(x, tw);
 }
 
 }  // namespace ipgfft

@@ -60,9 +60,21 @@ struct FftPlanDev {
     const double2* tw;     // exp(-2 pi i m / N), m < N/2
     const double* s2;      // 4 sin^2(pi q / N), q < N
 };
+// launch shapes of the three setV passes (0: 256-thread blocks, 1: 512; for
+// the forward row and the column passes 2: one wavefront per transformed
+// row, synchronising itself only; the same arithmetic) and the event pairs
+// that bracket each when timed
+struct SetvVariant {
+    int fwd = 0, cols = 0, inv = 0;
+};
+struct SetvEvents {
+    hipEvent_t fwd[2] = {nullptr, nullptr};
+    hipEvent_t cols[2] = {nullptr, nullptr};
+    hipEvent_t inv[2] = {nullptr, nullptr};
+};
 // slices [k0, k0 + kc) of every event of the launch (one lattice size)
-void ipg_launch_setv_chunk(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k0, int kc, int inv_variant,
-                           hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st);
+void ipg_launch_setv_chunk(const EvDesc* d_evs, int nev, const FftPlanDev& plan, int k0, int kc,
+                           const SetvVariant& v, const SetvEvents& t, hipStream_t st);
 void ipg_launch_unit_v(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
 
 // tau = 0 fields
@@ -102,19 +114,23 @@ void ipg_adj_links_newton(const EvDesc* d_evs, long long S, const double* s0, co
                           const double* db, double* vb, hipStream_t st);
 // setV in reverse for slices [k0, k0+kc) of event 0: vv/vbar are the current
 // Wilson lines and their cotangents (2 nuclei x 18 planes), updated in place;
-// grad[2][Ny][S][8] receives d/d rho
+// grad[2][Ny][S][8] receives d/d rho; v: launch shapes of the recomputed
+// forward row pass and of the column passes (as ipg_launch_setv_chunk)
 void ipg_adj_setv_chunk(const EvDesc* d_evs, const FftPlanDev& plan, int k0, int kc, double* vv, double* vbar,
-                        double* grad, hipStream_t st);
+                        double* grad, const SetvVariant& v, hipStream_t st);
 
 
 // Coulomb gauge fixing and gluon spectra (one event, evs[0]; par: the buffer
-// holding the current U and phi).  Stage 0 computes chi = Im tr(div A t^a)
+// holding the current U and phi).  A step computes chi = Im tr(div A t^a)
 // and its row transform into buf (4 complex planes) with the residual's row
-// partials in part[N]; stage 1 filters and transforms back into the site
-// rotations G (18 planes); the transform applies G to the fields.
-void ipg_launch_gf_iteration(const EvDesc* d, int par, int N, const FftPlanDev& pl, double2* buf, double* G,
-                             double* part, int stage, hipStream_t st);
-void ipg_launch_gf_transform(const EvDesc* d, int par, long long S, const double* G, hipStream_t st);
+// partials in part[N], applies the stopping rule to the state gs[4]
+// (stopped, step index, residual of the last tenth step, last residual),
+// then filters, transforms back into the site rotations G (18 planes) and
+// applies them to the fields; once gs says stopped every kernel returns at
+// once.  ipg_launch_gf_reset starts gs.
+void ipg_launch_gf_reset(double* gs, hipStream_t st);
+void ipg_launch_gf_step(const EvDesc* d, int par, int N, const FftPlanDev& pl, double2* buf, double* G,
+                        double* part, double* gs, hipStream_t st);
 // the 2D transform of the Hermitian field F (9 planes) into buf (5 complex
 // planes) and the per-block sums of n(k) = 2/(omega N^2) pref Re tr F(k)F(-k),
 // omega n(k) ahbarc and the k_T histogram into part (ipg_mult_slots() doubles

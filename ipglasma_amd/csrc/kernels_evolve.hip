@@ -93,8 +93,12 @@ __device__ __forceinline__ M3 taylor2(double c, const M3& E) {
 #define IPG_PHASE() __builtin_amdgcn_sched_barrier(0)
 
 // OCC: waves per SIMD the register budget is sized for (1: no spill, 2:
-// half the registers); the autotuner picks one, the results are identical
-template <int OCC>
+// half the registers).  TILED: a block takes a tile of 4 rows x 64 sites (one
+// wavefront per row, coalesced along y as in the row-major mapping) instead
+// of 256 consecutive sites, so that the x-neighbour rows a block reads are
+// mostly its own (events with N a multiple of 64; others keep the row-major
+// mapping).  The autotuner picks the build; the results are identical.
+template <int OCC, bool TILED = false>
 __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restrict__ evs, int nev, int it, int b0) {
     // b0: first block of a launch that covers one event's blocks only (a
     // multiple of 8, so the XCD of a block is unchanged)
@@ -103,8 +107,17 @@ __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restric
     const EvDesc& ev = evs[e];
     if (!ev.active || it > ev.itmax) return;
     const int lb = xcd_remap(bg - ev.block0, ev.nblocks);
-    const long long s = (long long)lb * IPG_BLOCK + threadIdx.x;
-    if (s >= ev.S) return;
+    long long s;
+    if (TILED && (ev.N & 63) == 0) {
+        static_assert(IPG_BLOCK == 256, "4 x 64 tiles");
+        const int tpr = ev.N >> 6;  // tiles along y
+        if (lb >= (ev.N >> 2) * tpr) return;
+        const int ti = lb / tpr, tj = lb - ti * tpr;
+        s = (long long)(ti * 4 + (int)(threadIdx.x >> 6)) * ev.N + tj * 64 + (int)(threadIdx.x & 63);
+    } else {
+        s = (long long)lb * IPG_BLOCK + threadIdx.x;
+        if (s >= ev.S) return;
+    }
     const size_t S = ev.S;
     const int p = (it - 1) & 1;
     const double tau = it * ev.dtau;
@@ -584,9 +597,12 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_finish(const EvDesc* __restrict__
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_step0, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev);
 }
+// variant 0: k_step<1>, 1: k_step<2>, 2: k_step<1> on 4 x 64 tiles
 void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant) {
     if (variant == 1)
         hipLaunchKernelGGL(k_step<2>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
+    else if (variant == 2)
+        hipLaunchKernelGGL((k_step<1, true>), dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
     else
         hipLaunchKernelGGL(k_step<1>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
 }
@@ -594,6 +610,8 @@ void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks
                            int variant) {
     if (variant == 1)
         hipLaunchKernelGGL(k_step<2>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+    else if (variant == 2)
+        hipLaunchKernelGGL((k_step<1, true>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
     else
         hipLaunchKernelGGL(k_step<1>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
 }

@@ -19,6 +19,15 @@
 //   k_mult_accum      per k: Re tr(F(k) F(-k)) = sum_m w_m |F_m(k)|^2, the
 //                     multiplicity n(k) = 2 / (omega N^2) c |...|, its energy,
 //                     and the k_T histogram (Evolution.cpp:2454-2527)
+//   k_gf_decide       the stopping rule of FFTChi on the device: the residual
+//                     from the row partials (summed in row order), the value
+//                     of every tenth step kept, stop below 1e-9 or when it
+//                     rises while below 1e-6; the gauge-fixing kernels of a
+//                     stopped iteration return at once, so the host launches
+//                     several steps between reads of the state
+//
+// Stopping state gs (doubles): [0] stopped (0 or 1), [1] the step index,
+// [2] the residual of the last tenth step, [3] the last residual.
 //
 // The reference transforms with fftshift around FFTW (FFT.cpp:492-610) and
 // reads momenta in the shifted order; the kernels work in the unshifted
@@ -53,8 +62,10 @@ __device__ double bsum_gf(double v) {
 
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_gf_chi(const EvDesc* __restrict__ evs, FftPlanDev pl, int par,
-                                                     double2* __restrict__ buf, double* __restrict__ part) {
+                                                     double2* __restrict__ buf, double* __restrict__ part,
+                                                     const double* __restrict__ gs) {
     __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
+    if (gs[0] != 0.) return;
     const EvDesc& ev = evs[0];
     const int i = blockIdx.x;
     const size_t S = ev.S;
@@ -89,9 +100,11 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gf_chi(const EvDesc* __restrict__
 // kernel and transforms back, mode 1 leaves the forward transform
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_gf_cols(const EvDesc* __restrict__ evs, FftPlanDev pl,
-                                                      double2* __restrict__ buf, int nf, int mode) {
+                                                      double2* __restrict__ buf, int nf, int mode,
+                                                      const double* __restrict__ gs) {
     constexpr int C = ipgfft::cols_per_block(N);
     __shared__ double2 lds[ipgfft::lds_slots(N, C)];
+    if (gs && gs[0] != 0.) return;
     constexpr int per_pack = N / C;
     const int p = blockIdx.x / per_pack;
     const int j0 = (blockIdx.x % per_pack) * C;
@@ -125,8 +138,10 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gf_cols(const EvDesc* __restrict_
 
 template <int N>
 __global__ __launch_bounds__(IPG_BLOCK) void k_gf_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl,
-                                                          const double2* __restrict__ buf, double* __restrict__ G) {
+                                                          const double2* __restrict__ buf, double* __restrict__ G,
+                                                          const double* __restrict__ gs) {
     __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
+    if (gs[0] != 0.) return;
     const EvDesc& ev = evs[0];
     const int i = blockIdx.x;
     const size_t S = ev.S;
@@ -149,7 +164,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gf_rows_inv(const EvDesc* __restr
 }
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_gf_transform(const EvDesc* __restrict__ evs, int par,
-                                                           const double* __restrict__ G) {
+                                                           const double* __restrict__ G,
+                                                           const double* __restrict__ gs) {
+    if (gs[0] != 0.) return;
     const EvDesc& ev = evs[0];
     const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
     const size_t S = ev.S;
@@ -164,6 +181,35 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gf_transform(const EvDesc* __rest
     sth(ev.E2, S, s, mul_bd(g * ldh(ev.E2, S, s), g));
     sth(ev.phi[par], S, s, mul_bd(g * ldh(ev.phi[par], S, s), g));
     sth(ev.pi, S, s, mul_bd(g * ldh(ev.pi, S, s), g));
+}
+
+// one wavefront: GaugeFix.cpp:101-116 on the row partials of this step
+// (lane l sums rows l, l + 64, ...; then a fixed shuffle tree)
+__global__ void k_gf_decide(int N, const double* __restrict__ part, double* __restrict__ gs) {
+    if (gs[0] != 0.) return;
+    double v = 0.;
+    for (int i = (int)threadIdx.x; i < N; i += 64) v += part[i];
+    v = wsum_gf(v);
+    if (threadIdx.x != 0) return;
+    const double res = v / ((double)N * (double)N);
+    const int k = (int)gs[1];
+    double prev = gs[2];
+    if (k % 10 == 0) prev = res;
+    const bool stop = res < 1e-9 || (res > prev && res < 1e-6);
+    gs[2] = prev;
+    gs[3] = res;
+    if (stop)
+        gs[0] = 1.;
+    else
+        gs[1] = (double)(k + 1);
+}
+__global__ void k_gf_reset(double* __restrict__ gs) {
+    if (threadIdx.x == 0) {
+        gs[0] = 0.;
+        gs[1] = 0.;
+        gs[2] = 10000.;
+        gs[3] = 0.;
+    }
 }
 
 // Hermitian storage planes: 3 real diagonals, then (re, im) of the upper
@@ -259,21 +305,23 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_mult_accum(const EvDesc* __restri
 }
 
 template <int N>
-void launch_gf_iteration_n(const EvDesc* d, int par, const FftPlanDev& pl, double2* buf, double* G, double* part,
-                           int stage, hipStream_t st) {
+void launch_gf_step_n(const EvDesc* d, int par, long long S, const FftPlanDev& pl, double2* buf, double* G,
+                      double* part, double* gs, hipStream_t st) {
     constexpr int C = ipgfft::cols_per_block(N);
-    if (stage == 0) {
-        hipLaunchKernelGGL(k_gf_chi<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, par, buf, part);
-    } else {
-        hipLaunchKernelGGL(k_gf_cols<N>, dim3(4 * (N / C)), dim3(IPG_BLOCK), 0, st, d, pl, buf, 4, 0);
-        hipLaunchKernelGGL(k_gf_rows_inv<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, buf, G);
-    }
+    hipLaunchKernelGGL(k_gf_chi<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, par, buf, part, (const double*)gs);
+    hipLaunchKernelGGL(k_gf_decide, dim3(1), dim3(64), 0, st, N, (const double*)part, gs);
+    hipLaunchKernelGGL(k_gf_cols<N>, dim3(4 * (N / C)), dim3(IPG_BLOCK), 0, st, d, pl, buf, 4, 0,
+                       (const double*)gs);
+    hipLaunchKernelGGL(k_gf_rows_inv<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, buf, G, (const double*)gs);
+    hipLaunchKernelGGL(k_gf_transform, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0, st,
+                       d, par, (const double*)G, (const double*)gs);
 }
 template <int N>
 void launch_mult_fft_n(const EvDesc* d, const FftPlanDev& pl, const double* F, double2* buf, hipStream_t st) {
     constexpr int C = ipgfft::cols_per_block(N);
     hipLaunchKernelGGL(k_mult_rows<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, F, buf);
-    hipLaunchKernelGGL(k_gf_cols<N>, dim3(5 * (N / C)), dim3(IPG_BLOCK), 0, st, d, pl, buf, 5, 1);
+    hipLaunchKernelGGL(k_gf_cols<N>, dim3(5 * (N / C)), dim3(IPG_BLOCK), 0, st, d, pl, buf, 5, 1,
+                       (const double*)nullptr);
 }
 
 #define IPG_FOR_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
@@ -282,21 +330,21 @@ void launch_mult_fft_n(const EvDesc* d, const FftPlanDev& pl, const double* F, d
 
 int ipg_mult_slots() { return kMultSlots; }
 
-// stage 0: chi and its row transform (residual partials per row in part);
-// stage 1: the column pass and the inverse rows into the rotations G
-void ipg_launch_gf_iteration(const EvDesc* d, int par, int N, const FftPlanDev& pl, double2* buf, double* G,
-                             double* part, int stage, hipStream_t st) {
+void ipg_launch_gf_reset(double* gs, hipStream_t st) { hipLaunchKernelGGL(k_gf_reset, dim3(1), dim3(64), 0, st, gs); }
+// one gauge-fixing step: chi with its row transform and residual partials,
+// the stopping rule, the column pass, the inverse rows into the rotations G,
+// the transform of the fields; every kernel returns at once once gs says
+// stopped
+void ipg_launch_gf_step(const EvDesc* d, int par, int N, const FftPlanDev& pl, double2* buf, double* G,
+                        double* part, double* gs, hipStream_t st) {
+    const long long S = (long long)N * N;
     switch (N) {
 #define X(n) \
-    case n: launch_gf_iteration_n<n>(d, par, pl, buf, G, part, stage, st); break;
+    case n: launch_gf_step_n<n>(d, par, S, pl, buf, G, part, gs, st); break;
         IPG_FOR_SIZES(X)
 #undef X
         default: break;
     }
-}
-void ipg_launch_gf_transform(const EvDesc* d, int par, long long S, const double* G, hipStream_t st) {
-    hipLaunchKernelGGL(k_gf_transform, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0, st, d,
-                       par, G);
 }
 void ipg_launch_mult(const EvDesc* d, int N, const FftPlanDev& pl, const double* F, double2* buf, double pref,
                      double ahbarc, double* part, hipStream_t st) {

@@ -375,9 +375,11 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_stats_finish(const EvDesc* __rest
 
 // rows forward: generate rho for row i of (nucleus n, slice k), 4 packed
 // complex rows, forward transform along y
-template <int N>
-__global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl,
-                                                            int k0, int kc) {
+// WAVE: 4 wavefronts (BLK = 256), each transforms one of the four packed
+// rows, synchronising itself only
+template <int N, int BLK = IPG_BLOCK, bool WAVE = false>
+__global__ __launch_bounds__(BLK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+                                                      int kc) {
     __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
     const int per_ev = 2 * kc * N;
     const int e = blockIdx.x / per_ev;
@@ -405,7 +407,13 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
         for (int p = 0; p < 4; p++) lds[ipgfft::at(N, p, j)] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
     }
     __syncthreads();
-    ipgfft::lds_fft<N, 4, IPG_BLOCK, false>(lds, pl.tw);
+    if constexpr (WAVE) {
+        static_assert(BLK == 256, "one wavefront per packed row");
+        ipgfft::wave_fft<N, 1, false>(lds + ipgfft::at(N, (int)(threadIdx.x >> 6), 0), pl.tw);
+        __syncthreads();
+    } else {
+        ipgfft::lds_fft<N, 4, BLK, false>(lds, pl.tw);
+    }
     double2* out = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
         const int p = q / N, j = q - p * N;
@@ -414,10 +422,11 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_rows_fwd(const EvDesc* __res
 }
 
 // columns: forward transform, multiply by 1/(k^2+m^2) e^{-|k| UVdamp} / N^2,
-// inverse transform; C columns per block
-template <int N>
-__global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
-                                                        int kc) {
+// inverse transform; C columns per block.  WAVE: BLK = 64 x (waves), each
+// wavefront transforms and filters its own C / waves columns between the
+// block's load and store, synchronising itself only.
+template <int N, int BLK = IPG_BLOCK, bool WAVE = false>
+__global__ __launch_bounds__(BLK) void k_setv_cols(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0, int kc) {
     constexpr int C = ipgfft::cols_per_block(N);
     __shared__ double2 lds[ipgfft::lds_slots(N, C)];
     constexpr int per_pack = N / C;
@@ -438,11 +447,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
         lds[ipgfft::at(N, c, r)] = buf[(size_t)r * N + j0 + c];
     }
     __syncthreads();
-    ipgfft::lds_fft<N, C, IPG_BLOCK, false>(lds, pl.tw);
     const double m2 = ev.m_lat * ev.m_lat, uv = ev.uvdamp_lat;
     const double invN2 = 1.0 / ((double)N * (double)N);
-    for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
-        const int c = q / N, r = q - c * N;
+    auto filter = [&](int c, int r) {
         const double kt2 = pl.s2[r] + pl.s2[j0 + c];
         double f;
         if (m2 == 0.)
@@ -452,9 +459,24 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_setv_cols(const EvDesc* __restric
         f *= invN2;
         const int a = ipgfft::at(N, c, r);
         lds[a] = make_double2(lds[a].x * f, lds[a].y * f);
+    };
+    if constexpr (WAVE) {
+        constexpr int NW = BLK / 64;
+        constexpr int CPW = C / NW;
+        static_assert(CPW * NW == C, "columns per wavefront");
+        const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+        double2* mine = lds + ipgfft::at(N, w * CPW, 0);
+        ipgfft::wave_fft<N, CPW, false>(mine, pl.tw);
+        for (int q = lane; q < CPW * N; q += 64) filter(w * CPW + q / N, q % N);
+        __syncwarp();
+        ipgfft::wave_fft<N, CPW, true>(mine, pl.tw);
+        __syncthreads();
+    } else {
+        ipgfft::lds_fft<N, C, BLK, false>(lds, pl.tw);
+        for (int q = threadIdx.x; q < C * N; q += blockDim.x) filter(q / N, q % N);
+        __syncthreads();
+        ipgfft::lds_fft<N, C, BLK, true>(lds, pl.tw);
     }
-    __syncthreads();
-    ipgfft::lds_fft<N, C, IPG_BLOCK, true>(lds, pl.tw);
     for (int q = threadIdx.x; q < C * N; q += blockDim.x) {
         const int r = q / C, c = q - r * C;
         buf[(size_t)r * N + j0 + c] = lds[ipgfft::at(N, c, r)];
@@ -1061,18 +1083,36 @@ void ipg_launch_stats(const EvDesc* d, int nev, int nblk, double* out, hipStream
     hipLaunchKernelGGL(k_stats, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
     hipLaunchKernelGGL(k_stats_finish, dim3(nev), dim3(IPG_BLOCK), 0, st, d, nev, out);
 }
-// inv_variant: 0 = 256-thread blocks, 1 = 512-thread blocks for the inverse
-// row pass (the same arithmetic; chosen by the autotuner); the inverse row pass
-// is bracketed by ev_a / ev_b when they are given
+// variants (the same arithmetic in other launch shapes; the autotuner picks
+// each): v.inv, v.fwd, v.cols = 0 for 256-thread blocks, 1 for 512-thread
+// blocks of the inverse row, forward row and column passes.  Each pass is
+// bracketed by its pair of events when they are given.
 template <int N>
-static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, int inv_variant,
-                       hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st) {
+static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, const SetvVariant& v,
+                       const SetvEvents& t, hipStream_t st) {
     constexpr int C = ipgfft::cols_per_block(N);
     const int nrow_fwd = nev * 2 * kc * N;
     const int ncol = nev * 2 * kc * 4 * (N / C);
     const int nrow_inv = nev * 2 * N;
-    hipLaunchKernelGGL(k_setv_rows_fwd<N>, dim3(nrow_fwd), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
-    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    const int inv_variant = v.inv;
+    const hipEvent_t ev_a = t.inv[0], ev_b = t.inv[1];
+    constexpr int CW = C < 8 ? C : 8;  // wavefronts of the wavefront column pass
+    if (t.fwd[0]) (void)hipEventRecord(t.fwd[0], st);
+    if (v.fwd == 1 && N >= 256)
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow_fwd), dim3(512), 0, st, d, pl, k0, kc);
+    else if (v.fwd == 2 && N >= 256)
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow_fwd), dim3(256), 0, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow_fwd), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    if (t.fwd[1]) (void)hipEventRecord(t.fwd[1], st);
+    if (t.cols[0]) (void)hipEventRecord(t.cols[0], st);
+    if (v.cols == 1 && N >= 256)
+        hipLaunchKernelGGL((k_setv_cols<N, 512>), dim3(ncol), dim3(512), 0, st, d, pl, k0, kc);
+    else if (v.cols == 2 && N >= 256)
+        hipLaunchKernelGGL((k_setv_cols<N, 64 * CW, true>), dim3(ncol), dim3(64 * CW), 0, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL((k_setv_cols<N, IPG_BLOCK>), dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    if (t.cols[1]) (void)hipEventRecord(t.cols[1], st);
     if (ev_a) (void)hipEventRecord(ev_a, st);
     if (inv_variant == 1)
         hipLaunchKernelGGL((k_setv_rows_inv<N, 512>), dim3(nrow_inv), dim3(512), 0, st, d, pl, k0, kc);
@@ -1080,10 +1120,10 @@ static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, i
         hipLaunchKernelGGL((k_setv_rows_inv<N, 256>), dim3(nrow_inv), dim3(256), 0, st, d, pl, k0, kc);
     if (ev_b) (void)hipEventRecord(ev_b, st);
 }
-void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, int inv_variant,
-                           hipEvent_t ev_a, hipEvent_t ev_b, hipStream_t st) {
+void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, const SetvVariant& v,
+                           const SetvEvents& t, hipStream_t st) {
 #define IPG_SETV_CASE(n) \
-    case n: setv_chunk<n>(d, nev, pl, k0, kc, inv_variant, ev_a, ev_b, st); break;
+    case n: setv_chunk<n>(d, nev, pl, k0, kc, v, t, st); break;
     switch (pl.N) {
         IPG_SETV_CASE(16)
         IPG_SETV_CASE(32)
@@ -1101,24 +1141,41 @@ void ipg_launch_setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k
     }
 #undef IPG_SETV_CASE
 }
+// the column pass in the launch shape of variant cv (as setv_chunk)
+template <int N>
+static void launch_cols(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, int ncol, int cv, hipStream_t st) {
+    constexpr int C = ipgfft::cols_per_block(N);
+    constexpr int CW = C < 8 ? C : 8;
+    if (cv == 1 && N >= 256)
+        hipLaunchKernelGGL((k_setv_cols<N, 512>), dim3(ncol), dim3(512), 0, st, d, pl, k0, kc);
+    else if (cv == 2 && N >= 256)
+        hipLaunchKernelGGL((k_setv_cols<N, 64 * CW, true>), dim3(ncol), dim3(64 * CW), 0, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL((k_setv_cols<N, IPG_BLOCK>), dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+}
 template <int N>
 static void adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
-                           double* grad, hipStream_t st) {
+                           double* grad, const SetvVariant& v, hipStream_t st) {
     constexpr int C = ipgfft::cols_per_block(N);
     const int nrow = 2 * kc * N;
     const int ncol = 2 * kc * 4 * (N / C);
     // recompute the chunk's A_k (forward row and column passes)
-    hipLaunchKernelGGL(k_setv_rows_fwd<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
-    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    if (v.fwd == 1 && N >= 256)
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow), dim3(512), 0, st, d, pl, k0, kc);
+    else if (v.fwd == 2 && N >= 256)
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow), dim3(256), 0, st, d, pl, k0, kc);
+    else
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);
     hipLaunchKernelGGL((k_adj_setv_rows<N, 256>), dim3(2 * N), dim3(256), 0, st, d, pl, k0, kc, vv, vbar);
     hipLaunchKernelGGL(k_adj_filter_rows_fwd<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
-    hipLaunchKernelGGL(k_setv_cols<N>, dim3(ncol), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+    launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);
     hipLaunchKernelGGL(k_adj_filter_rows_inv<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc, grad);
 }
 void ipg_adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
-                        double* grad, hipStream_t st) {
+                        double* grad, const SetvVariant& v, hipStream_t st) {
 #define IPG_ADJ_CASE(n) \
-    case n: adj_setv_chunk<n>(d, pl, k0, kc, vv, vbar, grad, st); break;
+    case n: adj_setv_chunk<n>(d, pl, k0, kc, vv, vbar, grad, v, st); break;
     switch (pl.N) {
         IPG_ADJ_CASE(16)
         IPG_ADJ_CASE(32)

@@ -5,12 +5,13 @@ matrices, site-by-site gauge transform), on the same fields.
 
 Tolerances: gauge-fixing step counts equal; the residual 1e-6 relative;
 dN/dy, dE/dy, the k_T spectra and their cuts 1e-8 relative; bin counts
-exact.  The fixed fields are compared through tr U_x(x), tr U_y(x) (1e-11),
+exact.  The fixed fields are compared through tr U_x(x), tr U_y(x) (1e-9),
 which a local gauge rotation changes and a global one does not, and
 directly to 1e-7: a constant rotation leaves the Coulomb condition and the
 residual unchanged, so nothing in the iteration damps it and round-off
 differences of the two computations drift along it (1e-8 after a few
-hundred steps).
+hundred steps; the slowest non-constant modes drift less, 3e-11 in the
+traces).
 
 The fix uses periodic neighbours (GaugeFix.cpp:53-64) and the evolution
 clamped ones, so at the lattice edges the gauge-transformed fields do not
@@ -63,7 +64,7 @@ def test_gauge_fix_after_first_step(system, N, seed):
     for k in ("Ux", "Uy"):
         tg = np.trace(O.from_c(ev.field(k)), axis1=-2, axis2=-1)
         to = np.trace(O.from_c(st[k]), axis1=-2, axis2=-1)
-        np.testing.assert_allclose(tg, to, rtol=0, atol=1e-11, err_msg=k)
+        np.testing.assert_allclose(tg, to, rtol=0, atol=1e-9, err_msg=k)
     for k in O.FIELDS:
         np.testing.assert_allclose(ev.field(k), st[k], rtol=0, atol=1e-7, err_msg=k)
     ev.close()

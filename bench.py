@@ -451,6 +451,8 @@ def main():
                     help="run only the timed single event of the roofline figure (for rocprofv3)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the reverse-mode timing (for rocprofv3)")
+    ap.add_argument("--adjoint-pbpb-only", action="store_true",
+                    help="run only the reverse-mode timing of Pb+Pb b=0 at N=512 (for rocprofv3)")
     ap.add_argument("--no-large-adjoint", action="store_true",
                     help="skip the batched reverse mode of 8 Pb+Pb events at N=1024, Ny=100")
     ap.add_argument("--large-adjoint-only", action="store_true",
@@ -474,6 +476,9 @@ def main():
         return
     if a.adjoint_only:
         print(json.dumps(adjoint_timing(I)))
+        return
+    if a.adjoint_pbpb_only:
+        print(json.dumps(adjoint_timing(I, seed=78, system="PbPb", N=512)))
         return
     if a.large_adjoint_only:
         print(json.dumps(large_batched_adjoint(I)))

@@ -122,10 +122,12 @@ int ipg_read_input(const char *path, ipg_params *p);
 void ipg_derive(ipg_params *p);
 const char *ipg_last_error(void);
 int ipg_version(void);
-/* Per-event device memory is kept, per device and size, when a context or
- * batch is destroyed (up to a quarter of the device's memory) and handed,
- * cleared, to the next one that needs that size.  This frees the kept
- * blocks.  No counterpart in the reference. */
+/* Per-event device memory, the reverse-mode workspace and a batch's
+ * gradients are kept, per device and size, when a context or batch is
+ * destroyed (up to four fifths of the device's memory; an allocation that
+ * fails hands them back first) and handed, cleared, to the next one that
+ * needs that size.  This frees the kept blocks.  No counterpart in the
+ * reference. */
 int ipg_release_cached_memory(void);
 /* sizeof(ipg_params) (which=0), sizeof(ipg_observables) (1) or
  * sizeof(ipg_multiplicity) (2), so that bindings can check their struct

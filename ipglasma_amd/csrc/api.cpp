@@ -1018,7 +1018,7 @@ struct Engine {
                     // full steps of every active event are timed for the tuner
                     const std::string sk = step_key();
                     bool sm = false;
-                    const int v = timing ? step_decided() : g_tuner.pick(sk, 3, &sm);
+                    const int v = timing ? step_decided() : g_tuner.pick(sk, 4, &sm);
                     hipEvent_t a = nullptr, b = nullptr;
                     if (timed_begin(sk, v, sm && !fin && !timing, &a, &b)) (void)hipEventRecord(a, st);
                     ipg_launch_step(d_desc, nev, nblk, it, st, v);
@@ -1457,6 +1457,8 @@ struct Engine {
             ipg_adj_tmunu(d_adesc, S, sT, itm, tb, cA, st);
         }
         double *cin = cA, *cout = cB;
+        // the reverse step's kernel split (the tuner's choice)
+        const std::string rkey = "adj_step_kernels/N=" + std::to_string(d.N);
         for (int k = nck - 1; k >= 0; k--) {
             const int s0 = k * K, s1 = std::min(s0 + K, T);
             if (s1 <= s0) continue;
@@ -1467,13 +1469,19 @@ struct Engine {
                 const double* xb = it - 1 == s0 ? x0 : s_seg + (size_t)(it - 1 - s0) * slot;
                 // K = 1: the state after the step is the next stored state
                 const double* xa = K > 1 ? s_seg + (size_t)(it - s0) * slot : s_ck + (size_t)it * slot;
-                ipg_adj_step(d_adesc, S, xb, xa, it, cin, cout, st);
+                bool rm = false;
+                const int rv = g_tuner.pick(rkey, 4, &rm);
+                hipEvent_t ra = nullptr, rb = nullptr;
+                if (timed_begin(rkey, rv, rm, &ra, &rb)) (void)hipEventRecord(ra, st);
+                ipg_adj_step(d_adesc, S, xb, xa, it, cin, cout, st, rv);
+                if (rb) (void)hipEventRecord(rb, st);
                 std::swap(cin, cout);
             }
         }
         ipg_adj_kick(d_adesc, S, s_init, cin, c0, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        timed_resolve();
         adj_phase_ms[2] += ms_since(tr);
         if (grad) {
             // [6][S][18]: U_x, U_y, E1, E2, phi, pi

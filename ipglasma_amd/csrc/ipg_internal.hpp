@@ -34,7 +34,8 @@ inline void ipg_obs_from_device(const double* o, ipg_observables* q) {
 
 // site-parallel helpers: nblk = total blocks over all events of the launch
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st);
-// variant 0: register budget for 1 wave per SIMD, 1: for 2 (same results)
+// variant 0: register budget for 1 wave per SIMD, 1: for 2, 2: 4 x 64 site
+// tiles along y, 3: the tiles down 64-wide column strips (same results)
 void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant = 0);
 // the step for the blocks [block0, block0 + nblocks) of one event only
 void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st,
@@ -96,8 +97,12 @@ void ipg_adj_state_load(const EvDesc* d_evs, long long S, int parity, const doub
 void ipg_adj_zero(double* p, long long n, hipStream_t st);
 void ipg_adj_loss(const EvDesc* d_evs, long long S, const double* xs, int it, double yb, double* cout,
                   hipStream_t st);
+// variant 0: three kernels (own-field planes, link outboxes, phi outboxes),
+// 1: one kernel running the three passes per site, 2: one kernel keeping
+// the partial sums of the link cotangents in LDS, 3: the first kernel, then
+// passes 2 and 3 as one with those sums in LDS (the same results)
 void ipg_adj_step(const EvDesc* d_evs, long long S, const double* xb, const double* xa, int it, const double* cin,
-                  double* cout, hipStream_t st);
+                  double* cout, hipStream_t st, int variant = 0);
 // T^{mu nu} at it in reverse: tb[10][S] cotangents of the stored components,
 // field cotangents of the state xs into a zeroed pending set
 void ipg_adj_tmunu(const EvDesc* d_evs, long long S, const double* xs, int it, const double* tb, double* pout,

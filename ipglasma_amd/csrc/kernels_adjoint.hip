@@ -134,6 +134,13 @@ __device__ __forceinline__ M3 cin_ph(const double* P, size_t S, int N, int i, in
     return r;
 }
 
+// the site of this thread in a launch of pad8(ceil(S / IPG_BLOCK)) blocks:
+// the XCD-aware block order of the forward kernels (xcd_remap), so that the
+// blocks of neighbouring lattice rows run on one XCD and share its L2
+__device__ __forceinline__ long long xcd_site() {
+    return (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * IPG_BLOCK + threadIdx.x;
+}
+
 struct Lat {
     int N;
     long long S;
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict
                                                        int it, double yb, double* __restrict__ cout) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_a(const EvDesc* __restr
                                                           double* __restrict__ pout) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
                                                           double* __restrict__ pout) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int i = (int)(s / L.N), j = (int)(s - (long long)i * L.N);
@@ -529,7 +536,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_c(const EvDesc* __restr
                                                           double* __restrict__ pout) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int i = (int)(s / L.N), j = (int)(s - (long long)i * L.N);
@@ -576,12 +583,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_fill_tb(long long S, const do
 // pass 1, site-local: U' = M(E') U, phi' = phi + (tau + dtau/2) dtau pi',
 // and the traceless part taken by E'.  Leaves in pout: the U cotangents from
 // the link update, Z_i = TL(E_i'b) (the own E_i cotangent), phi'b and pi'b.
-__global__ __launch_bounds__(IPG_BLOCK) void k_adj_local(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
-                                                        const double* __restrict__ xa, int it,
-                                                        const double* __restrict__ pin, double* __restrict__ pout) {
-    const EvDesc& ev = evs[0];
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
-    if (s >= ev.S) return;
+__device__ __forceinline__ void adj_local_site(const EvDesc& ev, long long s, const double* __restrict__ xb,
+                                               const double* __restrict__ xa, int it, const double* __restrict__ pin,
+                                               double* __restrict__ pout) {
     const size_t S = ev.S;
     const int N = ev.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
     const double dt = ev.dtau, tau = it * dt, g = ev.g;
@@ -606,12 +610,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_local(const EvDesc* __restric
 
 // pass 2: the plaquettes of E' (U12 = ux uy_px ux_py^+ uy^+,
 // U1m2 = ux uy_pxmy^+ ux_my^+ uy_my, U2m1 = uy ux_mxpy^+ uy_mx^+ ux_mx)
-__global__ __launch_bounds__(IPG_BLOCK) void k_adj_plaq(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
-                                                       int it, double* __restrict__ pout) {
-    const EvDesc& ev = evs[0];
+__device__ __forceinline__ void adj_plaq_site(const EvDesc& ev, long long s, const double* __restrict__ xb, int it,
+                                              double* __restrict__ pout) {
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
-    if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
     const double dt = ev.dtau, tau = it * dt, g = ev.g;
@@ -646,12 +647,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_plaq(const EvDesc* __restrict
 
 // pass 3: the commutators of E' and the pi' stencil
 //   pi' = pi + (dtau/tau) [Ux phi(x+x) Ux^+ + Ux(x-x)^+ phi(x-x) Ux(x-x) + (y) - 4 phi]
-__global__ __launch_bounds__(IPG_BLOCK) void k_adj_phi(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
-                                                      int it, double* __restrict__ pout) {
-    const EvDesc& ev = evs[0];
+__device__ __forceinline__ void adj_phi_site(const EvDesc& ev, long long s, const double* __restrict__ xb, int it,
+                                             double* __restrict__ pout) {
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
-    if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
     const double dt = ev.dtau, tau = it * dt;
@@ -693,13 +691,198 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_phi(const EvDesc* __restrict_
     add_h(pout + kPPh * S, S, s, hproj(phs));
 }
 
+// The three passes as kernels of their own (few matrices live at a time)
+// or as one kernel: each pass reads only its own site's planes of pout, so a
+// site can run them back to back.  Same arithmetic in the same order either
+// way.
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_local(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                        const double* __restrict__ xa, int it,
+                                                        const double* __restrict__ pin, double* __restrict__ pout) {
+    const long long s = xcd_site();
+    if (s < evs[0].S) adj_local_site(evs[0], s, xb, xa, it, pin, pout);
+}
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_plaq(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                       int it, double* __restrict__ pout) {
+    const long long s = xcd_site();
+    if (s < evs[0].S) adj_plaq_site(evs[0], s, xb, it, pout);
+}
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_phi(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                      int it, double* __restrict__ pout) {
+    const long long s = xcd_site();
+    if (s < evs[0].S) adj_phi_site(evs[0], s, xb, it, pout);
+}
+__global__ __launch_bounds__(IPG_BLOCK) void k_adj_step(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
+                                                       const double* __restrict__ xa, int it,
+                                                       const double* __restrict__ pin, double* __restrict__ pout) {
+    const long long s = xcd_site();
+    if (s >= evs[0].S) return;
+    adj_local_site(evs[0], s, xb, xa, it, pin, pout);
+    adj_plaq_site(evs[0], s, xb, it, pout);
+    adj_phi_site(evs[0], s, xb, it, pout);
+}
+
+
+// One kernel for the three passes whose partial sums stay on chip: the link
+// cotangents of the own site (pass 1 starts them, passes 2 and 3 add) and the
+// two link outboxes that passes 2 and 3 both write (U_x to (-1,0), U_y to
+// (0,-1)) live in LDS, 72 doubles per thread, until their last term; every
+// other plane of pout is written once, when final.  In the one-kernel form
+// without them each of those planes crosses HBM three times (written, read
+// back after the L2 has evicted it, written again).  Same sums in the same
+// order as the three kernels.
+constexpr int kAccSlots = 72;  // U_x, U_y, outbox U_y (0,-1), outbox U_x (-1,0)
+__device__ __forceinline__ M3 acc_ld(const double* a) {
+    M3 m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        m.e[k].r = a[(2 * k) * IPG_BLOCK];
+        m.e[k].i = a[(2 * k + 1) * IPG_BLOCK];
+    }
+    return m;
+}
+__device__ __forceinline__ void acc_st(double* a, const M3& m) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        a[(2 * k) * IPG_BLOCK] = m.e[k].r;
+        a[(2 * k + 1) * IPG_BLOCK] = m.e[k].i;
+    }
+}
+// passes 2 and 3 of one site with the partial sums in LDS; the link
+// cotangents of pass 1 come from LDS (P1_LDS) or from pout
+template <bool P1_LDS>
+__device__ __forceinline__ void adj_plaq_phi_lds(const EvDesc& ev, long long s, const double* __restrict__ xb, int it,
+                                                 double* __restrict__ pout, double* aU1) {
+    double* const aU2 = aU1 + 18 * IPG_BLOCK;
+    double* const aOy36 = aU1 + 36 * IPG_BLOCK;
+    double* const aOx54 = aU1 + 54 * IPG_BLOCK;
+    const Lat L{ev.N, ev.S};
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const double dt = ev.dtau, tau = it * dt, g = ev.g;
+    const Slot B = slot_of(xb, S);
+    // pass 2: the plaquettes of E'
+    {
+        M3 U12b, U1m2b, U2m1b;
+        plaq_bars(ldh(pout + kPE1 * S, S, s), ldh(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
+        M3 B1, B2, B3, B4, uxs, uys;
+        {
+            const M3 ux = ldm(B.Ux, S, s);
+            adj_prod4(ux, ldm(B.Uy, S, L.at(i + 1, j)), dag(ldm(B.Ux, S, L.at(i, j + 1))), dag(ldm(B.Uy, S, s)),
+                      U12b, B1, B2, B3, B4);
+            uxs = B1;
+            stm(pout + (kPOy + 0) * S, S, s, B2);
+            stm(pout + (kPOx + 0) * S, S, s, dag(B3));
+            uys = dag(B4);
+            adj_prod4(ux, dag(ldm(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm(B.Ux, S, L.at(i, j - 1))),
+                      ldm(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
+        }
+        uxs = uxs + B1;
+        stm(pout + (kPOy + 18) * S, S, s, dag(B2));
+        stm(pout + (kPOx + 18) * S, S, s, dag(B3));
+        acc_st(aOy36, B4);
+        adj_prod4(ldm(B.Uy, S, s), dag(ldm(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm(B.Uy, S, L.at(i - 1, j))),
+                  ldm(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
+        uys = uys + B1;
+        stm(pout + (kPOx + 36) * S, S, s, dag(B2));
+        stm(pout + (kPOy + 54) * S, S, s, dag(B3));
+        acc_st(aOx54, B4);
+        acc_st(aU1, (P1_LDS ? acc_ld(aU1) : ldm(pout + kPU1 * S, S, s)) + uxs);
+        acc_st(aU2, (P1_LDS ? acc_ld(aU2) : ldm(pout + kPU2 * S, S, s)) + uys);
+    }
+    asm volatile("" ::: "memory");
+    // pass 3: the commutators of E' and the pi' stencil
+    {
+        const cd c2c{0., -dt / tau};  // conj(i dtau / tau)
+        const M3 Bb = (dt / tau) * ldh(pout + kPPi * S, S, s);
+        const M3 ph = ldh(B.phi, S, s);
+        M3 phs = -4. * Bb;
+        {
+            const M3 u = ldm(B.Ux, S, s), pn = ldh(B.phi, S, L.at(i + 1, j));
+            M3 ub = m_zero(), pnb = m_zero();
+            comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE1 * S, S, s)), phs, ub, pnb);
+            adj_conj(u, pn, Bb, ub, pnb);
+            stm(pout + kPU1 * S, S, s, acc_ld(aU1) + ub);
+            sth(pout + (kPOph + 0) * S, S, s, hproj(pnb));
+        }
+        {
+            const M3 u = ldm(B.Uy, S, s), pn = ldh(B.phi, S, L.at(i, j + 1));
+            M3 ub = m_zero(), pnb = m_zero();
+            comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE2 * S, S, s)), phs, ub, pnb);
+            adj_conj(u, pn, Bb, ub, pnb);
+            stm(pout + kPU2 * S, S, s, acc_ld(aU2) + ub);
+            sth(pout + (kPOph + 9) * S, S, s, hproj(pnb));
+        }
+        {
+            const size_t sm = L.at(i - 1, j);
+            M3 ub = m_zero(), pnb = m_zero();
+            adj_conjd(ldm(B.Ux, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+            stm(pout + (kPOx + 54) * S, S, s, acc_ld(aOx54) + ub);
+            sth(pout + (kPOph + 18) * S, S, s, hproj(pnb));
+        }
+        {
+            const size_t sm = L.at(i, j - 1);
+            M3 ub = m_zero(), pnb = m_zero();
+            adj_conjd(ldm(B.Uy, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+            stm(pout + (kPOy + 36) * S, S, s, acc_ld(aOy36) + ub);
+            sth(pout + (kPOph + 27) * S, S, s, hproj(pnb));
+        }
+        add_h(pout + kPPh * S, S, s, hproj(phs));
+    }
+}
+__global__ __launch_bounds__(IPG_BLOCK, 1) void k_adj_step_lds(const EvDesc* __restrict__ evs,
+                                                              const double* __restrict__ xb,
+                                                              const double* __restrict__ xa, int it,
+                                                              const double* __restrict__ pin,
+                                                              double* __restrict__ pout) {
+    __shared__ double acc[kAccSlots * IPG_BLOCK];
+    double* const aU1 = acc + threadIdx.x;
+    double* const aU2 = aU1 + 18 * IPG_BLOCK;
+    const EvDesc& ev = evs[0];
+    const Lat L{ev.N, ev.S};
+    const long long s = xcd_site();
+    if (s >= L.S) return;
+    const size_t S = L.S;
+    const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
+    const double dt = ev.dtau, tau = it * dt, g = ev.g;
+    const Slot B = slot_of(xb, S), A = slot_of(xa, S);
+    // pass 1: the link update, the traceless part of E', phi' and pi'
+    {
+        const double kappa = g * g * dt / (tau + dt / 2.);
+        {
+            M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
+            u_adj(ldm(B.Ux, S, s), ldh(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+            acc_st(aU1, ub);
+            sth(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
+        }
+        {
+            M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
+            u_adj(ldm(B.Uy, S, s), ldh(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+            acc_st(aU2, ub);
+            sth(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
+        }
+        const M3 phb = cin_ph(pin, S, N, i, j, s);
+        sth(pout + kPPh * S, S, s, phb);
+        sth(pout + kPPi * S, S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
+    }
+    // the partial sums are read back from LDS, not kept in registers
+    asm volatile("" ::: "memory");
+    adj_plaq_phi_lds<true>(ev, s, xb, it, pout, aU1);
+}
+// passes 2 and 3 as one kernel after k_adj_local
+__global__ __launch_bounds__(IPG_BLOCK, 1) void k_adj_plaq_phi(const EvDesc* __restrict__ evs,
+                                                              const double* __restrict__ xb, int it,
+                                                              double* __restrict__ pout) {
+    __shared__ double acc[kAccSlots * IPG_BLOCK];
+    const long long s = xcd_site();
+    if (s < evs[0].S) adj_plaq_phi_lds<false>(evs[0], s, xb, it, pout, acc + threadIdx.x);
+}
 // ---- the tau = 0 kick (phi += dtau^2/2 pi, U = M(E) U) in reverse ----------------
 // pin: pending cotangents after the kick; cout: compact cotangents of the
 // tau = 0 fields
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_kick(const EvDesc* __restrict__ evs, const double* __restrict__ x0,
                                                        const double* __restrict__ pin, double* __restrict__ cout) {
     const EvDesc& ev = evs[0];
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= ev.S) return;
     const size_t S = ev.S;
     const int N = ev.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
@@ -741,7 +924,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_pi_init(const EvDesc* __restr
                                                           double* __restrict__ db) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
@@ -781,7 +964,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __
                                                                double* __restrict__ vb) {
     const EvDesc& ev = evs[0];
     const Lat L{ev.N, ev.S};
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    const long long s = xcd_site();
     if (s >= L.S) return;
     const size_t S = L.S;
     const int N = L.N, i = (int)(s / N), j = (int)(s - (long long)i * N);
@@ -809,19 +992,21 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __
 
 }  // namespace
 
+static unsigned nb_of(long long n) { return (unsigned)((n + IPG_BLOCK - 1) / IPG_BLOCK); }
+// blocks of a site-parallel launch: a multiple of 8 for xcd_site
+static unsigned nb8_of(long long S) { return (nb_of(S) + 7) / 8 * 8; }
+
 void ipg_adj_pi_init(const EvDesc* d, long long S, const double* s0, const double* vab, const double* cs0,
                      double* ulb, double* db, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_pi_init, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0, st, d,
-                       s0, vab, cs0, ulb, db);
+    hipLaunchKernelGGL(k_adj_pi_init, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, s0, vab, cs0, ulb, db);
 }
 void ipg_adj_links_newton(const EvDesc* d, long long S, const double* s0, const double* vab, const double* ulb,
                           const double* db, double* vb, hipStream_t st) {
-    const dim3 g((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK));
+    const dim3 g(nb8_of(S));
     hipLaunchKernelGGL(k_adj_links_newton<0>, g, dim3(IPG_BLOCK), 0, st, d, s0, vab, ulb, db, vb);
     hipLaunchKernelGGL(k_adj_links_newton<1>, g, dim3(IPG_BLOCK), 0, st, d, s0, vab, ulb, db, vb);
 }
 
-static unsigned nb_of(long long n) { return (unsigned)((n + IPG_BLOCK - 1) / IPG_BLOCK); }
 
 void ipg_adj_state_save(const EvDesc* d, long long S, int parity, double* slot, hipStream_t st) {
     hipLaunchKernelGGL(k_state_save, dim3(nb_of(72 * S)), dim3(IPG_BLOCK), 0, st, d, parity, slot);
@@ -833,23 +1018,35 @@ void ipg_adj_zero(double* p, long long n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(nb_of(n)), dim3(IPG_BLOCK), 0, st, p, n);
 }
 void ipg_adj_loss(const EvDesc* d, long long S, const double* xs, int it, double yb, double* cout, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_loss, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, yb, cout);
+    hipLaunchKernelGGL(k_adj_loss, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, yb, cout);
 }
 void ipg_adj_step(const EvDesc* d, long long S, const double* xb, const double* xa, int it, const double* cin,
-                  double* cout, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_local, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
-    hipLaunchKernelGGL(k_adj_plaq, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
-    hipLaunchKernelGGL(k_adj_phi, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
+                  double* cout, hipStream_t st, int variant) {
+    if (variant == 1) {
+        hipLaunchKernelGGL(k_adj_step, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
+        return;
+    }
+    if (variant == 2) {
+        hipLaunchKernelGGL(k_adj_step_lds, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
+        return;
+    }
+    hipLaunchKernelGGL(k_adj_local, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
+    if (variant == 3) {
+        hipLaunchKernelGGL(k_adj_plaq_phi, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
+        return;
+    }
+    hipLaunchKernelGGL(k_adj_plaq, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
+    hipLaunchKernelGGL(k_adj_phi, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
 }
 void ipg_adj_kick(const EvDesc* d, long long S, const double* x0, const double* cin, double* cout, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_kick, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, x0, cin, cout);
+    hipLaunchKernelGGL(k_adj_kick, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, x0, cin, cout);
 }
 
 void ipg_adj_tmunu(const EvDesc* d, long long S, const double* xs, int it, const double* tb, double* pout,
                    hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_tmunu_a, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
-    hipLaunchKernelGGL(k_adj_tmunu_b, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
-    hipLaunchKernelGGL(k_adj_tmunu_c, dim3(nb_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+    hipLaunchKernelGGL(k_adj_tmunu_a, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+    hipLaunchKernelGGL(k_adj_tmunu_b, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
+    hipLaunchKernelGGL(k_adj_tmunu_c, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xs, it, tb, pout);
 }
 void ipg_adj_fill_tb(long long S, const double* w, double* tb, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_fill_tb, dim3(nb_of(10 * S)), dim3(IPG_BLOCK), 0, st, S, w, tb);

@@ -97,8 +97,11 @@ __device__ __forceinline__ M3 taylor2(double c, const M3& E) {
 // wavefront per row, coalesced along y as in the row-major mapping) instead
 // of 256 consecutive sites, so that the x-neighbour rows a block reads are
 // mostly its own (events with N a multiple of 64; others keep the row-major
-// mapping).  The autotuner picks the build; the results are identical.
-template <int OCC, bool TILED = false>
+// mapping); tiles follow each other along y (TILED 1) or down a 64-wide
+// column strip along x (TILED 2), so that the blocks in flight on an XCD
+// share their halo rows.  The autotuner picks the build; the results are
+// identical.
+template <int OCC, int TILED = 0>
 __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restrict__ evs, int nev, int it, int b0) {
     // b0: first block of a launch that covers one event's blocks only (a
     // multiple of 8, so the XCD of a block is unchanged)
@@ -111,8 +114,10 @@ __global__ __launch_bounds__(IPG_BLOCK, OCC) void k_step(const EvDesc* __restric
     if (TILED && (ev.N & 63) == 0) {
         static_assert(IPG_BLOCK == 256, "4 x 64 tiles");
         const int tpr = ev.N >> 6;  // tiles along y
-        if (lb >= (ev.N >> 2) * tpr) return;
-        const int ti = lb / tpr, tj = lb - ti * tpr;
+        const int tpc = ev.N >> 2;  // tiles along x
+        if (lb >= tpc * tpr) return;
+        const int ti = TILED == 1 ? lb / tpr : lb % tpc;
+        const int tj = TILED == 1 ? lb - ti * tpr : lb / tpc;
         s = (long long)(ti * 4 + (int)(threadIdx.x >> 6)) * ev.N + tj * 64 + (int)(threadIdx.x & 63);
     } else {
         s = (long long)lb * IPG_BLOCK + threadIdx.x;
@@ -597,21 +602,19 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_finish(const EvDesc* __restrict__
 void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_step0, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev);
 }
-// variant 0: k_step<1>, 1: k_step<2>, 2: k_step<1> on 4 x 64 tiles
+// variant 0: k_step<1>, 1: k_step<2>, 2: k_step<1> on 4 x 64 tiles along y,
+// 3: k_step<1> on 4 x 64 tiles down column strips
 void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant) {
-    if (variant == 1)
-        hipLaunchKernelGGL(k_step<2>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
-    else if (variant == 2)
-        hipLaunchKernelGGL((k_step<1, true>), dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
-    else
-        hipLaunchKernelGGL(k_step<1>, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, 0);
+    ipg_launch_step_event(d_evs, nev, 0, nblk, it, st, variant);
 }
 void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks, int it, hipStream_t st,
                            int variant) {
     if (variant == 1)
         hipLaunchKernelGGL(k_step<2>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
     else if (variant == 2)
-        hipLaunchKernelGGL((k_step<1, true>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+        hipLaunchKernelGGL((k_step<1, 1>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+    else if (variant == 3)
+        hipLaunchKernelGGL((k_step<1, 2>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
     else
         hipLaunchKernelGGL(k_step<1>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
 }

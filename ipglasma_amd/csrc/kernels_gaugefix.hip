@@ -168,7 +168,9 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_gf_transform(const EvDesc* __rest
                                                            const double* __restrict__ gs) {
     if (gs[0] != 0.) return;
     const EvDesc& ev = evs[0];
-    const long long s = (long long)blockIdx.x * IPG_BLOCK + threadIdx.x;
+    // XCD-aware block order (grid padded to a multiple of 8): the blocks of
+    // neighbouring rows share one XCD's L2 for the rotations of row i + 1
+    const long long s = (long long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * IPG_BLOCK + threadIdx.x;
     const size_t S = ev.S;
     if (s >= ev.S) return;
     const int N = ev.N;
@@ -313,7 +315,7 @@ void launch_gf_step_n(const EvDesc* d, int par, long long S, const FftPlanDev& p
     hipLaunchKernelGGL(k_gf_cols<N>, dim3(4 * (N / C)), dim3(IPG_BLOCK), 0, st, d, pl, buf, 4, 0,
                        (const double*)gs);
     hipLaunchKernelGGL(k_gf_rows_inv<N>, dim3(N), dim3(IPG_BLOCK), 0, st, d, pl, buf, G, (const double*)gs);
-    hipLaunchKernelGGL(k_gf_transform, dim3((unsigned)((S + IPG_BLOCK - 1) / IPG_BLOCK)), dim3(IPG_BLOCK), 0, st,
+    hipLaunchKernelGGL(k_gf_transform, dim3((unsigned)(((S + IPG_BLOCK - 1) / IPG_BLOCK + 7) / 8 * 8)), dim3(IPG_BLOCK), 0, st,
                        d, par, (const double*)G, (const double*)gs);
 }
 template <int N>

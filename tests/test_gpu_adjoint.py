@@ -295,3 +295,33 @@ def test_batched_adjoint_matches_single_event_contexts():
             gb = b.rho_grad(e, p.Ny, p.size ** 2)
             assert np.abs(gb - g).max() <= 1e-12 * np.abs(g).max(), (loss, e)
         b.close()
+
+
+def test_reverse_step_kernels_agree():
+    """the reverse leapfrog step as three kernels (tuning off) and, while the
+    tuner measures them, alternating with the one-kernel form, then with the
+    tuner's choice: the same arithmetic per site, so the gradients agree to
+    the round-off of the loss's atomic sums"""
+    import os
+
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=48, Ny=3, maxtime=0.4, L=8.0)
+    key = "adj_step_kernels/N=48"
+    assert key not in I.autotune_report()
+    runs = []
+    for tune in ("0", "1", "1"):
+        os.environ["IPG_AUTOTUNE"] = tune
+        try:
+            ev = I.Event(p, seed=21)
+            assert ev.init()
+            runs.append(ev.adjoint_dedeta())
+            ev.close()
+        finally:
+            os.environ.pop("IPG_AUTOTUNE")
+    rep = I.autotune_report()
+    assert key in rep and rep[key]["chosen"] is not None, rep
+    L0, g0 = runs[0]
+    for L1, g1 in runs[1:]:
+        assert L1 == pytest.approx(L0, rel=1e-14)
+        for k in O.FIELDS:
+            assert np.abs(g1[k] - g0[k]).max() <= 1e-12 * np.abs(g0[k]).max(), k

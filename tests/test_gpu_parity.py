@@ -318,6 +318,35 @@ def test_batch_setv_schedules_match_plain():
     assert "setv_schedule/N=32/events=3" in I.autotune_report()
 
 
+def test_step_builds_agree():
+    """N a multiple of 64: while the tuner measures the four k_step builds
+    (one or two waves per SIMD; 4 x 64 site tiles along y or down column
+    strips) a one-event batch runs them in turn, step by step; the plain
+    event before it runs the first build throughout.  Same arithmetic per
+    site, so the observables agree to round-off of the reductions."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=192, Ny=3)
+    key = "step_waves_per_simd/N=192/events=1"
+    assert key not in I.autotune_report()
+    ev = I.Event(p, seed=41)
+    tries = 0
+    while not ev.init() and tries < 20:
+        tries += 1
+    ev.run()
+    plain = ev.observables()
+    ev.close()
+    b = I.Batch([p], [41])
+    b.run(use_graph=False)
+    o = b.observables(0)
+    b.close()
+    rep = I.autotune_report()
+    assert key in rep and len(rep[key]["times_ms"]) == 4, rep
+    assert all(t > 0 for t in rep[key]["times_ms"]), rep[key]
+    assert o["dEdeta"] == pytest.approx(plain["dEdeta"], rel=1e-13)
+    for k in ("eccentricity", "Psi"):
+        np.testing.assert_allclose(o[k], plain[k], rtol=1e-11, atol=1e-13, err_msg=k)
+
+
 def test_full_size_pbpb_event_invariants():
     """The throughput workload at its full size (Pb+Pb b=0, N=512, L=30 fm,
     Ny=50, tau_max=0.4 fm/c, itmax=69), where an oracle event takes minutes,
@@ -442,7 +471,9 @@ def test_driver_writes_output_files(tmp_path):
     """writeOutputs 7 in the drop-in driver: the epsilon-u-Hydro-TauHydro,
     Tmunu-t<tau> and Jazma-Hydro-t<tau> files of MyEigen::flowVelocity4D at
     itmax (MyEigen.cpp:326-1293), on the sizeOutput x sizeOutput grid, and the
-    printed Etot, which the Jazma file integrates to."""
+    printed Etot, which the Jazma file integrates to; with
+    computeGluonMultiplicity (its default), the eccentricity line, the
+    spectra after steps 1 and itmax and the NpartdNdy file."""
     import os
     import re
     import subprocess
@@ -465,6 +496,15 @@ def test_driver_writes_output_files(tmp_path):
     # computeGluonMultiplicity (default 1): the itmax line of eccentricities0.dat
     ecc = (tmp_path / "eccentricities0.dat").read_text().splitlines()
     assert len(ecc) == 1 and len(ecc[0].split()) == 23
+    # and the gauge-fixed spectra after steps 1 and itmax (Evolution.cpp:644-651):
+    # the printed dN/dy lines and NpartdNdy-t<tau>-0.dat at itmax
+    dn = [float(x) for x in re.findall(r"dN/dy 1 = (\S+),", r.stdout)]
+    assert len(dn) == 2 and 0 < dn[0] < dn[1]
+    npd = [n for n in names if n.startswith("NpartdNdy-t")]
+    assert len(npd) == 1, names
+    f = (tmp_path / npd[0]).read_text().split()
+    assert len(f) == 14 and f[6:9] == ["N/A"] * 3 and int(f[5]) == 5
+    assert float(f[1]) == pytest.approx(dn[1], rel=1e-5)
     hyd = np.array([ln.split() for ln in (tmp_path / "epsilon-u-Hydro-TauHydro-0.dat").read_text().splitlines()[1:]
                     if ln], dtype=float)
     assert hyd.shape == (40 * 40, 18) and hyd[:, 3].max() > 0 and etot > 0

@@ -162,7 +162,9 @@ void hipFreeVoid(double* p) { (void)hipFree(p); }
 // up to four fifths of the device's memory.  Allocating and freeing a few GB
 // costs milliseconds per context, more than the kernels of a small event,
 // and a hipMalloc of ~100 GB after a free of as much took seconds.  A reused
-// block is cleared first, so a context sees what a fresh allocation gives it.
+// block is cleared first (clear), so a context sees what a fresh allocation
+// gives it; the reverse-mode workspace, whose every word a kernel writes
+// before any reads it, is not (clearing its 12 GB at N = 512 took 1.5 ms).
 // An allocation that fails hands the kept blocks of its device back and tries
 // once more; ipg_release_cached_memory hands them all back.
 struct BlockCache {
@@ -176,7 +178,7 @@ BlockCache& block_cache() {
     static BlockCache* c = new BlockCache;
     return *c;
 }
-hipError_t block_alloc(void** p, size_t n) {
+hipError_t block_alloc(void** p, size_t n, bool clear = true) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
@@ -188,7 +190,7 @@ hipError_t block_alloc(void** p, size_t n) {
             *p = it->second;
             c.blocks.erase(it);
             c.bytes -= n;
-            return hipMemset(*p, 0, n);
+            return clear ? hipMemset(*p, 0, n) : hipSuccess;
         }
     }
     e = hipMalloc(p, n);
@@ -1161,8 +1163,8 @@ struct Engine {
         adj_ws_n = 0;
         rho_grad = nullptr;
         rho_grad_n = 0;
-        // through the block cache, like the per-event blocks
-        HIPCHK(block_alloc((void**)&adj_ws, sizeof(double) * n));
+        // through the block cache, like the per-event blocks (not cleared)
+        HIPCHK(block_alloc((void**)&adj_ws, sizeof(double) * n, false));
         adj_ws_n = n;
         note_memory();
         return IPG_OK;
@@ -1470,7 +1472,7 @@ struct Engine {
                 // K = 1: the state after the step is the next stored state
                 const double* xa = K > 1 ? s_seg + (size_t)(it - s0) * slot : s_ck + (size_t)it * slot;
                 bool rm = false;
-                const int rv = g_tuner.pick(rkey, 4, &rm);
+                const int rv = g_tuner.pick(rkey, 3, &rm);
                 hipEvent_t ra = nullptr, rb = nullptr;
                 if (timed_begin(rkey, rv, rm, &ra, &rb)) (void)hipEventRecord(ra, st);
                 ipg_adj_step(d_adesc, S, xb, xa, it, cin, cout, st, rv);

@@ -98,9 +98,9 @@ void ipg_adj_zero(double* p, long long n, hipStream_t st);
 void ipg_adj_loss(const EvDesc* d_evs, long long S, const double* xs, int it, double yb, double* cout,
                   hipStream_t st);
 // variant 0: three kernels (own-field planes, link outboxes, phi outboxes),
-// 1: one kernel running the three passes per site, 2: one kernel keeping
-// the partial sums of the link cotangents in LDS, 3: the first kernel, then
-// passes 2 and 3 as one with those sums in LDS (the same results)
+// 1: one kernel running the three passes per site with the partial sums of
+// the link cotangents in LDS, 2: the first kernel, then passes 2 and 3 as
+// one with those sums in LDS (the same results)
 void ipg_adj_step(const EvDesc* d_evs, long long S, const double* xb, const double* xa, int it, const double* cin,
                   double* cout, hipStream_t st, int variant = 0);
 // T^{mu nu} at it in reverse: tb[10][S] cotangents of the stored components,

@@ -691,10 +691,9 @@ __device__ __forceinline__ void adj_phi_site(const EvDesc& ev, long long s, cons
     add_h(pout + kPPh * S, S, s, hproj(phs));
 }
 
-// The three passes as kernels of their own (few matrices live at a time)
-// or as one kernel: each pass reads only its own site's planes of pout, so a
-// site can run them back to back.  Same arithmetic in the same order either
-// way.
+// The three passes as kernels of their own (few matrices live at a time).
+// Each pass reads only its own site's planes of pout, so a site can also run
+// them back to back (below).
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_local(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
                                                         const double* __restrict__ xa, int it,
                                                         const double* __restrict__ pin, double* __restrict__ pout) {
@@ -711,25 +710,16 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_phi(const EvDesc* __restrict_
     const long long s = xcd_site();
     if (s < evs[0].S) adj_phi_site(evs[0], s, xb, it, pout);
 }
-__global__ __launch_bounds__(IPG_BLOCK) void k_adj_step(const EvDesc* __restrict__ evs, const double* __restrict__ xb,
-                                                       const double* __restrict__ xa, int it,
-                                                       const double* __restrict__ pin, double* __restrict__ pout) {
-    const long long s = xcd_site();
-    if (s >= evs[0].S) return;
-    adj_local_site(evs[0], s, xb, xa, it, pin, pout);
-    adj_plaq_site(evs[0], s, xb, it, pout);
-    adj_phi_site(evs[0], s, xb, it, pout);
-}
 
 
-// One kernel for the three passes whose partial sums stay on chip: the link
+// The passes back to back with their partial sums on chip: the link
 // cotangents of the own site (pass 1 starts them, passes 2 and 3 add) and the
 // two link outboxes that passes 2 and 3 both write (U_x to (-1,0), U_y to
 // (0,-1)) live in LDS, 72 doubles per thread, until their last term; every
-// other plane of pout is written once, when final.  In the one-kernel form
-// without them each of those planes crosses HBM three times (written, read
-// back after the L2 has evicted it, written again).  Same sums in the same
-// order as the three kernels.
+// other plane of pout is written once, when final.  Kept in pout instead,
+// each of those planes crosses HBM three times (written, read back after the
+// L2 has evicted it, written again).  Same sums in the same order as the
+// three kernels.
 constexpr int kAccSlots = 72;  // U_x, U_y, outbox U_y (0,-1), outbox U_x (-1,0)
 __device__ __forceinline__ M3 acc_ld(const double* a) {
     M3 m;
@@ -868,7 +858,9 @@ __global__ __launch_bounds__(IPG_BLOCK, 1) void k_adj_step_lds(const EvDesc* __r
     asm volatile("" ::: "memory");
     adj_plaq_phi_lds<true>(ev, s, xb, it, pout, aU1);
 }
-// passes 2 and 3 as one kernel after k_adj_local
+// passes 2 and 3 as one kernel after k_adj_local (the whole-step kernel
+// above spills: pass 1's link exponential reverse and pass 3 do not fit the
+// register file together)
 __global__ __launch_bounds__(IPG_BLOCK, 1) void k_adj_plaq_phi(const EvDesc* __restrict__ evs,
                                                               const double* __restrict__ xb, int it,
                                                               double* __restrict__ pout) {
@@ -1023,15 +1015,11 @@ void ipg_adj_loss(const EvDesc* d, long long S, const double* xs, int it, double
 void ipg_adj_step(const EvDesc* d, long long S, const double* xb, const double* xa, int it, const double* cin,
                   double* cout, hipStream_t st, int variant) {
     if (variant == 1) {
-        hipLaunchKernelGGL(k_adj_step, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
-        return;
-    }
-    if (variant == 2) {
         hipLaunchKernelGGL(k_adj_step_lds, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
         return;
     }
     hipLaunchKernelGGL(k_adj_local, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, xa, it, cin, cout);
-    if (variant == 3) {
+    if (variant == 2) {
         hipLaunchKernelGGL(k_adj_plaq_phi, dim3(nb8_of(S)), dim3(IPG_BLOCK), 0, st, d, xb, it, cout);
         return;
     }

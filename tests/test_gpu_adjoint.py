@@ -299,14 +299,14 @@ def test_batched_adjoint_matches_single_event_contexts():
 
 def test_reverse_step_kernels_agree():
     """the reverse leapfrog step as three kernels (tuning off) and, while the
-    tuner measures them, alternating with the one-kernel form, then with the
-    tuner's choice: the same arithmetic per site, so the gradients agree to
-    the round-off of the loss's atomic sums"""
+    tuner measures them, alternating with the fused forms, then
+    with the tuner's choice: the same arithmetic per site, so the gradients
+    agree to the round-off of the loss's atomic sums"""
     import os
 
     import ipglasma_amd as I
-    p = K.params("AuAu", N=48, Ny=3, maxtime=0.4, L=8.0)
-    key = "adj_step_kernels/N=48"
+    p = K.params("AuAu", N=96, Ny=3, maxtime=0.4, L=8.0)
+    key = "adj_step_kernels/N=96"
     assert key not in I.autotune_report()
     runs = []
     for tune in ("0", "1", "1"):

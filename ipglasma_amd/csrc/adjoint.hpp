@@ -496,22 +496,44 @@ IPG_HD void gen_project(const M3& G, double (&out)[8]) {
     out[7] = s3 * (G.e[0].r + G.e[4].r - 2. * G.e[8].r);
 }
 
-IPG_HD void expq_adj(const double (&Q)[8], const M3& Eb, double (&Qb)[8]) {
-    const M3 Qm = gen_matrix(Q);
-    const M3 Q2 = Qm * Qm;
+// exp(i Q^a t^a) = f0 + f1 Q + f2 Q^2 and what its reverse needs: Q, Q^2,
+// the coefficients f_j(c0, c1) and their derivatives (computed once when a
+// caller wants both the exponential and its reverse)
+struct ExpQ {
+    M3 Qm, Q2;
+    cd f[3], fa[3], fb[3];
+};
+IPG_HD void expq_prep(const double (&Q)[8], ExpQ& x) {
+    x.Qm = gen_matrix(Q);
+    x.Q2 = x.Qm * x.Qm;
     double c1 = 0.;
 #pragma unroll
     for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
     c1 *= 0.25;
-    const double c0 = retrprod(Q2, Qm) / 3.;  // det Q = tr(Q^3) / 3
-    cd f[3], fa[3], fb[3];
-    expq_fcoef(c0, c1, f, fa, fb);
+    const double c0 = retrprod(x.Q2, x.Qm) / 3.;  // det Q = tr(Q^3) / 3
+    expq_fcoef(c0, c1, x.f, x.fa, x.fb);
+}
+// the exponential from the coefficients (equal to exp_iQt to round-off)
+IPG_HD M3 expq_value(const ExpQ& x) {
+    M3 E = cscale(x.f[1], x.Qm) + cscale(x.f[2], x.Q2);
+    E.e[0] = E.e[0] + x.f[0];
+    E.e[4] = E.e[4] + x.f[0];
+    E.e[8] = E.e[8] + x.f[0];
+    return E;
+}
+// Qb^a = Re tr(Eb^+ dE/dQ^a)
+IPG_HD void expq_back(const ExpQ& x, const M3& Eb, double (&Qb)[8]) {
     const M3 Ebd = dag(Eb);
-    const cd t0 = tr(Ebd), t1 = trprod(Ebd, Qm), t2 = trprod(Ebd, Q2);
-    const double b0 = (fa[0] * t0 + fa[1] * t1 + fa[2] * t2).r;
-    const double b1 = (fb[0] * t0 + fb[1] * t1 + fb[2] * t2).r;
-    const M3 G = (b0 * Q2 + b1 * Qm) + (cscale(f[1], Ebd) + cscale(f[2], Qm * Ebd + Ebd * Qm));
+    const cd t0 = tr(Ebd), t1 = trprod(Ebd, x.Qm), t2 = trprod(Ebd, x.Q2);
+    const double b0 = (x.fa[0] * t0 + x.fa[1] * t1 + x.fa[2] * t2).r;
+    const double b1 = (x.fb[0] * t0 + x.fb[1] * t1 + x.fb[2] * t2).r;
+    const M3 G = (b0 * x.Q2 + b1 * x.Qm) + (cscale(x.f[1], Ebd) + cscale(x.f[2], x.Qm * Ebd + Ebd * x.Qm));
     gen_project(G, Qb);
+}
+IPG_HD void expq_adj(const double (&Q)[8], const M3& Eb, double (&Qb)[8]) {
+    ExpQ x;
+    expq_prep(Q, x);
+    expq_back(x, Eb, Qb);
 }
 
 }  // namespace ipgad

@@ -539,6 +539,9 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
 // V = E_{Ny-1} ... E_0, E_k = exp(-i A_k^a t^a).  Walking the slices of a chunk
 // backwards with the current V and its cotangent Vb in registers:
 //   V_k = E_k^+ V,  Eb_k = Vb V_k^+,  Vb <- E_k^+ Vb,  Ab_k = -Qb(Q = -A_k, Eb_k).
+// E_k is rebuilt from the Cayley-Hamilton coefficients its reverse needs
+// anyway (equal to the forward's exp_iQt to round-off, so V_k is recovered
+// to round-off).
 // The chunk's A_k are recomputed by the forward row and column passes; Ab_k
 // replaces A_k in the transform buffer, row by row.
 template <int N, int BLK>
@@ -580,13 +583,16 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
-                const M3 E = exp_iQt(Q);
+                // one set of Cayley-Hamilton coefficients for E and its reverse
+                ipgad::ExpQ x;
+                ipgad::expq_prep(Q, x);
+                const M3 E = ipgad::expq_value(x);
                 const M3 Vk = mul_ad(E, V[t]);
                 const M3 Eb = mul_bd(Vb[t], Vk);
                 Vb[t] = mul_ad(E, Vb[t]);
                 V[t] = Vk;
                 double Qb[8];
-                ipgad::expq_adj(Q, Eb, Qb);
+                ipgad::expq_back(x, Eb, Qb);
 #pragma unroll
                 for (int p = 0; p < 4; p++)
                     io[(size_t)p * S + (size_t)i * N + j] = make_double2(-Qb[2 * p], -Qb[2 * p + 1]);

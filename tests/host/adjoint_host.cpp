@@ -88,6 +88,14 @@ void adjh_acc(const double* x, const double* ab, double* y, double* xb) {
     acc_dir_adj(D, U, Dm, Um, ld(ab), b[0], b[1], b[2], b[3]);
     for (int k = 0; k < 4; k++) st(xb + 18 * k, b[k]);
 }
+// the exponential from the reverse's coefficients (expq_value), against exp_iQt
+void adjh_expq_value(const double* q, double* e) {
+    double Q[8];
+    for (int a = 0; a < 8; a++) Q[a] = q[a];
+    ExpQ x;
+    expq_prep(Q, x);
+    st(e, expq_value(x));
+}
 void adjh_expq(const double* q, const double* eb, double* e, double* qb) {
     double Q[8], Qb[8];
     for (int a = 0; a < 8; a++) Q[a] = q[a];

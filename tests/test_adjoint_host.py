@@ -35,6 +35,7 @@ def lib():
         L.adjh_newton.argtypes = [dp] * 6
         L.adjh_acc.argtypes = [dp] * 4
         L.adjh_expq.argtypes = [dp] * 4
+        L.adjh_expq_value.argtypes = [dp] * 2
         _lib = L
     return _lib
 
@@ -219,6 +220,7 @@ def test_su3_exponential_adjoint_against_frechet():
         exact_degenerate.append(len(cases))
         cases.append(np.eye(8)[7] * q8)
     cases.append(np.eye(8)[2] * 1.7)
+    near_degenerate = [len(cases), len(cases) + 1]
     cases.append(np.eye(8)[7] * 2.5 + rng.normal(size=8) * 1e-7)
     cases.append(np.eye(8)[7] * -1.3 + rng.normal(size=8) * 1e-4)
     for n, Q in enumerate(cases):
@@ -238,3 +240,16 @@ def test_su3_exponential_adjoint_against_frechet():
         ref = np.array([np.real(np.trace(Eb.conj().T @ sl.expm_frechet(H, 1j * lam[a], compute_expm=False)))
                         for a in range(8)])
         assert np.allclose(qb, ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max()), (scale, qb, ref)
+        # the exponential rebuilt from the reverse's coefficients (what the
+        # reverse setV pass uses to walk the Wilson lines back): the matrix
+        # exponential to round-off everywhere, and the forward's exp_iQt to
+        # round-off away from (nearly) degenerate spectra, where the closed
+        # form exp_iQt follows loses digits (1.7e-9 at a 1e-7 splitting of
+        # 2.5 t^8; such a Q among Gaussian colour charges has probability
+        # ~ splitting^3)
+        ev = np.zeros(18)
+        L.adjh_expq_value(Q, ev)
+        ev = ev[0::2] + 1j * ev[1::2]
+        assert np.abs(ev - E.reshape(-1)).max() <= 2e-15 * max(1.0, scale), (n, scale)
+        if n not in exact_degenerate + near_degenerate:
+            assert np.abs(ev - (e[0::2] + 1j * e[1::2])).max() <= 1e-13 * max(1.0, scale), (n, scale)

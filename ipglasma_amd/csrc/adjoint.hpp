@@ -403,6 +403,14 @@ IPG_HD void expq_fcoef(double c0, double c1, cd (&f)[3], cd (&fa)[3], cd (&fb)[3
                 br[j] += cr * v1[j];
                 bi[j] += ci * v1[j];
             }
+            // the recursion's matrix has 1-norm max(1, |c0| + c1) = 1 here
+            // (c1 < 0.5), and its derivatives add |d| to a0 and b1, so the
+            // sum s of |v|, |v0|, |v1| at most triples per term while 1/n!
+            // falls at least 5-fold from n = 4: once coef s < 1e-19 the terms
+            // left add less than 1.5e-19 (|f_j| is of order 1)
+            const double sz = fabs(a) + fabs(b) + fabs(d) + fabs(a0) + fabs(b0) + fabs(d0) + fabs(a1) + fabs(b1) +
+                              fabs(d1);
+            if (n >= 4 && coef * sz < 1e-19) break;
             const double na = c0 * d, nb = a + c1 * d, nd = b;
             const double na0 = d + c0 * d0, nb0 = a0 + c1 * d0, nd0 = b0;
             const double na1 = c0 * d1, nb1 = a1 + d + c1 * d1, nd1 = b1;

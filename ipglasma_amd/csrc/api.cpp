@@ -83,6 +83,7 @@ struct Tuner {
         std::vector<int> seen;
         int chosen = -1;
         int next = 0;
+        int run = 0;  // launches of `next` so far in its current run
     };
     std::mutex mu;
     std::map<std::string, Entry> tab;
@@ -103,12 +104,20 @@ struct Tuner {
         }
         *measuring = e.chosen < 0;
         if (e.chosen >= 0) return e.chosen;
-        const int c = e.next;  // round robin over the launches being measured
-        e.next = (e.next + 1) % ncand;
+        // each candidate takes kRun consecutive launches, so that it is timed
+        // after its own previous launch (what that left in the L2 and MALL)
+        // and not after another candidate's
+        const int c = e.next;
+        if (++e.run == kRun) {
+            e.run = 0;
+            e.next = (e.next + 1) % ncand;
+        }
         return c;
     }
+    static constexpr int kRun = 3;
     // the first sample of a candidate is discarded (code-object load, clock
-    // ramp); the best of the next two decides
+    // ramp, the previous candidate's cache state); the best of the next two
+    // decides
     void report(const std::string& key, int cand, float ms) {
         std::lock_guard<std::mutex> g(mu);
         Entry& e = tab[key];

@@ -80,7 +80,7 @@ __device__ __forceinline__ void atomic_add_h(double* f, size_t S, size_t s, cons
     atomicAdd(f + 8 * S + s, h.e[5].i);
 }
 __device__ __forceinline__ void add_h(double* f, size_t S, size_t s, const M3& h) {
-    sth(f, S, s, ldh(f, S, s) + h);
+    sth<true>(f, S, s, ldh<true>(f, S, s) + h);
 }
 
 // The sites u along one axis whose clamped neighbour clamp(u + d) is t: one
@@ -103,14 +103,14 @@ __device__ __forceinline__ M3 gather(const double* f, size_t S, int N, int i, in
         for (int q = 0; q < b.n; q++) {
             const int sj = q ? b.b : b.a;
             const size_t u = (size_t)si * N + sj;
-            r = r + (HERM ? ldh(f, S, u) : ldm(f, S, u));
+            r = r + (HERM ? ldh<true>(f, S, u) : ldm<true>(f, S, u));
         }
     }
     return r;
 }
 // total cotangents of U_x, U_y and phi at (i, j) = s in a pending set
 __device__ __forceinline__ M3 cin_ux(const double* P, size_t S, int N, int i, int j, size_t s) {
-    M3 r = ldm(P + kPU1 * S, S, s);
+    M3 r = ldm<true>(P + kPU1 * S, S, s);
     r = r + gather<false>(P + (kPOx + 0) * S, S, N, i, j, 0, 1);
     r = r + gather<false>(P + (kPOx + 18) * S, S, N, i, j, 0, -1);
     r = r + gather<false>(P + (kPOx + 36) * S, S, N, i, j, -1, 1);
@@ -118,7 +118,7 @@ __device__ __forceinline__ M3 cin_ux(const double* P, size_t S, int N, int i, in
     return r;
 }
 __device__ __forceinline__ M3 cin_uy(const double* P, size_t S, int N, int i, int j, size_t s) {
-    M3 r = ldm(P + kPU2 * S, S, s);
+    M3 r = ldm<true>(P + kPU2 * S, S, s);
     r = r + gather<false>(P + (kPOy + 0) * S, S, N, i, j, 1, 0);
     r = r + gather<false>(P + (kPOy + 18) * S, S, N, i, j, 1, -1);
     r = r + gather<false>(P + (kPOy + 36) * S, S, N, i, j, 0, -1);
@@ -126,7 +126,7 @@ __device__ __forceinline__ M3 cin_uy(const double* P, size_t S, int N, int i, in
     return r;
 }
 __device__ __forceinline__ M3 cin_ph(const double* P, size_t S, int N, int i, int j, size_t s) {
-    M3 r = ldh(P + kPPh * S, S, s);
+    M3 r = ldh<true>(P + kPPh * S, S, s);
     r = r + gather<true>(P + (kPOph + 0) * S, S, N, i, j, 1, 0);
     r = r + gather<true>(P + (kPOph + 9) * S, S, N, i, j, 0, 1);
     r = r + gather<true>(P + (kPOph + 18) * S, S, N, i, j, -1, 0);
@@ -204,22 +204,22 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict
     const Slot X = slot_of(xs, S);
     const size_t pX = L.at(i + 1, j), pY = L.at(i, j + 1), pXY = L.at(i + 1, j + 1);
     TIn q;
-    q.e1 = ldh(X.E1, S, s);
-    q.e1py = ldh(X.E1, S, pY);
-    q.e2 = ldh(X.E2, S, s);
-    q.e2px = ldh(X.E2, S, pX);
-    q.p = ldh(X.pi, S, s);
-    q.ppx = ldh(X.pi, S, pX);
-    q.ppy = ldh(X.pi, S, pY);
-    q.ppxy = ldh(X.pi, S, pXY);
-    q.ux = ldm(X.Ux, S, s);
-    q.uy = ldm(X.Uy, S, s);
-    q.uy_px = ldm(X.Uy, S, pX);
-    q.ux_py = ldm(X.Ux, S, pY);
-    q.ph = ldh(X.phi, S, s);
-    q.phpx = ldh(X.phi, S, pX);
-    q.phpy = ldh(X.phi, S, pY);
-    q.phpxy = ldh(X.phi, S, pXY);
+    q.e1 = ldh<true>(X.E1, S, s);
+    q.e1py = ldh<true>(X.E1, S, pY);
+    q.e2 = ldh<true>(X.E2, S, s);
+    q.e2px = ldh<true>(X.E2, S, pX);
+    q.p = ldh<true>(X.pi, S, s);
+    q.ppx = ldh<true>(X.pi, S, pX);
+    q.ppy = ldh<true>(X.pi, S, pY);
+    q.ppxy = ldh<true>(X.pi, S, pXY);
+    q.ux = ldm<true>(X.Ux, S, s);
+    q.uy = ldm<true>(X.Uy, S, s);
+    q.uy_px = ldm<true>(X.Uy, S, pX);
+    q.ux_py = ldm<true>(X.Ux, S, pY);
+    q.ph = ldh<true>(X.phi, S, s);
+    q.phpx = ldh<true>(X.phi, S, pX);
+    q.phpy = ldh<true>(X.phi, S, pY);
+    q.phpxy = ldh<true>(X.phi, S, pXY);
     q.plaq = i != N - 1;
     TBar b;
     M3* bb = reinterpret_cast<M3*>(&b);
@@ -273,9 +273,9 @@ struct TScat {
 // 1 = U_x, 2 = U_y), given Gb
 __device__ __forceinline__ void grad_adj(const TScat& sc, const Slot& X, size_t S, int f, size_t uq, size_t n,
                                          size_t m, const M3& Gb) {
-    const M3 u = ldm(f == 1 ? X.Ux : X.Uy, S, uq);
+    const M3 u = ldm<true>(f == 1 ? X.Ux : X.Uy, S, uq);
     M3 ub = m_zero(), pb = m_zero();
-    adj_conj(u, ldh(X.phi, S, n), Gb, ub, pb);
+    adj_conj(u, ldh<true>(X.phi, S, n), Gb, ub, pb);
     if (f == 1)
         sc.u1(uq, ub);
     else
@@ -284,8 +284,8 @@ __device__ __forceinline__ void grad_adj(const TScat& sc, const Slot& X, size_t 
     sc.ph(m, -1. * Gb);
 }
 __device__ __forceinline__ M3 grad_fwd(const Slot& X, size_t S, int f, size_t uq, size_t n, size_t m) {
-    const M3 u = ldm(f == 1 ? X.Ux : X.Uy, S, uq);
-    return (u * ldh(X.phi, S, n)) * dag(u) - ldh(X.phi, S, m);
+    const M3 u = ldm<true>(f == 1 ? X.Ux : X.Uy, S, uq);
+    return (u * ldh<true>(X.phi, S, n)) * dag(u) - ldh<true>(X.phi, S, m);
 }
 
 __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_a(const EvDesc* __restrict__ evs, const double* __restrict__ xs,
@@ -310,21 +310,21 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_a(const EvDesc* __restr
     const double reb = tb[6 * S + s] * g / (tau * tau * tau) / a5;  // T^{tau eta}
     const double bb = tb[7 * S + s] * 0.5 / (tau * tau) / a4;       // T^{xy}
     // pi: sum Re tr(pi^2) / 4
-    sc.pi(s, (lb / 2.) * ldh(X.pi, S, s));
-    sc.pi(pX, (lb / 2.) * ldh(X.pi, S, pX));
-    sc.pi(pY, (lb / 2.) * ldh(X.pi, S, pY));
-    sc.pi(pXY, (lb / 2.) * ldh(X.pi, S, pXY));
+    sc.pi(s, (lb / 2.) * ldh<true>(X.pi, S, s));
+    sc.pi(pX, (lb / 2.) * ldh<true>(X.pi, S, pX));
+    sc.pi(pY, (lb / 2.) * ldh<true>(X.pi, S, pY));
+    sc.pi(pXY, (lb / 2.) * ldh<true>(X.pi, S, pXY));
     // plaquette: 2/g^2 (3 - Re tr(ux uy(+x) ux(+y)^+ uy^+)), 1 on the last row
     if (i != N - 1) {
         M3 B1, B2, B3, B4;
-        adj_prod4(ldm(X.Ux, S, s), ldm(X.Uy, S, pX), dag(ldm(X.Ux, S, pY)), dag(ldm(X.Uy, S, s)),
+        adj_prod4(ldm<true>(X.Ux, S, s), ldm<true>(X.Uy, S, pX), dag(ldm<true>(X.Ux, S, pY)), dag(ldm<true>(X.Uy, S, s)),
                   (-2. / (g * g) * lb) * m_one(), B1, B2, B3, B4);
         sc.u1(s, B1);
         sc.u2(pX, B2);
         sc.u1(pY, dag(B3));
         sc.u2(s, dag(B4));
     }
-    const M3 e1 = ldh(X.E1, S, s), e1p = ldh(X.E1, S, pY), e2 = ldh(X.E2, S, s), e2p = ldh(X.E2, S, pX);
+    const M3 e1 = ldh<true>(X.E1, S, s), e1p = ldh<true>(X.E1, S, pY), e2 = ldh<true>(X.E2, S, s), e2p = ldh<true>(X.E2, S, pX);
     M3 e1b = (2. * t1b) * e1, e1pb = (2. * t1b) * e1p, e2b = (2. * t2b) * e2, e2pb = (2. * t2b) * e2p;
     const M3 gx = grad_fwd(X, S, 1, s, pX, s), gy = grad_fwd(X, S, 2, s, pY, s);
     const M3 gxy1 = grad_fwd(X, S, 1, pY, pXY, pY), gxy2 = grad_fwd(X, S, 2, pX, pXY, pX);
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_a(const EvDesc* __restr
     gxy2b = gxy2b + reb * dag(e2p);
     // T^{xy}: 4 Re tr(c P Q) + Re tr(gx gy) + Re tr(yl gy) + Re tr(gx xl) + Re tr(yl xl)
     {
-        const M3 ux = ldm(X.Ux, S, s), uy = ldm(X.Uy, S, s);
+        const M3 ux = ldm<true>(X.Ux, S, s), uy = ldm<true>(X.Uy, S, s);
         M3 uxb = m_zero(), uyb = m_zero();
         const double c = -0.25 * g * g;
         const M3 Pm = e1 + (uy * e1p) * dag(uy), Qm = e2 + (ux * e2p) * dag(ux);
@@ -377,7 +377,7 @@ struct Fac {
     bool d;
 };
 __device__ __forceinline__ M3 fac_ld(const Slot& X, size_t S, const Fac& a) {
-    const M3 m = ldm(a.f == 1 ? X.Ux : X.Uy, S, a.q);
+    const M3 m = ldm<true>(a.f == 1 ? X.Ux : X.Uy, S, a.q);
     return a.d ? dag(m) : m;
 }
 __device__ __forceinline__ void fac_st(const TScat& sc, const Fac& a, const M3& b) {
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
     {
         const Fac pc[4] = {{1, mX, true}, {2, mX, false}, {1, mXpY, false}, {2, s0, true}};
         const M3 K = ta + ah(prod4(X, S, pc));
-        const M3 e2 = ldh(X.E2, S, s0), gy = grad_fwd(X, S, 2, s0, pY, s0);
+        const M3 e2 = ldh<true>(X.E2, S, s0), gy = grad_fwd(X, S, 2, s0, pY, s0);
         // ttx_im = Im tr(E2 K), xe_im = -Im tr(K gY)
         const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gy));
         sc.e2(s0, mul_i(txb, dag(K)));
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
     {
         const Fac pc[4] = {{2, mY, true}, {1, mY, false}, {2, pXmY, false}, {1, s0, true}};
         const M3 K = ah(prod4(X, S, pc)) - ta;
-        const M3 e1 = ldh(X.E1, S, s0), gx = grad_fwd(X, S, 1, s0, pX, s0);
+        const M3 e1 = ldh<true>(X.E1, S, s0), gx = grad_fwd(X, S, 1, s0, pX, s0);
         const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gx));
         sc.e1(s0, mul_i(tyb, dag(K)));
         grad_adj(sc, X, S, 1, s0, pX, s0, mul_i(-yeb, dag(K)));
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
         const Fac pe[4] = {{2, pX, false}, {1, pXY, false}, {2, p2X, true}, {1, pX, true}};
         const Fac pg[4] = {{1, s0, true}, {2, s0, false}, {1, pY, false}, {2, pX, true}};
         const M3 K = ah(prod4(X, S, pe)) + ah(prod4(X, S, pg));
-        const M3 e2 = ldh(X.E2, S, pX), gg = grad_fwd(X, S, 2, pX, pXY, pX);
+        const M3 e2 = ldh<true>(X.E2, S, pX), gg = grad_fwd(X, S, 2, pX, pXY, pX);
         const M3 Kb = mul_i(txb, dag(e2)) + mul_i(-xeb, dag(gg));
         sc.e2(pX, mul_i(txb, dag(K)));
         grad_adj(sc, X, S, 2, pX, pXY, pX, mul_i(-xeb, dag(K)));
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
         const Fac qe[4] = {{1, pY, false}, {2, pXY, false}, {1, p2Y, true}, {2, pY, true}};
         const Fac qg[4] = {{2, s0, true}, {1, s0, false}, {2, pX, false}, {1, pY, true}};
         const M3 K = ah(prod4(X, S, qe)) + ah(prod4(X, S, qg));
-        const M3 e1 = ldh(X.E1, S, pY), gg = grad_fwd(X, S, 1, pY, pXY, pY);
+        const M3 e1 = ldh<true>(X.E1, S, pY), gg = grad_fwd(X, S, 1, pY, pXY, pY);
         const M3 Kb = mul_i(tyb, dag(e1)) + mul_i(-yeb, dag(gg));
         sc.e1(pY, mul_i(tyb, dag(K)));
         grad_adj(sc, X, S, 1, pY, pXY, pY, mul_i(-yeb, dag(K)));
@@ -493,13 +493,13 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_tmunu_b(const EvDesc* __restr
 __device__ __attribute__((noinline)) void pigrad_adj(const TScat& sc, const Slot& X, size_t S, int f, double c, size_t w,
                                            size_t uq, size_t n, size_t vq, size_t m) {
     const double* U = f == 1 ? X.Ux : X.Uy;
-    const M3 u = ldm(U, S, uq), v = ldm(U, S, vq), pw = ldh(X.pi, S, w);
-    const M3 G = (u * ldh(X.phi, S, n)) * dag(u) - (dag(v) * ldh(X.phi, S, m)) * v;
+    const M3 u = ldm<true>(U, S, uq), v = ldm<true>(U, S, vq), pw = ldh<true>(X.pi, S, w);
+    const M3 G = (u * ldh<true>(X.phi, S, n)) * dag(u) - (dag(v) * ldh<true>(X.phi, S, m)) * v;
     sc.pi(w, c * dag(G));
     const M3 Yb = c * dag(pw);
     M3 ub = m_zero(), vb = m_zero(), nb = m_zero(), mb = m_zero();
-    adj_conj(u, ldh(X.phi, S, n), Yb, ub, nb);
-    adj_conjd(v, ldh(X.phi, S, m), -1. * Yb, vb, mb);
+    adj_conj(u, ldh<true>(X.phi, S, n), Yb, ub, nb);
+    adj_conjd(v, ldh<true>(X.phi, S, m), -1. * Yb, vb, mb);
     if (f == 1) {
         sc.u1(uq, ub);
         sc.u1(vq, vb);
@@ -514,8 +514,8 @@ __device__ __attribute__((noinline)) void pigrad_adj(const TScat& sc, const Slot
 __device__ __attribute__((noinline)) void epi_adj(const TScat& sc, const Slot& X, size_t S, int fe, double c, size_t w,
                                         size_t uq, size_t n) {
     const double* U = fe == 1 ? X.Ux : X.Uy;
-    const M3 u = ldm(U, S, uq), e = ldh(fe == 1 ? X.E1 : X.E2, S, w);
-    const M3 Z = ldh(X.pi, S, w) + (u * ldh(X.pi, S, n)) * dag(u);
+    const M3 u = ldm<true>(U, S, uq), e = ldh<true>(fe == 1 ? X.E1 : X.E2, S, w);
+    const M3 Z = ldh<true>(X.pi, S, w) + (u * ldh<true>(X.pi, S, n)) * dag(u);
     if (fe == 1)
         sc.e1(w, c * dag(Z));
     else
@@ -523,7 +523,7 @@ __device__ __attribute__((noinline)) void epi_adj(const TScat& sc, const Slot& X
     const M3 Zb = c * dag(e);
     sc.pi(w, Zb);
     M3 ub = m_zero(), nb = m_zero();
-    adj_conj(u, ldh(X.pi, S, n), Zb, ub, nb);
+    adj_conj(u, ldh<true>(X.pi, S, n), Zb, ub, nb);
     if (fe == 1)
         sc.u1(uq, ub);
     else
@@ -592,20 +592,20 @@ __device__ __forceinline__ void adj_local_site(const EvDesc& ev, long long s, co
     const Slot B = slot_of(xb, S), A = slot_of(xa, S);
     const double kappa = g * g * dt / (tau + dt / 2.);
     {
-        M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
-        u_adj(ldm(B.Ux, S, s), ldh(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
-        stm(pout + kPU1 * S, S, s, ub);
-        sth(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
+        M3 e1b = ldh<true>(pin + kPE1 * S, S, s), ub = m_zero();
+        u_adj(ldm<true>(B.Ux, S, s), ldh<true>(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+        stm<true>(pout + kPU1 * S, S, s, ub);
+        sth<true>(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
     }
     {
-        M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
-        u_adj(ldm(B.Uy, S, s), ldh(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
-        stm(pout + kPU2 * S, S, s, ub);
-        sth(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
+        M3 e2b = ldh<true>(pin + kPE2 * S, S, s), ub = m_zero();
+        u_adj(ldm<true>(B.Uy, S, s), ldh<true>(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+        stm<true>(pout + kPU2 * S, S, s, ub);
+        sth<true>(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
     }
     const M3 phb = cin_ph(pin, S, N, i, j, s);
-    sth(pout + kPPh * S, S, s, phb);
-    sth(pout + kPPi * S, S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
+    sth<true>(pout + kPPh * S, S, s, phb);
+    sth<true>(pout + kPPi * S, S, s, ldh<true>(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
 }
 
 // pass 2: the plaquettes of E' (U12 = ux uy_px ux_py^+ uy^+,
@@ -618,29 +618,29 @@ __device__ __forceinline__ void adj_plaq_site(const EvDesc& ev, long long s, con
     const double dt = ev.dtau, tau = it * dt, g = ev.g;
     const Slot B = slot_of(xb, S);
     M3 U12b, U1m2b, U2m1b;
-    plaq_bars(ldh(pout + kPE1 * S, S, s), ldh(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
+    plaq_bars(ldh<true>(pout + kPE1 * S, S, s), ldh<true>(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
     M3 B1, B2, B3, B4, uxs, uys;
     {
-        const M3 ux = ldm(B.Ux, S, s);
-        adj_prod4(ux, ldm(B.Uy, S, L.at(i + 1, j)), dag(ldm(B.Ux, S, L.at(i, j + 1))), dag(ldm(B.Uy, S, s)), U12b,
+        const M3 ux = ldm<true>(B.Ux, S, s);
+        adj_prod4(ux, ldm<true>(B.Uy, S, L.at(i + 1, j)), dag(ldm<true>(B.Ux, S, L.at(i, j + 1))), dag(ldm<true>(B.Uy, S, s)), U12b,
                   B1, B2, B3, B4);
         uxs = B1;
-        stm(pout + (kPOy + 0) * S, S, s, B2);
-        stm(pout + (kPOx + 0) * S, S, s, dag(B3));
+        stm<true>(pout + (kPOy + 0) * S, S, s, B2);
+        stm<true>(pout + (kPOx + 0) * S, S, s, dag(B3));
         uys = dag(B4);
-        adj_prod4(ux, dag(ldm(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm(B.Ux, S, L.at(i, j - 1))),
-                  ldm(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
+        adj_prod4(ux, dag(ldm<true>(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm<true>(B.Ux, S, L.at(i, j - 1))),
+                  ldm<true>(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
     }
     uxs = uxs + B1;
-    stm(pout + (kPOy + 18) * S, S, s, dag(B2));
-    stm(pout + (kPOx + 18) * S, S, s, dag(B3));
-    stm(pout + (kPOy + 36) * S, S, s, B4);
-    adj_prod4(ldm(B.Uy, S, s), dag(ldm(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm(B.Uy, S, L.at(i - 1, j))),
-              ldm(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
+    stm<true>(pout + (kPOy + 18) * S, S, s, dag(B2));
+    stm<true>(pout + (kPOx + 18) * S, S, s, dag(B3));
+    stm<true>(pout + (kPOy + 36) * S, S, s, B4);
+    adj_prod4(ldm<true>(B.Uy, S, s), dag(ldm<true>(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm<true>(B.Uy, S, L.at(i - 1, j))),
+              ldm<true>(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
     uys = uys + B1;
-    stm(pout + (kPOx + 36) * S, S, s, dag(B2));
-    stm(pout + (kPOy + 54) * S, S, s, dag(B3));
-    stm(pout + (kPOx + 54) * S, S, s, B4);
+    stm<true>(pout + (kPOx + 36) * S, S, s, dag(B2));
+    stm<true>(pout + (kPOy + 54) * S, S, s, dag(B3));
+    stm<true>(pout + (kPOx + 54) * S, S, s, B4);
     add_m3(pout + kPU1 * S, S, s, uxs);
     add_m3(pout + kPU2 * S, S, s, uys);
 }
@@ -655,38 +655,38 @@ __device__ __forceinline__ void adj_phi_site(const EvDesc& ev, long long s, cons
     const double dt = ev.dtau, tau = it * dt;
     const Slot B = slot_of(xb, S);
     const cd c2c{0., -dt / tau};  // conj(i dtau / tau)
-    const M3 Bb = (dt / tau) * ldh(pout + kPPi * S, S, s);
-    const M3 ph = ldh(B.phi, S, s);
+    const M3 Bb = (dt / tau) * ldh<true>(pout + kPPi * S, S, s);
+    const M3 ph = ldh<true>(B.phi, S, s);
     M3 phs = -4. * Bb;
     {
-        const M3 u = ldm(B.Ux, S, s), pn = ldh(B.phi, S, L.at(i + 1, j));
+        const M3 u = ldm<true>(B.Ux, S, s), pn = ldh<true>(B.phi, S, L.at(i + 1, j));
         M3 ub = m_zero(), pnb = m_zero();
-        comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE1 * S, S, s)), phs, ub, pnb);
+        comm_adj(u, pn, ph, cscale(c2c, ldh<true>(pout + kPE1 * S, S, s)), phs, ub, pnb);
         adj_conj(u, pn, Bb, ub, pnb);
         add_m3(pout + kPU1 * S, S, s, ub);
-        sth(pout + (kPOph + 0) * S, S, s, hproj(pnb));
+        sth<true>(pout + (kPOph + 0) * S, S, s, hproj(pnb));
     }
     {
-        const M3 u = ldm(B.Uy, S, s), pn = ldh(B.phi, S, L.at(i, j + 1));
+        const M3 u = ldm<true>(B.Uy, S, s), pn = ldh<true>(B.phi, S, L.at(i, j + 1));
         M3 ub = m_zero(), pnb = m_zero();
-        comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE2 * S, S, s)), phs, ub, pnb);
+        comm_adj(u, pn, ph, cscale(c2c, ldh<true>(pout + kPE2 * S, S, s)), phs, ub, pnb);
         adj_conj(u, pn, Bb, ub, pnb);
         add_m3(pout + kPU2 * S, S, s, ub);
-        sth(pout + (kPOph + 9) * S, S, s, hproj(pnb));
+        sth<true>(pout + (kPOph + 9) * S, S, s, hproj(pnb));
     }
     {
         const size_t sm = L.at(i - 1, j);
         M3 ub = m_zero(), pnb = m_zero();
-        adj_conjd(ldm(B.Ux, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+        adj_conjd(ldm<true>(B.Ux, S, sm), ldh<true>(B.phi, S, sm), Bb, ub, pnb);
         add_m3(pout + (kPOx + 54) * S, S, s, ub);
-        sth(pout + (kPOph + 18) * S, S, s, hproj(pnb));
+        sth<true>(pout + (kPOph + 18) * S, S, s, hproj(pnb));
     }
     {
         const size_t sm = L.at(i, j - 1);
         M3 ub = m_zero(), pnb = m_zero();
-        adj_conjd(ldm(B.Uy, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
+        adj_conjd(ldm<true>(B.Uy, S, sm), ldh<true>(B.phi, S, sm), Bb, ub, pnb);
         add_m3(pout + (kPOy + 36) * S, S, s, ub);
-        sth(pout + (kPOph + 27) * S, S, s, hproj(pnb));
+        sth<true>(pout + (kPOph + 27) * S, S, s, hproj(pnb));
     }
     add_h(pout + kPPh * S, S, s, hproj(phs));
 }
@@ -753,68 +753,68 @@ __device__ __forceinline__ void adj_plaq_phi_lds(const EvDesc& ev, long long s, 
     // pass 2: the plaquettes of E'
     {
         M3 U12b, U1m2b, U2m1b;
-        plaq_bars(ldh(pout + kPE1 * S, S, s), ldh(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
+        plaq_bars(ldh<true>(pout + kPE1 * S, S, s), ldh<true>(pout + kPE2 * S, S, s), tau, dt, g, U12b, U1m2b, U2m1b);
         M3 B1, B2, B3, B4, uxs, uys;
         {
-            const M3 ux = ldm(B.Ux, S, s);
-            adj_prod4(ux, ldm(B.Uy, S, L.at(i + 1, j)), dag(ldm(B.Ux, S, L.at(i, j + 1))), dag(ldm(B.Uy, S, s)),
+            const M3 ux = ldm<true>(B.Ux, S, s);
+            adj_prod4(ux, ldm<true>(B.Uy, S, L.at(i + 1, j)), dag(ldm<true>(B.Ux, S, L.at(i, j + 1))), dag(ldm<true>(B.Uy, S, s)),
                       U12b, B1, B2, B3, B4);
             uxs = B1;
-            stm(pout + (kPOy + 0) * S, S, s, B2);
-            stm(pout + (kPOx + 0) * S, S, s, dag(B3));
+            stm<true>(pout + (kPOy + 0) * S, S, s, B2);
+            stm<true>(pout + (kPOx + 0) * S, S, s, dag(B3));
             uys = dag(B4);
-            adj_prod4(ux, dag(ldm(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm(B.Ux, S, L.at(i, j - 1))),
-                      ldm(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
+            adj_prod4(ux, dag(ldm<true>(B.Uy, S, L.at(i + 1, j - 1))), dag(ldm<true>(B.Ux, S, L.at(i, j - 1))),
+                      ldm<true>(B.Uy, S, L.at(i, j - 1)), U1m2b, B1, B2, B3, B4);
         }
         uxs = uxs + B1;
-        stm(pout + (kPOy + 18) * S, S, s, dag(B2));
-        stm(pout + (kPOx + 18) * S, S, s, dag(B3));
+        stm<true>(pout + (kPOy + 18) * S, S, s, dag(B2));
+        stm<true>(pout + (kPOx + 18) * S, S, s, dag(B3));
         acc_st(aOy36, B4);
-        adj_prod4(ldm(B.Uy, S, s), dag(ldm(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm(B.Uy, S, L.at(i - 1, j))),
-                  ldm(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
+        adj_prod4(ldm<true>(B.Uy, S, s), dag(ldm<true>(B.Ux, S, L.at(i - 1, j + 1))), dag(ldm<true>(B.Uy, S, L.at(i - 1, j))),
+                  ldm<true>(B.Ux, S, L.at(i - 1, j)), U2m1b, B1, B2, B3, B4);
         uys = uys + B1;
-        stm(pout + (kPOx + 36) * S, S, s, dag(B2));
-        stm(pout + (kPOy + 54) * S, S, s, dag(B3));
+        stm<true>(pout + (kPOx + 36) * S, S, s, dag(B2));
+        stm<true>(pout + (kPOy + 54) * S, S, s, dag(B3));
         acc_st(aOx54, B4);
-        acc_st(aU1, (P1_LDS ? acc_ld(aU1) : ldm(pout + kPU1 * S, S, s)) + uxs);
-        acc_st(aU2, (P1_LDS ? acc_ld(aU2) : ldm(pout + kPU2 * S, S, s)) + uys);
+        acc_st(aU1, (P1_LDS ? acc_ld(aU1) : ldm<true>(pout + kPU1 * S, S, s)) + uxs);
+        acc_st(aU2, (P1_LDS ? acc_ld(aU2) : ldm<true>(pout + kPU2 * S, S, s)) + uys);
     }
     asm volatile("" ::: "memory");
     // pass 3: the commutators of E' and the pi' stencil
     {
         const cd c2c{0., -dt / tau};  // conj(i dtau / tau)
-        const M3 Bb = (dt / tau) * ldh(pout + kPPi * S, S, s);
-        const M3 ph = ldh(B.phi, S, s);
+        const M3 Bb = (dt / tau) * ldh<true>(pout + kPPi * S, S, s);
+        const M3 ph = ldh<true>(B.phi, S, s);
         M3 phs = -4. * Bb;
         {
-            const M3 u = ldm(B.Ux, S, s), pn = ldh(B.phi, S, L.at(i + 1, j));
+            const M3 u = ldm<true>(B.Ux, S, s), pn = ldh<true>(B.phi, S, L.at(i + 1, j));
             M3 ub = m_zero(), pnb = m_zero();
-            comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE1 * S, S, s)), phs, ub, pnb);
+            comm_adj(u, pn, ph, cscale(c2c, ldh<true>(pout + kPE1 * S, S, s)), phs, ub, pnb);
             adj_conj(u, pn, Bb, ub, pnb);
-            stm(pout + kPU1 * S, S, s, acc_ld(aU1) + ub);
-            sth(pout + (kPOph + 0) * S, S, s, hproj(pnb));
+            stm<true>(pout + kPU1 * S, S, s, acc_ld(aU1) + ub);
+            sth<true>(pout + (kPOph + 0) * S, S, s, hproj(pnb));
         }
         {
-            const M3 u = ldm(B.Uy, S, s), pn = ldh(B.phi, S, L.at(i, j + 1));
+            const M3 u = ldm<true>(B.Uy, S, s), pn = ldh<true>(B.phi, S, L.at(i, j + 1));
             M3 ub = m_zero(), pnb = m_zero();
-            comm_adj(u, pn, ph, cscale(c2c, ldh(pout + kPE2 * S, S, s)), phs, ub, pnb);
+            comm_adj(u, pn, ph, cscale(c2c, ldh<true>(pout + kPE2 * S, S, s)), phs, ub, pnb);
             adj_conj(u, pn, Bb, ub, pnb);
-            stm(pout + kPU2 * S, S, s, acc_ld(aU2) + ub);
-            sth(pout + (kPOph + 9) * S, S, s, hproj(pnb));
+            stm<true>(pout + kPU2 * S, S, s, acc_ld(aU2) + ub);
+            sth<true>(pout + (kPOph + 9) * S, S, s, hproj(pnb));
         }
         {
             const size_t sm = L.at(i - 1, j);
             M3 ub = m_zero(), pnb = m_zero();
-            adj_conjd(ldm(B.Ux, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
-            stm(pout + (kPOx + 54) * S, S, s, acc_ld(aOx54) + ub);
-            sth(pout + (kPOph + 18) * S, S, s, hproj(pnb));
+            adj_conjd(ldm<true>(B.Ux, S, sm), ldh<true>(B.phi, S, sm), Bb, ub, pnb);
+            stm<true>(pout + (kPOx + 54) * S, S, s, acc_ld(aOx54) + ub);
+            sth<true>(pout + (kPOph + 18) * S, S, s, hproj(pnb));
         }
         {
             const size_t sm = L.at(i, j - 1);
             M3 ub = m_zero(), pnb = m_zero();
-            adj_conjd(ldm(B.Uy, S, sm), ldh(B.phi, S, sm), Bb, ub, pnb);
-            stm(pout + (kPOy + 36) * S, S, s, acc_ld(aOy36) + ub);
-            sth(pout + (kPOph + 27) * S, S, s, hproj(pnb));
+            adj_conjd(ldm<true>(B.Uy, S, sm), ldh<true>(B.phi, S, sm), Bb, ub, pnb);
+            stm<true>(pout + (kPOy + 36) * S, S, s, acc_ld(aOy36) + ub);
+            sth<true>(pout + (kPOph + 27) * S, S, s, hproj(pnb));
         }
         add_h(pout + kPPh * S, S, s, hproj(phs));
     }
@@ -839,20 +839,20 @@ __global__ __launch_bounds__(IPG_BLOCK, 1) void k_adj_step_lds(const EvDesc* __r
     {
         const double kappa = g * g * dt / (tau + dt / 2.);
         {
-            M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
-            u_adj(ldm(B.Ux, S, s), ldh(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+            M3 e1b = ldh<true>(pin + kPE1 * S, S, s), ub = m_zero();
+            u_adj(ldm<true>(B.Ux, S, s), ldh<true>(A.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
             acc_st(aU1, ub);
-            sth(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
+            sth<true>(pout + kPE1 * S, S, s, traceless(hproj(e1b)));
         }
         {
-            M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
-            u_adj(ldm(B.Uy, S, s), ldh(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+            M3 e2b = ldh<true>(pin + kPE2 * S, S, s), ub = m_zero();
+            u_adj(ldm<true>(B.Uy, S, s), ldh<true>(A.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
             acc_st(aU2, ub);
-            sth(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
+            sth<true>(pout + kPE2 * S, S, s, traceless(hproj(e2b)));
         }
         const M3 phb = cin_ph(pin, S, N, i, j, s);
-        sth(pout + kPPh * S, S, s, phb);
-        sth(pout + kPPi * S, S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
+        sth<true>(pout + kPPh * S, S, s, phb);
+        sth<true>(pout + kPPi * S, S, s, ldh<true>(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
     }
     // the partial sums are read back from LDS, not kept in registers
     asm volatile("" ::: "memory");
@@ -882,20 +882,20 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_kick(const EvDesc* __restrict
     const Slot X = slot_of(x0, S);
     const double kappa = g * g * dt / (tau + dt / 2.);
     {
-        M3 e1b = ldh(pin + kPE1 * S, S, s), ub = m_zero();
-        u_adj(ldm(X.Ux, S, s), ldh(X.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
-        stm(cot(cout, S, 0), S, s, ub);
-        stm(cot(cout, S, 2), S, s, hproj(e1b));
+        M3 e1b = ldh<true>(pin + kPE1 * S, S, s), ub = m_zero();
+        u_adj(ldm<true>(X.Ux, S, s), ldh<true>(X.E1, S, s), kappa, cin_ux(pin, S, N, i, j, s), ub, e1b);
+        stm<true>(cot(cout, S, 0), S, s, ub);
+        stm<true>(cot(cout, S, 2), S, s, hproj(e1b));
     }
     {
-        M3 e2b = ldh(pin + kPE2 * S, S, s), ub = m_zero();
-        u_adj(ldm(X.Uy, S, s), ldh(X.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
-        stm(cot(cout, S, 1), S, s, ub);
-        stm(cot(cout, S, 3), S, s, hproj(e2b));
+        M3 e2b = ldh<true>(pin + kPE2 * S, S, s), ub = m_zero();
+        u_adj(ldm<true>(X.Uy, S, s), ldh<true>(X.E2, S, s), kappa, cin_uy(pin, S, N, i, j, s), ub, e2b);
+        stm<true>(cot(cout, S, 1), S, s, ub);
+        stm<true>(cot(cout, S, 3), S, s, hproj(e2b));
     }
     const M3 phb = cin_ph(pin, S, N, i, j, s);
-    stm(cot(cout, S, 4), S, s, phb);
-    stm(cot(cout, S, 5), S, s, ldh(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
+    stm<true>(cot(cout, S, 4), S, s, phb);
+    stm<true>(cot(cout, S, 5), S, s, ldh<true>(pin + kPPi * S, S, s) + ((tau + dt / 2.) * dt) * phb);
 }
 
 // ---- back from the tau = 0 fields to the Wilson lines ------------------------------
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_kick(const EvDesc* __restrict
 __device__ __forceinline__ M3 link_of(const double* V, const Lat& L, int i, int j, int dir) {
     const size_t s = (size_t)i * L.N + j;
     const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
-    return mul_bd(ldm(V, L.S, s), ldm(V, L.S, n));
+    return mul_bd(ldm<true>(V, L.S, s), ldm<true>(V, L.S, n));
 }
 
 // pi = -(i/4g) acc at tau = 0 (k_pi_init) in reverse: accb = conj(-i/4g) pib;
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_pi_init(const EvDesc* __restr
     const double* VA = vab;
     const double* VB = vab + 18 * S;
     const double c = -1. / (4. * ev.g);
-    const M3 accb = cscale(cd{0., -c}, hproj(ldm(cot(const_cast<double*>(cs0), S, 5), S, s)));
+    const M3 accb = cscale(cd{0., -c}, hproj(ldm<true>(cot(const_cast<double*>(cs0), S, 5), S, s)));
     for (int dir = 0; dir < 2; dir++) {
         const int im = dir == 0 ? i - 1 : i, jm = dir == 0 ? j : j - 1;
         const size_t sm = L.at(im, jm);
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_pi_init(const EvDesc* __restr
         const M3 Dm = link_of(VA, L, imc, jmc, dir) - link_of(VB, L, imc, jmc, dir);
         const double* Uf = dir == 0 ? X.Ux : X.Uy;
         M3 Db = m_zero(), Ub = m_zero(), Dmb = m_zero(), Umb = m_zero();
-        acc_dir_adj(D, ldm(Uf, S, s), Dm, ldm(Uf, S, sm), accb, Db, Ub, Dmb, Umb);
+        acc_dir_adj(D, ldm<true>(Uf, S, s), Dm, ldm<true>(Uf, S, sm), accb, Db, Ub, Dmb, Umb);
         double* ul = ulb + (size_t)dir * 18 * S;
         double* dd = db + (size_t)dir * 18 * S;
         atomic_add_m3(ul, S, s, Ub);
@@ -966,12 +966,12 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __
     {
         constexpr int dir = DIR;
         const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
-        const M3 va = ldm(VA, S, s), van = ldm(VA, S, n), vbs = ldm(VB, S, s), vbn = ldm(VB, S, n);
+        const M3 va = ldm<true>(VA, S, s), van = ldm<true>(VA, S, n), vbs = ldm<true>(VB, S, s), vbn = ldm<true>(VB, S, n);
         const M3 U1 = mul_bd(va, van), U2 = mul_bd(vbs, vbn);
-        const M3 U = ldm(dir == 0 ? X.Ux : X.Uy, S, s);
+        const M3 U = ldm<true>(dir == 0 ? X.Ux : X.Uy, S, s);
         M3 U1b = m_zero(), U2b = m_zero();
-        newton_adj(U1, U2, U, ldm(ulb + (size_t)dir * 18 * S, S, s), U1b, U2b);
-        const M3 Db = ldm(db + (size_t)dir * 18 * S, S, s);
+        newton_adj(U1, U2, U, ldm<true>(ulb + (size_t)dir * 18 * S, S, s), U1b, U2b);
+        const M3 Db = ldm<true>(db + (size_t)dir * 18 * S, S, s);
         U1b = U1b + Db;
         U2b = U2b - Db;
         // L = V(s) V(n)^+:  V(s) b += Lb V(n),  V(n) b += Lb^+ V(s)

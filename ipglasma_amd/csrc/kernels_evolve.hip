@@ -277,26 +277,26 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_a(const EvDesc* __restrict_
         [[maybe_unused]] const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
         // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
         // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
-        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
-        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
-        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
-        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
+        auto gX = [&]() { M3 u = ldm<true>(Ux, S, s); return m3(u, ldh<true>(phi, S, pX), dag(u)) - ldh<true>(phi, S, s); };
+        auto gY = [&]() { M3 u = ldm<true>(Uy, S, s); return m3(u, ldh<true>(phi, S, pY), dag(u)) - ldh<true>(phi, S, s); };
+        auto gXY1 = [&]() { M3 u = ldm<true>(Ux, S, pY); return m3(u, ldh<true>(phi, S, pXY), dag(u)) - ldh<true>(phi, S, pY); };
+        auto gXY2 = [&]() { M3 u = ldm<true>(Uy, S, pX); return m3(u, ldh<true>(phi, S, pXY), dag(u)) - ldh<true>(phi, S, pX); };
 
         // electric and longitudinal (Evolution.cpp:754-814)
         double tt, xx, yy, ee;
         {
-            const M3 e1 = ldh(E1, S, s), e1p = ldh(E1, S, pY), e2 = ldh(E2, S, s), e2p = ldh(E2, S, pX);
+            const M3 e1 = ldh<true>(E1, S, s), e1p = ldh<true>(E1, S, pY), e2 = ldh<true>(E2, S, s), e2p = ldh<true>(E2, S, pX);
             const double t11 = retrprod(e1, e1), t1p = retrprod(e1p, e1p), t22 = retrprod(e2, e2),
                          t2p = retrprod(e2p, e2p);
             double lng = 0.;
             {
-                M3 q = ldh(PI, S, s);
+                M3 q = ldh<true>(PI, S, s);
                 lng += retrprod(q, q);
-                q = ldh(PI, S, pX);
+                q = ldh<true>(PI, S, pX);
                 lng += retrprod(q, q);
-                q = ldh(PI, S, pY);
+                q = ldh<true>(PI, S, pY);
                 lng += retrprod(q, q);
-                q = ldh(PI, S, pXY);
+                q = ldh<true>(PI, S, pXY);
                 lng += retrprod(q, q);
                 lng /= 4.;
             }
@@ -311,8 +311,8 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_a(const EvDesc* __restrict_
         {
             double rplaq = 3.0;
             if (i != N - 1) {
-                const M3 ux = ldm(Ux, S, s);
-                rplaq = retrprod(ux, ldm(Uy, S, pX) * mul_bd(dag(ldm(Ux, S, pY)), ldm(Uy, S, s)));
+                const M3 ux = ldm<true>(Ux, S, s);
+                rplaq = retrprod(ux, ldm<true>(Uy, S, pX) * mul_bd(dag(ldm<true>(Ux, S, pY)), ldm<true>(Uy, S, s)));
             }
             double A, B, C, D;
             {
@@ -346,20 +346,20 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_a(const EvDesc* __restrict_
         IPG_FENCE();
         // T^{tau eta}
         {
-            double re = retrprod(ldh(E1, S, s), gX());
-            re += retrprod(ldh(E1, S, pY), gXY1());
-            re += retrprod(ldh(E2, S, s), gY());
-            re += retrprod(ldh(E2, S, pX), gXY2());
+            double re = retrprod(ldh<true>(E1, S, s), gX());
+            re += retrprod(ldh<true>(E1, S, pY), gXY1());
+            re += retrprod(ldh<true>(E2, S, s), gY());
+            re += retrprod(ldh<true>(E2, S, pX), gXY2());
             T[6 * S + s] = (g / tau / tau / tau * re) / a5;
         }
         IPG_FENCE();
         // T^{xy}
         {
-            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
+            const M3 ux = ldm<true>(Ux, S, s), uy = ldm<true>(Uy, S, s);
             double b;
             {
-                const M3 el = (-1. / 4. * g * g * (ldh(E1, S, s) + m3(uy, ldh(E1, S, pY), dag(uy)))) *
-                              (ldh(E2, S, s) + m3(ux, ldh(E2, S, pX), dag(ux)));
+                const M3 el = (-1. / 4. * g * g * (ldh<true>(E1, S, s) + m3(uy, ldh<true>(E1, S, pY), dag(uy)))) *
+                              (ldh<true>(E2, S, s) + m3(ux, ldh<true>(E2, S, pX), dag(ux)));
                 b = 4. * tr(el).r;
             }
             const M3 yl = m3(uy, gXY1(), dag(uy));  // U_y gXY1 U_y^+
@@ -402,10 +402,10 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_b(const EvDesc* __restrict_
         [[maybe_unused]] const double a2 = a * a, a4 = a2 * a2, a5 = a4 * a, a6 = a4 * a2;
         // phi gradients: gX = Ux phi(x+x) Ux^+ - phi, gY, gXY1 = Ux(+y) phi(+x+y) Ux(+y)^+ - phi(+y),
         // gXY2 = Uy(+x) phi(+x+y) Uy(+x)^+ - phi(+x)
-        auto gX = [&]() { M3 u = ldm(Ux, S, s); return m3(u, ldh(phi, S, pX), dag(u)) - ldh(phi, S, s); };
-        auto gY = [&]() { M3 u = ldm(Uy, S, s); return m3(u, ldh(phi, S, pY), dag(u)) - ldh(phi, S, s); };
-        auto gXY1 = [&]() { M3 u = ldm(Ux, S, pY); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pY); };
-        auto gXY2 = [&]() { M3 u = ldm(Uy, S, pX); return m3(u, ldh(phi, S, pXY), dag(u)) - ldh(phi, S, pX); };
+        auto gX = [&]() { M3 u = ldm<true>(Ux, S, s); return m3(u, ldh<true>(phi, S, pX), dag(u)) - ldh<true>(phi, S, s); };
+        auto gY = [&]() { M3 u = ldm<true>(Uy, S, s); return m3(u, ldh<true>(phi, S, pY), dag(u)) - ldh<true>(phi, S, s); };
+        auto gXY1 = [&]() { M3 u = ldm<true>(Ux, S, pY); return m3(u, ldh<true>(phi, S, pXY), dag(u)) - ldh<true>(phi, S, pY); };
+        auto gXY2 = [&]() { M3 u = ldm<true>(Uy, S, pX); return m3(u, ldh<true>(phi, S, pXY), dag(u)) - ldh<true>(phi, S, pX); };
 
         // T^{tau x}, T^{x eta}, T^{tau y}, T^{y eta} (Evolution.cpp:1047-1234).
         // The clover combinations enter with opposite signs, e.g. the T^{x eta}
@@ -414,53 +414,53 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_b(const EvDesc* __restrict_
         {
             // X1 = (Pa - Pb)_TL + (Pc - Pd)_TL,
             // Pa = Uy Ux(+y) Uy(+x)^+ Ux^+, Pb = Ux Uy(+x) Ux(+y)^+ Uy^+
-            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
-            M3 X = traceless(m4(uy, ldm(Ux, S, pY), dag(ldm(Uy, S, pX)), dag(ux)) -
-                             m4(ux, ldm(Uy, S, pX), dag(ldm(Ux, S, pY)), dag(uy)));
+            const M3 ux = ldm<true>(Ux, S, s), uy = ldm<true>(Uy, S, s);
+            M3 X = traceless(m4(uy, ldm<true>(Ux, S, pY), dag(ldm<true>(Uy, S, pX)), dag(ux)) -
+                             m4(ux, ldm<true>(Uy, S, pX), dag(ldm<true>(Ux, S, pY)), dag(uy)));
             {
-                const M3 UxmX = ldm(Ux, S, mX), UxmXpY = ldm(Ux, S, mXpY), UymX = ldm(Uy, S, mX);
+                const M3 UxmX = ldm<true>(Ux, S, mX), UxmXpY = ldm<true>(Ux, S, mXpY), UymX = ldm<true>(Uy, S, mX);
                 X = X + traceless(m4(dag(UxmX), UymX, UxmXpY, dag(uy)) - m4(uy, dag(UxmXpY), dag(UymX), UxmX));
             }
-            ttx_im = imtrprod(ldh(E2, S, s), X);
+            ttx_im = imtrprod(ldh<true>(E2, S, s), X);
             xe_im = -imtrprod(X, gY());
         }
         IPG_FENCE();
         {
             // Y1 = (Pb - Pa)_TL + (Qc - Qd)_TL
-            const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s);
-            M3 Y = traceless(m4(ux, ldm(Uy, S, pX), dag(ldm(Ux, S, pY)), dag(uy)) -
-                             m4(uy, ldm(Ux, S, pY), dag(ldm(Uy, S, pX)), dag(ux)));
+            const M3 ux = ldm<true>(Ux, S, s), uy = ldm<true>(Uy, S, s);
+            M3 Y = traceless(m4(ux, ldm<true>(Uy, S, pX), dag(ldm<true>(Ux, S, pY)), dag(uy)) -
+                             m4(uy, ldm<true>(Ux, S, pY), dag(ldm<true>(Uy, S, pX)), dag(ux)));
             {
-                const M3 UymY = ldm(Uy, S, mY), UxmY = ldm(Ux, S, mY), UypXmY = ldm(Uy, S, pXmY);
+                const M3 UymY = ldm<true>(Uy, S, mY), UxmY = ldm<true>(Ux, S, mY), UypXmY = ldm<true>(Uy, S, pXmY);
                 Y = Y + traceless(m4(dag(UymY), UxmY, UypXmY, dag(ux)) - m4(ux, dag(UypXmY), dag(UxmY), UymY));
             }
-            tty_im = imtrprod(ldh(E1, S, s), Y);
+            tty_im = imtrprod(ldh<true>(E1, S, s), Y);
             ye_im = -imtrprod(Y, gX());
         }
         IPG_FENCE();
         {
             // X2 = (Pe - Pf)_TL + (Pg - Ph)_TL
-            const M3 UxpX = ldm(Ux, S, pX), UxpXpY = ldm(Ux, S, pXY), Uyp2X = ldm(Uy, S, p2X);
-            const M3 UypX = ldm(Uy, S, pX);
+            const M3 UxpX = ldm<true>(Ux, S, pX), UxpXpY = ldm<true>(Ux, S, pXY), Uyp2X = ldm<true>(Uy, S, p2X);
+            const M3 UypX = ldm<true>(Uy, S, pX);
             M3 X = traceless(m4(UypX, UxpXpY, dag(Uyp2X), dag(UxpX)) - m4(UxpX, Uyp2X, dag(UxpXpY), dag(UypX)));
             {
-                const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s), UxpY = ldm(Ux, S, pY);
+                const M3 ux = ldm<true>(Ux, S, s), uy = ldm<true>(Uy, S, s), UxpY = ldm<true>(Ux, S, pY);
                 X = X + traceless(m4(dag(ux), uy, UxpY, dag(UypX)) - m4(UypX, dag(UxpY), dag(uy), ux));
             }
-            ttx_im += imtrprod(ldh(E2, S, pX), X);
+            ttx_im += imtrprod(ldh<true>(E2, S, pX), X);
             xe_im -= imtrprod(X, gXY2());
         }
         IPG_FENCE();
         {
             // Y2 = (Qe - Qf)_TL + (Qg - Qh)_TL
-            const M3 UxpY = ldm(Ux, S, pY), UypXpY = ldm(Uy, S, pXY), Uxp2Y = ldm(Ux, S, p2Y);
-            const M3 UypY = ldm(Uy, S, pY);
+            const M3 UxpY = ldm<true>(Ux, S, pY), UypXpY = ldm<true>(Uy, S, pXY), Uxp2Y = ldm<true>(Ux, S, p2Y);
+            const M3 UypY = ldm<true>(Uy, S, pY);
             M3 Y = traceless(m4(UxpY, UypXpY, dag(Uxp2Y), dag(UypY)) - m4(UypY, Uxp2Y, dag(UypXpY), dag(UxpY)));
             {
-                const M3 ux = ldm(Ux, S, s), uy = ldm(Uy, S, s), UypX = ldm(Uy, S, pX);
+                const M3 ux = ldm<true>(Ux, S, s), uy = ldm<true>(Uy, S, s), UypX = ldm<true>(Uy, S, pX);
                 Y = Y + traceless(m4(dag(uy), ux, UypX, dag(UxpY)) - m4(UxpY, dag(UypX), dag(ux), uy));
             }
-            tty_im += imtrprod(ldh(E1, S, pY), Y);
+            tty_im += imtrprod(ldh<true>(E1, S, pY), Y);
             ye_im -= imtrprod(Y, gXY1());
         }
         // raw clover sums, finished by k_tmunu_c
@@ -502,49 +502,49 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tmunu_c(const EvDesc* __restrict_
         const double ttx_im = T[4 * S + s], tty_im = T[5 * S + s], xe_im = T[8 * S + s], ye_im = T[9 * S + s];
         IPG_FENCE();
         {
-            M3 u = ldm(Ux, S, s), v = ldm(Ux, S, mX);
-            ttx_re = retrprod(ldh(PI, S, s), m3(u, ldh(phi, S, pX), dag(u)) - m3(dag(v), ldh(phi, S, mX), v));
-            u = ldm(Ux, S, pY);
-            v = ldm(Ux, S, mXpY);
-            ttx_re += retrprod(ldh(PI, S, pY), m3(u, ldh(phi, S, pXY), dag(u)) - m3(dag(v), ldh(phi, S, mXpY), v));
-            u = ldm(Ux, S, pX);
-            v = ldm(Ux, S, s);
-            ttx_re += retrprod(ldh(PI, S, pX), m3(u, ldh(phi, S, p2X), dag(u)) - m3(dag(v), ldh(phi, S, s), v));
-            u = ldm(Ux, S, pXY);
-            v = ldm(Ux, S, pY);
-            ttx_re += retrprod(ldh(PI, S, pXY), m3(u, ldh(phi, S, p2XY), dag(u)) - m3(dag(v), ldh(phi, S, pY), v));
+            M3 u = ldm<true>(Ux, S, s), v = ldm<true>(Ux, S, mX);
+            ttx_re = retrprod(ldh<true>(PI, S, s), m3(u, ldh<true>(phi, S, pX), dag(u)) - m3(dag(v), ldh<true>(phi, S, mX), v));
+            u = ldm<true>(Ux, S, pY);
+            v = ldm<true>(Ux, S, mXpY);
+            ttx_re += retrprod(ldh<true>(PI, S, pY), m3(u, ldh<true>(phi, S, pXY), dag(u)) - m3(dag(v), ldh<true>(phi, S, mXpY), v));
+            u = ldm<true>(Ux, S, pX);
+            v = ldm<true>(Ux, S, s);
+            ttx_re += retrprod(ldh<true>(PI, S, pX), m3(u, ldh<true>(phi, S, p2X), dag(u)) - m3(dag(v), ldh<true>(phi, S, s), v));
+            u = ldm<true>(Ux, S, pXY);
+            v = ldm<true>(Ux, S, pY);
+            ttx_re += retrprod(ldh<true>(PI, S, pXY), m3(u, ldh<true>(phi, S, p2XY), dag(u)) - m3(dag(v), ldh<true>(phi, S, pY), v));
         }
         IPG_FENCE();
         {
-            M3 u = ldm(Uy, S, s), v = ldm(Uy, S, mY);
-            tty_re = retrprod(ldh(PI, S, s), m3(u, ldh(phi, S, pY), dag(u)) - m3(dag(v), ldh(phi, S, mY), v));
-            u = ldm(Uy, S, pX);
-            v = ldm(Uy, S, pXmY);
-            tty_re += retrprod(ldh(PI, S, pX), m3(u, ldh(phi, S, pXY), dag(u)) - m3(dag(v), ldh(phi, S, pXmY), v));
-            u = ldm(Uy, S, pY);
-            v = ldm(Uy, S, s);
-            tty_re += retrprod(ldh(PI, S, pY), m3(u, ldh(phi, S, p2Y), dag(u)) - m3(dag(v), ldh(phi, S, s), v));
-            u = ldm(Uy, S, pXY);
-            v = ldm(Uy, S, pX);
-            tty_re += retrprod(ldh(PI, S, pXY), m3(u, ldh(phi, S, pX2Y), dag(u)) - m3(dag(v), ldh(phi, S, pX), v));
+            M3 u = ldm<true>(Uy, S, s), v = ldm<true>(Uy, S, mY);
+            tty_re = retrprod(ldh<true>(PI, S, s), m3(u, ldh<true>(phi, S, pY), dag(u)) - m3(dag(v), ldh<true>(phi, S, mY), v));
+            u = ldm<true>(Uy, S, pX);
+            v = ldm<true>(Uy, S, pXmY);
+            tty_re += retrprod(ldh<true>(PI, S, pX), m3(u, ldh<true>(phi, S, pXY), dag(u)) - m3(dag(v), ldh<true>(phi, S, pXmY), v));
+            u = ldm<true>(Uy, S, pY);
+            v = ldm<true>(Uy, S, s);
+            tty_re += retrprod(ldh<true>(PI, S, pY), m3(u, ldh<true>(phi, S, p2Y), dag(u)) - m3(dag(v), ldh<true>(phi, S, s), v));
+            u = ldm<true>(Uy, S, pXY);
+            v = ldm<true>(Uy, S, pX);
+            tty_re += retrprod(ldh<true>(PI, S, pXY), m3(u, ldh<true>(phi, S, pX2Y), dag(u)) - m3(dag(v), ldh<true>(phi, S, pX), v));
         }
         T[4 * S + s] = (2. / tau / 8. * ttx_im - 2. / 8. / tau * ttx_re) / a4;
         T[5 * S + s] = (2. / tau / 8. * tty_im - 2. / 8. / tau * tty_re) / a4;
         IPG_FENCE();
         // E pi parts of T^{x eta}, T^{y eta}
         {
-            M3 u = ldm(Ux, S, s);
-            double r1 = retrprod(ldh(E1, S, s), ldh(PI, S, s) + m3(u, ldh(PI, S, pX), dag(u)));
-            u = ldm(Ux, S, pY);
-            r1 += retrprod(ldh(E1, S, pY), ldh(PI, S, pY) + m3(u, ldh(PI, S, pXY), dag(u)));
+            M3 u = ldm<true>(Ux, S, s);
+            double r1 = retrprod(ldh<true>(E1, S, s), ldh<true>(PI, S, s) + m3(u, ldh<true>(PI, S, pX), dag(u)));
+            u = ldm<true>(Ux, S, pY);
+            r1 += retrprod(ldh<true>(E1, S, pY), ldh<true>(PI, S, pY) + m3(u, ldh<true>(PI, S, pXY), dag(u)));
             T[8 * S + s] = (-2. / tau / tau * (1. / 4. * g * r1 + 1. / 8. / g * xe_im)) / a5;
         }
         IPG_FENCE();
         {
-            M3 u = ldm(Uy, S, s);
-            double r1 = retrprod(ldh(E2, S, s), ldh(PI, S, s) + m3(u, ldh(PI, S, pY), dag(u)));
-            u = ldm(Uy, S, pX);
-            r1 += retrprod(ldh(E2, S, pX), ldh(PI, S, pX) + m3(u, ldh(PI, S, pXY), dag(u)));
+            M3 u = ldm<true>(Uy, S, s);
+            double r1 = retrprod(ldh<true>(E2, S, s), ldh<true>(PI, S, s) + m3(u, ldh<true>(PI, S, pY), dag(u)));
+            u = ldm<true>(Uy, S, pX);
+            r1 += retrprod(ldh<true>(E2, S, pX), ldh<true>(PI, S, pX) + m3(u, ldh<true>(PI, S, pXY), dag(u)));
             T[9 * S + s] = (-2. / tau / tau * (1. / 4. * g * r1 + 1. / 8. / g * ye_im)) / a5;
         }
     }
@@ -603,7 +603,9 @@ void ipg_launch_step0(const EvDesc* d_evs, int nev, int nblk, hipStream_t st) {
     hipLaunchKernelGGL(k_step0, dim3(nblk), dim3(IPG_BLOCK), 0, st, d_evs, nev);
 }
 // variant 0: k_step<1>, 1: k_step<2>, 2: k_step<1> on 4 x 64 tiles along y,
-// 3: k_step<1> on 4 x 64 tiles down column strips
+// 3: k_step<1> on 4 x 64 tiles down column strips (k_step<2> with opaque
+// lane offsets spills 284 B per lane instead of 508, and still takes 158 us
+// against 83 at N = 512)
 void ipg_launch_step(const EvDesc* d_evs, int nev, int nblk, int it, hipStream_t st, int variant) {
     ipg_launch_step_event(d_evs, nev, 0, nblk, it, st, variant);
 }
@@ -615,6 +617,7 @@ void ipg_launch_step_event(const EvDesc* d_evs, int nev, int block0, int nblocks
         hipLaunchKernelGGL((k_step<1, 1>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
     else if (variant == 3)
         hipLaunchKernelGGL((k_step<1, 2>), dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
+
     else
         hipLaunchKernelGGL(k_step<1>, dim3(nblocks), dim3(IPG_BLOCK), 0, st, d_evs, nev, it, block0);
 }

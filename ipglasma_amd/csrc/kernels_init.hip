@@ -560,8 +560,8 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
 #pragma unroll
     for (int t = 0; t < SPT; t++) {
         const int j = threadIdx.x + t * BLK;
-        V[t] = j < N ? ldm(Vg, S, (size_t)i * N + j) : m_one();
-        Vb[t] = j < N ? ldm(Bg, S, (size_t)i * N + j) : m_zero();
+        V[t] = j < N ? ldm<true>(Vg, S, (size_t)i * N + j) : m_one();
+        Vb[t] = j < N ? ldm<true>(Bg, S, (size_t)i * N + j) : m_zero();
     }
     const int kend = min(kc, ev.Ny - k0);
     for (int kk = kend - 1; kk >= 0; kk--) {
@@ -604,8 +604,8 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
     for (int t = 0; t < SPT; t++) {
         const int j = threadIdx.x + t * BLK;
         if (j < N) {
-            stm(Vg, S, (size_t)i * N + j, V[t]);
-            stm(Bg, S, (size_t)i * N + j, Vb[t]);
+            stm<true>(Vg, S, (size_t)i * N + j, V[t]);
+            stm<true>(Bg, S, (size_t)i * N + j, Vb[t]);
         }
     }
 }

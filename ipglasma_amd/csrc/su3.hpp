@@ -160,7 +160,18 @@ IPG_HD double square(const M3& a) {
 // ---- SoA field access ------------------------------------------------------
 // Field pointers are cast to the global address space so that each plane is
 // addressed as (uniform plane base in SGPRs) + (32-bit lane offset): one
-// offset register per site instead of a 64-bit address per plane.
+// offset register per site instead of a 64-bit address per plane.  With
+// Opaque the offset passes through an empty asm statement: left visible, the
+// compiler may widen it and fold it into 64-bit per-plane addresses in
+// VGPRs, which in the reverse-mode kernels costs 50-100 registers per lane
+// or spills (the forward kernels are faster with it visible: neighbour
+// offsets fold into the instructions' immediate offsets).
+template <bool Opaque>
+__device__ __forceinline__ unsigned lane_offset(size_t s) {
+    unsigned ob = (unsigned)s * 8u;
+    if (Opaque) asm volatile("" : "+v"(ob));
+    return ob;
+}
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) char gchar;
 __device__ __forceinline__ const gdouble* gp(const double* p) { return (const gdouble*)p; }
@@ -173,8 +184,9 @@ __device__ __forceinline__ void gst(double* f, size_t plane, unsigned ob, double
     *(gdouble*)((gchar*)gp(f + plane) + ob) = v;
 }
 
+template <bool Opaque = false>
 __device__ __forceinline__ M3 ldm(const double* f, size_t S, size_t s) {
-    const unsigned ob = (unsigned)s * 8u;
+    const unsigned ob = lane_offset<Opaque>(s);
     M3 m;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -183,8 +195,9 @@ __device__ __forceinline__ M3 ldm(const double* f, size_t S, size_t s) {
     }
     return m;
 }
+template <bool Opaque = false>
 __device__ __forceinline__ void stm(double* f, size_t S, size_t s, const M3& m) {
-    const unsigned ob = (unsigned)s * 8u;
+    const unsigned ob = lane_offset<Opaque>(s);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         gst(f, (2 * k) * S, ob, m.e[k].r);
@@ -198,8 +211,9 @@ __device__ __forceinline__ void stm(double* f, size_t S, size_t s, const M3& m) 
 // commutators of Hermitian matrices; pi = -(2i/g) A^eta with A^eta
 // anti-Hermitian, Init.cpp:2669-2673); storing the upper triangle drops only
 // the round-off-level anti-Hermitian part.  pi and phi keep their trace.
+template <bool Opaque = false>
 __device__ __forceinline__ M3 ldh(const double* f, size_t S, size_t s0) {
-    const unsigned s = (unsigned)s0 * 8u;
+    const unsigned s = lane_offset<Opaque>(s0);
     M3 m;
     const double r01 = gld(f, 3 * S, s), i01 = gld(f, 4 * S, s);
     const double r02 = gld(f, 5 * S, s), i02 = gld(f, 6 * S, s);
@@ -215,8 +229,9 @@ __device__ __forceinline__ M3 ldh(const double* f, size_t S, size_t s0) {
     m.e[7] = cd{r12, -i12};
     return m;
 }
+template <bool Opaque = false>
 __device__ __forceinline__ void sth(double* f, size_t S, size_t s0, const M3& m) {
-    const unsigned s = (unsigned)s0 * 8u;
+    const unsigned s = lane_offset<Opaque>(s0);
     gst(f, 0, s, m.e[0].r);
     gst(f, S, s, m.e[4].r);
     gst(f, 2 * S, s, m.e[8].r);

@@ -276,11 +276,11 @@ def test_batch_matches_plain():
         ev.run()
         plain.append(ev.observables())
         ev.close()
-    # six runs: while it measures them, the tuner takes the three evolution
+    # nine runs: while it measures them, the tuner takes the three evolution
     # schedules (all events per launch; groups of events taking their runs of
-    # steps one group after the other, two group sizes) in turn, so every
-    # pairing of schedule and graph replay is run once
-    for graph in (False, True, True, True, False, False):
+    # steps one group after the other, two group sizes) for three runs each,
+    # so every schedule runs with and without the graph replay
+    for graph in (False, True, False, True, False, True, True, False, True):
         b = I.Batch(ps, seeds)
         b.run(use_graph=graph)
         for e in range(len(ps)):
@@ -321,7 +321,7 @@ def test_batch_setv_schedules_match_plain():
 def test_step_builds_agree():
     """N a multiple of 64: while the tuner measures the four k_step builds
     (one or two waves per SIMD; 4 x 64 site tiles along y or down column
-    strips) a one-event batch runs them in turn, step by step; the plain
+    strips) a one-event batch runs them in turn, three steps each; the plain
     event before it runs the first build throughout.  Same arithmetic per
     site, so the observables agree to round-off of the reductions."""
     import ipglasma_amd as I

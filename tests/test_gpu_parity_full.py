@@ -189,6 +189,29 @@ def test_deformed_au_nucleons_and_event():
     ev.close()
 
 
+@pytest.mark.parametrize("force_dmin,gamma", [(1, 0.0), (0, 0.5), (1, 0.5)])
+def test_forced_dmin_and_triaxial_nucleons(force_dmin, gamma):
+    """the forced-d_min and triaxial Woods-Saxon samplers (Init.cpp:2906-2982,
+    2984-3037) on Pb with beta2 = 0.25, beta4 = 0.05: nucleons bit-exact
+    against the oracle, the collision counts equal, and one event through
+    the chain"""
+    import ipglasma_amd as I
+    for seed in (3, 8):
+        p = K.params("PbPb", N=32, Ny=3, seed=seed, beta2=0.25, beta4=0.05, gamma=gamma,
+                     force_dmin_flag=force_dmin)
+        ev = I.Event(p, seed=seed)
+        ev.init_nucleons()
+        A, B = ev.nucleons()
+        st, nA, nB, info = K.oracle_event(p, seed)
+        np.testing.assert_array_equal(A, K.nucleon_array(nA))
+        np.testing.assert_array_equal(B, K.nucleon_array(nB))
+        assert ev.observables()["Ncoll"] == info.Ncoll and ev.observables()["Npart"] == info.Npart
+        ev.close()
+    p = K.params("PbPb", N=64, Ny=3, seed=9, beta2=0.25, beta4=0.05, gamma=gamma, force_dmin_flag=force_dmin)
+    ev, _, _, _ = run_chain(p, 9, mt_normals=False)
+    ev.close()
+
+
 def test_c3_heterogeneous_batch_matches_plain_and_oracle():
     import ipglasma_amd as I
     ps = K.c3_params()

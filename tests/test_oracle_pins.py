@@ -393,6 +393,51 @@ def test_nucleon_dmin_recentring_and_rotation():
             assert (d2 < 0.9 ** 2).sum() / 2 < 3
 
 
+def test_forced_dmin_keeps_every_pair_apart():
+    """force_dmin_flag 1 (Init.cpp:2906-2982): each nucleon is redrawn until
+    it is at least d_min from every earlier one, with no try limit, so no
+    pair is closer (the recentring shift and the rotation keep distances)"""
+    p = K.params("PbPb", N=16, beta2=0.2, beta4=0.05, force_dmin_flag=1, d_min=1.2)
+    for seed in range(4):
+        st, nA, nB, info = K.oracle_event(p, seed)
+        for arr in (K.nucleon_array(nA), K.nucleon_array(nB)):
+            xyz = arr[:, :3]
+            d2 = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(-1)
+            np.fill_diagonal(d2, 1e9)
+            assert d2.min() >= 1.2 ** 2 * (1 - 1e-12)
+
+
+def _shape_ratio(p, seeds):
+    """mean of (l2 - l1) / (l3 - l1) over the eigenvalues l1 <= l2 <= l3 of
+    each nucleus' second-moment tensor: rotation invariant, 0 for a prolate,
+    1 for an oblate spheroid"""
+    out = []
+    for seed in seeds:
+        st, nA, nB, info = K.oracle_event(p, seed)
+        for arr, sgn in ((K.nucleon_array(nA), -1), (K.nucleon_array(nB), 1)):
+            xyz = arr[:, :3].copy()
+            xyz[:, 0] -= sgn * info.b / 2 * np.cos(info.phiRP)
+            xyz[:, 1] -= sgn * info.b / 2 * np.sin(info.phiRP)
+            lam = np.linalg.eigvalsh(xyz.T @ xyz / len(xyz))
+            out.append((lam[1] - lam[0]) / (lam[2] - lam[0]))
+    return float(np.mean(out))
+
+
+def test_triaxial_sampler_shape():
+    """R(theta, phi) = R (1 + beta2 (cos(gamma) Y20 + sin(gamma) Y22))
+    (Init.cpp:2984-3037): with beta2 = 0.45 the nuclei are prolate at
+    gamma = 0 (two small second moments), oblate at gamma = pi/3 (two large
+    ones), in between at pi/6; Y20 and Y22 swapped would make the gamma = 0
+    nuclei triaxial"""
+    base = dict(beta2=0.45, d_min=0.0)
+    pro = _shape_ratio(K.params("PbPb", N=16, gamma=0.0, force_dmin_flag=1, **base), range(60))
+    tri = _shape_ratio(K.params("PbPb", N=16, gamma=np.pi / 6, **base), range(60))
+    obl = _shape_ratio(K.params("PbPb", N=16, gamma=np.pi / 3, **base), range(60))
+    # 0, 0.5 and 1 to first order in beta2 for a smooth surface; the
+    # finite-A fluctuations of 208 nucleons pull them to ~0.13, 0.39, 0.75
+    assert pro < 0.2 and obl > 0.68 and 0.3 < tri < 0.5, (pro, tri, obl)
+
+
 def test_wounding_brute_force_and_counts():
     """N_part/N_coll against a brute-force pair loop over the recorded draws."""
     p = K.params("OO", N=16, gaussianWounding=0)  # hard sphere: no draws

@@ -104,6 +104,9 @@ def adjoint_timing(I, seed=77, system="AuAu", N=256):
         p.set(Projectile="Au", Target="Au", R_WS=6.38, a_WS=0.535, SigmaNN=42.0, roots=200.0, bmin=5.0, bmax=5.0)
     def timed(adj):
         ev = I.Event(p, seed=seed)
+        if adj:
+            # the reverse mode takes the forward's filtered colour fields
+            ev.keep_slices(True)
         t0 = time.perf_counter()
         while not ev.init():
             pass

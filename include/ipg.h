@@ -247,6 +247,17 @@ int ipg_adjoint_loss(ipg_ctx *c, int loss, const double *weights, int wrt, doubl
  * once), else the K in between that fits.  K > 0 is used as given.
  * Gradients do not depend on it.  No counterpart in the reference. */
 int ipg_set_checkpoint_interval(ipg_ctx *c, int K);
+
+/* on != 0: the Wilson lines of the following ipg_init (or
+ * ipg_init_wilson_lines) keep the filtered colour fields of every slice
+ * (2 Ny x 4 complex planes of N^2, in a device block of their own), so that
+ * the next reverse mode to rho (ipg_adjoint_rho, ipg_adjoint_loss with
+ * IPG_WRT_RHO) takes them instead of recomputing the forward row and
+ * column transforms of setV; that reverse pass overwrites them.  Setting a
+ * field with ipg_set_field or ipg_read_wilson_lines drops them.  on = 0
+ * releases the block.  Gradients do not depend on it.  No counterpart in
+ * the reference. */
+int ipg_set_keep_slices(ipg_ctx *c, int on);
 /* the checkpoint interval the last reverse pass of the context used */
 int ipg_get_checkpoint_interval(ipg_ctx *c, int *K);
 

@@ -165,6 +165,7 @@ def lib():
             "ipg_adjoint_rho": (C.c_int, [vp, dp, dp]),
             "ipg_get_rho_grad": (C.c_int, [vp, dp]),
             "ipg_set_checkpoint_interval": (C.c_int, [vp, C.c_int]),
+            "ipg_set_keep_slices": (C.c_int, [vp, C.c_int]),
             "ipg_get_checkpoint_interval": (C.c_int, [vp, ip]),
             "ipg_batch_adjoint": (C.c_int, [vp, C.c_int, dp, C.c_int, dp]),
             "ipg_batch_get_rho_grad": (C.c_int, [vp, C.c_int, dp]),
@@ -386,6 +387,11 @@ class Event:
         free device memory, the full tape when it fits; 1: every state
         stored, nothing recomputed; K: every K-th state)"""
         _check(lib().ipg_set_checkpoint_interval(self._c, int(K)))
+
+    def keep_slices(self, on=True):
+        """keep the filtered colour fields of the next Wilson lines for the
+        next reverse mode to rho (which then skips recomputing them)"""
+        _check(lib().ipg_set_keep_slices(self._c, int(bool(on))))
 
     def checkpoint_interval(self):
         """the interval the last reverse pass used"""

@@ -372,6 +372,19 @@ struct Engine {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     }
     EvDesc* d_adesc = nullptr;   // one-event descriptor of the event being reversed
+    // Kept slices (single-event contexts, ipg_set_keep_slices): the setV
+    // chunks of the last Wilson lines write their filtered colour fields
+    // (after the column pass: kc slices x 2 nuclei x 4 packed planes per
+    // chunk, the transform buffer's chunk layout) into `kept`, chunk after
+    // chunk, instead of the event's transform buffer, so that the reverse
+    // mode to rho finds them there instead of recomputing the forward row
+    // and column passes.  The reverse pass overwrites them (kept_valid).
+    bool keep_slices = false;
+    double* kept = nullptr;
+    size_t kept_n = 0;
+    bool kept_valid = false;
+    EvDesc* d_kdesc = nullptr;   // one descriptor per kept chunk
+    size_t kdesc_n = 0;
 
     ~Engine() {
         // the per-event blocks go back to the cache, not through hipFree
@@ -398,6 +411,8 @@ struct Engine {
         if (d_out) (void)hipFree(d_out);
         if (d_obs) (void)hipFree(d_obs);
         if (adj_ws) block_release(adj_ws, sizeof(double) * adj_ws_n);
+        if (kept) block_release(kept, sizeof(double) * kept_n);
+        if (d_kdesc) (void)hipFree(d_kdesc);
         if (d_rdesc) (void)hipFree(d_rdesc);
         if (d_adesc) (void)hipFree(d_adesc);
         if (bgrad) block_release(bgrad, sizeof(double) * bgrad_n);
@@ -761,10 +776,47 @@ struct Engine {
         timed.clear();
     }
 
+    // nch descriptors equal to d but for the transform buffer, which points
+    // at kept chunk c, in d_kdesc
+    int kept_descriptors(const EvDesc& d, int nch) {
+        const size_t per = 16 * (size_t)d.kc * (size_t)d.S;  // doubles per chunk
+        std::vector<EvDesc> kd((size_t)nch, d);
+        for (int c = 0; c < nch; c++) kd[(size_t)c].fftbuf = (double2*)(kept + (size_t)c * per);
+        if (kdesc_n < (size_t)nch) {
+            if (d_kdesc) HIPCHK(hipFree(d_kdesc));
+            d_kdesc = nullptr;
+            kdesc_n = 0;
+            HIPCHK(hipMalloc(&d_kdesc, sizeof(EvDesc) * nch));
+            kdesc_n = (size_t)nch;
+        }
+        HIPCHK(hipMemcpy(d_kdesc, kd.data(), sizeof(EvDesc) * nch, hipMemcpyHostToDevice));
+        return IPG_OK;
+    }
+
     int wilson_lines() {
         int rc = prepare_normals();
         if (rc) return rc;
         const int nev = (int)ev.size();
+        kept_valid = false;
+        const bool keep = keep_slices && nev == 1 && desc[0].active;
+        if (keep) {
+            const EvDesc& d0 = desc[0];
+            const int nch = (d0.Ny + d0.kc - 1) / d0.kc;
+            const size_t need = (size_t)nch * 16 * (size_t)d0.kc * (size_t)d0.S;
+            if (need != kept_n) {
+                if (kept) {
+                    HIPCHK(hipStreamSynchronize(st));
+                    block_release(kept, sizeof(double) * kept_n);
+                }
+                kept = nullptr;
+                kept_n = 0;
+                // every word is written by the forward passes before it is read
+                HIPCHK(block_alloc((void**)&kept, sizeof(double) * need, false));
+                kept_n = need;
+            }
+            rc = kept_descriptors(d0, nch);
+            if (rc) return rc;
+        }
         ipg_launch_unit_v(d_desc, nev, nblk, st);
         // group events by lattice size; each group is contiguous in a copy
         std::map<int, std::vector<int>> groups;
@@ -819,7 +871,8 @@ struct Engine {
             hipEvent_t sa = nullptr, sb = nullptr;
             if (timed_begin(skey, sched, smeas, &sa, &sb)) (void)hipEventRecord(sa, st);
             if (sched == 0) {
-                for (int k0 = 0; k0 < maxNy; k0 += kc) chunk(d_desc + es.front(), (int)es.size(), k0, kc, minNy);
+                for (int k0 = 0; k0 < maxNy; k0 += kc)
+                    chunk(keep ? d_kdesc + k0 / kc : d_desc + es.front(), (int)es.size(), k0, kc, minNy);
             } else {
                 for (int e : es) {
                     const int kce = sched == 2 ? std::max(1, desc[e].kc / 2) : desc[e].kc;
@@ -829,6 +882,7 @@ struct Engine {
             if (sb) (void)hipEventRecord(sb, st);
         }
         HIPCHK(hipGetLastError());
+        kept_valid = keep;
         return IPG_OK;
     }
 
@@ -1263,7 +1317,16 @@ struct Engine {
             sv.fwd = std::max(0, g_tuner.chosen("setv_rows_fwd_block" + tail));
             sv.cols = std::max(0, g_tuner.chosen("setv_cols_block" + tail));
         }
-        for (int c = nch - 1; c >= 0; c--) ipg_adj_setv_chunk(d_adesc, dp, c * kc, kc, vab, vb, gr, sv, st);
+        // the forward's filtered fields when the last Wilson lines kept them
+        // (this event alone, same chunking), else recomputed chunk by chunk
+        const bool use_kept = kept_valid && ev.size() == 1 && e == 0 && kept_n == (size_t)nch * 16 * (size_t)kc * S;
+        if (use_kept) {
+            rc = kept_descriptors(d, nch);
+            if (rc) return rc;
+        }
+        for (int c = nch - 1; c >= 0; c--)
+            ipg_adj_setv_chunk(use_kept ? d_kdesc + c : d_adesc, dp, c * kc, kc, vab, vb, gr, sv, st, !use_kept);
+        kept_valid = false;  // the reverse pass wrote its cotangents over them
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         adj_phase_ms[3] += ms_since(t2);
@@ -1998,6 +2061,7 @@ int ipg_get_field(ipg_ctx* c, int which, double* out) {
 int ipg_set_field(ipg_ctx* c, int which, const double* in) {
     if (!c || !in) return fail(IPG_ERR_ARG, "null argument");
     Engine& g = c->eng;
+    g.kept_valid = false;  // the kept slices no longer belong to these fields
     int kind;
     double* f = field_ptr(g, which, &kind);
     if (!f) return fail(IPG_ERR_ARG, "unknown field id");
@@ -2036,6 +2100,19 @@ int ipg_adjoint_loss(ipg_ctx* c, int loss, const double* weights, int wrt, doubl
     if (wrt == IPG_WRT_RHO) return c->eng.adjoint_rho_loss(loss, weights, value, grad);
     if (wrt == IPG_WRT_TMUNU) return c->eng.adjoint_tmunu(loss, weights, value, grad);
     return fail(IPG_ERR_ARG, "wrt must be IPG_WRT_FIELDS, IPG_WRT_RHO or IPG_WRT_TMUNU");
+}
+int ipg_set_keep_slices(ipg_ctx* c, int on) {
+    if (!c) return fail(IPG_ERR_ARG, "null context");
+    Engine& g = c->eng;
+    g.keep_slices = on != 0;
+    if (!g.keep_slices && g.kept) {
+        HIPCHK(hipStreamSynchronize(g.st));
+        block_release(g.kept, sizeof(double) * g.kept_n);
+        g.kept = nullptr;
+        g.kept_n = 0;
+        g.kept_valid = false;
+    }
+    return IPG_OK;
 }
 int ipg_set_checkpoint_interval(ipg_ctx* c, int K) {
     if (!c || K < 0) return fail(IPG_ERR_ARG, "bad argument");

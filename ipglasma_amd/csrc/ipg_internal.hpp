@@ -120,9 +120,11 @@ void ipg_adj_links_newton(const EvDesc* d_evs, long long S, const double* s0, co
 // setV in reverse for slices [k0, k0+kc) of event 0: vv/vbar are the current
 // Wilson lines and their cotangents (2 nuclei x 18 planes), updated in place;
 // grad[2][Ny][S][8] receives d/d rho; v: launch shapes of the recomputed
-// forward row pass and of the column passes (as ipg_launch_setv_chunk)
+// forward row pass and of the column passes (as ipg_launch_setv_chunk);
+// recompute false: the chunk's transform buffer already holds the filtered
+// fields of the forward pass
 void ipg_adj_setv_chunk(const EvDesc* d_evs, const FftPlanDev& plan, int k0, int kc, double* vv, double* vbar,
-                        double* grad, const SetvVariant& v, hipStream_t st);
+                        double* grad, const SetvVariant& v, hipStream_t st, bool recompute = true);
 
 
 // Coulomb gauge fixing and gluon spectra (one event, evs[0]; par: the buffer

@@ -1161,27 +1161,30 @@ static void launch_cols(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, i
 }
 template <int N>
 static void adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
-                           double* grad, const SetvVariant& v, hipStream_t st) {
+                           double* grad, const SetvVariant& v, hipStream_t st, bool recompute) {
     constexpr int C = ipgfft::cols_per_block(N);
     const int nrow = 2 * kc * N;
     const int ncol = 2 * kc * 4 * (N / C);
-    // recompute the chunk's A_k (forward row and column passes)
-    if (v.fwd == 1 && N >= 256)
-        hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow), dim3(512), 0, st, d, pl, k0, kc);
-    else if (v.fwd == 2 && N >= 256)
-        hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow), dim3(256), 0, st, d, pl, k0, kc);
-    else
-        hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
-    launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);
+    // recompute the chunk's A_k (forward row and column passes) unless the
+    // buffer already holds them
+    if (recompute) {
+        if (v.fwd == 1 && N >= 256)
+            hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow), dim3(512), 0, st, d, pl, k0, kc);
+        else if (v.fwd == 2 && N >= 256)
+            hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow), dim3(256), 0, st, d, pl, k0, kc);
+        else
+            hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
+        launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);
+    }
     hipLaunchKernelGGL((k_adj_setv_rows<N, 256>), dim3(2 * N), dim3(256), 0, st, d, pl, k0, kc, vv, vbar);
     hipLaunchKernelGGL(k_adj_filter_rows_fwd<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
     launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);
     hipLaunchKernelGGL(k_adj_filter_rows_inv<N>, dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc, grad);
 }
 void ipg_adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc, double* vv, double* vbar,
-                        double* grad, const SetvVariant& v, hipStream_t st) {
+                        double* grad, const SetvVariant& v, hipStream_t st, bool recompute) {
 #define IPG_ADJ_CASE(n) \
-    case n: adj_setv_chunk<n>(d, pl, k0, kc, vv, vbar, grad, v, st); break;
+    case n: adj_setv_chunk<n>(d, pl, k0, kc, vv, vbar, grad, v, st, recompute); break;
     switch (pl.N) {
         IPG_ADJ_CASE(16)
         IPG_ADJ_CASE(32)

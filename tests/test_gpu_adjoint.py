@@ -325,3 +325,30 @@ def test_reverse_step_kernels_agree():
         assert L1 == pytest.approx(L0, rel=1e-14)
         for k in O.FIELDS:
             assert np.abs(g1[k] - g0[k]).max() <= 1e-12 * np.abs(g0[k]).max(), k
+
+
+def test_kept_slices_give_the_same_rho_gradient():
+    """ipg_set_keep_slices: the reverse setV takes the forward's filtered
+    colour fields (Ny = 11: two chunks of 8 slices, the second short)
+    instead of recomputing them; the gradient equals the recomputed one.
+    After a field is set (here pi, to its own values) the kept fields are
+    dropped and the reverse pass recomputes them."""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=32, Ny=11, maxtime=0.4, L=8.0)
+
+    def grad(keep, touch=False):
+        ev = I.Event(p, seed=21)
+        if keep:
+            ev.keep_slices(True)
+        assert ev.init()
+        if touch:
+            ev.set_field("pi", ev.field("pi"))
+        L, g = ev.adjoint(loss="dEdeta", wrt="rho")
+        ev.close()
+        return L, np.asarray(g)
+
+    L0, g0 = grad(False)
+    for kw in ({"keep": True}, {"keep": True, "touch": True}):
+        L1, g1 = grad(**kw)
+        assert L1 == pytest.approx(L0, rel=1e-14), kw
+        assert np.abs(g1 - g0).max() <= 1e-12 * np.abs(g0).max(), kw

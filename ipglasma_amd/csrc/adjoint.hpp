@@ -273,34 +273,46 @@ IPG_HD void tgen(int a, M3& t) {
         default: t.e[0] = cd{s3, 0.}; t.e[4] = cd{s3, 0.}; t.e[8] = cd{-2. * s3, 0.}; break;
     }
 }
-// solve A x = b (8x8, partial pivoting), A row-major, overwritten
-IPG_HD void solve8(double* A, double* b) {
+// solve A x = b (8x8, partial pivoting), A row-major, overwritten.  The
+// loops are unrolled and the pivot rows exchanged by selects, so that A and
+// b stay in registers (an exchange through a run-time row index puts them
+// in scratch memory on the GPU)
+IPG_HD void solve8(double (&A)[64], double (&b)[8]) {
+#pragma unroll
     for (int j = 0; j < 8; j++) {
         int p = j;
         double mx = fabs(A[8 * j + j]);
+#pragma unroll
         for (int r = j + 1; r < 8; r++)
             if (fabs(A[8 * r + j]) > mx) {
                 mx = fabs(A[8 * r + j]);
                 p = r;
             }
-        if (p != j) {
+#pragma unroll
+        for (int r = j + 1; r < 8; r++) {
+            const bool sw = r == p;
+#pragma unroll
             for (int c = 0; c < 8; c++) {
-                const double t = A[8 * j + c];
-                A[8 * j + c] = A[8 * p + c];
-                A[8 * p + c] = t;
+                const double x = A[8 * j + c], y = A[8 * r + c];
+                A[8 * j + c] = sw ? y : x;
+                A[8 * r + c] = sw ? x : y;
             }
-            const double t = b[j];
-            b[j] = b[p];
-            b[p] = t;
+            const double x = b[j], y = b[r];
+            b[j] = sw ? y : x;
+            b[r] = sw ? x : y;
         }
+#pragma unroll
         for (int r = j + 1; r < 8; r++) {
             const double f = A[8 * r + j] / A[8 * j + j];
+#pragma unroll
             for (int c = j; c < 8; c++) A[8 * r + c] -= f * A[8 * j + c];
             b[r] -= f * b[j];
         }
     }
+#pragma unroll
     for (int r = 7; r >= 0; r--) {
         double t = b[r];
+#pragma unroll
         for (int c = r + 1; c < 8; c++) t -= A[8 * r + c] * b[c];
         b[r] = t / A[8 * r + r];
     }
@@ -308,23 +320,26 @@ IPG_HD void solve8(double* A, double* b) {
 IPG_HD void newton_adj(const M3& U1, const M3& U2, const M3& U, const M3& Ub, M3& U1b, M3& U2b) {
     const M3 W = U1 + U2, U0 = U1 * U2;
     const M3 A = mul_bd(W, U), B = mul_bd(U, W);  // W U^+, U W^+
+    // unrolled: the generators are constants, their zeros fold away
     M3 t[8];
+#pragma unroll
     for (int a = 0; a < 8; a++) tgen(a, t[a]);
-    double J[64], rhs[8];
+    double JT[64], rhs[8];  // J^T: JT[8 b + a] = J_ab
+#pragma unroll
     for (int a = 0; a < 8; a++) {
         const M3 ta_A = t[a] * A;
-        for (int b = 0; b < 8; b++) J[8 * a + b] = retrprod(ta_A, t[b]) + retrprod(t[a] * t[b], B);
+#pragma unroll
+        for (int b = 0; b < 8; b++) JT[8 * b + a] = retrprod(ta_A, t[b]) + retrprod(t[a] * t[b], B);
     }
     // betab_b = Re tr(Ub^+ i t^b U); solve J^T lambda = -betab
-    double JT[64];
-    for (int a = 0; a < 8; a++)
-        for (int b = 0; b < 8; b++) JT[8 * a + b] = J[8 * b + a];
+#pragma unroll
     for (int b = 0; b < 8; b++) {
         const M3 d = mul_i(1., t[b] * U);
         rhs[b] = -retrprod(dag(Ub), d);
     }
     solve8(JT, rhs);
     M3 T = m_zero();
+#pragma unroll
     for (int a = 0; a < 8; a++) T = T + rhs[a] * t[a];
     const M3 iT = mul_i(1., T);
     const M3 Wb = (-2.) * (iT * (m_one() + U));

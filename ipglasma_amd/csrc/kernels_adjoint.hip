@@ -221,9 +221,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_loss(const EvDesc* __restrict
     q.phpy = ldh<true>(X.phi, S, pY);
     q.phpxy = ldh<true>(X.phi, S, pXY);
     q.plaq = i != N - 1;
-    TBar b;
-    M3* bb = reinterpret_cast<M3*>(&b);
-    for (int k = 0; k < 16; k++) bb[k] = m_zero();
+    TBar b{};  // zeros (named members: registers, not an indexed array)
     tt_adj(q, it * ev.dtau, ev.g, yb, b);
     double* cU1 = cout + kPU1 * S;
     double* cU2 = cout + kPU2 * S;
@@ -966,19 +964,23 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_adj_links_newton(const EvDesc* __
     {
         constexpr int dir = DIR;
         const size_t n = dir == 0 ? L.at(i + 1, j) : L.at(i, j + 1);
-        const M3 va = ldm<true>(VA, S, s), van = ldm<true>(VA, S, n), vbs = ldm<true>(VB, S, s), vbn = ldm<true>(VB, S, n);
-        const M3 U1 = mul_bd(va, van), U2 = mul_bd(vbs, vbn);
-        const M3 U = ldm<true>(dir == 0 ? X.Ux : X.Uy, S, s);
         M3 U1b = m_zero(), U2b = m_zero();
-        newton_adj(U1, U2, U, ldm<true>(ulb + (size_t)dir * 18 * S, S, s), U1b, U2b);
+        {
+            const M3 U1 = mul_bd(ldm<true>(VA, S, s), ldm<true>(VA, S, n));
+            const M3 U2 = mul_bd(ldm<true>(VB, S, s), ldm<true>(VB, S, n));
+            newton_adj(U1, U2, ldm<true>(dir == 0 ? X.Ux : X.Uy, S, s), ldm<true>(ulb + (size_t)dir * 18 * S, S, s),
+                       U1b, U2b);
+        }
         const M3 Db = ldm<true>(db + (size_t)dir * 18 * S, S, s);
         U1b = U1b + Db;
         U2b = U2b - Db;
-        // L = V(s) V(n)^+:  V(s) b += Lb V(n),  V(n) b += Lb^+ V(s)
-        atomic_add_m3(vb, S, s, U1b * van);
-        atomic_add_m3(vb, S, n, mul_ad(U1b, va));
-        atomic_add_m3(vb + 18 * S, S, s, U2b * vbn);
-        atomic_add_m3(vb + 18 * S, S, n, mul_ad(U2b, vbs));
+        // L = V(s) V(n)^+:  V(s) b += Lb V(n),  V(n) b += Lb^+ V(s); the
+        // Wilson lines are loaded again here (cache hits) rather than kept
+        // live across the solve
+        atomic_add_m3(vb, S, s, U1b * ldm<true>(VA, S, n));
+        atomic_add_m3(vb, S, n, mul_ad(U1b, ldm<true>(VA, S, s)));
+        atomic_add_m3(vb + 18 * S, S, s, U2b * ldm<true>(VB, S, n));
+        atomic_add_m3(vb + 18 * S, S, n, mul_ad(U2b, ldm<true>(VB, S, s)));
     }
 }
 

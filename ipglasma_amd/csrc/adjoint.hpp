@@ -544,6 +544,46 @@ IPG_HD M3 expq_value(const ExpQ& x) {
     E.e[8] = E.e[8] + x.f[0];
     return E;
 }
+// exp(i Q^a t^a) without its reverse (the setV slices): for c1 = tr(Q^2)/2
+// below 0.5 the Cayley-Hamilton form f0 + f1 Q + f2 Q^2 with the f_j summed
+// as the series of exp(i x) in the basis {1, Q, Q^2} (Q^3 = c0 + c1 Q),
+// stopped once the terms left are below 1.5e-19 (the bound of expq_fcoef):
+// no transcendental functions; above, exp_iQt's closed form.  Equal to the
+// matrix exponential to round-off (exp_iQt, which follows Matrix.cpp:310-426,
+// loses digits at nearly degenerate spectra and returns 1 at degenerate ones).
+IPG_HD M3 expq_forward(const double (&Q)[8]) {
+    double c1 = 0.;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
+    c1 *= 0.25;
+    if (!(c1 < 0.5)) return exp_iQt(Q);
+    const M3 Qm = gen_matrix(Q);
+    const M3 Q2 = Qm * Qm;
+    const double c0 = retrprod(Q2, Qm) / 3.;
+    double a = 1., b = 0., d = 0.;
+    double fr[3] = {0., 0., 0.}, fi[3] = {0., 0., 0.};
+    double coef = 1.;  // 1/n!
+    for (int n = 0; n < 26; n++) {
+        const int m = n & 3;
+        const double cr = m == 0 ? coef : (m == 2 ? -coef : 0.);
+        const double ci = m == 1 ? coef : (m == 3 ? -coef : 0.);
+        fr[0] += cr * a;
+        fi[0] += ci * a;
+        fr[1] += cr * b;
+        fi[1] += ci * b;
+        fr[2] += cr * d;
+        fi[2] += ci * d;
+        if (n >= 4 && coef * (fabs(a) + fabs(b) + fabs(d)) < 1e-19) break;
+        const double na = c0 * d, nb = a + c1 * d, nd = b;
+        a = na, b = nb, d = nd;
+        coef /= (double)(n + 1);
+    }
+    M3 E = cscale(cd{fr[1], fi[1]}, Qm) + cscale(cd{fr[2], fi[2]}, Q2);
+    E.e[0] = E.e[0] + cd{fr[0], fi[0]};
+    E.e[4] = E.e[4] + cd{fr[0], fi[0]};
+    E.e[8] = E.e[8] + cd{fr[0], fi[0]};
+    return E;
+}
 // Qb^a = Re tr(Eb^+ dE/dQ^a)
 IPG_HD void expq_back(const ExpQ& x, const M3& Eb, double (&Qb)[8]) {
     const M3 Ebd = dag(Eb);

@@ -484,7 +484,9 @@ __global__ __launch_bounds__(BLK) void k_setv_cols(const EvDesc* __restrict__ ev
 }
 
 // rows inverse for every slice of the chunk in order; each thread keeps the
-// Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V
+// Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V (the
+// exponential by its Cayley-Hamilton series where that converges fast,
+// expq_forward)
 template <int N, int kInvBlock>
 __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                             int kc) {
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
-                V[t] = exp_iQt(Q) * V[t];
+                V[t] = ipgad::expq_forward(Q) * V[t];
             }
         }
         __syncthreads();  // the next slice overwrites the LDS rows
@@ -540,8 +542,8 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
 // backwards with the current V and its cotangent Vb in registers:
 //   V_k = E_k^+ V,  Eb_k = Vb V_k^+,  Vb <- E_k^+ Vb,  Ab_k = -Qb(Q = -A_k, Eb_k).
 // E_k is rebuilt from the Cayley-Hamilton coefficients its reverse needs
-// anyway (equal to the forward's exp_iQt to round-off, so V_k is recovered
-// to round-off).
+// anyway (equal to the forward's expq_forward to round-off, so V_k is
+// recovered to round-off).
 // The chunk's A_k are recomputed by the forward row and column passes; Ab_k
 // replaces A_k in the transform buffer, row by row.
 template <int N, int BLK>

@@ -96,6 +96,12 @@ void adjh_expq_value(const double* q, double* e) {
     expq_prep(Q, x);
     st(e, expq_value(x));
 }
+// the forward slices' exponential (expq_forward)
+void adjh_expq_forward(const double* q, double* e) {
+    double Q[8];
+    for (int a = 0; a < 8; a++) Q[a] = q[a];
+    st(e, expq_forward(Q));
+}
 void adjh_expq(const double* q, const double* eb, double* e, double* qb) {
     double Q[8], Qb[8];
     for (int a = 0; a < 8; a++) Q[a] = q[a];

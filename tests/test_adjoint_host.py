@@ -36,6 +36,7 @@ def lib():
         L.adjh_acc.argtypes = [dp] * 4
         L.adjh_expq.argtypes = [dp] * 4
         L.adjh_expq_value.argtypes = [dp] * 2
+        L.adjh_expq_forward.argtypes = [dp] * 2
         _lib = L
     return _lib
 
@@ -253,3 +254,12 @@ def test_su3_exponential_adjoint_against_frechet():
         assert np.abs(ev - E.reshape(-1)).max() <= 2e-15 * max(1.0, scale), (n, scale)
         if n not in exact_degenerate + near_degenerate:
             assert np.abs(ev - (e[0::2] + 1j * e[1::2])).max() <= 1e-13 * max(1.0, scale), (n, scale)
+        # the forward setV exponential: the series below c1 = 0.5, exp_iQt
+        # above; the matrix exponential to round-off except where exp_iQt
+        # itself is off (degenerate spectra above c1 = 0.5)
+        ef = np.zeros(18)
+        L.adjh_expq_forward(Q, ef)
+        ef = ef[0::2] + 1j * ef[1::2]
+        c1 = 0.25 * float(np.sum(Q * Q))
+        if c1 < 0.5 or n not in exact_degenerate + near_degenerate:
+            assert np.abs(ef - E.reshape(-1)).max() <= 2e-13 * max(1.0, scale), (n, scale, c1)

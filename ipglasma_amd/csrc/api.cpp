@@ -394,6 +394,10 @@ struct Engine {
             if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
         }
+        for (int k = 0; k < 2; k++) {
+            if (gf_exec[k]) (void)hipGraphExecDestroy(gf_exec[k]);
+            if (gf_graph[k]) (void)hipGraphDestroy(gf_graph[k]);
+        }
         for (Event& e : ev) {
             if (e.mem) block_release(e.mem, e.mem_bytes);
             if (e.d_nuc) (void)hipFree(e.d_nuc);
@@ -1674,6 +1678,17 @@ struct Engine {
     // on the device (k_gf_decide), so the host launches batches of steps (8,
     // growing to 64) and reads the 4-double state between them; the steps a
     // batch holds after the stop return at once.
+    // kGfGraphSteps gauge-fixing steps (5 launches each) as one HIP graph,
+    // per field parity; replayed for the full groups of steps of a batch.
+    // The kernels take every pointer as an argument, so a graph is valid for
+    // as long as the event's buffers and the descriptor table stay put
+    // (gf_key).  Launched one by one, the 176 x 5 small kernels of a fix
+    // left ~15 us between launches.
+    static constexpr int kGfGraphSteps = 8;
+    hipGraph_t gf_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gf_exec[2] = {nullptr, nullptr};
+    const void* gf_key[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+
     int gauge_fix(int max_iter, int* iters, double* residual) {
         if (ev.size() != 1) return fail(IPG_ERR_ARG, "the gauge fixing runs on single-event contexts");
         EvDesc& d = desc[0];
@@ -1688,12 +1703,29 @@ struct Engine {
         double* gs = d.partial + ((N + 31) / 32) * 32;  // the stopping state
         int rc = upload_desc();
         if (rc) return rc;
+        const int pk = par & 1;
+        const void* key[3] = {d_desc, buf, pl.tw};
+        if (gf_exec[pk] && (gf_key[pk][0] != key[0] || gf_key[pk][1] != key[1] || gf_key[pk][2] != key[2])) {
+            (void)hipGraphExecDestroy(gf_exec[pk]);
+            (void)hipGraphDestroy(gf_graph[pk]);
+            gf_exec[pk] = nullptr;
+            gf_graph[pk] = nullptr;
+        }
+        if (!gf_exec[pk]) {
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int b = 0; b < kGfGraphSteps; b++) ipg_launch_gf_step(d_desc, par, N, dp, buf, G, part, gs, st);
+            HIPCHK(hipStreamEndCapture(st, &gf_graph[pk]));
+            HIPCHK(hipGraphInstantiate(&gf_exec[pk], gf_graph[pk], nullptr, nullptr, 0));
+            for (int k = 0; k < 3; k++) gf_key[pk][k] = key[k];
+        }
         ipg_launch_gf_reset(gs, st);
         double h[4] = {0., 0., 0., 0.};
         int launched = 0, batch = 8;
         while (launched < max_iter) {
             const int nb = std::min(batch, max_iter - launched);
-            for (int b = 0; b < nb; b++) ipg_launch_gf_step(d_desc, par, N, dp, buf, G, part, gs, st);
+            for (int b = 0; b + kGfGraphSteps <= nb; b += kGfGraphSteps) HIPCHK(hipGraphLaunch(gf_exec[pk], st));
+            for (int b = nb - nb % kGfGraphSteps; b < nb; b++)
+                ipg_launch_gf_step(d_desc, par, N, dp, buf, G, part, gs, st);
             HIPCHK(hipGetLastError());
             launched += nb;
             HIPCHK(hipMemcpyAsync(h, gs, sizeof(h), hipMemcpyDeviceToHost, st));

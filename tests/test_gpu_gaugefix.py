@@ -189,3 +189,31 @@ def test_full_size_pbpb_gauge_fix_invariants():
     assert d1 == pytest.approx(d0, rel=1e-8)
     plain.close()
     ev.close()
+
+
+def test_gauge_fix_graph_replayed_on_other_fields():
+    """the gauge-fixing steps replay from a HIP graph cached in the context:
+    fixing a second field configuration in the same context (eager launches,
+    copies and synchronisations in between) gives what a fresh context
+    gives, step count and fields bit for bit"""
+    p = K.params("AuAu", N=32, Ny=3)
+    src = at_fields(p, 6)
+    src.step(0)
+    src.step(1)
+    stB = {k: src.field(k).copy() for k in O.FIELDS}
+    it_fresh, res_fresh = src.gauge_fix(4000)
+    fixed_fresh = {k: src.field(k).copy() for k in O.FIELDS}
+    src.close()
+    ev = at_fields(p, 5)
+    ev.step(0)
+    ev.step(1)
+    it_a, _ = ev.gauge_fix(4000)
+    assert it_a > 8
+    for k, v in stB.items():
+        ev.set_field(k, v)
+    it_b, res_b = ev.gauge_fix(4000)
+    say(f"replayed graph: {it_b} steps (fresh {it_fresh}), residual {res_b:.6e} (fresh {res_fresh:.6e})")
+    assert it_b == it_fresh and res_b == res_fresh
+    for k in O.FIELDS:
+        assert np.array_equal(ev.field(k), fixed_fresh[k]), k
+    ev.close()

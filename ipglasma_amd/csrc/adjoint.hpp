@@ -548,15 +548,26 @@ IPG_HD M3 expq_value(const ExpQ& x) {
 // below 0.5 the Cayley-Hamilton form f0 + f1 Q + f2 Q^2 with the f_j summed
 // as the series of exp(i x) in the basis {1, Q, Q^2} (Q^3 = c0 + c1 Q),
 // stopped once the terms left are below 1.5e-19 (the bound of expq_fcoef):
-// no transcendental functions; above, exp_iQt's closed form.  Equal to the
-// matrix exponential to round-off (exp_iQt, which follows Matrix.cpp:310-426,
-// loses digits at nearly degenerate spectra and returns 1 at degenerate ones).
-IPG_HD M3 expq_forward(const double (&Q)[8]) {
+// no transcendental functions.  Above, exp_iQt's closed form
+// (Matrix.cpp:310-426).  The two agree to round-off except at (nearly)
+// degenerate spectra, where the closed form loses digits or returns 1, and
+// at |Q| below ~1e-8, where it cancels catastrophically (its 9u^2 - w^2
+// denominator and a floor of 1e-15 on c0max) and its value, off the
+// exponential by up to |Q|, depends on the last bits of acos and sincos.
+constexpr double kExpqSeriesMax = 0.5;
+IPG_HD bool expq_series_range(const double (&Q)[8]) {
     double c1 = 0.;
 #pragma unroll
     for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
     c1 *= 0.25;
-    if (!(c1 < 0.5)) return exp_iQt(Q);
+    return c1 < kExpqSeriesMax;
+}
+IPG_HD M3 expq_forward(const double (&Q)[8]) {
+    if (!expq_series_range(Q)) return exp_iQt(Q);
+    double c1 = 0.;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
+    c1 *= 0.25;
     const M3 Qm = gen_matrix(Q);
     const M3 Q2 = Qm * Qm;
     const double c0 = retrprod(Q2, Qm) / 3.;
@@ -583,6 +594,11 @@ IPG_HD M3 expq_forward(const double (&Q)[8]) {
     E.e[4] = E.e[4] + cd{fr[0], fi[0]};
     E.e[8] = E.e[8] + cd{fr[0], fi[0]};
     return E;
+}
+// the forward slices' exponential (expq_forward) from the coefficients of
+// its reverse where they agree, exp_iQt elsewhere
+IPG_HD M3 expq_forward_from(const ExpQ& x, const double (&Q)[8]) {
+    return expq_series_range(Q) ? expq_value(x) : exp_iQt(Q);
 }
 // Qb^a = Re tr(Eb^+ dE/dQ^a)
 IPG_HD void expq_back(const ExpQ& x, const M3& Eb, double (&Qb)[8]) {

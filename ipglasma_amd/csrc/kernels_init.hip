@@ -542,8 +542,8 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
 // backwards with the current V and its cotangent Vb in registers:
 //   V_k = E_k^+ V,  Eb_k = Vb V_k^+,  Vb <- E_k^+ Vb,  Ab_k = -Qb(Q = -A_k, Eb_k).
 // E_k is rebuilt from the Cayley-Hamilton coefficients its reverse needs
-// anyway (equal to the forward's expq_forward to round-off, so V_k is
-// recovered to round-off).
+// anyway where the forward's expq_forward takes the series (the same sums,
+// so V_k is recovered to round-off), and by exp_iQt above.
 // The chunk's A_k are recomputed by the forward row and column passes; Ab_k
 // replaces A_k in the transform buffer, row by row.
 template <int N, int BLK>
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
                 // one set of Cayley-Hamilton coefficients for E and its reverse
                 ipgad::ExpQ x;
                 ipgad::expq_prep(Q, x);
-                const M3 E = ipgad::expq_value(x);
+                const M3 E = ipgad::expq_forward_from(x, Q);
                 const M3 Vk = mul_ad(E, V[t]);
                 const M3 Eb = mul_bd(Vb[t], Vk);
                 Vb[t] = mul_ad(E, Vb[t]);

@@ -10,6 +10,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import _oracle as O
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "host", "adjoint_host.cpp")
 OUT = os.path.join(HERE, "_build", "libadjoint_host.so")
@@ -263,3 +265,19 @@ def test_su3_exponential_adjoint_against_frechet():
         c1 = 0.25 * float(np.sum(Q * Q))
         if c1 < 0.5 or n not in exact_degenerate + near_degenerate:
             assert np.abs(ef - E.reshape(-1)).max() <= 2e-13 * max(1.0, scale), (n, scale, c1)
+    # tiny charges: the series keeps the exponential (1 + iQ - Q^2/2 to
+    # round-off) where the reference's closed form, in this host build as in
+    # the oracle, is off by up to |Q| (catastrophic cancellation)
+    for sc in (1e-8, 1e-9, 1e-11):
+        Q = rng.normal(size=8) * sc
+        H = 1j * np.einsum("a,aij->ij", Q, lam)
+        ef = np.zeros(18)
+        L.adjh_expq_forward(Q, ef)
+        ef = (ef[0::2] + 1j * ef[1::2]).reshape(3, 3)
+        assert np.abs(ef - (np.eye(3) + H + H @ H / 2)).max() <= 1e-17, sc
+        e = np.zeros(18)
+        qb = np.zeros(8)
+        L.adjh_expq(Q, cm(np.eye(3) + 0j), e, qb)
+        u = np.zeros(18)
+        O.lib().orc_u_from_exponent(np.ascontiguousarray(Q), u)
+        assert np.abs(e - u).max() <= 1e-20, sc  # the two closed forms agree with each other

@@ -544,33 +544,22 @@ IPG_HD M3 expq_value(const ExpQ& x) {
     E.e[8] = E.e[8] + x.f[0];
     return E;
 }
-// exp(i Q^a t^a) without its reverse (the setV slices): for c1 = tr(Q^2)/2
-// below 0.5 the Cayley-Hamilton form f0 + f1 Q + f2 Q^2 with the f_j summed
-// as the series of exp(i x) in the basis {1, Q, Q^2} (Q^3 = c0 + c1 Q),
-// stopped once the terms left are below 1.5e-19 (the bound of expq_fcoef):
-// no transcendental functions.  Above, exp_iQt's closed form
-// (Matrix.cpp:310-426).  The two agree to round-off except at (nearly)
-// degenerate spectra, where the closed form loses digits or returns 1, and
-// at |Q| below ~1e-8, where it cancels catastrophically (its 9u^2 - w^2
-// denominator and a floor of 1e-15 on c0max) and its value, off the
-// exponential by up to |Q|, depends on the last bits of acos and sincos.
+// The setV slices' exponential exp(i Q^a t^a) without its reverse: the
+// Cayley-Hamilton form f0 + f1 Q + f2 Q^2 with the f_j summed as the series
+// of exp(i x) in the basis {1, Q, Q^2} (Q^3 = c0 + c1 Q) for c1 = tr(Q^2)/2
+// below kExpqSeriesMax, stopped once the terms left are below 1.5e-19 (the
+// bound of expq_fcoef); larger Q scaled into that range and the factor
+// applied repeatedly (expq_apply).  No transcendental functions.  It agrees
+// with exp_iQt's closed form (Matrix.cpp:310-426) to round-off except at
+// (nearly) degenerate spectra, where the closed form loses digits or
+// returns 1, and at |Q| below ~1e-8, where the closed form cancels
+// catastrophically (its 9u^2 - w^2 denominator and a floor of 1e-15 on
+// c0max) and its value, off the exponential by up to |Q|, depends on the
+// last bits of acos and sincos.
 constexpr double kExpqSeriesMax = 0.5;
-IPG_HD bool expq_series_range(const double (&Q)[8]) {
-    double c1 = 0.;
-#pragma unroll
-    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
-    c1 *= 0.25;
-    return c1 < kExpqSeriesMax;
-}
-IPG_HD M3 expq_forward(const double (&Q)[8]) {
-    if (!expq_series_range(Q)) return exp_iQt(Q);
-    double c1 = 0.;
-#pragma unroll
-    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
-    c1 *= 0.25;
-    const M3 Qm = gen_matrix(Q);
-    const M3 Q2 = Qm * Qm;
-    const double c0 = retrprod(Q2, Qm) / 3.;
+// the f_j of the series: exp(i x) summed in the basis {1, Q, Q^2} with
+// Q^3 = c0 + c1 Q (the powers Q^n = a_n + b_n Q + d_n Q^2)
+IPG_HD void expq_series(double c0, double c1, cd (&f)[3]) {
     double a = 1., b = 0., d = 0.;
     double fr[3] = {0., 0., 0.}, fi[3] = {0., 0., 0.};
     double coef = 1.;  // 1/n!
@@ -589,16 +578,70 @@ IPG_HD M3 expq_forward(const double (&Q)[8]) {
         a = na, b = nb, d = nd;
         coef /= (double)(n + 1);
     }
-    M3 E = cscale(cd{fr[1], fi[1]}, Qm) + cscale(cd{fr[2], fi[2]}, Q2);
-    E.e[0] = E.e[0] + cd{fr[0], fi[0]};
-    E.e[4] = E.e[4] + cd{fr[0], fi[0]};
-    E.e[8] = E.e[8] + cd{fr[0], fi[0]};
-    return E;
+#pragma unroll
+    for (int j = 0; j < 3; j++) f[j] = cd{fr[j], fi[j]};
 }
-// the forward slices' exponential (expq_forward) from the coefficients of
-// its reverse where they agree, exp_iQt elsewhere
-IPG_HD M3 expq_forward_from(const ExpQ& x, const double (&Q)[8]) {
-    return expq_series_range(Q) ? expq_value(x) : exp_iQt(Q);
+// det(Q^a t^a) from the components (gen_matrix's entries: real diagonal
+// d_i, upper triangle q01, q02, q12):
+// d0 d1 d2 - d0 |q12|^2 - d1 |q02|^2 - d2 |q01|^2 + 2 Re(q01 q12 conj q02)
+IPG_HD double gen_det(const double (&Q)[8]) {
+    const double s3 = 1. / (2. * sqrt(3.));
+    const double d0 = 0.5 * Q[2] + s3 * Q[7], d1 = -0.5 * Q[2] + s3 * Q[7], d2 = -2. * s3 * Q[7];
+    const cd q01{0.5 * Q[0], -0.5 * Q[1]}, q02{0.5 * Q[3], -0.5 * Q[4]}, q12{0.5 * Q[5], -0.5 * Q[6]};
+    const double n01 = q01.r * q01.r + q01.i * q01.i, n02 = q02.r * q02.r + q02.i * q02.i,
+                 n12 = q12.r * q12.r + q12.i * q12.i;
+    return d0 * d1 * d2 - d0 * n12 - d1 * n02 - d2 * n01 + 2. * mulc(q01 * q12, q02).r;
+}
+// V <- (f0 + f1 Q + f2 Q^2) V column by column: v <- f0 v + f1 (Q v) +
+// f2 Q (Q v), Q = Q^a t^a from its components.  Neither Q^2 nor the
+// exponential is formed, so far fewer values are live than in E * V.
+IPG_HD void expq_coef_apply(const double (&Q)[8], const cd (&f)[3], M3& V) {
+    const double s3 = 1. / (2. * sqrt(3.));
+    const double d0 = 0.5 * Q[2] + s3 * Q[7], d1 = -0.5 * Q[2] + s3 * Q[7], d2 = -2. * s3 * Q[7];
+    const cd q01{0.5 * Q[0], -0.5 * Q[1]}, q02{0.5 * Q[3], -0.5 * Q[4]}, q12{0.5 * Q[5], -0.5 * Q[6]};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const cd v0 = V.e[j], v1 = V.e[3 + j], v2 = V.e[6 + j];
+        const cd w0 = v0 * d0 + q01 * v1 + q02 * v2;
+        const cd w1 = cmul(q01, v0) + v1 * d1 + q12 * v2;
+        const cd w2 = cmul(q02, v0) + cmul(q12, v1) + v2 * d2;
+        const cd x0 = w0 * d0 + q01 * w1 + q02 * w2;
+        const cd x1 = cmul(q01, w0) + w1 * d1 + q12 * w2;
+        const cd x2 = cmul(q02, w0) + cmul(q12, w1) + w2 * d2;
+        V.e[j] = f[0] * v0 + f[1] * w0 + f[2] * x0;
+        V.e[3 + j] = f[0] * v1 + f[1] * w1 + f[2] * x1;
+        V.e[6 + j] = f[0] * v2 + f[1] * w2 + f[2] * x2;
+    }
+}
+// V <- exp(i Q^a t^a) V for any Q (the setV slices): Q scaled by 2^-s
+// (exact) until c1 = tr(Q^2)/2 is below kExpqSeriesMax, the series
+// coefficients of the scaled Q (expq_series, c0 = det Q), applied 2^s times
+// by expq_coef_apply.  No transcendental functions; s is at most 12
+// (c1 < 8.4e6), so that no input can make the loop long.
+IPG_HD void expq_apply(const double (&Q)[8], M3& V) {
+    double c1 = 0.;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += Q[a] * Q[a];
+    c1 *= 0.25;
+    double sc = 1.;
+    int reps = 1;
+    while (c1 >= kExpqSeriesMax && reps < 4096) {
+        c1 *= 0.25;
+        sc *= 0.5;
+        reps *= 2;
+    }
+    double Qs[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) Qs[a] = Q[a] * sc;
+    cd f[3];
+    expq_series(gen_det(Qs), c1, f);
+    for (int r = 0; r < reps; r++) expq_coef_apply(Qs, f, V);
+}
+// the slice exponential itself (expq_apply to the identity)
+IPG_HD M3 expq_forward(const double (&Q)[8]) {
+    M3 E = m_one();
+    expq_apply(Q, E);
+    return E;
 }
 // Qb^a = Re tr(Eb^+ dE/dQ^a)
 IPG_HD void expq_back(const ExpQ& x, const M3& Eb, double (&Qb)[8]) {

@@ -850,12 +850,13 @@ struct Engine {
                 bool meas = false;
                 // only full chunks are compared (the last one may be short)
                 const bool full = k0 + kcc <= ny;
-                v.inv = g_tuner.pick("setv_rows_inv_block" + tail, 2, &meas);
+                v.inv = g_tuner.pick("setv_rows_inv_block" + tail, 6, &meas);
                 timed_begin("setv_rows_inv_block" + tail, v.inv, meas && full, &t.inv[0], &t.inv[1]);
                 if (kv.first >= 256) {
                     // 0: 256-thread blocks, 1: 512-thread blocks, 2: one
                     // wavefront per transformed row, synchronised alone
-                    v.fwd = g_tuner.pick("setv_rows_fwd_block" + tail, 3, &meas);
+                    // (forward rows 3: 2 with the twiddles in LDS)
+                    v.fwd = g_tuner.pick("setv_rows_fwd_block" + tail, 4, &meas);
                     timed_begin("setv_rows_fwd_block" + tail, v.fwd, meas && full, &t.fwd[0], &t.fwd[1]);
                     v.cols = g_tuner.pick("setv_cols_block" + tail, 3, &meas);
                     timed_begin("setv_cols_block" + tail, v.cols, meas && full, &t.cols[0], &t.cols[1]);

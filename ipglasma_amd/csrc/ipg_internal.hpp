@@ -63,8 +63,11 @@ struct FftPlanDev {
 };
 // launch shapes of the three setV passes (0: 256-thread blocks, 1: 512; for
 // the forward row and the column passes 2: one wavefront per transformed
-// row, synchronising itself only; the same arithmetic) and the event pairs
-// that bracket each when timed
+// row, synchronising itself only; for the inverse row pass 2 and 3: the
+// next slice prefetched into registers, in 512-thread blocks and in
+// 256-thread blocks of wavefront-synchronised transforms; inverse row 4, 5
+// and forward row 3: shapes 0, 1 and 2 with the twiddles read from LDS; the
+// same arithmetic) and the event pairs that bracket each when timed
 struct SetvVariant {
     int fwd = 0, cols = 0, inv = 0;
 };

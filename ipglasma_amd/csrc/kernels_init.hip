@@ -377,10 +377,13 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_stats_finish(const EvDesc* __rest
 // complex rows, forward transform along y
 // WAVE: 4 wavefronts (BLK = 256), each transforms one of the four packed
 // rows, synchronising itself only
-template <int N, int BLK = IPG_BLOCK, bool WAVE = false>
+// TWL: the twiddle table copied into LDS first (LDS reads in the stages
+// instead of global ones)
+template <int N, int BLK = IPG_BLOCK, bool WAVE = false, bool TWL = false>
 __global__ __launch_bounds__(BLK) void k_setv_rows_fwd(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                       int kc) {
     __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
+    __shared__ double2 tws[TWL ? N : 1];
     const int per_ev = 2 * kc * N;
     const int e = blockIdx.x / per_ev;
     const int rem = blockIdx.x - e * per_ev;
@@ -406,13 +409,18 @@ __global__ __launch_bounds__(BLK) void k_setv_rows_fwd(const EvDesc* __restrict_
 #pragma unroll
         for (int p = 0; p < 4; p++) lds[ipgfft::at(N, p, j)] = make_double2(amp * z[2 * p], amp * z[2 * p + 1]);
     }
+    const double2* tw = pl.tw;
+    if constexpr (TWL) {
+        for (int m = threadIdx.x; m < N; m += BLK) tws[m] = pl.tw[m];
+        tw = tws;
+    }
     __syncthreads();
     if constexpr (WAVE) {
         static_assert(BLK == 256, "one wavefront per packed row");
-        ipgfft::wave_fft<N, 1, false>(lds + ipgfft::at(N, (int)(threadIdx.x >> 6), 0), pl.tw);
+        ipgfft::wave_fft<N, 1, false>(lds + ipgfft::at(N, (int)(threadIdx.x >> 6), 0), tw);
         __syncthreads();
     } else {
-        ipgfft::lds_fft<N, 4, BLK, false>(lds, pl.tw);
+        ipgfft::lds_fft<N, 4, BLK, false>(lds, tw);
     }
     double2* out = ev.fftbuf + ((size_t)n * kc + kk) * 4 * S;
     for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
@@ -484,14 +492,14 @@ __global__ __launch_bounds__(BLK) void k_setv_cols(const EvDesc* __restrict__ ev
 }
 
 // rows inverse for every slice of the chunk in order; each thread keeps the
-// Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V (the
-// exponential by its Cayley-Hamilton series where that converges fast,
-// expq_forward)
-template <int N, int kInvBlock>
-__global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+// Wilson lines of its SPT sites in registers, V <- exp(-i A_k^a t^a) V
+// (expq_apply)
+template <int N, int kInvBlock, bool TWL = false, int WPE = 1>
+__global__ __launch_bounds__(kInvBlock, WPE) void k_setv_rows_inv(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
                                                             int kc) {
     constexpr int SPT = (N + kInvBlock - 1) / kInvBlock;
     __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
+    __shared__ double2 tws[TWL ? N : 1];
     const int e = blockIdx.x / (2 * N);
     const int n = (blockIdx.x / N) & 1;
     const int i = blockIdx.x % N;
@@ -499,6 +507,11 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
     if (!ev.active || k0 >= ev.Ny) return;
     const size_t S = ev.S;
     double* Vg = n ? ev.VB : ev.VA;
+    const double2* tw = pl.tw;
+    if constexpr (TWL) {
+        for (int m = threadIdx.x; m < N; m += kInvBlock) tws[m] = pl.tw[m];
+        tw = tws;  // visible after the first barrier of the slice loop
+    }
     M3 V[SPT];
 #pragma unroll
     for (int t = 0; t < SPT; t++) {
@@ -513,7 +526,7 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
             lds[ipgfft::at(N, p, j)] = in[(size_t)p * S + (size_t)i * N + j];
         }
         __syncthreads();
-        ipgfft::lds_fft<N, 4, kInvBlock, true>(lds, pl.tw);
+        ipgfft::lds_fft<N, 4, kInvBlock, true>(lds, tw);
 #pragma unroll
         for (int t = 0; t < SPT; t++) {
             const int j = threadIdx.x + t * kInvBlock;
@@ -525,7 +538,7 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
                     Q[2 * p] = -v.x;
                     Q[2 * p + 1] = -v.y;
                 }
-                V[t] = ipgad::expq_forward(Q) * V[t];
+                ipgad::expq_apply(Q, V[t]);
             }
         }
         __syncthreads();  // the next slice overwrites the LDS rows
@@ -537,13 +550,108 @@ __global__ __launch_bounds__(kInvBlock) void k_setv_rows_inv(const EvDesc* __res
     }
 }
 
+// this thread's share of the four rows at `in` (plane stride S), value l
+// of it at q = threadIdx.x + l BLK (row q / N, element q % N); unrolled by
+// recursion, so that every index into `pre` is a constant from the start
+// and the array never lands in scratch or LDS
+template <int N, int BLK, int L, int l = 0>
+__device__ __forceinline__ void setv_fetch_rows(double2 (&pre)[L], const double2* in, size_t S) {
+    if constexpr (l < L) {
+        const int q = (int)threadIdx.x + l * BLK;
+        if (q < 4 * N) {
+            const int p = q / N, j = q - p * N;
+            pre[l] = in[(size_t)p * S + j];
+        }
+        setv_fetch_rows<N, BLK, L, l + 1>(pre, in, S);
+    }
+}
+// the same values into the LDS rows
+template <int N, int BLK, int L, int l = 0>
+__device__ __forceinline__ void setv_put_rows(const double2 (&pre)[L], double2* lds) {
+    if constexpr (l < L) {
+        const int q = (int)threadIdx.x + l * BLK;
+        if (q < 4 * N) {
+            const int p = q / N, j = q - p * N;
+            lds[ipgfft::at(N, p, j)] = pre[l];
+        }
+        setv_put_rows<N, BLK, L, l + 1>(pre, lds);
+    }
+}
+
+// The same pass, software-pipelined: while slice kk is transformed and
+// exponentiated, each thread's share of slice kk+1's four rows is already
+// on its way into registers, and the twiddles sit in LDS, so that no
+// global load inside the loop queues behind that prefetch (vector-memory
+// loads retire in order).  WAVE: BLK = 256, each wavefront transforms one
+// of the four rows, synchronising itself only.  Same arithmetic as
+// k_setv_rows_inv, bit for bit.
+template <int N, int BLK, bool WAVE>
+__global__ __launch_bounds__(BLK) void k_setv_rows_inv_pf(const EvDesc* __restrict__ evs, FftPlanDev pl, int k0,
+                                                         int kc) {
+    constexpr int SPT = (N + BLK - 1) / BLK;
+    constexpr int L = (4 * N + BLK - 1) / BLK;  // prefetched values per thread
+    __shared__ double2 lds[ipgfft::lds_slots(N, 4)];
+    __shared__ double2 tws[N];
+    const int e = blockIdx.x / (2 * N);
+    const int n = (blockIdx.x / N) & 1;
+    const int i = blockIdx.x % N;
+    const EvDesc& ev = evs[e];
+    if (!ev.active || k0 >= ev.Ny) return;
+    const size_t S = ev.S;
+    for (int m = threadIdx.x; m < N; m += BLK) tws[m] = pl.tw[m];
+    double* Vg = n ? ev.VB : ev.VA;
+    const int kend = min(kc, ev.Ny - k0);
+    const double2* rows = ev.fftbuf + (size_t)n * kc * 4 * S + (size_t)i * N;
+    double2 pre[L];
+    setv_fetch_rows<N, BLK>(pre, rows, S);
+    M3 V[SPT];
+#pragma unroll
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * BLK;
+        V[t] = j < N ? ldm(Vg, S, (size_t)i * N + j) : m_one();
+    }
+    for (int kk = 0; kk < kend; kk++) {
+        setv_put_rows<N, BLK>(pre, lds);
+        if (kk + 1 < kend) setv_fetch_rows<N, BLK>(pre, rows + (size_t)(kk + 1) * 4 * S, S);
+        __syncthreads();
+        if constexpr (WAVE) {
+            static_assert(BLK == 256, "one wavefront per row");
+            ipgfft::wave_fft<N, 1, true>(lds + ipgfft::at(N, (int)(threadIdx.x >> 6), 0), tws);
+            __syncthreads();
+        } else {
+            ipgfft::lds_fft<N, 4, BLK, true>(lds, tws);
+        }
+#pragma unroll
+        for (int t = 0; t < SPT; t++) {
+            const int j = threadIdx.x + t * BLK;
+            if (j < N) {
+                double Q[8];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    double2 v = lds[ipgfft::at(N, p, j)];
+                    Q[2 * p] = -v.x;
+                    Q[2 * p + 1] = -v.y;
+                }
+                ipgad::expq_apply(Q, V[t]);
+            }
+        }
+        __syncthreads();  // the next slice overwrites the LDS rows
+    }
+#pragma unroll
+    for (int t = 0; t < SPT; t++) {
+        const int j = threadIdx.x + t * BLK;
+        if (j < N) stm(Vg, S, (size_t)i * N + j, V[t]);
+    }
+}
+
 // ---- reverse mode of setV (one event) ---------------------------------------------
 // V = E_{Ny-1} ... E_0, E_k = exp(-i A_k^a t^a).  Walking the slices of a chunk
 // backwards with the current V and its cotangent Vb in registers:
 //   V_k = E_k^+ V,  Eb_k = Vb V_k^+,  Vb <- E_k^+ Vb,  Ab_k = -Qb(Q = -A_k, Eb_k).
 // E_k is rebuilt from the Cayley-Hamilton coefficients its reverse needs
-// anyway where the forward's expq_forward takes the series (the same sums,
-// so V_k is recovered to round-off), and by exp_iQt above.
+// anyway (expq_fcoef: the forward's series below kExpqSeriesMax, its own
+// closed form above), equal to the forward's expq_apply to round-off, so
+// V_k is recovered to round-off.
 // The chunk's A_k are recomputed by the forward row and column passes; Ab_k
 // replaces A_k in the transform buffer, row by row.
 template <int N, int BLK>
@@ -588,7 +696,7 @@ __global__ __launch_bounds__(BLK) void k_adj_setv_rows(const EvDesc* __restrict_
                 // one set of Cayley-Hamilton coefficients for E and its reverse
                 ipgad::ExpQ x;
                 ipgad::expq_prep(Q, x);
-                const M3 E = ipgad::expq_forward_from(x, Q);
+                const M3 E = ipgad::expq_value(x);
                 const M3 Vk = mul_ad(E, V[t]);
                 const M3 Eb = mul_bd(Vb[t], Vk);
                 Vb[t] = mul_ad(E, Vb[t]);
@@ -1093,8 +1201,13 @@ void ipg_launch_stats(const EvDesc* d, int nev, int nblk, double* out, hipStream
 }
 // variants (the same arithmetic in other launch shapes; the autotuner picks
 // each): v.inv, v.fwd, v.cols = 0 for 256-thread blocks, 1 for 512-thread
-// blocks of the inverse row, forward row and column passes.  Each pass is
-// bracketed by its pair of events when they are given.
+// blocks of the inverse row, forward row and column passes, 2 for the
+// wavefront-synchronised transforms of the forward row and column passes;
+// v.inv = 2, 3: the prefetching inverse row pass in 512-thread blocks and
+// in 256-thread blocks with wavefront-synchronised transforms; v.inv = 4, 5
+// and v.fwd = 3: variants 0, 1 and 2 with the twiddles read from LDS (5 held
+// to 128 registers, four waves per SIMD).  Each pass is bracketed by its
+// pair of events when they are given.
 template <int N>
 static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, int kc, const SetvVariant& v,
                        const SetvEvents& t, hipStream_t st) {
@@ -1110,6 +1223,8 @@ static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, i
         hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow_fwd), dim3(512), 0, st, d, pl, k0, kc);
     else if (v.fwd == 2 && N >= 256)
         hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow_fwd), dim3(256), 0, st, d, pl, k0, kc);
+    else if (v.fwd == 3 && N >= 256)
+        hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true, true>), dim3(nrow_fwd), dim3(256), 0, st, d, pl, k0, kc);
     else
         hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow_fwd), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
     if (t.fwd[1]) (void)hipEventRecord(t.fwd[1], st);
@@ -1124,6 +1239,14 @@ static void setv_chunk(const EvDesc* d, int nev, const FftPlanDev& pl, int k0, i
     if (ev_a) (void)hipEventRecord(ev_a, st);
     if (inv_variant == 1)
         hipLaunchKernelGGL((k_setv_rows_inv<N, 512>), dim3(nrow_inv), dim3(512), 0, st, d, pl, k0, kc);
+    else if (inv_variant == 2)
+        hipLaunchKernelGGL((k_setv_rows_inv_pf<N, 512, false>), dim3(nrow_inv), dim3(512), 0, st, d, pl, k0, kc);
+    else if (inv_variant == 3)
+        hipLaunchKernelGGL((k_setv_rows_inv_pf<N, 256, true>), dim3(nrow_inv), dim3(256), 0, st, d, pl, k0, kc);
+    else if (inv_variant == 4)
+        hipLaunchKernelGGL((k_setv_rows_inv<N, 256, true>), dim3(nrow_inv), dim3(256), 0, st, d, pl, k0, kc);
+    else if (inv_variant == 5)
+        hipLaunchKernelGGL((k_setv_rows_inv<N, 512, true, 4>), dim3(nrow_inv), dim3(512), 0, st, d, pl, k0, kc);
     else
         hipLaunchKernelGGL((k_setv_rows_inv<N, 256>), dim3(nrow_inv), dim3(256), 0, st, d, pl, k0, kc);
     if (ev_b) (void)hipEventRecord(ev_b, st);
@@ -1174,6 +1297,8 @@ static void adj_setv_chunk(const EvDesc* d, const FftPlanDev& pl, int k0, int kc
             hipLaunchKernelGGL((k_setv_rows_fwd<N, 512>), dim3(nrow), dim3(512), 0, st, d, pl, k0, kc);
         else if (v.fwd == 2 && N >= 256)
             hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true>), dim3(nrow), dim3(256), 0, st, d, pl, k0, kc);
+        else if (v.fwd == 3 && N >= 256)
+            hipLaunchKernelGGL((k_setv_rows_fwd<N, 256, true, true>), dim3(nrow), dim3(256), 0, st, d, pl, k0, kc);
         else
             hipLaunchKernelGGL((k_setv_rows_fwd<N, IPG_BLOCK>), dim3(nrow), dim3(IPG_BLOCK), 0, st, d, pl, k0, kc);
         launch_cols<N>(d, pl, k0, kc, ncol, v.cols, st);

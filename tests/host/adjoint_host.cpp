@@ -102,6 +102,19 @@ void adjh_expq_forward(const double* q, double* e) {
     for (int a = 0; a < 8; a++) Q[a] = q[a];
     st(e, expq_forward(Q));
 }
+// the setV slice step V <- exp(i Q^a t^a) V (expq_apply) and det Q (gen_det)
+void adjh_expq_apply(const double* q, const double* v, double* out) {
+    double Q[8];
+    for (int a = 0; a < 8; a++) Q[a] = q[a];
+    M3 V = ld(v);
+    expq_apply(Q, V);
+    st(out, V);
+}
+double adjh_gen_det(const double* q) {
+    double Q[8];
+    for (int a = 0; a < 8; a++) Q[a] = q[a];
+    return gen_det(Q);
+}
 void adjh_expq(const double* q, const double* eb, double* e, double* qb) {
     double Q[8], Qb[8];
     for (int a = 0; a < 8; a++) Q[a] = q[a];

@@ -39,6 +39,9 @@ def lib():
         L.adjh_expq.argtypes = [dp] * 4
         L.adjh_expq_value.argtypes = [dp] * 2
         L.adjh_expq_forward.argtypes = [dp] * 2
+        L.adjh_expq_apply.argtypes = [dp] * 3
+        L.adjh_gen_det.argtypes = [dp]
+        L.adjh_gen_det.restype = C.c_double
         _lib = L
     return _lib
 
@@ -256,15 +259,23 @@ def test_su3_exponential_adjoint_against_frechet():
         assert np.abs(ev - E.reshape(-1)).max() <= 2e-15 * max(1.0, scale), (n, scale)
         if n not in exact_degenerate + near_degenerate:
             assert np.abs(ev - (e[0::2] + 1j * e[1::2])).max() <= 1e-13 * max(1.0, scale), (n, scale)
-        # the forward setV exponential: the series below c1 = 0.5, exp_iQt
-        # above; the matrix exponential to round-off except where exp_iQt
-        # itself is off (degenerate spectra above c1 = 0.5)
+        # the forward setV exponential: the series below c1 = 0.5, above it
+        # the series of Q / 2^s applied 2^s times; the matrix exponential to
+        # round-off everywhere, degenerate spectra included (where exp_iQt
+        # returns the identity or loses digits)
         ef = np.zeros(18)
         L.adjh_expq_forward(Q, ef)
         ef = ef[0::2] + 1j * ef[1::2]
         c1 = 0.25 * float(np.sum(Q * Q))
-        if c1 < 0.5 or n not in exact_degenerate + near_degenerate:
-            assert np.abs(ef - E.reshape(-1)).max() <= 2e-13 * max(1.0, scale), (n, scale, c1)
+        assert np.abs(ef - E.reshape(-1)).max() <= 1e-14 * max(1.0, scale), (n, scale, c1)
+        # applied to a Wilson line: E V to round-off; det Q from the
+        # components against the determinant of the matrix
+        V = sl.expm(1j * np.einsum("a,aij->ij", rng.normal(size=8), lam))
+        va = np.zeros(18)
+        L.adjh_expq_apply(Q, cm(V), va)
+        assert np.abs(va[0::2] + 1j * va[1::2] - (E @ V).reshape(-1)).max() <= 1e-14 * max(1.0, scale), (n, c1)
+        Hq = np.einsum("a,aij->ij", Q, lam)
+        assert abs(L.adjh_gen_det(Q) - np.linalg.det(Hq).real) <= 1e-15 * max(1.0, scale) ** 3, n
     # tiny charges: the series keeps the exponential (1 + iQ - Q^2/2 to
     # round-off) where the reference's closed form, in this host build as in
     # the oracle, is off by up to |Q| (catastrophic cancellation)

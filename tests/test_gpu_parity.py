@@ -347,6 +347,45 @@ def test_step_builds_agree():
         np.testing.assert_allclose(o[k], plain[k], rtol=1e-11, atol=1e-13, err_msg=k)
 
 
+def test_setv_launch_shapes_agree():
+    """N=256, one chunk of six slices: while the tuner measures the six
+    inverse-row launch shapes (256- and 512-thread blocks, each also with
+    the next slice prefetched into registers and with the twiddles in LDS),
+    the four forward-row and the three column shapes, three runs each,
+    repeated initialisations of one event give the same Wilson lines bit for
+    bit (the same arithmetic per site)"""
+    import ipglasma_amd as I
+    p = K.params("AuAu", N=256, Ny=6)
+    key = "setv_rows_inv_block/N=256/events=1/kc=6"
+    assert key not in I.autotune_report()
+    seed, ref = None, None
+    for run in range(19):
+        for s in ([seed] if seed is not None else range(61, 81)):
+            ev = I.Event(p, seed=s)
+            ev.init_nucleons()
+            if ev.init_color_charge():
+                seed = s
+                break
+            ev.close()
+        assert seed is not None
+        ev.init_wilson_lines()
+        V = (ev.field("VA"), ev.field("VB"))
+        ev.init_fields()  # synchronises and hands the timings to the tuner
+        ev.close()
+        if ref is None:
+            ref = V
+            # Wilson lines in SU(3): unitary to round-off
+            M = (V[0][:, 0::2] + 1j * V[0][:, 1::2]).reshape(-1, 3, 3)
+            err = np.abs(M @ np.conj(np.swapaxes(M, 1, 2)) - np.eye(3)).max()
+            assert err < 1e-12, err
+        else:
+            for a, b in zip(V, ref):
+                np.testing.assert_array_equal(a, b, err_msg=f"run {run}")
+    rep = I.autotune_report()
+    assert key in rep and len(rep[key]["times_ms"]) == 6, rep.get(key)
+    assert all(t > 0 for t in rep[key]["times_ms"]), rep[key]
+
+
 def test_full_size_pbpb_event_invariants():
     """The throughput workload at its full size (Pb+Pb b=0, N=512, L=30 fm,
     Ny=50, tau_max=0.4 fm/c, itmax=69), where an oracle event takes minutes,

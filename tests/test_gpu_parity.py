@@ -347,16 +347,17 @@ def test_step_builds_agree():
         np.testing.assert_allclose(o[k], plain[k], rtol=1e-11, atol=1e-13, err_msg=k)
 
 
-def test_setv_launch_shapes_agree():
-    """N=256, one chunk of six slices: while the tuner measures the six
-    inverse-row launch shapes (256- and 512-thread blocks, each also with
-    the next slice prefetched into registers and with the twiddles in LDS),
-    the four forward-row and the three column shapes, three runs each,
-    repeated initialisations of one event give the same Wilson lines bit for
-    bit (the same arithmetic per site)"""
+@pytest.mark.parametrize("system,N,Ny", [("AuAu", 256, 6), ("PbPb", 512, 5)])
+def test_setv_launch_shapes_agree(system, N, Ny):
+    """One chunk of Ny slices: while the tuner measures the six inverse-row
+    launch shapes (256- and 512-thread blocks, each also with the next slice
+    prefetched into registers and with the twiddles in LDS), the four
+    forward-row and the three column shapes, three runs each, repeated
+    initialisations of one event give the same Wilson lines bit for bit (the
+    same arithmetic per site)"""
     import ipglasma_amd as I
-    p = K.params("AuAu", N=256, Ny=6)
-    key = "setv_rows_inv_block/N=256/events=1/kc=6"
+    p = K.params(system, N=N, Ny=Ny)
+    key = f"setv_rows_inv_block/N={N}/events=1/kc={Ny}"
     assert key not in I.autotune_report()
     seed, ref = None, None
     for run in range(19):

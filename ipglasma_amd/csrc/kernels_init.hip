@@ -56,6 +56,32 @@ __device__ __forceinline__ void normals8(uint64_t seed, uint32_t c1, uint32_t c2
 }
 
 // ---- T_p -----------------------------------------------------------------------
+// A nucleon's thickness is a Gaussian in the distance b to it; without the
+// proton anisotropy (xi = 0) it factorises over the lattice axes,
+// exp(-b^2 / (2 BG hbarc^2)) = gx(n, i) gy(n, j), so k_tp_g tabulates the
+// 2 (A1 + A2) N factors (one exp per nucleon and lattice line, in the
+// transform buffer, which setV has not yet taken) and k_tp sums their
+// products: one multiply per nucleon and site instead of one exp.  With
+// xi != 0, or a table that would not fit, k_tp takes the exp per nucleon and
+// site (the reference's own order, Init.cpp:1009-1115).
+__device__ __forceinline__ bool tp_separable(const EvDesc& ev) {
+    return ev.xi == 0. && 2LL * (ev.A1 + ev.A2) * ev.N <= 72LL * (long long)ev.S;
+}
+__global__ __launch_bounds__(IPG_BLOCK) void k_tp_g(const EvDesc* __restrict__ evs, int nev) {
+    const int e = find_event(evs, nev, blockIdx.x);
+    const EvDesc& ev = evs[e];
+    if (!ev.active || !tp_separable(ev)) return;
+    const int A = ev.A1 + ev.A2, N = ev.N;
+    const long long AN = (long long)A * N, stride = (long long)ev.nblocks * IPG_BLOCK;
+    double* g = reinterpret_cast<double*>(ev.fftbuf);  // [gx: A][N], then [gy: A][N]
+    for (long long t = (long long)(blockIdx.x - ev.block0) * IPG_BLOCK + threadIdx.x; t < 2 * AN; t += stride) {
+        const int xy = t >= AN;
+        const long long r = t - xy * AN;
+        const int n = (int)(r / N), i = (int)(r - (long long)n * N);
+        const double d = ev.nuc[4 * n + xy] - (-ev.L / 2. + ev.a * i);
+        g[t] = exp(-(d * d / (HBARC * HBARC)) / (2. * ev.BG));
+    }
+}
 __global__ __launch_bounds__(IPG_BLOCK) void k_tp(const EvDesc* __restrict__ evs, int nev) {
     const int e = find_event(evs, nev, blockIdx.x);
     const EvDesc& ev = evs[e];
@@ -64,10 +90,20 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_tp(const EvDesc* __restrict__ evs
     if (s >= ev.S) return;
     const int N = ev.N;
     const int ix = (int)(s / N), iy = (int)(s - (long long)ix * N);
-    const double x = -ev.L / 2. + ev.a * ix, y = -ev.L / 2. + ev.a * iy;
     const double xi = ev.xi, BG = ev.BG;
     const double pre = sqrt(1 + xi), nrm = 2. * M_PI * BG;
     double tA = 0., tB = 0.;
+    if (tp_separable(ev)) {
+        const int A = ev.A1 + ev.A2;
+        const double* gx = reinterpret_cast<const double*>(ev.fftbuf) + ix;
+        const double* gy = reinterpret_cast<const double*>(ev.fftbuf) + (size_t)A * N + iy;
+        for (int n = 0; n < ev.A1; n++) tA += gx[(size_t)n * N] * gy[(size_t)n * N];
+        for (int n = ev.A1; n < A; n++) tB += gx[(size_t)n * N] * gy[(size_t)n * N];
+        ev.TpA[s] = pre * tA / nrm;
+        ev.TpB[s] = pre * tB / nrm;
+        return;
+    }
+    const double x = -ev.L / 2. + ev.a * ix, y = -ev.L / 2. + ev.a * iy;
     for (int n = 0; n < ev.A1 + ev.A2; n++) {
         const double xm = ev.nuc[4 * n], ym = ev.nuc[4 * n + 1];
         double dx = xm - x, dy = ym - y;
@@ -1187,6 +1223,7 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_zero_fields(const EvDesc* __restr
 }  // namespace
 
 void ipg_launch_tp(const EvDesc* d, int nev, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_tp_g, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
     hipLaunchKernelGGL(k_tp, dim3(nblk), dim3(IPG_BLOCK), 0, st, d, nev);
 }
 void ipg_launch_qs(const EvDesc* d, int nev, int nblk, const double* Tl, const double* Q, int* flag,

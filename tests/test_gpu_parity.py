@@ -56,9 +56,12 @@ def test_nucleons_bit_exact(system, seed):
     assert o["b"] == info.b and o["phiRP"] == info.phiRP
 
 
-@pytest.mark.parametrize("system,seed", [("AuAu", 5), ("PbPb", 7), ("pPb", 2)])
-def test_color_charge(system, seed):
-    p = K.params(system, N=64)
+@pytest.mark.parametrize("system,seed,xi", [("AuAu", 5, 0.0), ("PbPb", 7, 0.0), ("pPb", 2, 0.0), ("pPb", 2, 0.6)])
+def test_color_charge(system, seed, xi):
+    """T_p (by axis factors without the proton anisotropy, per nucleon and
+    site with it), Q_s^2, g^2 mu^2 and the event statistics against the
+    oracle"""
+    p = K.params(system, N=64, protonAnisotropy=xi)
     ev = event(p, seed)
     ev.init_nucleons()
     ok = ev.init_color_charge()

@@ -343,17 +343,22 @@ __global__ __launch_bounds__(IPG_BLOCK) void k_stats(const EvDesc* __restrict__ 
         const double g2B = (s > 0 && s < (long long)(N - 1) * N + N - 1) ? ev.g2B[s] : 0.;
         const double fA = rA * rA / a / a * HBARC * HBARC * g * g, fB = rB * rB / a / a * HBARC * HBARC * g * g;
         v[4] = g2B >= g2A ? g2A * fA : g2B * fB;
+        // a wounded nucleon of A, then one of B, within rcut; the squared
+        // distance decides away from rcut^2 (margins of 1e-12 keep every
+        // such decision equal to sqrt(d^2) < rcut), sqrt only next to it
+        const double in2 = rcut * rcut * (1. - 1e-12), out2 = rcut * rcut * (1. + 1e-12);
+        auto near = [&](int n) {
+            const double dx = x - ev.nuc[4 * n], dy = y - ev.nuc[4 * n + 1];
+            const double d2 = dx * dx + dy * dy;
+            const bool inside = d2 < in2 || (d2 <= out2 && sqrt(d2) < rcut);
+            return inside && ev.nuc[4 * n + 3] == 1.;
+        };
         int check = 0;
-        for (int n = 0; n < ev.A1; n++) {
-            double dx = x - ev.nuc[4 * n], dy = y - ev.nuc[4 * n + 1];
-            if (sqrt(dx * dx + dy * dy) < rcut && ev.nuc[4 * n + 3] == 1.) check = 1;
-        }
-        if (check == 1) {
-            for (int n = ev.A1; n < ev.A1 + ev.A2; n++) {
-                double dx = x - ev.nuc[4 * n], dy = y - ev.nuc[4 * n + 1];
-                if (sqrt(dx * dx + dy * dy) < rcut && ev.nuc[4 * n + 3] == 1.) check = 2;
-            }
-        }
+        for (int n = 0; n < ev.A1 && check == 0; n++)
+            if (near(n)) check = 1;
+        if (check == 1)
+            for (int n = ev.A1; n < ev.A1 + ev.A2 && check == 1; n++)
+                if (near(n)) check = 2;
         if (check == 2) {
             if (g2B > g2A) {
                 v[0] = sqrt(g2B * fB);

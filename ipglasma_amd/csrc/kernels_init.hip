@@ -48,8 +48,11 @@ __device__ __forceinline__ void normals8(uint64_t seed, uint32_t c1, uint32_t c2
         uint32_t c[4] = {s, c1, c2, p};
         philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
         double r = sqrt(-2.0 * log(u53(c[0], c[1])));
+        // sin and cos of 2 pi u2 as sincospi(2 u2): the doubling is exact,
+        // the reduction of the argument cheaper than sincos's (the oracle's
+        // cos(2 pi u2) differs by the rounding of 2 pi u2, ~1e-16)
         double sn, cs;
-        sincos(6.283185307179586 * u53(c[2], c[3]), &sn, &cs);
+        sincospi(2.0 * u53(c[2], c[3]), &sn, &cs);
         z[2 * p] = r * cs;
         z[2 * p + 1] = r * sn;
     }
